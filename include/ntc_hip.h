@@ -1,0 +1,319 @@
+/* ntc_hip.h - C ABI of the MI355X (gfx950) kernel library.
+ *
+ * This is the layer that replaces the reference's kernel launch functions
+ * (libopenntc/ntc_kernel.cuh:1-78) and its three cuBLAS GEMM wrappers plus
+ * the Saxpy/Sdot loss calls (libopenntc/libopenntc.cpp:397-417, 659-668).
+ * Differences from those, by design:
+ *   - every entry point returns an int status (0 = success, NTC_E_* or a
+ *     positive hipError_t) instead of void;
+ *   - every entry point takes an explicit stream;
+ *   - quantisation noise is regenerated inside the gather from a counter-based
+ *     generator (Philox4x32-10, Salmon et al., SC'11) keyed by
+ *     (seed, step, grid, level, cell, channel) instead of being read from a
+ *     per-step noise buffer filled by cuRAND;
+ *   - buffers are caller-owned device memory; no entry point allocates except
+ *     where stated.
+ * All pointers are device pointers unless the comment says "host".
+ * The header is plain C and needs no HIP header.
+ */
+#ifndef NTC_HIP_H
+#define NTC_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ihipStream_t* hipStream_t;
+
+#define NTC_OK 0
+#define NTC_E_INVALID (-1)    /* argument out of range / inconsistent shapes */
+#define NTC_E_WORKSPACE (-2)  /* workspace too small */
+#define NTC_E_NODEVICE (-3)   /* no usable gfx950 device */
+
+/* g0 feature modes */
+#define NTC_G0_CORNERS 0   /* 4*c0 slots: the 4 bilinear corners, unweighted */
+#define NTC_G0_BILINEAR 1  /* c0 slots: bilinearly weighted */
+
+/* activations */
+#define NTC_ACT_HARDGELU 0
+#define NTC_ACT_RELU 1
+
+/* GEMM forms, row-major (libopenntc.cpp:395-417):
+ *   AB : C[n][m] = A[n][k] * B[k][m]
+ *   ATB: C[n][m] = A[k][n]^T * B[k][m]
+ *   ABT: C[n][m] = A[n][k] * B[m][k]^T   (k split in chunks of
+ *        NTC_GEMM_KCHUNK, each summed in order, chunk sums added in order) */
+#define NTC_GEMM_AB 0
+#define NTC_GEMM_ATB 1
+#define NTC_GEMM_ABT 2
+#define NTC_GEMM_KCHUNK 1024
+
+typedef struct ntc_feature_desc
+{
+  int32_t g0_dim;          /* side of the active g0 level */
+  int32_t g1_dim;          /* side of the active g1 level */
+  int32_t g0_channels;
+  int32_t g1_channels;
+  int32_t g0_mode;         /* NTC_G0_CORNERS or NTC_G0_BILINEAR */
+  int32_t pe_first_period; /* 8 for the reference layout */
+  int32_t pe_count;        /* periods first, first/2, ...; 4 slots each */
+  int32_t lod_slot;        /* 1: one slot holding tiles.norm_lod */
+  int32_t feature_dim;     /* F; slots past the above are zero */
+} ntc_feature_desc;
+
+typedef struct ntc_tiles
+{
+  const int32_t* origins;  /* 2*n_tiles ints (x0, y0) per tile */
+  int32_t n_tiles;
+  int32_t tile_dim;        /* T: texels per tile side (wrapping at mip_dim) */
+  int32_t mip_dim;         /* M: side of the sampled mip */
+  float norm_lod;          /* lod / (mip_count - 1) */
+} ntc_tiles;
+
+typedef struct ntc_noise
+{
+  uint32_t seed;
+  uint32_t step;      /* training step index */
+  int32_t level;      /* active pyramid level */
+  int32_t enabled;    /* 0: no noise (after the lock, eval) */
+  float delta0;       /* g0 quantisation step 2/2^b0 */
+  float delta1;       /* g1 quantisation step 2/2^b1 */
+} ntc_noise;
+
+/* ---- library ------------------------------------------------------------ */
+const char* ntc_version(void);
+/* 0 when a gfx950 device is usable; fills *count with the device count. */
+int ntc_device_check(int32_t* count);
+const char* ntc_status_string(int status);
+
+/* ---- gather (replaces launch_draw_features, ntc_kernel.cu:75-214) -------- */
+/* X[f][b*T*T + xy], f < F, B = n_tiles*T*T. */
+int ntc_draw_features_f32(const ntc_feature_desc* fd, const ntc_tiles* tiles,
+                          const ntc_noise* noise, const float* g0,
+                          const float* g1, float* x, hipStream_t stream);
+int ntc_draw_features_f64(const ntc_feature_desc* fd, const ntc_tiles* tiles,
+                          const ntc_noise* noise, const double* g0,
+                          const double* g1, double* x, hipStream_t stream);
+
+/* ---- targets (replaces launch_draw_targets, ntc_kernel.cu:216-259) ------- */
+/* Y[o][b] = package[(y*M + x)*O + o] / 255 */
+int ntc_draw_targets_f32(const ntc_tiles* tiles, int32_t out_dim,
+                         const uint8_t* package, float* y, hipStream_t stream);
+int ntc_draw_targets_f64(const ntc_tiles* tiles, int32_t out_dim,
+                         const uint8_t* package, double* y, hipStream_t stream);
+
+/* ---- GEMM (replaces matmulAB/ATB/ABT + cublasSgemm) ----------------------- */
+size_t ntc_gemm_workspace_bytes(int32_t op, int32_t n, int32_t m, int32_t k,
+                                int32_t elem_size);
+int ntc_gemm_f32(int32_t op, int32_t n, int32_t m, int32_t k, const float* a,
+                 const float* b, float* c, void* workspace,
+                 size_t workspace_bytes, hipStream_t stream);
+int ntc_gemm_f64(int32_t op, int32_t n, int32_t m, int32_t k, const double* a,
+                 const double* b, double* c, void* workspace,
+                 size_t workspace_bytes, hipStream_t stream);
+
+/* ---- activations (replace forward/backward_hardgelu, ntc_kernel.cu:13-48) */
+int ntc_act_forward_f32(int32_t act, int64_t n, const float* z, float* a,
+                        hipStream_t stream);
+int ntc_act_backward_f32(int32_t act, int64_t n, const float* z,
+                         const float* g, float* out, hipStream_t stream);
+int ntc_act_forward_f64(int32_t act, int64_t n, const double* z, double* a,
+                        hipStream_t stream);
+int ntc_act_backward_f64(int32_t act, int64_t n, const double* z,
+                         const double* g, double* out, hipStream_t stream);
+
+/* ---- loss (replaces Saxpy + Sdot + scalar_product, libopenntc.cpp:659-668)
+ * r = y_target - y_pred; dy = scale * r; *sse = sum r^2 accumulated in FP64
+ * (two-pass, deterministic). workspace: ntc_loss_workspace_bytes(n). */
+size_t ntc_loss_workspace_bytes(int64_t n);
+int ntc_loss_f32(int64_t n, float scale, const float* y_target,
+                 const float* y_pred, float* dy, double* sse, void* workspace,
+                 hipStream_t stream);
+int ntc_loss_f64(int64_t n, double scale, const double* y_target,
+                 const double* y_pred, double* dy, double* sse,
+                 void* workspace, hipStream_t stream);
+
+/* Per-channel SSE: sse[o] += sum_b (y_target[o][b] - y_pred[o][b])^2 in FP64,
+ * O <= 16, deterministic. workspace: ntc_channel_sse_workspace_bytes(). */
+size_t ntc_channel_sse_workspace_bytes(void);
+int ntc_channel_sse_f32(int32_t out_dim, int64_t batch, const float* y_target,
+                        const float* y_pred, double* sse, void* workspace,
+                        hipStream_t stream);
+
+/* ---- latent-gradient scatter (replaces accumulate_grid_gradients,
+ * ntc_kernel.cu:261-373). Adds into dg0/dg1; the caller zeroes them. -------- */
+int ntc_accumulate_grid_gradients_f32(const ntc_feature_desc* fd,
+                                      const ntc_tiles* tiles, const float* dx,
+                                      float* dg0, float* dg1,
+                                      hipStream_t stream);
+int ntc_accumulate_grid_gradients_f64(const ntc_feature_desc* fd,
+                                      const ntc_tiles* tiles, const double* dx,
+                                      double* dg0, double* dg1,
+                                      hipStream_t stream);
+
+/* ---- optimiser and grid maintenance ------------------------------------- */
+/* Dense Adam (update_adam, ntc_kernel.cu:375-410). */
+int ntc_adam_f32(int64_t n, float lr, float beta1, float beta2, float bias1,
+                 float bias2, const float* grad, float* m, float* v, float* p,
+                 hipStream_t stream);
+/* Fused grid update: Adam, then clamp p to [lo, hi], then grad = 0; one pass
+ * over p, m, v, grad (update_adam + clamp_grid + the next step's FillZero). */
+int ntc_grid_update_f32(int64_t n, float lr, float beta1, float beta2,
+                        float bias1, float bias2, float lo, float hi,
+                        float* grad, float* m, float* v, float* p,
+                        hipStream_t stream);
+/* clamp_grid (ntc_kernel.cu:468-482): g = clamp(g, -1, 1 - delta) */
+int ntc_clamp_grid_f32(int64_t n, float delta, float* g, hipStream_t stream);
+/* quantize_grid (ntc_kernel.cu:448-466) */
+int ntc_quantize_grid_f32(int64_t n, int32_t bits, float delta, float* g,
+                          hipStream_t stream);
+/* quantize_pack (ntc_kernel.cu:486-507): n*bits must be a multiple of 32 */
+int ntc_quantize_pack_f32(int64_t n, int32_t bits, const float* in,
+                          uint32_t* out, hipStream_t stream);
+/* max_abs (ntc_kernel.cu:511-531): *result = max(*result, max|x|) */
+int ntc_max_abs_f32(int64_t n, const float* x, float* result,
+                    hipStream_t stream);
+/* zero-fill by kernel (graph-safe replacement for memset) */
+int ntc_fill_zero(void* p, size_t bytes, hipStream_t stream);
+
+/* ---- mip targets (prepare_tex / filter_lanczos / package_tex,
+ * ntc_kernel.cu:533-686) ---------------------------------------------------- */
+/* out[y][x][i] = tex[y][x][cmap[i]] for i < count of leading valid cmap
+ * entries (cmap is host memory, 4 ints, -1 terminates). */
+int ntc_prepare_tex(int32_t dim, int32_t src_channels, const int32_t* cmap,
+                    const uint8_t* tex, uint8_t* out, hipStream_t stream);
+/* Separable 2:1 Lanczos-3 downsample of a dim_src^2 x channels uint8 image
+ * (channels <= 4); tmp holds (dim_src/2)*dim_src*channels floats. */
+int ntc_filter_lanczos(int32_t dim_src, int32_t channels, const uint8_t* src,
+                       float* tmp, uint8_t* dst, hipStream_t stream);
+/* Interleave sources into [dim][dim][out_channels]. sources, source_channels,
+ * feat_source and feat_channel are host arrays. */
+int ntc_package_tex(int32_t dim, int32_t out_channels, int32_t n_sources,
+                    const uint8_t* const* sources,
+                    const int32_t* source_channels,
+                    const int32_t* feat_source, const int32_t* feat_channel,
+                    uint8_t* out, hipStream_t stream);
+
+/* ---- Context shim --------------------------------------------------------
+ * Flat C entry points over openntc::Context (include/libopenntc/libopenntc.h)
+ * for callers that bind through an FFI (ctypes, cgo, JNI). Status values are
+ * openntc::Result cast to int. */
+typedef struct ntc_context ntc_context;
+
+typedef struct ntc_layout_info
+{
+  int32_t g0_bits, g1_bits, g0_channels, g1_channels, g0_scale, hidden_dim;
+  int32_t feature_layout; /* 0 reference, 1 bilinear */
+  int32_t activation;     /* NTC_ACT_* */
+} ntc_layout_info;
+
+typedef struct ntc_init_ex
+{
+  int32_t profile;
+  int32_t use_layout;
+  ntc_layout_info layout;
+  int32_t min_dimension;
+  int32_t mip_min_dim;
+  uint32_t seed;
+} ntc_init_ex;
+
+typedef struct ntc_train_ex
+{
+  int32_t steps, grids_per_batch;
+  float lr_grid, lr_decoder, init_lo, init_hi;
+  int32_t path; /* 0 fused bf16, 1 plain fp32 */
+} ntc_train_ex;
+
+typedef struct ntc_progress
+{
+  int32_t phase, result, batches_complete, total_batches;
+} ntc_progress;
+
+/* Derived sizes of a loaded context. */
+typedef struct ntc_ctx_layout
+{
+  int32_t dim, mip_count, level_count, out_dim;
+  int32_t mip_dim[12];
+  int32_t g0_dim[6], g1_dim[6];
+  int32_t g0_channels, g1_channels, g0_bits, g1_bits, g0_scale;
+  int32_t feature_dim, feature_dim_padded, hidden_dim, out_dim_padded;
+  int32_t activation;
+  ntc_feature_desc feat; /* with g0_dim/g1_dim of level 0 */
+  float g0_delta, g1_delta;
+  int32_t lock_step, total_steps, grids_per_batch, batch_index;
+  uint32_t seed;
+} ntc_ctx_layout;
+
+/* One step of the precomputed schedule. */
+typedef struct ntc_step_info
+{
+  int32_t lod, level, tile_dim, n_tiles;
+  int32_t origins[16];
+  float lr_grid, lr_decoder;
+  float bias1, bias2, grid_bias1, grid_bias2;
+  int32_t grid_update; /* 1 before the lock */
+  int32_t lock;        /* 1 on the lock step */
+} ntc_step_info;
+
+ntc_context* ntc_ctx_create(void);
+void ntc_ctx_destroy(ntc_context* ctx);
+int ntc_ctx_init(ntc_context* ctx, int32_t profile);
+int ntc_ctx_init_ex(ntc_context* ctx, const ntc_init_ex* info);
+/* rgba8: host pointers, count entries; channels: strings like "RGB". */
+int ntc_ctx_load_textures(ntc_context* ctx, int32_t count,
+                          const uint8_t* const* rgba8,
+                          const int32_t* semantics,
+                          const char* const* channels, int32_t dim);
+int ntc_ctx_load_manifest(ntc_context* ctx, const char* path);
+void ntc_ctx_unload_manifest(ntc_context* ctx);
+int ntc_ctx_begin_training(ntc_context* ctx, int32_t quality,
+                           int32_t grids_per_batch);
+int ntc_ctx_begin_training_ex(ntc_context* ctx, const ntc_train_ex* info);
+int ntc_ctx_train(ntc_context* ctx, int32_t n, ntc_progress* out);
+int ntc_ctx_train_until_complete(ntc_context* ctx, ntc_progress* out);
+int ntc_ctx_eval(ntc_context* ctx, int32_t precision, double* mse,
+                 double* psnr, double* channel_sse /* host, 16 or NULL */);
+int ntc_ctx_calibrate(ntc_context* ctx, float headroom, float* out4);
+int ntc_ctx_get_last_error(ntc_context* ctx);
+int ntc_ctx_get_layout(ntc_context* ctx, ntc_ctx_layout* out);
+int ntc_ctx_step_info(ntc_context* ctx, int32_t step, ntc_step_info* out);
+int ntc_ctx_feature_level_for_lod(ntc_context* ctx, int32_t lod);
+/* which: 0 g0, 1 g1, 2 g0 Adam m, 3 g0 Adam v, 4 g1 m, 5 g1 v, 6 dg0, 7 dg1 */
+int ntc_ctx_read_grid(ntc_context* ctx, int32_t which, int32_t level,
+                      float* host);
+int ntc_ctx_write_grid(ntc_context* ctx, int32_t which, int32_t level,
+                       const float* host);
+/* which: 0 weights, 1 Adam m, 2 Adam v; layout W0[H][F] | W1[H][H] |
+ * Wout[16][H] */
+int ntc_ctx_read_decoder(ntc_context* ctx, int32_t which, float* host);
+int ntc_ctx_write_decoder(ntc_context* ctx, int32_t which, const float* host);
+int ntc_ctx_read_package(ntc_context* ctx, int32_t mip, uint8_t* host);
+/* Copies the compressed view: packed g0/g1 words (all levels), decoder blob.
+ * Sizes in bytes via the size query with NULL buffers. */
+int ntc_ctx_get_compressed(ntc_context* ctx, uint8_t* g0, size_t* g0_size,
+                           uint8_t* g1, size_t* g1_size, uint8_t* decoder,
+                           size_t* decoder_size, float* caldata4);
+int ntc_ctx_dump(ntc_context* ctx, const char* path);
+/* Device pointer to the context's stream (for timing with events). */
+void* ntc_ctx_stream(ntc_context* ctx);
+/* Average duration (ms) of the dominant kernels over the last Train call,
+ * measured with events on the context's stream; -1 when not recorded. */
+int ntc_ctx_kernel_times(ntc_context* ctx, float* out, int32_t n);
+int ntc_ctx_set_timing(ntc_context* ctx, int32_t enabled);
+
+/* ---- synthetic PBR set ----------------------------------------------------
+ * Seeded 9-channel set: albedo RGB (value noise through sRGB decode),
+ * normal RGB (3-octave value-noise height field), roughness / metallic / AO
+ * (independent value noise). Writes 3 RGBA8 images of dim*dim*4 bytes (host):
+ * albedo, normal, orm (R = AO, G = roughness, B = metallic). */
+int ntc_synthetic_pbr(int32_t dim, uint32_t seed, uint8_t* albedo,
+                      uint8_t* normal, uint8_t* orm);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NTC_HIP_H */

@@ -1,0 +1,180 @@
+"""ctypes binding of the C ABI in include/ntc_hip.h (libopenntc_amd.so).
+
+The library is built in-tree by ``python -m openntc_amd.build`` (or
+``__graft_entry__.build()``). Loading fails loudly when it is missing: there
+is no fallback implementation.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libopenntc_amd.so")
+
+NTC_OK = 0
+NTC_E_INVALID = -1
+NTC_E_WORKSPACE = -2
+NTC_E_NODEVICE = -3
+
+G0_CORNERS, G0_BILINEAR = 0, 1
+ACT_HARDGELU, ACT_RELU = 0, 1
+GEMM_AB, GEMM_ATB, GEMM_ABT = 0, 1, 2
+GEMM_KCHUNK = 1024
+
+
+class FeatureDesc(C.Structure):
+    _fields_ = [("g0_dim", C.c_int32), ("g1_dim", C.c_int32), ("g0_channels", C.c_int32),
+                ("g1_channels", C.c_int32), ("g0_mode", C.c_int32), ("pe_first_period", C.c_int32),
+                ("pe_count", C.c_int32), ("lod_slot", C.c_int32), ("feature_dim", C.c_int32)]
+
+
+class Tiles(C.Structure):
+    _fields_ = [("origins", C.c_void_p), ("n_tiles", C.c_int32), ("tile_dim", C.c_int32),
+                ("mip_dim", C.c_int32), ("norm_lod", C.c_float)]
+
+
+class Noise(C.Structure):
+    _fields_ = [("seed", C.c_uint32), ("step", C.c_uint32), ("level", C.c_int32),
+                ("enabled", C.c_int32), ("delta0", C.c_float), ("delta1", C.c_float)]
+
+
+class LayoutInfo(C.Structure):
+    _fields_ = [("g0_bits", C.c_int32), ("g1_bits", C.c_int32), ("g0_channels", C.c_int32),
+                ("g1_channels", C.c_int32), ("g0_scale", C.c_int32), ("hidden_dim", C.c_int32),
+                ("feature_layout", C.c_int32), ("activation", C.c_int32)]
+
+
+class InitEx(C.Structure):
+    _fields_ = [("profile", C.c_int32), ("use_layout", C.c_int32), ("layout", LayoutInfo),
+                ("min_dimension", C.c_int32), ("mip_min_dim", C.c_int32), ("seed", C.c_uint32)]
+
+
+class TrainEx(C.Structure):
+    _fields_ = [("steps", C.c_int32), ("grids_per_batch", C.c_int32), ("lr_grid", C.c_float),
+                ("lr_decoder", C.c_float), ("init_lo", C.c_float), ("init_hi", C.c_float),
+                ("path", C.c_int32)]
+
+
+class Progress(C.Structure):
+    _fields_ = [("phase", C.c_int32), ("result", C.c_int32), ("batches_complete", C.c_int32),
+                ("total_batches", C.c_int32)]
+
+
+class CtxLayout(C.Structure):
+    _fields_ = [("dim", C.c_int32), ("mip_count", C.c_int32), ("level_count", C.c_int32),
+                ("out_dim", C.c_int32), ("mip_dim", C.c_int32 * 12), ("g0_dim", C.c_int32 * 6),
+                ("g1_dim", C.c_int32 * 6), ("g0_channels", C.c_int32), ("g1_channels", C.c_int32),
+                ("g0_bits", C.c_int32), ("g1_bits", C.c_int32), ("g0_scale", C.c_int32),
+                ("feature_dim", C.c_int32), ("feature_dim_padded", C.c_int32),
+                ("hidden_dim", C.c_int32), ("out_dim_padded", C.c_int32), ("activation", C.c_int32),
+                ("feat", FeatureDesc), ("g0_delta", C.c_float), ("g1_delta", C.c_float),
+                ("lock_step", C.c_int32), ("total_steps", C.c_int32),
+                ("grids_per_batch", C.c_int32), ("batch_index", C.c_int32), ("seed", C.c_uint32)]
+
+
+class StepInfo(C.Structure):
+    _fields_ = [("lod", C.c_int32), ("level", C.c_int32), ("tile_dim", C.c_int32),
+                ("n_tiles", C.c_int32), ("origins", C.c_int32 * 16), ("lr_grid", C.c_float),
+                ("lr_decoder", C.c_float), ("bias1", C.c_float), ("bias2", C.c_float),
+                ("grid_bias1", C.c_float), ("grid_bias2", C.c_float), ("grid_update", C.c_int32),
+                ("lock", C.c_int32)]
+
+
+_P = C.c_void_p
+_I = C.c_int32
+_I64 = C.c_int64
+_U = C.c_uint32
+_F = C.c_float
+_D = C.c_double
+_SZ = C.c_size_t
+
+# name -> (restype, argtypes)
+SIGNATURES = {
+    "ntc_version": (C.c_char_p, []),
+    "ntc_device_check": (_I, [_P]),
+    "ntc_status_string": (C.c_char_p, [_I]),
+    "ntc_draw_features_f32": (_I, [_P, _P, _P, _P, _P, _P, _P]),
+    "ntc_draw_features_f64": (_I, [_P, _P, _P, _P, _P, _P, _P]),
+    "ntc_draw_targets_f32": (_I, [_P, _I, _P, _P, _P]),
+    "ntc_draw_targets_f64": (_I, [_P, _I, _P, _P, _P]),
+    "ntc_gemm_workspace_bytes": (_SZ, [_I, _I, _I, _I, _I]),
+    "ntc_gemm_f32": (_I, [_I, _I, _I, _I, _P, _P, _P, _P, _SZ, _P]),
+    "ntc_gemm_f64": (_I, [_I, _I, _I, _I, _P, _P, _P, _P, _SZ, _P]),
+    "ntc_act_forward_f32": (_I, [_I, _I64, _P, _P, _P]),
+    "ntc_act_backward_f32": (_I, [_I, _I64, _P, _P, _P, _P]),
+    "ntc_act_forward_f64": (_I, [_I, _I64, _P, _P, _P]),
+    "ntc_act_backward_f64": (_I, [_I, _I64, _P, _P, _P, _P]),
+    "ntc_loss_workspace_bytes": (_SZ, [_I64]),
+    "ntc_loss_f32": (_I, [_I64, _F, _P, _P, _P, _P, _P, _P]),
+    "ntc_loss_f64": (_I, [_I64, _D, _P, _P, _P, _P, _P, _P]),
+    "ntc_channel_sse_workspace_bytes": (_SZ, []),
+    "ntc_channel_sse_f32": (_I, [_I, _I64, _P, _P, _P, _P, _P]),
+    "ntc_accumulate_grid_gradients_f32": (_I, [_P, _P, _P, _P, _P, _P]),
+    "ntc_accumulate_grid_gradients_f64": (_I, [_P, _P, _P, _P, _P, _P]),
+    "ntc_adam_f32": (_I, [_I64, _F, _F, _F, _F, _F, _P, _P, _P, _P, _P]),
+    "ntc_grid_update_f32": (_I, [_I64, _F, _F, _F, _F, _F, _F, _F, _P, _P, _P, _P, _P]),
+    "ntc_clamp_grid_f32": (_I, [_I64, _F, _P, _P]),
+    "ntc_quantize_grid_f32": (_I, [_I64, _I, _F, _P, _P]),
+    "ntc_quantize_pack_f32": (_I, [_I64, _I, _P, _P, _P]),
+    "ntc_max_abs_f32": (_I, [_I64, _P, _P, _P]),
+    "ntc_fill_zero": (_I, [_P, _SZ, _P]),
+    "ntc_prepare_tex": (_I, [_I, _I, _P, _P, _P, _P]),
+    "ntc_filter_lanczos": (_I, [_I, _I, _P, _P, _P, _P]),
+    "ntc_package_tex": (_I, [_I, _I, _I, _P, _P, _P, _P, _P, _P]),
+    "ntc_ctx_create": (_P, []),
+    "ntc_ctx_destroy": (None, [_P]),
+    "ntc_ctx_init": (_I, [_P, _I]),
+    "ntc_ctx_init_ex": (_I, [_P, _P]),
+    "ntc_ctx_load_textures": (_I, [_P, _I, _P, _P, _P, _I]),
+    "ntc_ctx_load_manifest": (_I, [_P, C.c_char_p]),
+    "ntc_ctx_unload_manifest": (None, [_P]),
+    "ntc_ctx_begin_training": (_I, [_P, _I, _I]),
+    "ntc_ctx_begin_training_ex": (_I, [_P, _P]),
+    "ntc_ctx_train": (_I, [_P, _I, _P]),
+    "ntc_ctx_train_until_complete": (_I, [_P, _P]),
+    "ntc_ctx_eval": (_I, [_P, _I, _P, _P, _P]),
+    "ntc_ctx_calibrate": (_I, [_P, _F, _P]),
+    "ntc_ctx_get_last_error": (_I, [_P]),
+    "ntc_ctx_get_layout": (_I, [_P, _P]),
+    "ntc_ctx_step_info": (_I, [_P, _I, _P]),
+    "ntc_ctx_feature_level_for_lod": (_I, [_P, _I]),
+    "ntc_ctx_read_grid": (_I, [_P, _I, _I, _P]),
+    "ntc_ctx_write_grid": (_I, [_P, _I, _I, _P]),
+    "ntc_ctx_read_decoder": (_I, [_P, _I, _P]),
+    "ntc_ctx_write_decoder": (_I, [_P, _I, _P]),
+    "ntc_ctx_read_package": (_I, [_P, _I, _P]),
+    "ntc_ctx_get_compressed": (_I, [_P, _P, _P, _P, _P, _P, _P, _P]),
+    "ntc_ctx_dump": (_I, [_P, C.c_char_p]),
+    "ntc_ctx_stream": (_P, [_P]),
+    "ntc_ctx_kernel_times": (_I, [_P, _P, _I]),
+    "ntc_ctx_set_timing": (_I, [_P, _I]),
+    "ntc_synthetic_pbr": (_I, [_I, _U, _P, _P, _P]),
+}
+
+_lib = None
+
+
+def load():
+    """Loads libopenntc_amd.so and declares every C-ABI signature."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(
+            f"openntc_amd native library not built: {LIB_PATH} is missing "
+            "(run `python -m openntc_amd.build`)")
+    lib = C.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(status: int, what: str = "call"):
+    if status != NTC_OK:
+        msg = load().ntc_status_string(status).decode()
+        raise RuntimeError(f"{what} failed: status {status} ({msg})")
+    return status

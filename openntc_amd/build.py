@@ -1,0 +1,79 @@
+"""Builds openntc_amd/libopenntc_amd.so for gfx950.
+
+HIP sources (*.hip) go through hipcc with --offload-arch=gfx950; host sources
+(*.cpp) through g++ against the HIP runtime headers. Both with
+-ffp-contract=off: every fused multiply-add in the library is an explicit fma.
+Usage: python -m openntc_amd.build [-j N]
+"""
+from __future__ import annotations
+
+import argparse
+import concurrent.futures as cf
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+OBJ = os.path.join(HERE, "_build")
+LIB = os.path.join(HERE, "libopenntc_amd.so")
+ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
+
+HIP_SOURCES = ["kernels_plain.hip", "kernels_grid.hip", "kernels_tex.hip", "kernels_fused.hip"]
+CXX_SOURCES = ["context.cpp", "runtime.cpp", "capi.cpp", "synthetic.cpp"]
+
+INCLUDES = ["-I", os.path.join(ROOT, "include"), "-I", CSRC]
+
+HIPCC_FLAGS = [
+    "--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17", "-ffp-contract=off",
+    "-munsafe-fp-atomics",
+    # no packed FP32 instructions (v_pk_*_f32)
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wno-unused-result",
+]
+CXX_FLAGS = [
+    "-O2", "-fPIC", "-std=c++20", "-ffp-contract=off", "-D__HIP_PLATFORM_AMD__",
+    "-I", os.path.join(ROCM, "include"), "-Wall", "-Wno-unused-result", "-Wno-unused-function",
+]
+
+
+def _compile(src: str) -> str:
+    path = os.path.join(CSRC, src)
+    obj = os.path.join(OBJ, src + ".o")
+    if src.endswith(".hip"):
+        cmd = [os.path.join(ROCM, "bin", "hipcc"), *HIPCC_FLAGS, *INCLUDES, "-c", path, "-o", obj]
+    else:
+        cmd = ["g++", *CXX_FLAGS, *INCLUDES, "-c", path, "-o", obj]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"compile failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    return obj
+
+
+def build(jobs: int = 8, verbose: bool = True) -> str:
+    os.makedirs(OBJ, exist_ok=True)
+    sources = [s for s in HIP_SOURCES + CXX_SOURCES if os.path.exists(os.path.join(CSRC, s))]
+    with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
+        objs = list(ex.map(_compile, sources))
+    tmp = LIB + ".tmp"
+    cmd = [os.path.join(ROCM, "bin", "hipcc"), "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp,
+           *objs, "-lpthread"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    os.replace(tmp, LIB)
+    if verbose:
+        print(f"built {LIB}")
+    return LIB
+
+
+if __name__ == "__main__":
+    ap = argparse.ArgumentParser()
+    ap.add_argument("-j", type=int, default=8)
+    a = ap.parse_args()
+    try:
+        build(a.j)
+    except RuntimeError as e:
+        print(e, file=sys.stderr)
+        sys.exit(1)

@@ -1,0 +1,131 @@
+// openntc::Context implementation state (host side, C++). Shared by the
+// Context methods (context.cpp) and the flat C shim (capi.cpp).
+#pragma once
+
+#include <libopenntc/libopenntc.h>
+
+#include <hip/hip_runtime.h>
+
+#include <random>
+#include <string>
+#include <vector>
+
+#include "ntc_hip.h"
+
+namespace openntc
+{
+// Distributions over std::mt19937 used for the parameter init and the step
+// schedule. std::mt19937 itself is fully specified by the C++ standard; the
+// reference's <random> distributions are not (libopenntc.cpp:137, 564-583),
+// so these are defined here and restated by the CPU oracle.
+float DrawUniform01(std::mt19937& g);       // [0,1), 24 random bits
+float DrawUniform01Open0(std::mt19937& g);  // (0,1], 24 random bits
+float DrawUniformRange(std::mt19937& g, float lo, float hi);
+int32_t DrawUniformInt(std::mt19937& g, int32_t lo, int32_t hi);
+bool DrawBernoulli(std::mt19937& g, double p);
+
+struct StepPlan
+{
+  int32_t lod, level, tile_dim, n_tiles;
+  int32_t origins[16];
+  float lr_grid, lr_decoder;
+  float bias1, bias2, grid_bias1, grid_bias2;
+  int32_t grid_update, lock;
+};
+
+struct DeviceBuffer
+{
+  void* ptr = nullptr;
+  size_t bytes = 0;
+  hipError_t Alloc(size_t n);
+  void Free();
+  template <typename T>
+  T* as() const
+  {
+    return static_cast<T*>(ptr);
+  }
+};
+
+struct Context::Impl
+{
+  // ---- configuration (Init / InitEx) ----
+  bool initialized = false;
+  bool manifest_loaded = false;
+  Result last_error = Result::Success;
+  TrainPhase phase = TrainPhase::TrainError;
+  Profile profile = Profile::Bpp_0_2;
+  int32_t g0_bits = 0, g1_bits = 0, g0_channels = 0, g1_channels = 0, g0_scale = 0;
+  int32_t hidden_dim = 64;
+  FeatureLayout feature_layout = FeatureLayout::Reference;
+  Activation activation = Activation::HardGELU;
+  int32_t feature_dim = 0, feature_dim_padded = 0, out_dim = 0;
+  static constexpr int32_t kOutPadded = 16;
+  int32_t min_dimension = Context::kMinDimension;
+  int32_t mip_min_dim = 4;
+  uint32_t seed = 123;
+  float g0_delta = 0.f, g1_delta = 0.f;
+  std::mt19937 gen{123};
+
+  // ---- manifest-dependent layout ----
+  int32_t mip_count = 0, level_count = 0;
+  int32_t mip_dim[kMaxMips] = {};
+  int32_t g0_dim[kMaxLevels] = {}, g1_dim[kMaxLevels] = {};
+  Manifest manifest = {};
+
+  // ---- device state ----
+  hipStream_t stream = nullptr;
+  DeviceBuffer g0[kMaxLevels], g1[kMaxLevels];
+  DeviceBuffer g0m[kMaxLevels], g0v[kMaxLevels], g1m[kMaxLevels], g1v[kMaxLevels];
+  DeviceBuffer dg0[kMaxLevels], dg1[kMaxLevels];
+  DeviceBuffer dec, dec_m, dec_v, dec_grad;  // W0[H][F] | W1[H][H] | Wout[16][H]
+  DeviceBuffer package[kMaxMips];
+  DeviceBuffer mips[kMaxSources][kMaxMips];
+  DeviceBuffer origins_table;  // 16 ints per scheduled step
+  DeviceBuffer eval_origins;   // 16 ints
+  DeviceBuffer scalars;        // doubles: [0..15] channel SSE, [16] step SSE
+  DeviceBuffer pack_scratch;
+  // plain path activations (allocated on first use)
+  int64_t plain_batch = 0;
+  DeviceBuffer x, dx, yt, z0, a0, z1, a1, yp, dy, da1, dz1, da0, dz0, gemm_ws, loss_ws, chan_ws;
+
+  // ---- training state ----
+  int32_t batch_count = 0, batch_i = 0, lock_i = 0, grids_per_batch = 1;
+  int32_t grid_batch_i[kMaxLevels] = {};
+  float lr_grid_max = 0.01f, lr_dec_max = 0.005f;
+  float init_lo = -0.1f, init_hi = 0.1f;
+  TrainPath path = TrainPath::Fused;
+  std::vector<StepPlan> plan;
+  double last_step_sse = 0.0;
+
+  // ---- timing (optional) ----
+  bool timing = false;
+  std::vector<float> kernel_ms;
+
+  // ---- helpers ----
+  int64_t DecoderElems() const
+  {
+    return (int64_t)hidden_dim * feature_dim + (int64_t)hidden_dim * hidden_dim + (int64_t)kOutPadded * hidden_dim;
+  }
+  int64_t GridElems(int which, int level) const
+  {
+    return which == 0 ? (int64_t)g0_dim[level] * g0_dim[level] * g0_channels
+                      : (int64_t)g1_dim[level] * g1_dim[level] * g1_channels;
+  }
+  int32_t LevelForLod(int32_t lod) const;
+  ntc_feature_desc FeatureDesc(int level) const;
+  Result ApplyLayout(const LayoutInfo& l);
+  Result ApplyProfile(Profile p);
+  Result InitDevice();
+  void FreeManifestBuffers();
+  void FreeAll();
+  Result SetupTextures(const TextureInput* tex, int32_t count, int32_t dim);
+  Result EnsurePlainBuffers(int64_t batch);
+  void BuildPlan(int32_t steps);
+  Result RunPlainStep(const StepPlan& st, int32_t step_index);
+  Result LockGrids();
+  Result Finalize(Context& ctx);
+  Result ForwardTiles(int32_t mip, const int32_t* origins, int32_t n_tiles, int32_t T, bool with_targets);
+  EvalResults Evaluate(EvalPrecision p, EvalDetail* detail, Result& err);
+  CalibrationData RunCalibration(float headroom, Result& err);
+};
+} // namespace openntc

@@ -1,0 +1,260 @@
+// Optimiser and latent-grid maintenance kernels: Adam, the fused dense grid
+// update, clamp, quantise, bit packing, max|x| and a graph-safe zero fill.
+//
+//   k_adam          ntc_kernel.cu:375-410 (update_adam)
+//   k_grid_update   update_adam + clamp_grid (ntc_kernel.cu:468-482) + the
+//                   next step's gradient FillZero (libopenntc.cpp:695-696),
+//                   one HBM pass: reads p, m, v, g, writes p, m, v, g = 0.
+//   k_quantize      ntc_kernel.cu:448-466 (quantize_grid)
+//   k_pack          ntc_kernel.cu:486-507 (quantize_pack)
+//   k_max_abs       ntc_kernel.cu:511-531 (max_abs)
+#include "ntc_device.h"
+#include "ntc_host.h"
+
+using namespace ntc;
+
+namespace
+{
+constexpr int kThreads = 256;
+
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+struct AdamConst
+{
+  float lr, b1, b2, omb1, omb2, bias1, bias2;
+};
+
+// m = b1 m + (1-b1) g ; v = b2 v + ((1-b2) g) g ; p -= (lr (m bias1)) / (sqrt(v bias2) + 1e-8)
+__device__ __forceinline__ void adam_one(const AdamConst& c, float g, float& m, float& v, float& p)
+{
+  m = fmaf(c.b1, m, c.omb1 * g);
+  v = fmaf(c.b2, v, (c.omb2 * g) * g);
+  p = p - (c.lr * (m * c.bias1)) / (__builtin_sqrtf(v * c.bias2) + 1e-8f);
+}
+
+__global__ __launch_bounds__(kThreads) void k_adam(int64_t n, AdamConst c, const float* __restrict__ g,
+                                                   float* __restrict__ m, float* __restrict__ v,
+                                                   float* __restrict__ p)
+{
+  for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * kThreads)
+  {
+    float mi = m[i], vi = v[i], pi = p[i];
+    adam_one(c, g[i], mi, vi, pi);
+    m[i] = mi;
+    v[i] = vi;
+    p[i] = pi;
+  }
+}
+
+// Vectorised: 4 params per thread per iteration with 16-byte loads/stores;
+// streaming (non-temporal) hints since nothing is reused within the step.
+__global__ __launch_bounds__(kThreads) void k_grid_update(int64_t n4, AdamConst c, float lo, float hi,
+                                                          f4* __restrict__ g, f4* __restrict__ m,
+                                                          f4* __restrict__ v, f4* __restrict__ p)
+{
+  const f4 zero = {0.f, 0.f, 0.f, 0.f};
+  for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n4;
+       i += (int64_t)gridDim.x * kThreads)
+  {
+    const f4 gi = __builtin_nontemporal_load(&g[i]);
+    f4 mi = __builtin_nontemporal_load(&m[i]);
+    f4 vi = __builtin_nontemporal_load(&v[i]);
+    f4 pi = __builtin_nontemporal_load(&p[i]);
+#pragma unroll
+    for (int e = 0; e < 4; ++e)
+    {
+      float me = mi[e], ve = vi[e], pe = pi[e];
+      adam_one(c, gi[e], me, ve, pe);
+      mi[e] = me;
+      vi[e] = ve;
+      pi[e] = fmaxf(lo, fminf(hi, pe));
+    }
+    __builtin_nontemporal_store(mi, &m[i]);
+    __builtin_nontemporal_store(vi, &v[i]);
+    __builtin_nontemporal_store(pi, &p[i]);
+    __builtin_nontemporal_store(zero, &g[i]);
+  }
+}
+
+__global__ __launch_bounds__(kThreads) void k_grid_update_tail(int64_t beg, int64_t n, AdamConst c,
+                                                               float lo, float hi, float* __restrict__ g,
+                                                               float* __restrict__ m, float* __restrict__ v,
+                                                               float* __restrict__ p)
+{
+  const int64_t i = beg + (int64_t)blockIdx.x * kThreads + threadIdx.x;
+  if (i >= n) return;
+  float mi = m[i], vi = v[i], pi = p[i];
+  adam_one(c, g[i], mi, vi, pi);
+  m[i] = mi;
+  v[i] = vi;
+  p[i] = fmaxf(lo, fminf(hi, pi));
+  g[i] = 0.0f;
+}
+
+__global__ __launch_bounds__(kThreads) void k_clamp(int64_t n, float lo, float hi, float* __restrict__ g)
+{
+  for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * kThreads)
+    g[i] = fmaxf(lo, fminf(hi, g[i]));
+}
+
+__global__ __launch_bounds__(kThreads) void k_quantize(int64_t n, float kmin, float kmax, float delta,
+                                                       float* __restrict__ g)
+{
+  for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * kThreads)
+  {
+    const float k = __builtin_rintf(g[i] / delta);
+    g[i] = fminf(fmaxf(k, kmin), kmax) * delta;
+  }
+}
+
+// One 32-bit word per thread: codes of elements w*(32/b)+s at bits s*b.
+__global__ __launch_bounds__(kThreads) void k_pack(int64_t words, int bits, const float* __restrict__ in,
+                                                   uint32_t* __restrict__ out)
+{
+  const int64_t w = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+  if (w >= words) return;
+  const int per = 32 / bits;
+  const int half = 1 << (bits - 1);
+  uint32_t packed = 0;
+  for (int s = 0; s < per; ++s)
+  {
+    int q = (int)__builtin_roundf(in[w * per + s] * (float)half);
+    q = clampi(q, -half, half - 1);
+    packed |= (uint32_t)(q + half) << (s * bits);
+  }
+  out[w] = packed;
+}
+
+__global__ __launch_bounds__(kThreads) void k_max_abs(int64_t n, const float* __restrict__ x,
+                                                      float* __restrict__ result)
+{
+  __shared__ float red[kThreads];
+  float mx = 0.0f;
+  for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * kThreads)
+    mx = fmaxf(mx, fabsf(x[i]));
+  red[threadIdx.x] = mx;
+  __syncthreads();
+  for (int w = kThreads / 2; w > 0; w >>= 1)
+  {
+    if ((int)threadIdx.x < w) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + w]);
+    __syncthreads();
+  }
+  // non-negative floats order like their bit patterns
+  if (threadIdx.x == 0) atomicMax((int*)result, __float_as_int(red[0]));
+}
+
+__global__ __launch_bounds__(kThreads) void k_fill_zero(size_t n16, uint4* __restrict__ p16, size_t tail,
+                                                        uint8_t* __restrict__ ptail)
+{
+  const uint4 z = make_uint4(0, 0, 0, 0);
+  for (size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x; i < n16; i += (size_t)gridDim.x * kThreads)
+    p16[i] = z;
+  if (blockIdx.x == 0 && threadIdx.x < tail) ptail[threadIdx.x] = 0;
+}
+
+unsigned grid_stride_blocks(int64_t n)
+{
+  const int64_t b = (n + kThreads - 1) / kThreads;
+  return (unsigned)(b < 4096 ? (b < 1 ? 1 : b) : 4096);
+}
+
+AdamConst make_adam(float lr, float b1, float b2, float bias1, float bias2)
+{
+  AdamConst c;
+  c.lr = lr;
+  c.b1 = b1;
+  c.b2 = b2;
+  c.omb1 = 1.0f - b1;
+  c.omb2 = 1.0f - b2;
+  c.bias1 = bias1;
+  c.bias2 = bias2;
+  return c;
+}
+} // namespace
+
+extern "C" {
+int ntc_adam_f32(int64_t n, float lr, float beta1, float beta2, float bias1, float bias2,
+                 const float* grad, float* m, float* v, float* p, hipStream_t stream)
+{
+  NTC_RETURN_IF(n < 1 || !grad || !m || !v || !p, NTC_E_INVALID);
+  hipLaunchKernelGGL(k_adam, dim3(grid_stride_blocks(n)), dim3(kThreads), 0, stream, n,
+                     make_adam(lr, beta1, beta2, bias1, bias2), grad, m, v, p);
+  return NTC_LAUNCH_STATUS();
+}
+
+int ntc_grid_update_f32(int64_t n, float lr, float beta1, float beta2, float bias1, float bias2,
+                        float lo, float hi, float* grad, float* m, float* v, float* p,
+                        hipStream_t stream)
+{
+  NTC_RETURN_IF(n < 1 || !grad || !m || !v || !p, NTC_E_INVALID);
+  const AdamConst c = make_adam(lr, beta1, beta2, bias1, bias2);
+  const bool aligned = (((uintptr_t)grad | (uintptr_t)m | (uintptr_t)v | (uintptr_t)p) & 15) == 0;
+  const int64_t n4 = aligned ? n / 4 : 0;
+  if (n4 > 0)
+  {
+    // 8 resident blocks per CU over 256 CUs; each thread then streams many float4s
+    const int64_t want = (n4 + kThreads - 1) / kThreads;
+    const unsigned blocks = (unsigned)(want < 2048 ? want : 2048);
+    hipLaunchKernelGGL(k_grid_update, dim3(blocks), dim3(kThreads), 0, stream, n4, c, lo, hi,
+                       (f4*)grad, (f4*)m, (f4*)v, (f4*)p);
+  }
+  const int64_t beg = n4 * 4;
+  if (beg < n)
+    hipLaunchKernelGGL(k_grid_update_tail, dim3(blocks_for(n - beg, kThreads)), dim3(kThreads), 0, stream,
+                       beg, n, c, lo, hi, grad, m, v, p);
+  return NTC_LAUNCH_STATUS();
+}
+
+int ntc_clamp_grid_f32(int64_t n, float delta, float* g, hipStream_t stream)
+{
+  NTC_RETURN_IF(n < 1 || !g, NTC_E_INVALID);
+  hipLaunchKernelGGL(k_clamp, dim3(grid_stride_blocks(n)), dim3(kThreads), 0, stream, n, -1.0f,
+                     1.0f - delta, g);
+  return NTC_LAUNCH_STATUS();
+}
+
+int ntc_quantize_grid_f32(int64_t n, int32_t bits, float delta, float* g, hipStream_t stream)
+{
+  NTC_RETURN_IF(n < 1 || !g || bits < 1 || bits > 8, NTC_E_INVALID);
+  const float kmin = -(float)(1 << (bits - 1));
+  const float kmax = (float)(1 << (bits - 1)) - 1.0f;
+  hipLaunchKernelGGL(k_quantize, dim3(grid_stride_blocks(n)), dim3(kThreads), 0, stream, n, kmin, kmax,
+                     delta, g);
+  return NTC_LAUNCH_STATUS();
+}
+
+int ntc_quantize_pack_f32(int64_t n, int32_t bits, const float* in, uint32_t* out, hipStream_t stream)
+{
+  NTC_RETURN_IF(n < 1 || !in || !out, NTC_E_INVALID);
+  NTC_RETURN_IF(bits != 1 && bits != 2 && bits != 4 && bits != 8, NTC_E_INVALID);
+  NTC_RETURN_IF((n * bits) % 32 != 0, NTC_E_INVALID);
+  const int64_t words = n * bits / 32;
+  hipLaunchKernelGGL(k_pack, dim3(blocks_for(words, kThreads)), dim3(kThreads), 0, stream, words, bits, in,
+                     out);
+  return NTC_LAUNCH_STATUS();
+}
+
+int ntc_max_abs_f32(int64_t n, const float* x, float* result, hipStream_t stream)
+{
+  NTC_RETURN_IF(n < 1 || !x || !result, NTC_E_INVALID);
+  hipLaunchKernelGGL(k_max_abs, dim3(grid_stride_blocks(n)), dim3(kThreads), 0, stream, n, x, result);
+  return NTC_LAUNCH_STATUS();
+}
+
+int ntc_fill_zero(void* p, size_t bytes, hipStream_t stream)
+{
+  NTC_RETURN_IF(!p && bytes, NTC_E_INVALID);
+  if (bytes == 0) return NTC_OK;
+  const bool aligned = ((uintptr_t)p & 15) == 0;
+  const size_t n16 = aligned ? bytes / 16 : 0;
+  const size_t tail = bytes - n16 * 16;
+  NTC_RETURN_IF(tail > kThreads, NTC_E_INVALID);
+  hipLaunchKernelGGL(k_fill_zero, dim3(grid_stride_blocks((int64_t)(n16 ? n16 : 1))), dim3(kThreads), 0,
+                     stream, n16, (uint4*)p, tail, (uint8_t*)p + n16 * 16);
+  return NTC_LAUNCH_STATUS();
+}
+} // extern "C"

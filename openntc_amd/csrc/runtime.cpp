@@ -1,0 +1,600 @@
+// GPU-free runtime half of the API: name tables, decoder constants and the
+// ONTC container (libopenntc/openntc_runtime.cpp).
+//
+// Container layout (openntc_runtime.cpp:179-253): 16-byte header
+// {0x43544E4F 'ONTC', 4, json bytes, blob bytes}, compact JSON with keys in
+// sorted order, then blobs g0_0.., g1_0.., decoder, each starting at a
+// multiple of 256 bytes from the end of the JSON.
+#include <libopenntc/openntc_runtime.h>
+
+#include <cctype>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+namespace openntc
+{
+// ---- names (openntc_runtime.cpp:20-154) --------------------------------------------
+const char* SemanticToString(Semantic sem)
+{
+  switch (sem)
+  {
+  case Semantic::Albedo: return "Albedo";
+  case Semantic::Alpha: return "Alpha";
+  case Semantic::Displacement: return "Displacement";
+  case Semantic::Emissive: return "Emissive";
+  case Semantic::Gloss: return "Gloss";
+  case Semantic::Metallic: return "Metallic";
+  case Semantic::Normal: return "Normal";
+  case Semantic::AO: return "AO";
+  case Semantic::Roughness: return "Roughness";
+  case Semantic::Specular: return "Specular";
+  case Semantic::Transmission: return "Transmission";
+  default: return "None";
+  }
+}
+
+Semantic SemanticFromName(const std::string& s)
+{
+  struct Alias
+  {
+    const char* name;
+    Semantic sem;
+  };
+  static const Alias table[] = {
+      {"Albedo", Semantic::Albedo},         {"Diffuse", Semantic::Albedo},
+      {"Alpha", Semantic::Alpha},           {"Mask", Semantic::Alpha},
+      {"AlphaMask", Semantic::Alpha},       {"Displ", Semantic::Displacement},
+      {"Displacement", Semantic::Displacement}, {"Emissive", Semantic::Emissive},
+      {"Emission", Semantic::Emissive},     {"Glossiness", Semantic::Gloss},
+      {"Gloss", Semantic::Gloss},           {"Metalness", Semantic::Metallic},
+      {"Metallic", Semantic::Metallic},     {"Normal", Semantic::Normal},
+      {"Occlusion", Semantic::AO},          {"AO", Semantic::AO},
+      {"AmbientOcclusion", Semantic::AO},   {"Roughness", Semantic::Roughness},
+      {"SpecularColor", Semantic::Specular}, {"Specular", Semantic::Specular},
+      {"Transmission", Semantic::Transmission},
+  };
+  for (const Alias& a : table)
+    if (s == a.name) return a.sem;
+  return Semantic::None;
+}
+
+int GetChannelCountForSemantic(const Semantic sem)
+{
+  // None accepts any channel count (-1); Count is invalid (-1).
+  static const int32_t counts[] = {-1, 3, 1, 1, 1, 1, 1, 3, 1, 1, 3, 1, -1};
+  const int i = (int)sem;
+  if (i < 0 || i > 12) return -1;
+  return counts[i];
+}
+
+const char* ProfileToString(Profile p)
+{
+  switch (p)
+  {
+  case Profile::Bpp_0_2: return "bpp_0_2";
+  case Profile::Bpp_0_5: return "bpp_0_5";
+  case Profile::Bpp_1_0: return "bpp_1_0";
+  case Profile::Bpp_2_25: return "bpp_2_25";
+  default: return "unknown";
+  }
+}
+
+bool ProfileFromString(const std::string& s, Profile& o_profile)
+{
+  static const Profile all[] = {Profile::Bpp_0_2, Profile::Bpp_0_5, Profile::Bpp_1_0, Profile::Bpp_2_25};
+  for (Profile p : all)
+    if (s == ProfileToString(p))
+    {
+      o_profile = p;
+      return true;
+    }
+  return false;
+}
+
+// openntc_runtime.cpp:156-177
+void FillNTCConstants(const CompressedData& data, NTCConstants& o)
+{
+  o = {};
+  for (int i = 0; i < data.level_count_ && i < kMaxLevels; ++i)
+  {
+    o.g0_grid_dim_[i] = data.g0_grid_dim_[i];
+    o.g1_grid_dim_[i] = data.g1_grid_dim_[i];
+    o.g0_offset_[i] = (uint32_t)data.g0_offset_[i];
+    o.g1_offset_[i] = (uint32_t)data.g1_offset_[i];
+  }
+  o.g0_bits_per_channel_ = data.g0_bits_per_channel_;
+  o.g1_bits_per_channel_ = data.g1_bits_per_channel_;
+  o.g0_channels_ = data.g0_channels_;
+  o.g1_channels_ = data.g1_channels_;
+  o.dim_ = data.dim_;
+  o.mip_count_ = data.mip_count_;
+  o.rcp_s_a1_ = 1.0f / data.caldata_.s_a1_;
+  o.rcp_s_a2_ = 1.0f / data.caldata_.s_a2_;
+  o.channel_count_ = data.channel_count_;
+  for (int i = 0; i < data.channel_count_ && i < kMaxChannels; ++i)
+    o.channel_semantics_[i] = (uint32_t)data.channel_semantics_[i];
+}
+
+// ---- a small JSON value + parser, enough for the container metadata ----------------
+namespace
+{
+struct Json
+{
+  enum Kind
+  {
+    Null,
+    Bool,
+    Number,
+    String,
+    Array,
+    Object
+  } kind = Null;
+  bool b = false;
+  double num = 0.0;
+  bool is_int = false;
+  long long inum = 0;
+  std::string str;
+  std::vector<Json> arr;
+  std::vector<std::pair<std::string, Json>> obj;
+
+  const Json* find(const char* k) const
+  {
+    if (kind != Object) return nullptr;
+    for (const auto& kv : obj)
+      if (kv.first == k) return &kv.second;
+    return nullptr;
+  }
+};
+
+class Parser
+{
+public:
+  Parser(const char* b, const char* e) : p_(b), e_(e) {}
+  bool Parse(Json& out)
+  {
+    if (!Value(out, 0)) return false;
+    Ws();
+    return p_ == e_;
+  }
+
+private:
+  const char* p_;
+  const char* e_;
+  void Ws()
+  {
+    while (p_ < e_ && (*p_ == ' ' || *p_ == '\n' || *p_ == '\r' || *p_ == '\t')) ++p_;
+  }
+  bool Lit(const char* s)
+  {
+    const size_t n = strlen(s);
+    if ((size_t)(e_ - p_) < n || strncmp(p_, s, n) != 0) return false;
+    p_ += n;
+    return true;
+  }
+  bool Str(std::string& out)
+  {
+    if (p_ >= e_ || *p_ != '"') return false;
+    ++p_;
+    while (p_ < e_ && *p_ != '"')
+    {
+      char c = *p_++;
+      if (c == '\\')
+      {
+        if (p_ >= e_) return false;
+        char esc = *p_++;
+        switch (esc)
+        {
+        case '"': out += '"'; break;
+        case '\\': out += '\\'; break;
+        case '/': out += '/'; break;
+        case 'b': out += '\b'; break;
+        case 'f': out += '\f'; break;
+        case 'n': out += '\n'; break;
+        case 'r': out += '\r'; break;
+        case 't': out += '\t'; break;
+        case 'u':
+        {
+          if (e_ - p_ < 4) return false;
+          unsigned v = 0;
+          for (int i = 0; i < 4; ++i)
+          {
+            const char h = p_[i];
+            v <<= 4;
+            if (h >= '0' && h <= '9') v |= (unsigned)(h - '0');
+            else if (h >= 'a' && h <= 'f') v |= (unsigned)(h - 'a' + 10);
+            else if (h >= 'A' && h <= 'F') v |= (unsigned)(h - 'A' + 10);
+            else return false;
+          }
+          p_ += 4;
+          if (v < 0x80) out += (char)v;
+          else if (v < 0x800)
+          {
+            out += (char)(0xC0 | (v >> 6));
+            out += (char)(0x80 | (v & 0x3F));
+          }
+          else
+          {
+            out += (char)(0xE0 | (v >> 12));
+            out += (char)(0x80 | ((v >> 6) & 0x3F));
+            out += (char)(0x80 | (v & 0x3F));
+          }
+          break;
+        }
+        default: return false;
+        }
+      }
+      else
+        out += c;
+    }
+    if (p_ >= e_) return false;
+    ++p_;
+    return true;
+  }
+  bool Value(Json& v, int depth)
+  {
+    if (depth > 64) return false;
+    Ws();
+    if (p_ >= e_) return false;
+    const char c = *p_;
+    if (c == '{')
+    {
+      ++p_;
+      v.kind = Json::Object;
+      Ws();
+      if (p_ < e_ && *p_ == '}')
+      {
+        ++p_;
+        return true;
+      }
+      for (;;)
+      {
+        Ws();
+        std::string k;
+        if (!Str(k)) return false;
+        Ws();
+        if (p_ >= e_ || *p_ != ':') return false;
+        ++p_;
+        Json child;
+        if (!Value(child, depth + 1)) return false;
+        v.obj.emplace_back(std::move(k), std::move(child));
+        Ws();
+        if (p_ < e_ && *p_ == ',')
+        {
+          ++p_;
+          continue;
+        }
+        if (p_ < e_ && *p_ == '}')
+        {
+          ++p_;
+          return true;
+        }
+        return false;
+      }
+    }
+    if (c == '[')
+    {
+      ++p_;
+      v.kind = Json::Array;
+      Ws();
+      if (p_ < e_ && *p_ == ']')
+      {
+        ++p_;
+        return true;
+      }
+      for (;;)
+      {
+        Json child;
+        if (!Value(child, depth + 1)) return false;
+        v.arr.push_back(std::move(child));
+        Ws();
+        if (p_ < e_ && *p_ == ',')
+        {
+          ++p_;
+          continue;
+        }
+        if (p_ < e_ && *p_ == ']')
+        {
+          ++p_;
+          return true;
+        }
+        return false;
+      }
+    }
+    if (c == '"')
+    {
+      v.kind = Json::String;
+      return Str(v.str);
+    }
+    if (Lit("true"))
+    {
+      v.kind = Json::Bool;
+      v.b = true;
+      return true;
+    }
+    if (Lit("false"))
+    {
+      v.kind = Json::Bool;
+      return true;
+    }
+    if (Lit("null")) return true;
+    // number
+    const char* s = p_;
+    if (p_ < e_ && (*p_ == '-' || *p_ == '+')) ++p_;
+    bool frac = false;
+    while (p_ < e_ && (isdigit((unsigned char)*p_) || *p_ == '.' || *p_ == 'e' || *p_ == 'E' ||
+                       ((*p_ == '-' || *p_ == '+') && (p_[-1] == 'e' || p_[-1] == 'E'))))
+    {
+      if (*p_ == '.' || *p_ == 'e' || *p_ == 'E') frac = true;
+      ++p_;
+    }
+    if (p_ == s) return false;
+    const std::string tok(s, p_);
+    char* end = nullptr;
+    v.kind = Json::Number;
+    v.num = strtod(tok.c_str(), &end);
+    if (end != tok.c_str() + tok.size()) return false;
+    v.is_int = !frac;
+    if (v.is_int) v.inum = strtoll(tok.c_str(), nullptr, 10);
+    return true;
+  }
+};
+
+// Shortest decimal that reads back to the same double (".0" appended to
+// integral values, as the reference's JSON writer does).
+std::string FormatDouble(double d)
+{
+  char buf[64];
+  for (int prec = 1; prec <= 17; ++prec)
+  {
+    snprintf(buf, sizeof(buf), "%.*g", prec, d);
+    if (strtod(buf, nullptr) == d) break;
+  }
+  std::string s = buf;
+  if (s.find_first_of(".eEn") == std::string::npos) s += ".0";
+  return s;
+}
+
+std::string Quote(const std::string& s)
+{
+  std::string o = "\"";
+  for (char c : s)
+  {
+    if (c == '"' || c == '\\')
+    {
+      o += '\\';
+      o += c;
+    }
+    else
+      o += c;
+  }
+  return o + "\"";
+}
+
+uint64_t RoundUp256(uint64_t n) { return ((n + 255) / 256) * 256; }
+
+struct Blob
+{
+  std::string name;
+  uint64_t size;
+  const void* data;
+};
+} // namespace
+
+// openntc_runtime.cpp:179-253
+Result Dump(const std::string& path, const CompressedData& data)
+{
+  if (data.level_count_ < 1 || data.level_count_ > kMaxLevels) return Result::InvalidState;
+  std::vector<Blob> blobs;
+  for (int i = 0; i < data.level_count_; ++i) blobs.push_back({"g0_" + std::to_string(i), data.g0_size_[i], data.g0_[i]});
+  for (int i = 0; i < data.level_count_; ++i) blobs.push_back({"g1_" + std::to_string(i), data.g1_size_[i], data.g1_[i]});
+  blobs.push_back({"decoder", data.decoder_size_, data.decoder_});
+
+  auto dims = [](const int* d) {
+    std::string s = "[";
+    for (int i = 0; i < kMaxLevels; ++i) s += (i ? "," : "") + std::to_string(d[i]);
+    return s + "]";
+  };
+  std::string js = "{";
+  js += "\"blobs\":[";
+  uint64_t off = 0;
+  for (size_t i = 0; i < blobs.size(); ++i)
+  {
+    off = RoundUp256(off);
+    js += (i ? "," : "");
+    js += "{\"name\":" + Quote(blobs[i].name) + ",\"offset\":" + std::to_string(off) +
+          ",\"size\":" + std::to_string(blobs[i].size) + "}";
+    off += blobs[i].size;
+  }
+  js += "],";
+  js += "\"calibration\":{\"max_abs_a1\":" + FormatDouble(data.caldata_.max_abs_a1_) +
+        ",\"max_abs_a2\":" + FormatDouble(data.caldata_.max_abs_a2_) + ",\"s_a1\":" + FormatDouble(data.caldata_.s_a1_) +
+        ",\"s_a2\":" + FormatDouble(data.caldata_.s_a2_) + "},";
+  js += "\"channel_semantics\":[";
+  for (int i = 0; i < data.channel_count_; ++i)
+    js += std::string(i ? "," : "") + Quote(SemanticToString(data.channel_semantics_[i]));
+  js += "],";
+  js += "\"dim\":" + std::to_string(data.dim_) + ",";
+  js += "\"g0\":{\"bits\":" + std::to_string(data.g0_bits_per_channel_) + ",\"channels\":" +
+        std::to_string(data.g0_channels_) + ",\"grid_dims\":" + dims(data.g0_grid_dim_) + "},";
+  js += "\"g1\":{\"bits\":" + std::to_string(data.g1_bits_per_channel_) + ",\"channels\":" +
+        std::to_string(data.g1_channels_) + ",\"grid_dims\":" + dims(data.g1_grid_dim_) + "},";
+  js += "\"level_count\":" + std::to_string(data.level_count_) + ",";
+  js += "\"mip_count\":" + std::to_string(data.mip_count_) + ",";
+  js += "\"profile\":" + Quote(ProfileToString(data.profile_)) + ",";
+  js += "\"source\":{\"generator\":\"openntc\",\"version\":5}";
+  js += "}";
+
+  const uint32_t header[4] = {0x43544E4Fu, 4u, (uint32_t)js.size(), (uint32_t)off};
+  std::ofstream f(path, std::ios::binary);
+  if (!f) return Result::FileWriteFailure;
+  f.write((const char*)header, sizeof(header));
+  f.write(js.data(), (std::streamsize)js.size());
+  off = 0;
+  for (const Blob& b : blobs)
+  {
+    const uint64_t aligned = RoundUp256(off);
+    for (; off < aligned; ++off) f.put(0);
+    if (b.size) f.write((const char*)b.data, (std::streamsize)b.size);
+    off += b.size;
+  }
+  const bool ok = f.good();
+  f.close();
+  return ok ? Result::Success : Result::FileWriteFailure;
+}
+
+// openntc_runtime.cpp:307-456
+Result Load(const std::string& path, FileData& o)
+{
+  std::ifstream f(path, std::ios::binary | std::ios::ate);
+  if (!f) return Result::FileNotFound;
+  const std::streamoff raw_size = f.tellg();
+  if (raw_size < 16) return Result::InvalidFile;
+  delete[] o.raw_;
+  o.raw_ = new uint8_t[(size_t)raw_size];
+  f.seekg(0);
+  f.read((char*)o.raw_, raw_size);
+  if (!f) return Result::InvalidFile;
+  uint32_t header[4];
+  memcpy(header, o.raw_, sizeof(header));
+  if (header[0] != 0x43544E4Fu || header[1] != 4u) return Result::InvalidFile;
+  const uint64_t json_size = header[2], blob_size = header[3];
+  const uint64_t file_size = 16 + json_size + blob_size;
+  const uint64_t blob_off = 16 + json_size;
+  if (file_size > (uint64_t)raw_size) return Result::InvalidFile;
+  Json j;
+  Parser parser((const char*)o.raw_ + 16, (const char*)o.raw_ + 16 + json_size);
+  if (!parser.Parse(j) || j.kind != Json::Object) return Result::InvalidFile;
+
+  CompressedData& d = o.data_;
+  d = {};
+  const Json* jg0 = j.find("g0");
+  const Json* jg1 = j.find("g1");
+  const Json* jcal = j.find("calibration");
+  if (!jg0 || !jg1 || !jcal || jg0->kind != Json::Object || jg1->kind != Json::Object ||
+      jcal->kind != Json::Object)
+    return Result::InvalidFile;
+  auto get_int = [](const Json* obj, const char* k, int& out) {
+    const Json* v = obj->find(k);
+    if (!v || v->kind != Json::Number || !v->is_int) return false;
+    out = (int)v->inum;
+    return true;
+  };
+  auto get_float = [](const Json* obj, const char* k, float& out) {
+    const Json* v = obj->find(k);
+    if (!v || v->kind != Json::Number) return false;
+    out = (float)v->num;
+    return true;
+  };
+  auto get_dims = [](const Json* obj, const char* k, int* out) {
+    const Json* v = obj->find(k);
+    if (!v || v->kind != Json::Array || v->arr.size() != (size_t)kMaxLevels) return false;
+    for (int i = 0; i < kMaxLevels; ++i)
+    {
+      if (v->arr[i].kind != Json::Number || !v->arr[i].is_int) return false;
+      out[i] = (int)v->arr[i].inum;
+    }
+    return true;
+  };
+  bool ok = true;
+  ok &= get_dims(jg0, "grid_dims", d.g0_grid_dim_);
+  ok &= get_int(jg0, "bits", d.g0_bits_per_channel_);
+  ok &= get_int(jg0, "channels", d.g0_channels_);
+  ok &= get_dims(jg1, "grid_dims", d.g1_grid_dim_);
+  ok &= get_int(jg1, "bits", d.g1_bits_per_channel_);
+  ok &= get_int(jg1, "channels", d.g1_channels_);
+  ok &= get_int(&j, "dim", d.dim_);
+  ok &= get_int(&j, "mip_count", d.mip_count_);
+  ok &= get_int(&j, "level_count", d.level_count_);
+  ok &= get_float(jcal, "max_abs_a1", d.caldata_.max_abs_a1_);
+  ok &= get_float(jcal, "s_a1", d.caldata_.s_a1_);
+  ok &= get_float(jcal, "max_abs_a2", d.caldata_.max_abs_a2_);
+  ok &= get_float(jcal, "s_a2", d.caldata_.s_a2_);
+  if (!ok) return Result::InvalidFile;
+  const Json* jp = j.find("profile");
+  if (!jp || jp->kind != Json::String || !ProfileFromString(jp->str, d.profile_)) return Result::InvalidFile;
+  const Json* jsem = j.find("channel_semantics");
+  if (!jsem || jsem->kind != Json::Array || jsem->arr.size() > (size_t)kMaxChannels) return Result::InvalidFile;
+  d.channel_count_ = (int)jsem->arr.size();
+  for (int i = 0; i < d.channel_count_; ++i)
+  {
+    if (jsem->arr[i].kind != Json::String) return Result::InvalidFile;
+    const Semantic s = SemanticFromName(jsem->arr[i].str);
+    if (s == Semantic::None) return Result::InvalidFile;
+    d.channel_semantics_[i] = s;
+  }
+  if (d.level_count_ < 1 || d.level_count_ > kMaxLevels || d.mip_count_ < 1 || d.mip_count_ > kMaxMips)
+    return Result::InvalidFile;
+
+  struct Slot
+  {
+    std::string name;
+    void** dst;
+    size_t* size;
+    bool found;
+  };
+  std::vector<Slot> slots;
+  for (int i = 0; i < d.level_count_; ++i) slots.push_back({"g0_" + std::to_string(i), (void**)&d.g0_[i], &d.g0_size_[i], false});
+  for (int i = 0; i < d.level_count_; ++i) slots.push_back({"g1_" + std::to_string(i), (void**)&d.g1_[i], &d.g1_size_[i], false});
+  slots.push_back({"decoder", &d.decoder_, &d.decoder_size_, false});
+  const Json* jb = j.find("blobs");
+  if (!jb || jb->kind != Json::Array) return Result::InvalidFile;
+  for (const Json& b : jb->arr)
+  {
+    if (b.kind != Json::Object) return Result::InvalidFile;
+    const Json* n = b.find("name");
+    const Json* of = b.find("offset");
+    const Json* sz = b.find("size");
+    if (!n || n->kind != Json::String || !of || of->kind != Json::Number || !of->is_int || !sz ||
+        sz->kind != Json::Number || !sz->is_int || of->inum < 0 || sz->inum < 0)
+      return Result::InvalidFile;
+    if (blob_off + (uint64_t)of->inum + (uint64_t)sz->inum > file_size) return Result::InvalidFile;
+    for (Slot& s : slots)
+      if (n->str == s.name)
+      {
+        *s.dst = o.raw_ + blob_off + of->inum;
+        *s.size = (size_t)sz->inum;
+        s.found = true;
+      }
+  }
+  for (const Slot& s : slots)
+    if (!s.found) return Result::InvalidFile;
+  for (int i = 1; i < d.level_count_; ++i)
+  {
+    d.g0_offset_[i] = d.g0_offset_[i - 1] + d.g0_size_[i - 1];
+    d.g1_offset_[i] = d.g1_offset_[i - 1] + d.g1_size_[i - 1];
+  }
+  const int last = d.level_count_ - 1;
+  d.g0_total_size_ = d.g0_offset_[last] + d.g0_size_[last];
+  d.g1_total_size_ = d.g1_offset_[last] + d.g1_size_[last];
+  delete[] o.g0_packed_;
+  delete[] o.g1_packed_;
+  o.g0_packed_ = new uint8_t[d.g0_total_size_ ? d.g0_total_size_ : 1];
+  o.g1_packed_ = new uint8_t[d.g1_total_size_ ? d.g1_total_size_ : 1];
+  for (int i = 0; i < d.level_count_; ++i)
+  {
+    memcpy(o.g0_packed_ + d.g0_offset_[i], d.g0_[i], d.g0_size_[i]);
+    memcpy(o.g1_packed_ + d.g1_offset_[i], d.g1_[i], d.g1_size_[i]);
+    d.g0_[i] = (uint32_t*)(o.g0_packed_ + d.g0_offset_[i]);
+    d.g1_[i] = (uint32_t*)(o.g1_packed_ + d.g1_offset_[i]);
+  }
+  return Result::Success;
+}
+
+FileData::FileData() : data_{}, raw_(nullptr), g0_packed_(nullptr), g1_packed_(nullptr) {}
+
+FileData::~FileData()
+{
+  delete[] raw_;
+  delete[] g0_packed_;
+  delete[] g1_packed_;
+}
+
+const CompressedData& FileData::Data() { return data_; }
+} // namespace openntc
