@@ -1,0 +1,198 @@
+"""Training throughput of the openntc hot path on one MI355X.
+
+Workload (the configuration the headline metric is quoted on): a seeded
+synthetic 9-channel 4096x4096 PBR set with Lanczos-3 mips down to 1x1;
+latents g0 16 ch at 2048^2 (4-bit) and g1 16 ch at 1024^2 (4-bit); decoder
+40 -> 128 -> 128 -> 9 (bilinear features + 8 triangle waves, ReLU); 8 wrapping
+256^2 tiles = 524 288 texels per step; 8000-step schedule (cosine learning
+rates 1e-2 / 1e-3), of which `--warmup` untimed and `--steps` timed steps run.
+A step is the whole training step: gather, forward, loss, backward, latent
+scatter, decoder Adam and the dense Adam + clamp over the sampled pyramid
+level. Inputs are resident in HBM before the timed region.
+
+Prints one JSON line. `roofline` covers the dominant kernel, timed live with
+HIP events on the context's stream; `kernels` lists the others.
+`cpu_baseline` is the CPU oracle's rendition of the same step (all host
+threads) on a bounded sample of steps.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+BF16_PEAK_TFLOPS = 2500.0
+FP32_PEAK_TFLOPS = 157.3
+
+LAYOUT = dict(g0_bits=4, g1_bits=4, g0_channels=16, g1_channels=16, g0_scale=2, hidden=128)
+DIM = 4096
+TOTAL_STEPS = 8000
+TILES = 8
+SEED_SET = 20262
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def make_context(path):
+    import openntc_amd as ntc
+
+    tex = ntc.synthetic_pbr(DIM, SEED_SET)
+    ctx = ntc.Context()
+    lay = ntc.LayoutInfo(LAYOUT["g0_bits"], LAYOUT["g1_bits"], LAYOUT["g0_channels"], LAYOUT["g1_channels"],
+                         LAYOUT["g0_scale"], LAYOUT["hidden"], ntc.FeatureLayout.Bilinear, ntc.Activation.ReLU)
+    r = ctx.InitEx(layout=lay, min_dimension=512, mip_min_dim=1, seed=123)
+    if r != ntc.Result.Success:
+        raise SystemExit(f"InitEx failed: {r.name}")
+    r = ctx.LoadTextures(tex, DIM)
+    if r != ntc.Result.Success:
+        raise SystemExit(f"LoadTextures failed: {r.name}")
+    r = ctx.BeginTrainingEx(TOTAL_STEPS, TILES, 1e-2, 1e-3, -0.5, 0.5, path)
+    if r != ntc.Result.Success:
+        raise SystemExit(f"BeginTrainingEx failed: {r.name} ({ctx.GetLastError().name})")
+    return ctx, tex
+
+
+def step_texels(ctx, first, count):
+    tot = 0
+    infos = []
+    for s in range(first, first + count):
+        si = ctx.step_info(s)
+        tot += si.n_tiles * si.tile_dim * si.tile_dim
+        infos.append(si)
+    return tot, infos
+
+
+def cpu_baseline(ctx, first_step, n_steps):
+    """The CPU oracle (C, OpenMP, FP32) running the same schedule steps from
+    the same initial parameters."""
+    from oracle import oracle as O
+
+    lay = ctx.layout()
+    L = O.layout_custom(LAYOUT["g0_bits"], LAYOUT["g1_bits"], LAYOUT["g0_channels"], LAYOUT["g1_channels"],
+                        LAYOUT["g0_scale"], LAYOUT["hidden"], 1, 1, DIM, 9, 1)
+    mt = O.MT(123)
+    P = O.init_params(mt, L, -0.5, 0.5)
+    steps = O.schedule(mt, L, TOTAL_STEPS, TILES, 1e-2, 1e-3)
+    pk = [ctx.read_package(m) for m in range(lay.mip_count)]
+    tr = O.Trainer(L, P, pk, seed=123, max_batch=TILES * 65536)
+    tr.step(steps[0], 0)  # warm caches / page in
+    texels = 0
+    t0 = time.perf_counter()
+    for s in range(first_step, first_step + n_steps):
+        tr.step(steps[s], s)
+        texels += steps[s].n_tiles * steps[s].tile_dim ** 2
+    dt = time.perf_counter() - t0
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {"value": texels / dt, "unit": "texel-samples/s", "cores": cores, "kind": "port",
+            "sample": f"{n_steps} schedule steps ({texels} texels) of the same workload after 1 warm-up step, "
+                      f"CPU oracle (C, OpenMP, FP32), {dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--path", choices=["fused", "plain"], default="fused")
+    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (every GPU box has one GPU); "
+              "run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import openntc_amd as ntc
+
+    path = ntc.TrainPath.Fused if a.path == "fused" else ntc.TrainPath.Plain
+    log(f"setting up {DIM}^2 synthetic set, path={a.path}")
+    t_setup = time.perf_counter()
+    ctx, _ = make_context(path)
+    log(f"setup done in {time.perf_counter() - t_setup:.1f} s")
+    lib = ntc.load()
+
+    prog = ctx.Train(a.warmup)
+    if prog.result != ntc.Result.Success:
+        raise SystemExit(f"warmup failed: {prog}")
+    log(f"warmup {a.warmup} steps done")
+    lib.ntc_ctx_set_timing(ctx._h, 1)
+    lib.ntc_dev_synchronize()
+    t0 = time.perf_counter()
+    prog = ctx.Train(a.steps)
+    lib.ntc_dev_synchronize()
+    dt = time.perf_counter() - t0
+    if prog.result != ntc.Result.Success:
+        raise SystemExit(f"timed steps failed: {prog}")
+    log(f"timed {a.steps} steps in {dt * 1e3:.1f} ms")
+    texels, infos = step_texels(ctx, a.warmup, a.steps)
+
+    lay = ctx.layout()
+    H, F, O = lay.hidden_dim, lay.feature_dim, lay.out_dim
+    out = {
+        "metric": "train texel-samples/s (fwd+bwd+Adam) @4096² 9-ch; s to FP32 PSNR, BF16 vs FP32",
+        "value": texels / dt,
+        "unit": "texel-samples/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": dt * 1e3 / a.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "bf16" if a.path == "fused" else "fp32",
+        "data": "synthetic (seeded 9-channel PBR set, random-init latents and decoder)",
+        "config": {"workload": "4096^2 9-ch synthetic set; g0 16ch@2048^2 4-bit, g1 16ch@1024^2 4-bit; "
+                               "MLP 40->128->128->9 ReLU; 8 x 256^2 tiles = 524288 texels/step; "
+                               "8000-step schedule",
+                   "path": a.path, "global_batch": TILES * 65536, "parallelism": "single GPU"},
+    }
+
+    if a.path == "fused":
+        times = (C_float := __import__("ctypes").c_float * 5)()
+        lib.ntc_ctx_kernel_times(ctx._h, times, 5)
+        fused_ms, dec_ms, grid_ms, n_t, n_grid = list(times)
+        # algorithmic work per launch, averaged over the timed steps
+        flops = [6.0 * (F * H + H * H + H * O) * si.n_tiles * si.tile_dim ** 2 for si in infos]
+        params = []
+        for si in infos:
+            if si.grid_update:
+                params.append(lay.g0_dim[si.level] ** 2 * lay.g0_channels + lay.g1_dim[si.level] ** 2 * lay.g1_channels)
+        fused_tflops = np.mean(flops) / (fused_ms * 1e-3) / 1e12
+        kern = {"fused_train_step": {"ms": fused_ms, "achieved": fused_tflops, "unit": "TFLOP/s",
+                                     "peak": BF16_PEAK_TFLOPS, "frac": fused_tflops / BF16_PEAK_TFLOPS,
+                                     "flop_per_launch": float(np.mean(flops))},
+                "decoder_update": {"ms": dec_ms}}
+        if params:
+            grid_bytes = 32.0 * float(np.mean(params))
+            grid_gbs = grid_bytes / (grid_ms * 1e-3) / 1e9
+            kern["grid_update"] = {"ms": grid_ms, "achieved": grid_gbs, "unit": "GB/s", "peak": HBM_PEAK_GBS,
+                                   "frac": grid_gbs / HBM_PEAK_GBS, "bytes_per_launch": grid_bytes}
+        out["kernels"] = kern
+        if params and grid_ms >= fused_ms:
+            g = kern["grid_update"]
+            out["roofline"] = {"bound": "hbm", "kernel": "k_grid_update2", "achieved": g["achieved"],
+                               "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": g["frac"], "traffic": None}
+        else:
+            f = kern["fused_train_step"]
+            out["roofline"] = {"bound": "mfma", "kernel": "k_fused_train", "achieved": f["achieved"],
+                               "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": f["frac"], "traffic": None}
+
+    if not a.no_cpu_baseline:
+        log(f"cpu baseline: {a.cpu_steps} oracle steps")
+        out["cpu_baseline"] = cpu_baseline(ctx, a.warmup, a.cpu_steps)
+        log("cpu baseline done")
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

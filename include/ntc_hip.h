@@ -85,6 +85,13 @@ typedef struct ntc_noise
 
 /* ---- library ------------------------------------------------------------ */
 const char* ntc_version(void);
+/* Device memory helpers for FFI callers that have no HIP binding of their
+ * own (test harnesses, ctypes). Synchronous; status as elsewhere. */
+int ntc_dev_alloc(size_t bytes, void** out);
+int ntc_dev_free(void* p);
+int ntc_dev_upload(void* dst, const void* src, size_t bytes);
+int ntc_dev_download(void* dst, const void* src, size_t bytes);
+int ntc_dev_synchronize(void);
 /* 0 when a gfx950 device is usable; fills *count with the device count. */
 int ntc_device_check(int32_t* count);
 const char* ntc_status_string(int status);
@@ -165,6 +172,14 @@ int ntc_grid_update_f32(int64_t n, float lr, float beta1, float beta2,
                         float bias1, float bias2, float lo, float hi,
                         float* grad, float* m, float* v, float* p,
                         hipStream_t stream);
+/* The same for both grids of a pyramid level in one launch (sizes multiples
+ * of 4, 16-byte aligned buffers); per-grid clamp bounds. */
+int ntc_grid_update2_f32(int64_t n0, float* grad0, float* m0, float* v0,
+                         float* p0, float lo0, float hi0, int64_t n1,
+                         float* grad1, float* m1, float* v1, float* p1,
+                         float lo1, float hi1, float lr, float beta1,
+                         float beta2, float bias1, float bias2,
+                         hipStream_t stream);
 /* clamp_grid (ntc_kernel.cu:468-482): g = clamp(g, -1, 1 - delta) */
 int ntc_clamp_grid_f32(int64_t n, float delta, float* g, hipStream_t stream);
 /* quantize_grid (ntc_kernel.cu:448-466) */
@@ -178,6 +193,57 @@ int ntc_max_abs_f32(int64_t n, const float* x, float* result,
                     hipStream_t stream);
 /* zero-fill by kernel (graph-safe replacement for memset) */
 int ntc_fill_zero(void* p, size_t bytes, hipStream_t stream);
+
+/* ---- fused BF16 training step ------------------------------------------------
+ * One persistent launch per step does the gather, the decoder forward, the
+ * loss, the hand-written backward and the latent-gradient scatter
+ * (replacing launch_draw_features .. launch_accumulate_grid_gradients and the
+ * nine SGEMMs, libopenntc.cpp:621-709). It recomputes the forward, so it is
+ * the backward half of the forward/backward pair; the forward half for
+ * evaluation is the plain path. MFMA operands are BF16, accumulation FP32,
+ * master latents and weights stay FP32. */
+typedef struct ntc_fused_args
+{
+  ntc_feature_desc feat;
+  ntc_tiles tiles;
+  ntc_noise noise;
+  int32_t hidden_dim, out_dim, activation;
+  const float* g0;          /* active level, FP32 masters */
+  const float* g1;
+  const uint16_t* weights;  /* BF16 tables from ntc_pack_decoder_bf16 */
+  const uint8_t* targets;   /* target package of the sampled mip [M][M][O] */
+  float loss_scale;         /* -2 / (O * B) */
+  int32_t scatter;          /* 1: add latent gradients into dg0/dg1 */
+  float* dg0;
+  float* dg1;
+  float* dw_partials;       /* [blocks][decoder elems] */
+  double* sse_partials;     /* [blocks] */
+  int32_t blocks;           /* == ntc_fused_blocks_for(...) */
+} ntc_fused_args;
+
+/* 1 when a fused kernel is compiled for this feature layout / width. */
+int ntc_fused_supported(const ntc_feature_desc* fd, int32_t hidden_dim,
+                        int32_t activation);
+size_t ntc_fused_table_elems(const ntc_feature_desc* fd, int32_t hidden_dim);
+int ntc_fused_max_blocks(int32_t* blocks);
+int ntc_fused_blocks_for(const ntc_feature_desc* fd, int32_t hidden_dim,
+                         int32_t activation, int64_t batch,
+                         int32_t max_blocks);
+/* FP32 decoder W0[H][F] | W1[H][H] | Wout[16][H] -> BF16 MFMA tables. */
+int ntc_pack_decoder_bf16(const ntc_feature_desc* fd, int32_t hidden_dim,
+                          int32_t out_dim, const float* dec, uint16_t* table,
+                          hipStream_t stream);
+int ntc_fused_train_step(const ntc_fused_args* args, hipStream_t stream);
+/* Sums the workgroup partials in order into grad, then Adam on W0 | W1 |
+ * Wout rows < O (update_adam x3, libopenntc.cpp:722-754); sse_out (may be
+ * NULL) receives the ordered sum of sse_partials. */
+int ntc_decoder_update(int32_t hidden_dim, int32_t feature_dim,
+                       int32_t out_dim, int32_t blocks,
+                       const float* dw_partials, float lr, float beta1,
+                       float beta2, float bias1, float bias2, float* grad,
+                       float* m, float* v, float* p,
+                       const double* sse_partials, double* sse_out,
+                       hipStream_t stream);
 
 /* ---- mip targets (prepare_tex / filter_lanczos / package_tex,
  * ntc_kernel.cu:533-686) ---------------------------------------------------- */

@@ -39,6 +39,15 @@ class Noise(C.Structure):
                 ("enabled", C.c_int32), ("delta0", C.c_float), ("delta1", C.c_float)]
 
 
+class FusedArgs(C.Structure):
+    _fields_ = [("feat", FeatureDesc), ("tiles", Tiles), ("noise", Noise),
+                ("hidden_dim", C.c_int32), ("out_dim", C.c_int32), ("activation", C.c_int32),
+                ("g0", C.c_void_p), ("g1", C.c_void_p), ("weights", C.c_void_p),
+                ("targets", C.c_void_p), ("loss_scale", C.c_float), ("scatter", C.c_int32),
+                ("dg0", C.c_void_p), ("dg1", C.c_void_p), ("dw_partials", C.c_void_p),
+                ("sse_partials", C.c_void_p), ("blocks", C.c_int32)]
+
+
 class LayoutInfo(C.Structure):
     _fields_ = [("g0_bits", C.c_int32), ("g1_bits", C.c_int32), ("g0_channels", C.c_int32),
                 ("g1_channels", C.c_int32), ("g0_scale", C.c_int32), ("hidden_dim", C.c_int32),
@@ -150,6 +159,20 @@ SIGNATURES = {
     "ntc_ctx_kernel_times": (_I, [_P, _P, _I]),
     "ntc_ctx_set_timing": (_I, [_P, _I]),
     "ntc_synthetic_pbr": (_I, [_I, _U, _P, _P, _P]),
+    "ntc_dev_alloc": (_I, [_SZ, _P]),
+    "ntc_dev_free": (_I, [_P]),
+    "ntc_dev_upload": (_I, [_P, _P, _SZ]),
+    "ntc_dev_download": (_I, [_P, _P, _SZ]),
+    "ntc_dev_synchronize": (_I, []),
+    "ntc_grid_update2_f32": (_I, [_I64, _P, _P, _P, _P, _F, _F, _I64, _P, _P, _P, _P, _F, _F,
+                                  _F, _F, _F, _F, _F, _P]),
+    "ntc_fused_supported": (_I, [_P, _I, _I]),
+    "ntc_fused_table_elems": (_SZ, [_P, _I]),
+    "ntc_fused_max_blocks": (_I, [_P]),
+    "ntc_fused_blocks_for": (_I, [_P, _I, _I, _I64, _I]),
+    "ntc_pack_decoder_bf16": (_I, [_P, _I, _I, _P, _P, _P]),
+    "ntc_fused_train_step": (_I, [_P, _P]),
+    "ntc_decoder_update": (_I, [_I, _I, _I, _I, _P, _F, _F, _F, _F, _F, _P, _P, _P, _P, _P, _P, _P]),
 }
 
 _lib = None

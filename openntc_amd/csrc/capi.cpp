@@ -16,6 +16,35 @@ struct ntc_context
 extern "C" {
 const char* ntc_version(void) { return "openntc-mi355x 0.1 (gfx950)"; }
 
+int ntc_dev_alloc(size_t bytes, void** out)
+{
+  if (!out) return NTC_E_INVALID;
+  *out = nullptr;
+  const hipError_t e = hipMalloc(out, bytes ? bytes : 16);
+  if (e != hipSuccess) return (int)e;
+  return (int)hipMemset(*out, 0, bytes ? bytes : 16);
+}
+
+int ntc_dev_free(void* p) { return p ? (int)hipFree(p) : NTC_OK; }
+
+int ntc_dev_upload(void* dst, const void* src, size_t bytes)
+{
+  if (!bytes) return NTC_OK;
+  const hipError_t e = hipDeviceSynchronize();
+  if (e != hipSuccess) return (int)e;
+  return (int)hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice);
+}
+
+int ntc_dev_download(void* dst, const void* src, size_t bytes)
+{
+  if (!bytes) return NTC_OK;
+  const hipError_t e = hipDeviceSynchronize();
+  if (e != hipSuccess) return (int)e;
+  return (int)hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost);
+}
+
+int ntc_dev_synchronize(void) { return (int)hipDeviceSynchronize(); }
+
 int ntc_device_check(int32_t* count)
 {
   int n = 0;

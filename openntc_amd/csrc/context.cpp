@@ -651,6 +651,89 @@ Result Context::Impl::RunPlainStep(const StepPlan& st, int32_t step_index)
   return Result::Success;
 }
 
+Result Context::Impl::EnsureFusedBuffers()
+{
+  if (max_blocks == 0)
+  {
+    int32_t b = 0;
+    if (ntc_fused_max_blocks(&b) != NTC_OK || b < 1) return Result::CudaFailure;
+    max_blocks = b;
+  }
+  const ntc_feature_desc fd = FeatureDesc(0);
+  const size_t tab = sizeof(uint16_t) * ntc_fused_table_elems(&fd, hidden_dim);
+  if (wtab.bytes < tab) CTX_HIP(wtab.Alloc(tab), Result::AllocationFailure);
+  const size_t part = sizeof(float) * (size_t)max_blocks * DecoderElems();
+  if (dw_part.bytes < part) CTX_HIP(dw_part.Alloc(part), Result::AllocationFailure);
+  if (sse_part.bytes < sizeof(double) * max_blocks)
+    CTX_HIP(sse_part.Alloc(sizeof(double) * max_blocks), Result::AllocationFailure);
+  return Result::Success;
+}
+
+Result Context::Impl::PackDecoderTables()
+{
+  const ntc_feature_desc fd = FeatureDesc(0);
+  CTX_NTC(ntc_pack_decoder_bf16(&fd, hidden_dim, out_dim, dec.as<float>(), wtab.as<uint16_t>(), stream));
+  return Result::Success;
+}
+
+// One step of the fused path: one persistent kernel for gather, forward,
+// loss, backward and latent scatter; the ordered decoder-gradient reduction
+// with Adam; the BF16 table refresh; one dense update of both grids of the
+// level (Adam + clamp + gradient reset). ev: 6 events or null.
+Result Context::Impl::RunFusedStep(const StepPlan& st, int32_t step_index, hipEvent_t* ev)
+{
+  const int lvl = st.level, T = st.tile_dim, M = mip_dim[st.lod];
+  const int64_t B = (int64_t)st.n_tiles * T * T;
+  const int H = hidden_dim, O = out_dim;
+  if (st.lock)
+  {
+    Result r = LockGrids();
+    if (r != Result::Success) return r;
+  }
+  const ntc_feature_desc fd = FeatureDesc(lvl);
+  const int act = ActCode();
+  ntc_fused_args a = {};
+  a.feat = fd;
+  a.tiles = {origins_table.as<int32_t>() + (int64_t)step_index * 16, st.n_tiles, T, M,
+             (float)st.lod / (float)(mip_count - 1)};
+  a.noise = {seed, (uint32_t)step_index, lvl, st.grid_update, g0_delta, g1_delta};
+  a.hidden_dim = H;
+  a.out_dim = O;
+  a.activation = act;
+  a.g0 = g0[lvl].as<float>();
+  a.g1 = g1[lvl].as<float>();
+  a.weights = wtab.as<uint16_t>();
+  a.targets = package[st.lod].as<uint8_t>();
+  a.loss_scale = -2.0f / ((float)O * (float)B);
+  a.scatter = st.grid_update;
+  a.dg0 = dg0[lvl].as<float>();
+  a.dg1 = dg1[lvl].as<float>();
+  a.dw_partials = dw_part.as<float>();
+  a.sse_partials = sse_part.as<double>();
+  a.blocks = ntc_fused_blocks_for(&fd, H, act, B, max_blocks);
+  if (ev) CTX_HIP(hipEventRecord(ev[0], stream), Result::CudaFailure);
+  CTX_NTC(ntc_fused_train_step(&a, stream));
+  if (ev) CTX_HIP(hipEventRecord(ev[1], stream), Result::CudaFailure);
+  CTX_NTC(ntc_decoder_update(H, feature_dim, O, a.blocks, dw_part.as<float>(), st.lr_decoder, 0.9f, 0.999f, st.bias1,
+                             st.bias2, dec_grad.as<float>(), dec_m.as<float>(), dec_v.as<float>(), dec.as<float>(),
+                             sse_part.as<double>(), scalars.as<double>() + 16, stream));
+  Result r = PackDecoderTables();
+  if (r != Result::Success) return r;
+  if (ev) CTX_HIP(hipEventRecord(ev[2], stream), Result::CudaFailure);
+  if (st.grid_update)
+  {
+    if (ev) CTX_HIP(hipEventRecord(ev[3], stream), Result::CudaFailure);
+    CTX_NTC(ntc_grid_update2_f32(GridElems(0, lvl), dg0[lvl].as<float>(), g0m[lvl].as<float>(), g0v[lvl].as<float>(),
+                                 g0[lvl].as<float>(), -1.0f, 1.0f - g0_delta, GridElems(1, lvl),
+                                 dg1[lvl].as<float>(), g1m[lvl].as<float>(), g1v[lvl].as<float>(),
+                                 g1[lvl].as<float>(), -1.0f, 1.0f - g1_delta, st.lr_grid, 0.9f, 0.999f,
+                                 st.grid_bias1, st.grid_bias2, stream));
+    if (ev) CTX_HIP(hipEventRecord(ev[4], stream), Result::CudaFailure);
+  }
+  grid_batch_i[lvl] += 1;
+  return Result::Success;
+}
+
 // Forward of one eval/calibration batch into x, z0, a0, z1, a1, yp (+ yt).
 Result Context::Impl::ForwardTiles(int32_t mip, const int32_t* origins, int32_t n_tiles, int32_t T,
                                    bool with_targets)
@@ -992,7 +1075,7 @@ void Context::BeginTraining(const TrainInfo& train_info)
   ex.lr_decoder_ = 0.005f;
   ex.init_lo_ = -0.1f;
   ex.init_hi_ = 0.1f;
-  ex.path_ = TrainPath::Plain;
+  ex.path_ = TrainPath::Fused;
   BeginTrainingEx(ex);
 }
 
@@ -1052,6 +1135,16 @@ void Context::BeginTrainingEx(const TrainInfoEx& info)
       ntc_fill_zero(I.dec_grad.ptr, I.dec_grad.bytes, I.stream) != NTC_OK)
     return fail(Result::CudaFailure);
 
+  if (I.path == TrainPath::Fused)
+  {
+    const ntc_feature_desc fd = I.FeatureDesc(0);
+    if (!ntc_fused_supported(&fd, I.hidden_dim, I.ActCode())) return fail(Result::InvalidState);
+    Result r = I.EnsureFusedBuffers();
+    if (r != Result::Success) return fail(r);
+    r = I.PackDecoderTables();
+    if (r != Result::Success) return fail(r);
+  }
+
   I.BuildPlan(I.batch_count);
   std::vector<int32_t> table((size_t)I.batch_count * 16, 0);
   for (int t = 0; t < I.batch_count; ++t) std::memcpy(&table[(size_t)t * 16], I.plan[t].origins, sizeof(int32_t) * 16);
@@ -1096,13 +1189,46 @@ TrainProgress Context::Train(int32_t num_batches)
     e.total_batches_ = I.batch_count;
     return e;
   };
+  const bool timed = I.timing && I.path == TrainPath::Fused;
+  if (timed)
+  {
+    const size_t need = (size_t)n * 6;
+    while (I.events.size() < need)
+    {
+      hipEvent_t e;
+      if (hipEventCreate(&e) != hipSuccess) return error(Result::CudaFailure);
+      I.events.push_back(e);
+    }
+  }
+  const int first = I.batch_i;
   for (; I.batch_i < target; ++I.batch_i)
   {
     const StepPlan& st = I.plan[I.batch_i];
-    Result r = I.RunPlainStep(st, I.batch_i);
+    Result r = I.path == TrainPath::Fused
+                   ? I.RunFusedStep(st, I.batch_i, timed ? &I.events[(size_t)(I.batch_i - first) * 6] : nullptr)
+                   : I.RunPlainStep(st, I.batch_i);
     if (r != Result::Success) return error(r);
   }
   if (hipStreamSynchronize(I.stream) != hipSuccess) return error(Result::CudaFailure);
+  if (timed)
+  {
+    double fused = 0, decoder = 0, grid = 0;
+    int grids = 0;
+    for (int s = 0; s < n; ++s)
+    {
+      hipEvent_t* ev = &I.events[(size_t)s * 6];
+      float ms = 0;
+      if (hipEventElapsedTime(&ms, ev[0], ev[1]) == hipSuccess) fused += ms;
+      if (hipEventElapsedTime(&ms, ev[1], ev[2]) == hipSuccess) decoder += ms;
+      if (I.plan[first + s].grid_update && hipEventElapsedTime(&ms, ev[3], ev[4]) == hipSuccess)
+      {
+        grid += ms;
+        ++grids;
+      }
+    }
+    I.kernel_ms = {(float)(fused / n), (float)(decoder / n), grids ? (float)(grid / grids) : -1.0f, (float)n,
+                   (float)grids};
+  }
   tp.phase_ = TrainPhase::TrainInProgress;
   tp.result_ = Result::Success;
   tp.batches_complete_ = I.batch_i;

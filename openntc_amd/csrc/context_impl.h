@@ -97,9 +97,20 @@ struct Context::Impl
   std::vector<StepPlan> plan;
   double last_step_sse = 0.0;
 
+  // fused path state
+  DeviceBuffer wtab;          // BF16 MFMA weight tables
+  DeviceBuffer dw_part;       // [blocks][decoder elems]
+  DeviceBuffer sse_part;      // [blocks]
+  int32_t max_blocks = 0;
+
   // ---- timing (optional) ----
+  // When enabled, Train records events around the fused step kernel, the
+  // decoder update (+ table pack) and the dense grid update of every step on
+  // the context stream; kernel_ms holds {fused, decoder, grid, steps timed,
+  // grid-update launches timed} averaged over the last Train call.
   bool timing = false;
   std::vector<float> kernel_ms;
+  std::vector<hipEvent_t> events;
 
   // ---- helpers ----
   int64_t DecoderElems() const
@@ -122,6 +133,10 @@ struct Context::Impl
   Result EnsurePlainBuffers(int64_t batch);
   void BuildPlan(int32_t steps);
   Result RunPlainStep(const StepPlan& st, int32_t step_index);
+  Result EnsureFusedBuffers();
+  Result PackDecoderTables();
+  Result RunFusedStep(const StepPlan& st, int32_t step_index, hipEvent_t* ev);
+  int ActCode() const { return activation == Activation::HardGELU ? NTC_ACT_HARDGELU : NTC_ACT_RELU; }
   Result LockGrids();
   Result Finalize(Context& ctx);
   Result ForwardTiles(int32_t mip, const int32_t* origins, int32_t n_tiles, int32_t T, bool with_targets);

@@ -1,0 +1,929 @@
+// Fused training step on BF16 MFMA (gfx950): gather + decoder forward + loss
+// + hand-written backward + latent-gradient scatter + per-workgroup decoder
+// gradient accumulation, in one persistent kernel. Replaces, per step, the
+// reference's draw_features, draw_targets, 9 cuBLAS SGEMMs, 4 HardGELU
+// launches, Saxpy/Sdot/scale and accumulate_grid_gradients
+// (libopenntc.cpp:621-709, ntc_kernel.cu:75-373).
+//
+// Work split: a workgroup (4 waves) takes passes of 128 texels; each wave
+// owns 32 texels (two 16-texel MFMA column tiles) and runs the whole
+// forward/backward chain for them in registers with
+// v_mfma_f32_16x16x32_bf16 (FP32 accumulation):
+//   * layer outputs stay in the MFMA accumulator layout (lane l holds rows
+//     4(l/16)+i of column l%16) and feed the next layer's B operand directly:
+//     the next layer's weight columns are stored permuted so that B-operand
+//     position (l/16, e) of K-chunk j means hidden unit
+//     kperm = 32j + (e<4 ? 4(l/16)+e : 16+4(l/16)+e-4);
+//   * all weight tables (W0, W1, Wout and the transposes the backward needs,
+//     BF16, permuted) sit in LDS for the whole kernel;
+//   * decoder weight gradients need the texel index as the MFMA K dimension,
+//     so activations are staged through LDS in [unit][texel] layout and the
+//     four waves each accumulate a quarter of dW in registers across passes;
+//     the partial sums go to HBM once per workgroup and are reduced in order
+//     by ntc_decoder_update (deterministic);
+//   * latent gradients are added with global float atomics (the reference
+//     uses atomics too, ntc_kernel.cu:261-278).
+// Quantisation noise is the same Philox stream as the plain path.
+#include "ntc_device.h"
+#include "ntc_host.h"
+
+using namespace ntc;
+
+namespace
+{
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int kThreads = 256;
+
+__device__ __forceinline__ uint16_t f2bf(float f)
+{
+  uint32_t u = __float_as_uint(f);
+  u += 0x7FFFu + ((u >> 16) & 1u);  // round to nearest even
+  return (uint16_t)(u >> 16);
+}
+
+__device__ __forceinline__ bf16x8 frag_lds(const uint16_t* p)
+{
+  const uint4 v = *reinterpret_cast<const uint4*>(p);
+  return __builtin_bit_cast(bf16x8, v);
+}
+
+__device__ __forceinline__ bf16x8 frag_pack(const float* v)
+{
+  uint4 u;
+  u.x = (uint32_t)f2bf(v[0]) | ((uint32_t)f2bf(v[1]) << 16);
+  u.y = (uint32_t)f2bf(v[2]) | ((uint32_t)f2bf(v[3]) << 16);
+  u.z = (uint32_t)f2bf(v[4]) | ((uint32_t)f2bf(v[5]) << 16);
+  u.w = (uint32_t)f2bf(v[6]) | ((uint32_t)f2bf(v[7]) << 16);
+  return __builtin_bit_cast(bf16x8, u);
+}
+
+__device__ __forceinline__ uint16_t frag_elem(const bf16x8& f, int e)
+{
+  const uint4 u = __builtin_bit_cast(uint4, f);
+  const uint32_t w = e < 2 ? u.x : (e < 4 ? u.y : (e < 6 ? u.z : u.w));
+  return (uint16_t)((e & 1) ? (w >> 16) : (w & 0xFFFFu));
+}
+
+__device__ __forceinline__ f32x4 mfma(const bf16x8& a, const bf16x8& b, const f32x4& c)
+{
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+// Hidden unit held at B-operand position (g, e) of K-chunk j.
+__host__ __device__ __forceinline__ int kperm(int j, int g, int e)
+{
+  return 32 * j + (e < 4 ? 4 * g + e : 16 + 4 * g + (e - 4));
+}
+
+__host__ __device__ __forceinline__ int kperm_pos(int pos)
+{
+  const int j = pos >> 5, r = pos & 31;
+  return kperm(j, r >> 3, r & 7);
+}
+
+// Weight table layout (elements, BF16), identical in HBM and LDS.
+struct TableLayout
+{
+  int H, FP, G;
+  __host__ __device__ int oW0() const { return 0; }                   // [H][FP]
+  __host__ __device__ int oW1() const { return H * FP; }              // [H][H]  cols kperm
+  __host__ __device__ int oWo() const { return oW1() + H * H; }       // [16][H] cols kperm
+  __host__ __device__ int oWoT() const { return oWo() + 16 * H; }     // [H][32] cols (4g+e, e<4)
+  __host__ __device__ int oW1T() const { return oWoT() + 32 * H; }    // [H][H]  W1^T, cols kperm
+  __host__ __device__ int oW0T() const { return oW1T() + H * H; }     // [G][H]  W0^T, cols kperm
+  __host__ __device__ int total() const { return oW0T() + G * H; }
+};
+
+struct FusedParams
+{
+  ntc_feature_desc fd;
+  ntc_tiles tl;
+  ntc_noise nz;
+  const float* g0;
+  const float* g1;
+  const uint16_t* wtab;
+  const uint8_t* pkg;
+  int O;
+  int F;
+  float scale;
+  int64_t B;
+  int scatter;
+  float* dg0;
+  float* dg1;
+  float* dw_part;
+  double* sse_part;
+  int dec_elems;
+};
+
+// NT: 16-texel MFMA column tiles per wave; a workgroup pass covers 64*NT texels.
+template <int H, int FP, int G, int C0, int C1, int G0MODE, int PEC, int LODS, int ACT, int NT>
+struct Cfg
+{
+  static constexpr int PASS = 64 * NT;
+  static constexpr int MT = H / 16;      // hidden M-tiles
+  static constexpr int KF = FP / 32;     // K chunks over features
+  static constexpr int KH = H / 32;      // K chunks over hidden units
+  static constexpr int GT = G / 16;      // dX M-tiles (grid slots)
+  static constexpr int RW = H / 4;       // dW rows per wave
+  static constexpr int MTW = RW / 16;    // dW M-tiles per wave
+  static constexpr int NTO = H / 64;     // dWout column tiles per wave
+  static constexpr int G1B = G0MODE == NTC_G0_CORNERS ? 4 * C0 : C0;
+  static constexpr int GSLOTS = G1B + C1;
+  static constexpr int PE_FIRST = PEC == 3 ? 8 : 4;
+  static constexpr int WT = H * FP + 2 * H * H + 48 * H + G * H;
+  static constexpr int XFN = FP * PASS;
+  static constexpr int STAGE_B = 2 * H * 64 + 16 * 64;
+  static constexpr int UNION = (PASS * FP > STAGE_B) ? PASS * FP : STAGE_B;
+  static constexpr int LDS = WT + XFN + UNION;
+  static_assert(FP % 32 == 0 && H % 64 == 0 && G % 16 == 0, "tile shapes");
+  static_assert(GSLOTS <= G && G <= FP, "grid slots");
+  static_assert(LDS * 2 + 8 * 256 <= 160 * 1024, "LDS budget");
+};
+
+// Gather one texel's feature slots [lo, hi) into the LDS copies of X.
+template <class CF, int C0, int C1, int G0MODE, int PEC, int LODS>
+__device__ __forceinline__ void gather_part(const FusedParams& P, int part, int tl, bool valid, int px, int py,
+                                            uint16_t* xnf, uint16_t* xfn)
+{
+  constexpr int FP = CF::KF * 32;
+  auto put = [&](int slot, float v) {
+    const uint16_t b = f2bf(v);
+    xnf[tl * FP + slot] = b;
+    xfn[slot * CF::PASS + tl] = b;
+  };
+  const int M = P.tl.mip_dim;
+  if (part == 0)
+  {
+    if (!valid)
+    {
+      for (int s = 0; s < CF::G1B; ++s) put(s, 0.0f);
+      return;
+    }
+    const Footprint<float> f0 = footprint<float>(px, py, M, P.fd.g0_dim);
+    if (G0MODE == NTC_G0_CORNERS)
+    {
+#pragma unroll
+      for (int c = 0; c < 4; ++c)
+      {
+#pragma unroll
+        for (int q = 0; q < (C0 + 3) / 4; ++q)
+        {
+          const uint4 w = P.nz.enabled ? noise_words(P.nz.seed, P.nz.step, 0, P.nz.level, (uint32_t)f0.cell[c], q)
+                                       : make_uint4(0, 0, 0, 0);
+#pragma unroll
+          for (int e = 0; e < 4; ++e)
+          {
+            const int ch = 4 * q + e;
+            if (ch < C0)
+            {
+              float v = P.g0[(int64_t)f0.cell[c] * C0 + ch];
+              if (P.nz.enabled) v = v + noise_value(pick(w, e), P.nz.delta0);
+              put(c * C0 + ch, v);
+            }
+          }
+        }
+      }
+    }
+    else
+    {
+#pragma unroll
+      for (int q = 0; q < (C0 + 3) / 4; ++q)
+      {
+        uint4 w[4];
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+          w[c] = P.nz.enabled ? noise_words(P.nz.seed, P.nz.step, 0, P.nz.level, (uint32_t)f0.cell[c], q)
+                              : make_uint4(0, 0, 0, 0);
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+        {
+          const int ch = 4 * q + e;
+          if (ch < C0)
+          {
+            float acc = 0.0f;
+#pragma unroll
+            for (int c = 0; c < 4; ++c)
+            {
+              float v = P.g0[(int64_t)f0.cell[c] * C0 + ch];
+              if (P.nz.enabled) v = v + noise_value(pick(w[c], e), P.nz.delta0);
+              acc = fmaf(f0.w[c], v, acc);
+            }
+            put(ch, acc);
+          }
+        }
+      }
+    }
+    return;
+  }
+  // part 1: g1, positional encoding, lod, zero padding
+  if (!valid)
+  {
+    for (int s = CF::G1B; s < FP; ++s) put(s, 0.0f);
+    return;
+  }
+  const Footprint<float> f1 = footprint<float>(px, py, M, P.fd.g1_dim);
+#pragma unroll
+  for (int q = 0; q < (C1 + 3) / 4; ++q)
+  {
+    uint4 w[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+      w[c] = P.nz.enabled ? noise_words(P.nz.seed, P.nz.step, 1, P.nz.level, (uint32_t)f1.cell[c], q)
+                          : make_uint4(0, 0, 0, 0);
+#pragma unroll
+    for (int e = 0; e < 4; ++e)
+    {
+      const int ch = 4 * q + e;
+      if (ch < C1)
+      {
+        float acc = 0.0f;
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+        {
+          float v = P.g1[(int64_t)f1.cell[c] * C1 + ch];
+          if (P.nz.enabled) v = v + noise_value(pick(w[c], e), P.nz.delta1);
+          acc = fmaf(f1.w[c], v, acc);
+        }
+        put(CF::G1B + ch, acc);
+      }
+    }
+  }
+  int slot = CF::GSLOTS;
+#pragma unroll
+  for (int pi = 0; pi < PEC; ++pi)
+  {
+    const int Pd = CF::PE_FIRST >> pi;
+#pragma unroll
+    for (int axis = 0; axis < 2; ++axis)
+#pragma unroll
+      for (int h = 0; h < 2; ++h) put(slot++, tri_wave<float>(axis == 0 ? px : py, Pd, h));
+  }
+  if (LODS) put(slot++, P.tl.norm_lod);
+  for (; slot < FP; ++slot) put(slot, 0.0f);
+}
+
+template <int H, int FP, int G, int C0, int C1, int G0MODE, int PEC, int LODS, int ACT, int NT>
+__global__ __launch_bounds__(kThreads, 1) void k_fused_train(FusedParams P)
+{
+  using CF = Cfg<H, FP, G, C0, C1, G0MODE, PEC, LODS, ACT, NT>;
+  constexpr int kNT = NT, kPass = CF::PASS;
+  constexpr int MT = CF::MT, KF = CF::KF, KH = CF::KH, GT = CF::GT, MTW = CF::MTW, NTO = CF::NTO;
+  constexpr TableLayout TLY = {H, FP, G};
+  __shared__ __attribute__((aligned(16))) uint16_t smem[CF::LDS];
+  uint16_t* wt = smem;
+  uint16_t* xfn = smem + CF::WT;
+  uint16_t* uni = xfn + CF::XFN;
+  uint16_t* xnf = uni;                       // during gather/forward
+  uint16_t* stA = uni;                       // [H][64]  A0 | dZ0
+  uint16_t* stB = uni + H * 64;              // [H][64]  dZ1 | A1
+  uint16_t* stY = uni + 2 * H * 64;          // [16][64] dY
+
+  const int tid = threadIdx.x;
+  const int wave = tid >> 6, lane = tid & 63;
+  const int n16 = lane & 15, g = lane >> 4;
+
+  // weights -> LDS (16-byte copies)
+  {
+    const uint4* src = reinterpret_cast<const uint4*>(P.wtab);
+    uint4* dst = reinterpret_cast<uint4*>(wt);
+    for (int i = tid; i < CF::WT / 8; i += kThreads) dst[i] = src[i];
+  }
+
+  f32x4 dw1[MTW][MT], dw0[MTW][FP / 16], dwo[NTO];
+#pragma unroll
+  for (int a = 0; a < MTW; ++a)
+  {
+#pragma unroll
+    for (int c = 0; c < MT; ++c) dw1[a][c] = f32x4{0, 0, 0, 0};
+#pragma unroll
+    for (int c = 0; c < FP / 16; ++c) dw0[a][c] = f32x4{0, 0, 0, 0};
+  }
+#pragma unroll
+  for (int c = 0; c < NTO; ++c) dwo[c] = f32x4{0, 0, 0, 0};
+  double sse = 0.0;
+
+  const int64_t npass = (P.B + kPass - 1) / kPass;
+  const int T = P.tl.tile_dim, M = P.tl.mip_dim;
+  for (int64_t pass = blockIdx.x; pass < npass; pass += gridDim.x)
+  {
+    const int64_t tb = pass * kPass;
+    __syncthreads();  // previous pass done with the union region; weights visible on the first pass
+    // ---- P0: gather ----
+    {
+      // threads [0,128) gather g0 slots, [128,256) the rest; NT passes of 64 texels
+      const int part = tid >> 7;
+      for (int tl = tid & 127; tl < kPass; tl += 128)
+      {
+        const int64_t t = tb + tl;
+        const bool valid = t < P.B;
+        int px = 0, py = 0;
+        if (valid)
+        {
+          const TexelPos pos = texel_of(t, P.tl.origins, T, M);
+          px = pos.x;
+          py = pos.y;
+        }
+        gather_part<CF, C0, C1, G0MODE, PEC, LODS>(P, part, tl, valid, px, py, xnf, xfn);
+      }
+    }
+    __syncthreads();
+
+    // ---- P1: per-wave forward + backward for 32 texels ----
+    int tloc[kNT];
+    bool tvalid[kNT];
+    int tx[kNT], ty[kNT];
+#pragma unroll
+    for (int nt = 0; nt < kNT; ++nt)
+    {
+      tloc[nt] = wave * 16 * kNT + nt * 16 + n16;
+      const int64_t t = tb + tloc[nt];
+      tvalid[nt] = t < P.B;
+      tx[nt] = 0;
+      ty[nt] = 0;
+      if (tvalid[nt])
+      {
+        const TexelPos pos = texel_of(t, P.tl.origins, T, M);
+        tx[nt] = pos.x;
+        ty[nt] = pos.y;
+      }
+    }
+    // layer 0
+    f32x4 z0[MT][kNT];
+    {
+      bf16x8 xb[KF][kNT];
+#pragma unroll
+      for (int j = 0; j < KF; ++j)
+#pragma unroll
+        for (int nt = 0; nt < kNT; ++nt) xb[j][nt] = frag_lds(&xnf[tloc[nt] * FP + 32 * j + 8 * g]);
+#pragma unroll
+      for (int r = 0; r < MT; ++r)
+      {
+#pragma unroll
+        for (int nt = 0; nt < kNT; ++nt) z0[r][nt] = f32x4{0, 0, 0, 0};
+#pragma unroll
+        for (int j = 0; j < KF; ++j)
+        {
+          const bf16x8 a = frag_lds(&wt[TLY.oW0() + (16 * r + n16) * FP + 32 * j + 8 * g]);
+#pragma unroll
+          for (int nt = 0; nt < kNT; ++nt) z0[r][nt] = mfma(a, xb[j][nt], z0[r][nt]);
+        }
+      }
+    }
+    bf16x8 a0f[KH][kNT];
+#pragma unroll
+    for (int j = 0; j < KH; ++j)
+#pragma unroll
+      for (int nt = 0; nt < kNT; ++nt)
+      {
+        float v[8];
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+        {
+          v[i] = act_fwd<float>(ACT, z0[2 * j][nt][i]);
+          v[4 + i] = act_fwd<float>(ACT, z0[2 * j + 1][nt][i]);
+        }
+        a0f[j][nt] = frag_pack(v);
+      }
+    // layer 1
+    f32x4 z1[MT][kNT];
+#pragma unroll
+    for (int r = 0; r < MT; ++r)
+    {
+#pragma unroll
+      for (int nt = 0; nt < kNT; ++nt) z1[r][nt] = f32x4{0, 0, 0, 0};
+#pragma unroll
+      for (int j = 0; j < KH; ++j)
+      {
+        const bf16x8 a = frag_lds(&wt[TLY.oW1() + (16 * r + n16) * H + 32 * j + 8 * g]);
+#pragma unroll
+        for (int nt = 0; nt < kNT; ++nt) z1[r][nt] = mfma(a, a0f[j][nt], z1[r][nt]);
+      }
+    }
+    bf16x8 a1f[KH][kNT];
+#pragma unroll
+    for (int j = 0; j < KH; ++j)
+#pragma unroll
+      for (int nt = 0; nt < kNT; ++nt)
+      {
+        float v[8];
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+        {
+          v[i] = act_fwd<float>(ACT, z1[2 * j][nt][i]);
+          v[4 + i] = act_fwd<float>(ACT, z1[2 * j + 1][nt][i]);
+        }
+        a1f[j][nt] = frag_pack(v);
+      }
+    // output layer + loss
+    float dy[kNT][4];
+    bf16x8 dyf[kNT];
+#pragma unroll
+    for (int nt = 0; nt < kNT; ++nt)
+    {
+      f32x4 yh = {0, 0, 0, 0};
+#pragma unroll
+      for (int j = 0; j < KH; ++j)
+        yh = mfma(frag_lds(&wt[TLY.oWo() + n16 * H + 32 * j + 8 * g]), a1f[j][nt], yh);
+      const int64_t base = ((int64_t)ty[nt] * M + tx[nt]) * P.O;
+      float v[8];
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+      {
+        const int o = 4 * g + i;
+        float d = 0.0f;
+        if (tvalid[nt] && o < P.O)
+        {
+          const float yt = (float)P.pkg[base + o] / 255.0f;
+          const float r = yt - yh[i];
+          sse += (double)r * (double)r;
+          d = P.scale * r;
+        }
+        dy[nt][i] = d;
+        v[i] = d;
+        v[4 + i] = 0.0f;
+      }
+      dyf[nt] = frag_pack(v);
+    }
+    // backward through layer 2 and the activation of layer 1
+    bf16x8 dz1f[KH][kNT];
+    {
+      f32x4 dz1[MT][kNT];
+#pragma unroll
+      for (int r = 0; r < MT; ++r)
+      {
+        const bf16x8 a = frag_lds(&wt[TLY.oWoT() + (16 * r + n16) * 32 + 8 * g]);
+#pragma unroll
+        for (int nt = 0; nt < kNT; ++nt)
+        {
+          const f32x4 da = mfma(a, dyf[nt], f32x4{0, 0, 0, 0});
+#pragma unroll
+          for (int i = 0; i < 4; ++i) dz1[r][nt][i] = act_bwd<float>(ACT, z1[r][nt][i], da[i]);
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < KH; ++j)
+#pragma unroll
+        for (int nt = 0; nt < kNT; ++nt)
+        {
+          float v[8];
+#pragma unroll
+          for (int i = 0; i < 4; ++i)
+          {
+            v[i] = dz1[2 * j][nt][i];
+            v[4 + i] = dz1[2 * j + 1][nt][i];
+          }
+          dz1f[j][nt] = frag_pack(v);
+        }
+    }
+    // backward through layer 1 and the activation of layer 0
+    bf16x8 dz0f[KH][kNT];
+    {
+      f32x4 dz0[MT][kNT];
+#pragma unroll
+      for (int r = 0; r < MT; ++r)
+      {
+        f32x4 da[kNT];
+#pragma unroll
+        for (int nt = 0; nt < kNT; ++nt) da[nt] = f32x4{0, 0, 0, 0};
+#pragma unroll
+        for (int j = 0; j < KH; ++j)
+        {
+          const bf16x8 a = frag_lds(&wt[TLY.oW1T() + (16 * r + n16) * H + 32 * j + 8 * g]);
+#pragma unroll
+          for (int nt = 0; nt < kNT; ++nt) da[nt] = mfma(a, dz1f[j][nt], da[nt]);
+        }
+#pragma unroll
+        for (int nt = 0; nt < kNT; ++nt)
+#pragma unroll
+          for (int i = 0; i < 4; ++i) dz0[r][nt][i] = act_bwd<float>(ACT, z0[r][nt][i], da[nt][i]);
+      }
+#pragma unroll
+      for (int j = 0; j < KH; ++j)
+#pragma unroll
+        for (int nt = 0; nt < kNT; ++nt)
+        {
+          float v[8];
+#pragma unroll
+          for (int i = 0; i < 4; ++i)
+          {
+            v[i] = dz0[2 * j][nt][i];
+            v[4 + i] = dz0[2 * j + 1][nt][i];
+          }
+          dz0f[j][nt] = frag_pack(v);
+        }
+    }
+    // dX for the grid slots, scattered into the active level's gradients
+    if (P.scatter)
+    {
+#pragma unroll
+      for (int nt = 0; nt < kNT; ++nt)
+      {
+        if (!tvalid[nt]) continue;
+        const Footprint<float> f0 = footprint<float>(tx[nt], ty[nt], M, P.fd.g0_dim);
+        const Footprint<float> f1 = footprint<float>(tx[nt], ty[nt], M, P.fd.g1_dim);
+#pragma unroll
+        for (int r = 0; r < GT; ++r)
+        {
+          f32x4 dx = {0, 0, 0, 0};
+#pragma unroll
+          for (int j = 0; j < KH; ++j)
+            dx = mfma(frag_lds(&wt[TLY.oW0T() + (16 * r + n16) * H + 32 * j + 8 * g]), dz0f[j][nt], dx);
+#pragma unroll
+          for (int i = 0; i < 4; ++i)
+          {
+            const int slot = 16 * r + 4 * g + i;
+            if (slot < CF::G1B)
+            {
+              if (G0MODE == NTC_G0_CORNERS)
+              {
+                const int c = slot / C0, ch = slot - (slot / C0) * C0;
+                atomicAdd(&P.dg0[(int64_t)f0.cell[c] * C0 + ch], dx[i]);
+              }
+              else
+              {
+#pragma unroll
+                for (int c = 0; c < 4; ++c) atomicAdd(&P.dg0[(int64_t)f0.cell[c] * C0 + slot], f0.w[c] * dx[i]);
+              }
+            }
+            else if (slot < CF::GSLOTS)
+            {
+              const int ch = slot - CF::G1B;
+#pragma unroll
+              for (int c = 0; c < 4; ++c) atomicAdd(&P.dg1[(int64_t)f1.cell[c] * C1 + ch], f1.w[c] * dx[i]);
+            }
+          }
+        }
+      }
+    }
+
+    // ---- P2/P3: decoder weight gradients over the pass (K = texels) ----
+    // Round r stages column tile r of every wave: staged column k = 16*wave + n
+    // holds texel 16*NT*wave + 16*r + n.
+    __syncthreads();  // everyone done reading xnf
+#pragma unroll
+    for (int rnd = 0; rnd < kNT; ++rnd)
+    {
+      const int col = wave * 16 + n16;
+#pragma unroll
+      for (int j = 0; j < KH; ++j)
+#pragma unroll
+        for (int e = 0; e < 8; ++e)
+        {
+          const int h = kperm(j, g, e);
+          stA[h * 64 + col] = frag_elem(a0f[j][rnd], e);
+          stB[h * 64 + col] = frag_elem(dz1f[j][rnd], e);
+        }
+      __syncthreads();
+      // dW1[h_out][h_in] += dZ1 A0^T, rows of this wave
+#pragma unroll
+      for (int mt = 0; mt < MTW; ++mt)
+      {
+        const int row = CF::RW * wave + 16 * mt + n16;
+#pragma unroll
+        for (int kc = 0; kc < 2; ++kc)
+        {
+          const bf16x8 a = frag_lds(&stB[row * 64 + 32 * kc + 8 * g]);
+#pragma unroll
+          for (int c = 0; c < MT; ++c)
+            dw1[mt][c] = mfma(a, frag_lds(&stA[(16 * c + n16) * 64 + 32 * kc + 8 * g]), dw1[mt][c]);
+        }
+      }
+      __syncthreads();
+#pragma unroll
+      for (int j = 0; j < KH; ++j)
+#pragma unroll
+        for (int e = 0; e < 8; ++e)
+        {
+          const int h = kperm(j, g, e);
+          stA[h * 64 + col] = frag_elem(dz0f[j][rnd], e);
+          stB[h * 64 + col] = frag_elem(a1f[j][rnd], e);
+        }
+#pragma unroll
+      for (int i = 0; i < 4; ++i) stY[(4 * g + i) * 64 + col] = f2bf(dy[rnd][i]);
+      __syncthreads();
+      // dW0[h][f] += dZ0 X^T, X^T read from xfn in the staged texel order
+#pragma unroll
+      for (int mt = 0; mt < MTW; ++mt)
+      {
+        const int row = CF::RW * wave + 16 * mt + n16;
+#pragma unroll
+        for (int kc = 0; kc < 2; ++kc)
+        {
+          const int kk = 32 * kc + 8 * g;
+          const int xcol = (kk >> 4) * 16 * kNT + 16 * rnd + (kk & 15);
+          const bf16x8 a = frag_lds(&stA[row * 64 + kk]);
+#pragma unroll
+          for (int c = 0; c < FP / 16; ++c)
+            dw0[mt][c] = mfma(a, frag_lds(&xfn[(16 * c + n16) * kPass + xcol]), dw0[mt][c]);
+        }
+      }
+      // dWout[o][h] += dY A1^T, column tiles of this wave
+#pragma unroll
+      for (int kc = 0; kc < 2; ++kc)
+      {
+        const bf16x8 a = frag_lds(&stY[n16 * 64 + 32 * kc + 8 * g]);
+#pragma unroll
+        for (int c = 0; c < NTO; ++c)
+        {
+          const int hc = (wave * NTO + c) * 16 + n16;
+          dwo[c] = mfma(a, frag_lds(&stB[hc * 64 + 32 * kc + 8 * g]), dwo[c]);
+        }
+      }
+      __syncthreads();
+    }
+  }
+
+  // ---- partial decoder gradients of this workgroup: W0[H][F] | W1[H][H] | Wout[16][H]
+  float* part = P.dw_part + (int64_t)blockIdx.x * P.dec_elems;
+  const int F = P.F;
+  float* pw0 = part;
+  float* pw1 = part + H * F;
+  float* pwo = pw1 + H * H;
+#pragma unroll
+  for (int mt = 0; mt < MTW; ++mt)
+  {
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+    {
+      const int row = CF::RW * wave + 16 * mt + 4 * g + i;
+#pragma unroll
+      for (int c = 0; c < MT; ++c) pw1[row * H + 16 * c + n16] = dw1[mt][c][i];
+#pragma unroll
+      for (int c = 0; c < FP / 16; ++c)
+      {
+        const int col = 16 * c + n16;
+        if (col < F) pw0[row * F + col] = dw0[mt][c][i];
+      }
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < NTO; ++c)
+#pragma unroll
+    for (int i = 0; i < 4; ++i) pwo[(4 * g + i) * H + (wave * NTO + c) * 16 + n16] = dwo[c][i];
+
+  // SSE of this workgroup
+  __shared__ double red[kThreads];
+  red[tid] = sse;
+  __syncthreads();
+  for (int w = kThreads / 2; w > 0; w >>= 1)
+  {
+    if (tid < w) red[tid] += red[tid + w];
+    __syncthreads();
+  }
+  if (tid == 0) P.sse_part[blockIdx.x] = red[0];
+}
+
+// ---- decoder table packing (FP32 masters -> permuted BF16 tables) ----------------
+__global__ __launch_bounds__(256) void k_pack_tables(TableLayout L, int F, int O, int gslots,
+                                                     const float* __restrict__ dec, uint16_t* __restrict__ tab)
+{
+  const int e = blockIdx.x * 256 + threadIdx.x;
+  if (e >= L.total()) return;
+  const int H = L.H;
+  const float* w0 = dec;
+  const float* w1 = dec + H * F;
+  const float* wo = w1 + H * H;
+  float v = 0.0f;
+  if (e < L.oW1())
+  {
+    const int m = e / L.FP, c = e % L.FP;
+    v = c < F ? w0[m * F + c] : 0.0f;
+  }
+  else if (e < L.oWo())
+  {
+    const int i = e - L.oW1(), m = i / H, pos = i % H;
+    v = w1[m * H + kperm_pos(pos)];
+  }
+  else if (e < L.oWoT())
+  {
+    const int i = e - L.oWo(), o = i / H, pos = i % H;
+    v = o < O ? wo[o * H + kperm_pos(pos)] : 0.0f;
+  }
+  else if (e < L.oW1T())
+  {
+    const int i = e - L.oWoT(), h = i / 32, pos = i % 32, gg = pos >> 3, ee = pos & 7;
+    const int o = 4 * gg + ee;
+    v = (ee < 4 && o < O) ? wo[o * H + h] : 0.0f;
+  }
+  else if (e < L.oW0T())
+  {
+    const int i = e - L.oW1T(), hin = i / H, pos = i % H;
+    v = w1[kperm_pos(pos) * H + hin];
+  }
+  else
+  {
+    const int i = e - L.oW0T(), s = i / H, pos = i % H;
+    v = s < gslots ? w0[kperm_pos(pos) * F + s] : 0.0f;
+  }
+  tab[e] = f2bf(v);
+}
+
+// ---- decoder update: ordered sum of workgroup partials + Adam ----------------------
+struct DecAdam
+{
+  float lr, b1, b2, omb1, omb2, bias1, bias2;
+};
+
+__global__ __launch_bounds__(256) void k_decoder_update(int n_dec, int n_adam_w01, int H, int O, int blocks,
+                                                        const float* __restrict__ part, DecAdam c,
+                                                        float* __restrict__ grad, float* __restrict__ m,
+                                                        float* __restrict__ v, float* __restrict__ p,
+                                                        const double* __restrict__ sse_part,
+                                                        double* __restrict__ sse_out)
+{
+  const int e = blockIdx.x * 256 + threadIdx.x;
+  if (e == 0 && sse_out)
+  {
+    double s = 0.0;
+    for (int b = 0; b < blocks; ++b) s += sse_part[b];
+    *sse_out = s;
+  }
+  if (e >= n_dec) return;
+  // Wout rows >= O are outside the optimised set (libopenntc.cpp:722-732)
+  if (e >= n_adam_w01 && (e - n_adam_w01) / H >= O) return;
+  float gsum = 0.0f;
+  for (int b = 0; b < blocks; ++b) gsum = gsum + part[(int64_t)b * n_dec + e];
+  grad[e] = gsum;
+  const float mi = fmaf(c.b1, m[e], c.omb1 * gsum);
+  const float vi = fmaf(c.b2, v[e], (c.omb2 * gsum) * gsum);
+  m[e] = mi;
+  v[e] = vi;
+  p[e] = p[e] - (c.lr * (mi * c.bias1)) / (__builtin_sqrtf(vi * c.bias2) + 1e-8f);
+}
+
+// ---- dispatch ---------------------------------------------------------------------------
+struct Shape
+{
+  int H, FP, G, C0, C1, G0MODE, PEC, LODS, ACT;
+};
+
+// Column tiles per wave for a hidden width (register budget: H = 128 keeps
+// one 16-texel tile per wave in flight).
+constexpr int nt_for(int H) { return H >= 128 ? 1 : 2; }
+
+int round_up(int n, int k) { return ((n + k - 1) / k) * k; }
+
+bool shape_of(const ntc_feature_desc& fd, int H, int act, Shape& s)
+{
+  s.H = H;
+  s.FP = round_up(fd.feature_dim, 32);
+  const int g1b = fd.g0_mode == NTC_G0_CORNERS ? 4 * fd.g0_channels : fd.g0_channels;
+  s.G = round_up(g1b + fd.g1_channels, 16);
+  s.C0 = fd.g0_channels;
+  s.C1 = fd.g1_channels;
+  s.G0MODE = fd.g0_mode;
+  s.PEC = fd.pe_count;
+  s.LODS = fd.lod_slot;
+  s.ACT = act;
+  return true;
+}
+
+typedef void (*FusedKernel)(FusedParams);
+
+template <int H, int FP, int G, int C0, int C1, int G0MODE, int PEC, int LODS, int ACT>
+constexpr Shape shape_v()
+{
+  return Shape{H, FP, G, C0, C1, G0MODE, PEC, LODS, ACT};
+}
+
+struct Entry
+{
+  Shape s;
+  FusedKernel k;
+  int pass;  // texels per workgroup pass
+};
+
+#define NTC_FUSED(H, FP, G, C0, C1, MODE, PEC, LODS, ACT)                 \
+  Entry{shape_v<H, FP, G, C0, C1, MODE, PEC, LODS, ACT>(),                 \
+        k_fused_train<H, FP, G, C0, C1, MODE, PEC, LODS, ACT, nt_for(H)>, 64 * nt_for(H)}
+
+// Layouts with a compiled kernel: the reference profiles (corner g0 features,
+// HardGELU) and the measured configurations (bilinear g0, ReLU or HardGELU).
+const Entry kEntries[] = {
+    NTC_FUSED(64, 64, 48, 8, 12, NTC_G0_CORNERS, 3, 1, NTC_ACT_HARDGELU),   // Bpp_0_2
+    NTC_FUSED(64, 96, 80, 12, 20, NTC_G0_CORNERS, 3, 1, NTC_ACT_HARDGELU),  // Bpp_0_5
+    NTC_FUSED(64, 96, 64, 12, 10, NTC_G0_CORNERS, 3, 1, NTC_ACT_HARDGELU),  // Bpp_1_0
+    NTC_FUSED(64, 96, 80, 16, 12, NTC_G0_CORNERS, 3, 1, NTC_ACT_HARDGELU),  // Bpp_2_25
+    NTC_FUSED(64, 32, 32, 8, 16, NTC_G0_BILINEAR, 2, 0, NTC_ACT_RELU),      // 8+16+8 -> 64
+    NTC_FUSED(64, 64, 32, 12, 16, NTC_G0_BILINEAR, 2, 0, NTC_ACT_RELU),     // 12+16+8 -> 64
+    NTC_FUSED(128, 64, 32, 16, 16, NTC_G0_BILINEAR, 2, 0, NTC_ACT_RELU),    // 16+16+8 -> 128
+    NTC_FUSED(64, 32, 32, 8, 16, NTC_G0_BILINEAR, 2, 0, NTC_ACT_HARDGELU),
+    NTC_FUSED(128, 64, 32, 16, 16, NTC_G0_BILINEAR, 2, 0, NTC_ACT_HARDGELU),
+};
+
+bool same(const Shape& a, const Shape& b)
+{
+  return a.H == b.H && a.FP == b.FP && a.G == b.G && a.C0 == b.C0 && a.C1 == b.C1 && a.G0MODE == b.G0MODE &&
+         a.PEC == b.PEC && a.LODS == b.LODS && a.ACT == b.ACT;
+}
+
+const Entry* find_entry(const ntc_feature_desc& fd, int H, int act)
+{
+  Shape s;
+  shape_of(fd, H, act, s);
+  for (const Entry& e : kEntries)
+    if (same(e.s, s)) return &e;
+  return nullptr;
+}
+} // namespace
+
+extern "C" {
+int ntc_fused_supported(const ntc_feature_desc* fd, int32_t hidden_dim, int32_t activation)
+{
+  if (!valid_feature_desc(fd)) return 0;
+  return find_entry(*fd, hidden_dim, activation) != nullptr;
+}
+
+size_t ntc_fused_table_elems(const ntc_feature_desc* fd, int32_t hidden_dim)
+{
+  if (!valid_feature_desc(fd)) return 0;
+  Shape s;
+  shape_of(*fd, hidden_dim, 0, s);
+  return (size_t)TableLayout{s.H, s.FP, s.G}.total();
+}
+
+int ntc_fused_max_blocks(int32_t* blocks)
+{
+  int dev = 0, cus = 0;
+  NTC_HIP_TRY(hipGetDevice(&dev));
+  NTC_HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  if (blocks) *blocks = cus;
+  return NTC_OK;
+}
+
+int ntc_pack_decoder_bf16(const ntc_feature_desc* fd, int32_t hidden_dim, int32_t out_dim, const float* dec,
+                          uint16_t* table, hipStream_t stream)
+{
+  NTC_RETURN_IF(!valid_feature_desc(fd) || !dec || !table || out_dim < 1 || out_dim > 16, NTC_E_INVALID);
+  NTC_RETURN_IF(hidden_dim % 64 != 0 || hidden_dim < 64 || hidden_dim > 256, NTC_E_INVALID);
+  Shape s;
+  shape_of(*fd, hidden_dim, 0, s);
+  const TableLayout L{s.H, s.FP, s.G};
+  const int gslots = g1_base(*fd) + fd->g1_channels;
+  hipLaunchKernelGGL(k_pack_tables, dim3(blocks_for(L.total(), 256)), dim3(256), 0, stream, L, fd->feature_dim,
+                     out_dim, gslots, dec, table);
+  return NTC_LAUNCH_STATUS();
+}
+
+int ntc_fused_train_step(const ntc_fused_args* a, hipStream_t stream)
+{
+  NTC_RETURN_IF(!a || !valid_feature_desc(&a->feat) || !valid_tiles(&a->tiles), NTC_E_INVALID);
+  NTC_RETURN_IF(!a->g0 || !a->g1 || !a->weights || !a->targets || !a->dw_partials || !a->sse_partials,
+                NTC_E_INVALID);
+  NTC_RETURN_IF(a->scatter && (!a->dg0 || !a->dg1), NTC_E_INVALID);
+  NTC_RETURN_IF(a->out_dim < 1 || a->out_dim > 16 || a->blocks < 1, NTC_E_INVALID);
+  const Entry* e = find_entry(a->feat, a->hidden_dim, a->activation);
+  NTC_RETURN_IF(!e, NTC_E_INVALID);
+  FusedParams P = {};
+  P.fd = a->feat;
+  P.tl = a->tiles;
+  P.nz = a->noise;
+  P.g0 = a->g0;
+  P.g1 = a->g1;
+  P.wtab = a->weights;
+  P.pkg = a->targets;
+  P.O = a->out_dim;
+  P.F = a->feat.feature_dim;
+  P.scale = a->loss_scale;
+  P.B = batch_of(a->tiles);
+  P.scatter = a->scatter;
+  P.dg0 = a->dg0;
+  P.dg1 = a->dg1;
+  P.dw_part = a->dw_partials;
+  P.sse_part = a->sse_partials;
+  P.dec_elems = a->hidden_dim * a->feat.feature_dim + a->hidden_dim * a->hidden_dim + 16 * a->hidden_dim;
+  const int64_t npass = (P.B + e->pass - 1) / e->pass;
+  const int blocks = (int)(npass < a->blocks ? npass : a->blocks);
+  NTC_RETURN_IF(blocks != a->blocks, NTC_E_INVALID);  // every partial slot must be written
+  hipLaunchKernelGGL(e->k, dim3(blocks), dim3(kThreads), 0, stream, P);
+  return NTC_LAUNCH_STATUS();
+}
+
+int ntc_fused_blocks_for(const ntc_feature_desc* fd, int32_t hidden_dim, int32_t activation, int64_t batch,
+                         int32_t max_blocks)
+{
+  if (!valid_feature_desc(fd)) return 0;
+  const Entry* e = find_entry(*fd, hidden_dim, activation);
+  if (!e) return 0;
+  const int64_t npass = (batch + e->pass - 1) / e->pass;
+  return (int)(npass < max_blocks ? npass : max_blocks);
+}
+
+int ntc_decoder_update(int32_t hidden_dim, int32_t feature_dim, int32_t out_dim, int32_t blocks,
+                       const float* dw_partials, float lr, float beta1, float beta2, float bias1, float bias2,
+                       float* grad, float* m, float* v, float* p, const double* sse_partials, double* sse_out,
+                       hipStream_t stream)
+{
+  NTC_RETURN_IF(hidden_dim < 1 || feature_dim < 1 || out_dim < 1 || out_dim > 16 || blocks < 1, NTC_E_INVALID);
+  NTC_RETURN_IF(!dw_partials || !grad || !m || !v || !p, NTC_E_INVALID);
+  const int H = hidden_dim;
+  const int n_dec = H * feature_dim + H * H + 16 * H;
+  DecAdam c = {lr, beta1, beta2, 1.0f - beta1, 1.0f - beta2, bias1, bias2};
+  hipLaunchKernelGGL(k_decoder_update, dim3(blocks_for(n_dec, 256)), dim3(256), 0, stream, n_dec,
+                     H * feature_dim + H * H, H, out_dim, blocks, dw_partials, c, grad, m, v, p, sse_partials,
+                     sse_out);
+  return NTC_LAUNCH_STATUS();
+}
+} // extern "C"

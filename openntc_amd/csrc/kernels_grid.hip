@@ -77,6 +77,46 @@ __global__ __launch_bounds__(kThreads) void k_grid_update(int64_t n4, AdamConst 
   }
 }
 
+// Both grids of a level in one launch: items [0, a4) belong to grid 0, the
+// rest to grid 1 (float4 items; both sizes are multiples of 4).
+struct GridSeg
+{
+  f4* g;
+  f4* m;
+  f4* v;
+  f4* p;
+  float lo, hi;
+};
+
+__global__ __launch_bounds__(kThreads) void k_grid_update2(int64_t a4, int64_t n4, AdamConst c, GridSeg s0,
+                                                           GridSeg s1)
+{
+  const f4 zero = {0.f, 0.f, 0.f, 0.f};
+  for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n4; i += (int64_t)gridDim.x * kThreads)
+  {
+    const bool second = i >= a4;
+    const GridSeg& s = second ? s1 : s0;
+    const int64_t k = second ? i - a4 : i;
+    const f4 gi = __builtin_nontemporal_load(&s.g[k]);
+    f4 mi = __builtin_nontemporal_load(&s.m[k]);
+    f4 vi = __builtin_nontemporal_load(&s.v[k]);
+    f4 pi = __builtin_nontemporal_load(&s.p[k]);
+#pragma unroll
+    for (int e = 0; e < 4; ++e)
+    {
+      float me = mi[e], ve = vi[e], pe = pi[e];
+      adam_one(c, gi[e], me, ve, pe);
+      mi[e] = me;
+      vi[e] = ve;
+      pi[e] = fmaxf(s.lo, fminf(s.hi, pe));
+    }
+    __builtin_nontemporal_store(mi, &s.m[k]);
+    __builtin_nontemporal_store(vi, &s.v[k]);
+    __builtin_nontemporal_store(pi, &s.p[k]);
+    __builtin_nontemporal_store(zero, &s.g[k]);
+  }
+}
+
 __global__ __launch_bounds__(kThreads) void k_grid_update_tail(int64_t beg, int64_t n, AdamConst c,
                                                                float lo, float hi, float* __restrict__ g,
                                                                float* __restrict__ m, float* __restrict__ v,
@@ -206,6 +246,25 @@ int ntc_grid_update_f32(int64_t n, float lr, float beta1, float beta2, float bia
   if (beg < n)
     hipLaunchKernelGGL(k_grid_update_tail, dim3(blocks_for(n - beg, kThreads)), dim3(kThreads), 0, stream,
                        beg, n, c, lo, hi, grad, m, v, p);
+  return NTC_LAUNCH_STATUS();
+}
+
+int ntc_grid_update2_f32(int64_t n0, float* grad0, float* m0, float* v0, float* p0, float lo0, float hi0,
+                         int64_t n1, float* grad1, float* m1, float* v1, float* p1, float lo1, float hi1, float lr,
+                         float beta1, float beta2, float bias1, float bias2, hipStream_t stream)
+{
+  NTC_RETURN_IF(n0 < 4 || n1 < 4 || (n0 & 3) || (n1 & 3), NTC_E_INVALID);
+  NTC_RETURN_IF(!grad0 || !m0 || !v0 || !p0 || !grad1 || !m1 || !v1 || !p1, NTC_E_INVALID);
+  const uintptr_t bits = (uintptr_t)grad0 | (uintptr_t)m0 | (uintptr_t)v0 | (uintptr_t)p0 | (uintptr_t)grad1 |
+                         (uintptr_t)m1 | (uintptr_t)v1 | (uintptr_t)p1;
+  NTC_RETURN_IF(bits & 15, NTC_E_INVALID);
+  const AdamConst c = make_adam(lr, beta1, beta2, bias1, bias2);
+  const GridSeg s0 = {(f4*)grad0, (f4*)m0, (f4*)v0, (f4*)p0, lo0, hi0};
+  const GridSeg s1 = {(f4*)grad1, (f4*)m1, (f4*)v1, (f4*)p1, lo1, hi1};
+  const int64_t n4 = (n0 + n1) / 4;
+  const int64_t want = (n4 + kThreads - 1) / kThreads;
+  const unsigned blocks = (unsigned)(want < 2048 ? want : 2048);
+  hipLaunchKernelGGL(k_grid_update2, dim3(blocks), dim3(kThreads), 0, stream, n0 / 4, n4, c, s0, s1);
   return NTC_LAUNCH_STATUS();
 }
 

@@ -1,0 +1,142 @@
+"""openntc::Context on the GPU against the CPU oracle's rendition of the
+reference step loop (libopenntc.cpp:471-879): same init, same schedule, same
+mip targets, steps within tolerance, export bit-exact given the trained FP32
+latents and weights."""
+import numpy as np
+import pytest
+
+import openntc_amd as ntc
+
+pytestmark = pytest.mark.gpu
+
+DIM = 256
+LAYOUT_ARGS = (4, 4, 8, 16, 2, 64)  # b0 b1 c0 c1 s H (config-1 layout at 256^2)
+
+
+def _make(oracle, steps, path, grids=1, feature_layout=1, activation=1, lr=(1e-2, 1e-3),
+          init=(-0.5, 0.5)):
+    tex = ntc.synthetic_pbr(DIM, 20260)
+    ctx = ntc.Context()
+    lay = ntc.LayoutInfo(*LAYOUT_ARGS, ntc.FeatureLayout(feature_layout), ntc.Activation(activation))
+    assert ctx.InitEx(layout=lay, min_dimension=64) == ntc.Result.Success
+    assert ctx.LoadTextures(tex, DIM) == ntc.Result.Success
+    assert ctx.BeginTrainingEx(steps, grids, lr[0], lr[1], init[0], init[1], path) == ntc.Result.Success
+    L = oracle.layout_custom(*LAYOUT_ARGS, feature_layout, activation, DIM)
+    mt = oracle.MT(123)
+    P = oracle.init_params(mt, L, *init)
+    sched = oracle.schedule(mt, L, steps, grids, *lr)
+    pk = oracle.build_package([(t[0], int(t[1]), t[2]) for t in tex], L)
+    return ctx, L, P, sched, pk
+
+
+def test_targets_init_and_schedule_match_oracle(oracle):
+    ctx, L, P, sched, pk = _make(oracle, 50, ntc.TrainPath.Plain, grids=4)
+    lay = ctx.layout()
+    assert lay.mip_count == L.mip_count and lay.level_count == L.level_count
+    assert lay.feature_dim == L.feature_dim
+    for m in range(L.mip_count):
+        assert np.array_equal(ctx.read_package(m), pk[m]), f"mip {m}"
+    for l in range(L.level_count):
+        assert np.array_equal(ctx.read_grid(0, l), P.g0[l])
+        assert np.array_equal(ctx.read_grid(1, l), P.g1[l])
+    assert np.array_equal(ctx.read_decoder(), P.dec)
+    for t in range(50):
+        a, b = ctx.step_info(t), sched[t]
+        assert (a.lod, a.level, a.tile_dim, a.n_tiles) == (b.lod, b.level, b.tile_dim, b.n_tiles)
+        assert list(a.origins) == list(b.origins)
+        assert (a.lr_grid, a.lr_decoder, a.bias1, a.bias2, a.grid_bias1, a.grid_bias2) == \
+               (b.lr_grid, b.lr_decoder, b.bias1, b.bias2, b.grid_bias1, b.grid_bias2)
+        assert (a.grid_update, a.lock) == (b.grid_update, b.lock)
+
+
+@pytest.mark.parametrize("feature_layout,activation", [(1, 1), (0, 0)])
+def test_plain_steps_track_oracle(oracle, feature_layout, activation):
+    steps = 6
+    ctx, L, P, sched, pk = _make(oracle, steps, ntc.TrainPath.Plain, grids=2,
+                                 feature_layout=feature_layout, activation=activation)
+    tr = oracle.Trainer(L, P, pk, seed=123, max_batch=2 * 65536)
+    for t in range(steps):
+        prog = ctx.Train(1)
+        assert prog.result == ntc.Result.Success
+        tr.step(sched[t], t)
+        dec = ctx.read_decoder()
+        # the atomic scatter order differs from the oracle's texel order, so
+        # the latents (and through them everything else) drift by ulps
+        assert np.max(np.abs(dec - P.dec)) < 1e-4, f"step {t}"
+        lvl = sched[t].level
+        g = ctx.read_grid(0, lvl)
+        assert np.max(np.abs(g - P.g0[lvl])) < 1e-4, f"step {t}"
+    if steps >= 1:
+        d = ctx.read_decoder()
+        assert np.max(np.abs(d - P.dec)) < 1e-4
+
+
+def test_plain_training_to_completion_and_export(oracle):
+    steps = 40
+    ctx, L, P, sched, pk = _make(oracle, steps, ntc.TrainPath.Plain, grids=1)
+    tr = oracle.Trainer(L, P, pk, seed=123, max_batch=65536)
+    for t in range(steps):
+        tr.step(sched[t], t)
+    prog = ctx.TrainUntilComplete()
+    assert prog.phase == ntc.TrainPhase.TrainComplete and prog.batches_complete == steps
+    ev = ctx.Eval()
+    ref_mse = tr.eval()
+    ref_psnr = -10 * np.log10(ref_mse)
+    assert abs(ev.psnr - ref_psnr) < 0.05, (ev.psnr, ref_psnr)
+
+    # export is bit-exact given the context's own trained FP32 latents/weights
+    g0b, g1b, decb, cal = ctx.GetCompressedData()
+    g0_words = np.concatenate([oracle.quantize_pack(L.g0_bits, ctx.read_grid(0, l)) for l in range(L.level_count)])
+    g1_words = np.concatenate([oracle.quantize_pack(L.g1_bits, ctx.read_grid(1, l)) for l in range(L.level_count)])
+    assert np.array_equal(g0b.view(np.uint32), g0_words)
+    assert np.array_equal(g1b.view(np.uint32), g1_words)
+    # grids are frozen on the quantisation lattice after the lock
+    for l in range(L.level_count):
+        g = ctx.read_grid(0, l)
+        assert np.array_equal(g / L.g0_delta, np.rint(g / L.g0_delta))
+    dec = ctx.read_decoder()
+    H, F, Fp, O = L.hidden_dim, L.feature_dim, L.feature_dim_padded, L.out_dim
+    w0 = np.zeros((H, Fp), np.float32)
+    w0[:, :F] = dec[: H * F].reshape(H, F)
+    w1 = dec[H * F: H * F + H * H].reshape(H, H)
+    wout = dec[H * F + H * H:].reshape(16, H)[:O]
+    q0, s0 = oracle.quantize_weights(w0, 1.0 / 128.0)
+    q1, s1 = oracle.quantize_weights(w1, cal[2])
+    q2, s2 = oracle.quantize_weights(wout, cal[3])
+    blob = np.zeros(H * Fp + H * H + 16 * H + 4 * (H + H + 16), np.uint8)
+    off = 0
+    for part in (q0, q1):
+        blob[off: off + part.nbytes] = part.view(np.uint8)
+        off += part.nbytes
+    blob[off: off + q2.nbytes] = q2.view(np.uint8)
+    off += 16 * H
+    for sc, n in ((s0, H), (s1, H), (s2, 16)):
+        full = np.zeros(n, np.float32)
+        full[: sc.size] = sc
+        blob[off: off + 4 * n] = full.view(np.uint8)
+        off += 4 * n
+    assert np.array_equal(decb, blob)
+    # calibration against the oracle's forward over the context's own weights
+    P2 = oracle.Params(L)
+    for l in range(L.level_count):
+        P2.g0[l][...] = ctx.read_grid(0, l)
+        P2.g1[l][...] = ctx.read_grid(1, l)
+    P2.dec[...] = dec
+    mx = oracle.Trainer(L, P2, pk, max_batch=65536).calibrate()
+    assert np.allclose(cal[:2], mx, rtol=1e-6)
+
+
+def test_state_machine_errors(oracle):
+    ctx = ntc.Context()
+    # Train before Init / manifest
+    prog = ctx.Train(1)
+    assert prog.phase == ntc.TrainPhase.TrainError and prog.result == ntc.Result.InvalidState
+    assert ctx.Init(ntc.Profile.Bpp_0_5) == ntc.Result.Success
+    assert ctx.Init(ntc.Profile.Bpp_0_5) == ntc.Result.InvalidState
+    tex = ntc.synthetic_pbr(256, 1)
+    # below kMinDimension for the reference API
+    assert ctx.LoadTextures(tex, 256) == ntc.Result.InvalidDimension
+    assert ctx.Train(1).result == ntc.Result.InvalidState
+    bad = [(tex[0][0], ntc.Semantic.Albedo, "RG")]
+    assert ctx.LoadTextures(bad, 1024) == ntc.Result.InvalidManifest
+    assert ctx.LoadManifest("/nonexistent/manifest.json") == ntc.Result.FileNotFound

@@ -1,0 +1,155 @@
+"""Fused BF16 training step (ntc_fused_train_step + ntc_decoder_update) against
+the FP64 CPU oracle of the same step: gather with noise, forward, loss,
+backward, latent scatter. BF16 MFMA operands with FP32 accumulation bound
+the agreement at about 1e-2 relative in norm; the tolerance is 3e-2."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import openntc_amd as ntc
+from openntc_amd import _lib as L
+
+pytestmark = pytest.mark.gpu
+
+from tests import gpu_util as G  # noqa: E402
+
+REL_TOL = 3e-2
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30))
+
+
+def _case(oracle, kind):
+    if kind == "c1_bilinear_relu":
+        return oracle.layout_custom(4, 4, 8, 16, 2, 64, 1, 1, 256), 1
+    if kind == "c3_bilinear_h128":
+        return oracle.layout_custom(4, 4, 16, 16, 2, 128, 1, 1, 256), 1
+    if kind == "c2_bilinear":
+        return oracle.layout_custom(4, 2, 12, 16, 2, 64, 1, 1, 256), 1
+    if kind == "ref_bpp_2_25":
+        return oracle.layout_profile(3, 256), 0
+    if kind == "ref_bpp_1_0":
+        return oracle.layout_profile(2, 256), 0
+    if kind == "ref_bpp_0_5":
+        return oracle.layout_profile(1, 256), 0
+    raise KeyError(kind)
+
+
+@pytest.mark.parametrize("kind", ["c1_bilinear_relu", "c3_bilinear_h128", "c2_bilinear", "ref_bpp_2_25",
+                                  "ref_bpp_1_0", "ref_bpp_0_5"])
+@pytest.mark.parametrize("lod,n_tiles", [(0, 2), (1, 1), (2, 3)])
+def test_fused_step_vs_fp64_oracle(oracle, kind, lod, n_tiles):
+    O = oracle
+    Lay, act = _case(O, kind)
+    Lay.activation = act
+    lib = G.lib()
+    rng = np.random.default_rng(hash((kind, lod)) % 2**32)
+    level = O.level_for_lod(Lay, lod)
+    fd = O.feature_desc(Lay, level)
+    M = Lay.mip_dim[lod]
+    T = min(M, 256)
+    H, F, Od = Lay.hidden_dim, Lay.feature_dim, 9
+    origins = rng.integers(0, M, 2 * n_tiles).astype(np.int32)
+    g0 = rng.uniform(-0.5, 0.5, (fd["g0_dim"], fd["g0_dim"], fd["c0"])).astype(np.float32)
+    g1 = rng.uniform(-0.5, 0.5, (fd["g1_dim"], fd["g1_dim"], fd["c1"])).astype(np.float32)
+    w0 = rng.uniform(-1, 1, (H, F)).astype(np.float32) / np.sqrt(F)
+    w1 = rng.uniform(-1, 1, (H, H)).astype(np.float32) / np.sqrt(H)
+    wout = rng.uniform(-1, 1, (16, H)).astype(np.float32) / np.sqrt(H)
+    pkg = rng.integers(0, 256, (M, M, Od)).astype(np.uint8)
+    norm_lod = np.float32(lod / (Lay.mip_count - 1))
+    step, seed = 5, 123
+
+    # FP64 oracle of the step
+    x = O.draw_features(fd, origins, T, M, norm_lod, g0.astype(np.float64), g1.astype(np.float64), seed=seed,
+                        step=step, level=level, noise_on=True, d0=Lay.g0_delta, d1=Lay.g1_delta, dtype=np.float64)
+    yt = O.draw_targets(origins, T, M, Od, pkg, np.float64)
+    sse, dw0, dw1, dwout, dx = O.mlp_fwd_bwd(F, H, Od, act, x, yt, w0.astype(np.float64),
+                                             w1.astype(np.float64), wout.astype(np.float64))
+    rdg0, rdg1 = O.scatter(fd, origins, T, M, dx, g0.shape, g1.shape)
+
+    # fused step on the GPU
+    fdc = G.feature_desc(fd)
+    assert lib.ntc_fused_supported(G.byref(fdc), H, act) == 1
+    dec = np.concatenate([w0.ravel(), w1.ravel(), wout.ravel()])
+    d_dec = G.dev(dec)
+    tab = G.empty((lib.ntc_fused_table_elems(G.byref(fdc), H),), np.uint16)
+    G.ok(lib.ntc_pack_decoder_bf16(G.byref(fdc), H, Od, G.p(d_dec), G.p(tab), None))
+    maxb = C.c_int32()
+    G.ok(lib.ntc_fused_max_blocks(C.byref(maxb)))
+    B = n_tiles * T * T
+    blocks = lib.ntc_fused_blocks_for(G.byref(fdc), H, act, B, maxb.value)
+    n_dec = dec.size
+    part = G.empty((blocks * n_dec,), np.float32)
+    ssep = G.empty((blocks,), np.float64)
+    dg0 = G.empty(g0.shape, np.float32)
+    dg1 = G.empty(g1.shape, np.float32)
+    og = G.dev(origins)
+    d_g0, d_g1, d_pkg = G.dev(g0), G.dev(g1), G.dev(pkg)
+    a = L.FusedArgs()
+    a.feat = fdc
+    a.tiles = G.tiles(og, n_tiles, T, M, float(norm_lod))
+    a.noise = L.Noise(seed, step, level, 1, Lay.g0_delta, Lay.g1_delta)
+    a.hidden_dim, a.out_dim, a.activation = H, Od, act
+    a.g0, a.g1, a.weights, a.targets = G.p(d_g0), G.p(d_g1), G.p(tab), G.p(d_pkg)
+    a.loss_scale = float(np.float32(-2.0) / (np.float32(Od) * np.float32(B)))
+    a.scatter = 1
+    a.dg0, a.dg1, a.dw_partials, a.sse_partials, a.blocks = G.p(dg0), G.p(dg1), G.p(part), G.p(ssep), blocks
+    G.ok(lib.ntc_fused_train_step(G.byref(a), None), "ntc_fused_train_step")
+    grad = G.empty((n_dec,), np.float32)
+    m = G.empty((n_dec,), np.float32)
+    v = G.empty((n_dec,), np.float32)
+    p = G.dev(dec)
+    sse_out = G.empty((1,), np.float64)
+    G.ok(lib.ntc_decoder_update(H, F, Od, blocks, G.p(part), 0.0, 0.9, 0.999, 1.0, 1.0, G.p(grad), G.p(m), G.p(v),
+                                G.p(p), G.p(ssep), G.p(sse_out), None))
+    g = G.host(grad)
+    gw0, gw1, gwo = g[: H * F].reshape(H, F), g[H * F: H * F + H * H].reshape(H, H), g[H * F + H * H:].reshape(16, H)
+    assert abs(G.host(sse_out)[0] - sse) <= 1e-2 * sse
+    assert rel(gw0, dw0) < REL_TOL, rel(gw0, dw0)
+    assert rel(gw1, dw1) < REL_TOL, rel(gw1, dw1)
+    assert rel(gwo[:Od], dwout) < REL_TOL, rel(gwo[:Od], dwout)
+    assert not gwo[Od:].any()
+    assert rel(G.host(dg0), rdg0) < REL_TOL, rel(G.host(dg0), rdg0)
+    assert rel(G.host(dg1), rdg1) < REL_TOL, rel(G.host(dg1), rdg1)
+    # lr = 0 leaves the weights untouched
+    assert np.array_equal(G.host(p), dec)
+
+
+def test_grid_update2_bitexact(oracle):
+    rng = np.random.default_rng(3)
+    n0, n1 = 4 * 1001, 4 * 333
+    arrs = []
+    for n in (n0, n1):
+        arrs.append([(rng.standard_normal(n) * 1e-3).astype(np.float32),
+                     (rng.standard_normal(n) * 1e-3).astype(np.float32),
+                     rng.uniform(0, 1e-6, n).astype(np.float32),
+                     rng.uniform(-1.1, 1.1, n).astype(np.float32)])
+    d = [[G.dev(x) for x in a] for a in arrs]
+    lr, c1, c2 = 0.007, 2.5, 40.0
+    G.ok(G.lib().ntc_grid_update2_f32(n0, *[G.p(x) for x in d[0]], -1.0, 1.0 - 0.125,
+                                      n1, *[G.p(x) for x in d[1]], -1.0, 1.0 - 0.5, lr, 0.9, 0.999, c1, c2, None))
+    for (g, m, v, p), dd, delta in ((arrs[0], d[0], 0.125), (arrs[1], d[1], 0.5)):
+        oracle.adam(lr, 0.9, 0.999, c1, c2, g, m, v, p)
+        oracle.clamp_grid(delta, p)
+        assert np.array_equal(G.host(dd[3]), p) and np.array_equal(G.host(dd[1]), m)
+        assert np.array_equal(G.host(dd[2]), v) and not G.host(dd[0]).any()
+
+
+def test_fused_context_training_tracks_plain(oracle):
+    """The fused BF16 path and the plain FP32 path trained on the same seed
+    reach the same quality (config-1 layout at 256^2)."""
+    tex = ntc.synthetic_pbr(256, 20260)
+    psnr = {}
+    for path in (ntc.TrainPath.Plain, ntc.TrainPath.Fused):
+        ctx = ntc.Context()
+        lay = ntc.LayoutInfo(4, 4, 8, 16, 2, 64, ntc.FeatureLayout.Bilinear, ntc.Activation.ReLU)
+        assert ctx.InitEx(layout=lay, min_dimension=64) == ntc.Result.Success
+        assert ctx.LoadTextures(tex, 256) == ntc.Result.Success
+        assert ctx.BeginTrainingEx(300, 1, 1e-2, 1e-3, -0.5, 0.5, path) == ntc.Result.Success
+        prog = ctx.TrainUntilComplete()
+        assert prog.phase == ntc.TrainPhase.TrainComplete, prog
+        psnr[path] = ctx.Eval().psnr
+    assert psnr[ntc.TrainPath.Fused] > 15.0
+    assert abs(psnr[ntc.TrainPath.Fused] - psnr[ntc.TrainPath.Plain]) < 0.5, psnr

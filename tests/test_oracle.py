@@ -1,0 +1,418 @@
+"""Pins of the CPU oracle against known answers, closed forms, library
+routines and the reference's second statement of the formats (the HLSL
+decoder). The reference ships no tests or golden vectors; each pin here is
+chosen so that a plausible restatement mistake (dropped term, wrong index or
+sign, swapped operand) fails it."""
+import math
+
+import numpy as np
+import pytest
+
+
+# ---- RNG ------------------------------------------------------------------------
+def test_mt19937_known_answer(oracle):
+    # ISO C++ [rand.predef]: the 10000th output of a default-seeded
+    # std::mt19937 (seed 5489) is 4123659995.
+    m = oracle.MT(5489)
+    for _ in range(9999):
+        m.next()
+    assert m.next() == 4123659995
+
+
+@pytest.mark.parametrize("ctr,key,expect", [
+    ([0, 0, 0, 0], [0, 0], [0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8]),
+    ([0xFFFFFFFF] * 4, [0xFFFFFFFF] * 2, [0x408F276D, 0x41C83B0E, 0xA20BC7C6, 0x6D5451FD]),
+    ([0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344], [0xA4093822, 0x299F31D0],
+     [0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1]),
+])
+def test_philox_known_answers(oracle, ctr, key, expect):
+    # Random123 known-answer vectors for Philox4x32-10
+    assert oracle.philox(ctr, key) == expect
+
+
+def test_noise_range_and_moments(oracle):
+    delta = 0.125
+    vals = np.array([oracle.noise(123, 5, g, 1, cell, ch, delta)
+                     for g in (0, 1) for cell in range(0, 4000, 7) for ch in range(12)], np.float64)
+    assert vals.min() > -delta / 2 and vals.max() <= delta / 2
+    assert abs(vals.mean()) < 0.01 * delta
+    assert abs(vals.var() - delta ** 2 / 12) < 0.05 * delta ** 2 / 12
+    # values are k * 2^-24 * delta - delta/2: on the 24-bit lattice
+    k = (vals + delta / 2) / delta * 2 ** 24
+    assert np.array_equal(k, np.rint(k))
+    # consistent within a step, different across steps
+    assert oracle.noise(1, 2, 0, 0, 10, 3, delta) == oracle.noise(1, 2, 0, 0, 10, 3, delta)
+    assert oracle.noise(1, 2, 0, 0, 10, 3, delta) != oracle.noise(1, 3, 0, 0, 10, 3, delta)
+
+
+def test_uniform_int_unbiased_and_in_range(oracle):
+    m = oracle.MT(7)
+    draws = np.array([m.uniform_int(0, 10) for _ in range(55000)])
+    assert draws.min() == 0 and draws.max() == 10
+    counts = np.bincount(draws, minlength=11)
+    assert np.all(np.abs(counts - 5000) < 5 * math.sqrt(5000))
+
+
+def test_schedule_distribution(oracle):
+    L = oracle.layout_profile(3, 4096)
+    m = oracle.MT(123)
+    n = 40000
+    steps = oracle.schedule(m, L, n, 2)
+    lods = np.array([s.lod for s in steps])
+    mips = L.mip_count
+    for k in range(mips - 1):
+        p = 0.95 * 0.75 * 4.0 ** -k + 0.05 / mips
+        assert abs((lods == k).mean() - p) < 5 * math.sqrt(p * (1 - p) / n) + 1e-4, k
+    # tile origins uniform over the sampled mip
+    s0 = [s for s in steps if s.lod == 0]
+    xs = np.array([s.origins[0] for s in s0])
+    assert xs.min() >= 0 and xs.max() < 4096 and abs(xs.mean() - 2047.5) < 60
+    # lock at floor(95 n / 100); grid updates stop there
+    lock = (95 * n) // 100
+    assert steps[lock].lock == 1 and sum(s.lock for s in steps) == 1
+    assert all(s.grid_update == (i < lock) for i, s in enumerate(steps[lock - 2: lock + 2], lock - 2))
+    assert all(s.level == oracle.level_for_lod(L, s.lod) for s in steps[:100])
+
+
+def test_cosine_annealing_and_bias(oracle):
+    L = oracle.layout_profile(1, 1024)
+    steps = oracle.schedule(oracle.MT(1), L, 1000, 1, 0.01, 0.005)
+    assert steps[0].lr_grid == np.float32(0.01) and steps[0].lr_decoder == np.float32(0.005)
+    assert abs(steps[500].lr_grid - 0.005) < 1e-7
+    assert steps[0].bias1 == np.float32(1 / np.float32(1 - np.float32(0.9)))
+    assert abs(steps[9].bias2 - 1 / (1 - 0.999 ** 10)) < 1e-4 * 100.45  # float32 cancellation
+
+
+# ---- layout ----------------------------------------------------------------------
+@pytest.mark.parametrize("profile,size,F", [(0, 3924472, 60), (1, 9518920, 84), (2, 19024976, 72),
+                                            (3, 42514080, 92)])
+def test_readme_sizes(oracle, profile, size, F):
+    # README.md:48 compressed sizes at 4096^2 (3.93 / 9.52 / 19.0 / 42.5 MB)
+    L = oracle.layout_profile(profile, 4096)
+    assert oracle.compressed_bytes(L) == size
+    assert L.feature_dim == F and L.mip_count == 11 and L.level_count == 5
+    # HLSL FEATURE_DIM_PADDED (ntc_decode.hlsli:93)
+    assert L.feature_dim_padded == ((4 * L.g0_channels + L.g1_channels + 13 + 15) // 16) * 16
+
+
+def test_uncompressed_set_size(oracle):
+    # README.md:44: 9 channels at 4096^2 with mips down to 4x4 = 201 326 544 B
+    L = oracle.layout_profile(3, 4096)
+    assert sum(L.mip_dim[i] ** 2 * 9 for i in range(L.mip_count)) == 201326544
+
+
+@pytest.mark.parametrize("profile,level0_mips", [(0, 4), (1, 4), (2, 3), (3, 3)])
+def test_level_for_lod_matches_hlsl(oracle, profile, level0_mips):
+    # ntc_decode.hlsli:127-132 (no clamp there; the reference host clamps)
+    L = oracle.layout_profile(profile, 4096)
+    for lod in range(L.mip_count):
+        hl = 0 if lod < level0_mips else 1 + (lod - level0_mips) // 2
+        assert oracle.level_for_lod(L, lod) == min(hl, L.level_count - 1)
+
+
+# ---- gather ----------------------------------------------------------------------
+def _fd(oracle, bilinear=False, dim=64):
+    if bilinear:
+        L = oracle.layout_custom(4, 4, 4, 4, 2, 16, 1, 1, dim)
+    else:
+        L = oracle.layout_custom(4, 4, 4, 4, 2, 16, 0, 0, dim)
+    return L, oracle.feature_desc(L, 0)
+
+
+def test_gather_closed_form_cases(oracle):
+    L, fd = _fd(oracle)
+    M, T = 64, 64
+    D0, D1 = fd["g0_dim"], fd["g1_dim"]
+    rng = np.random.default_rng(0)
+    g0 = rng.uniform(-1, 1, (D0, D0, 4)).astype(np.float32)
+    g1 = rng.uniform(-1, 1, (D1, D1, 4)).astype(np.float32)
+    x = oracle.draw_features(fd, np.array([0, 0], np.int32), T, M, 0.25, g0, g1)
+    # texel (0,0): g = 0.5/64*32 - 0.5 = -0.25 -> x0 = -1, both x corners clamp to 0
+    t = 0
+    for c in range(4):
+        assert np.array_equal(x[c * 4: c * 4 + 4, t], g0[0, 0])
+    # PE: P=8, phase 0 at coordinate 0 -> 1 - 4|0.0625 - 0.5| = -0.75
+    base = 16 + 4
+    assert x[base, t] == np.float32(-0.75)
+    # LOD slot then zero padding
+    assert x[base + 12, t] == np.float32(0.25)
+    assert np.all(x[base + 13:, t] == 0)
+    # interior texel (5, 9): g0 x = 5.5/2 - 0.5 = 2.25 -> corners 2,3; y = 9.5/2-.5 = 4.25 -> 4,5
+    t = 9 * 64 + 5
+    assert np.array_equal(x[0:4, t], g0[4, 2]) and np.array_equal(x[4:8, t], g0[4, 3])
+    assert np.array_equal(x[8:12, t], g0[5, 2]) and np.array_equal(x[12:16, t], g0[5, 3])
+    # g1 (D1 = 16): gx = 5.5/4 - 0.5 = 0.875, gy = 9.5/4 - 0.5 = 1.875 -> bilinear
+    fx, fy = 0.875, 0.875
+    want = ((1 - fx) * (1 - fy) * g1[1, 0] + fx * (1 - fy) * g1[1, 1] + (1 - fx) * fy * g1[2, 0]
+            + fx * fy * g1[2, 1])
+    assert np.allclose(x[16:20, t], want, atol=1e-6)
+
+
+def test_gather_matches_hlsl_formulas(oracle):
+    # Recompute features the way the HLSL decoder does (ntc_decode.hlsli:
+    # 218-224 corners via floor(uv*dim - 0.5) with max/min clamps, 259-267
+    # bilinear g1 via frac(), 315-341 PE via floor(uv * (dim >> lod))) on
+    # float grids and compare with the oracle's training gather.
+    L, fd = _fd(oracle, dim=128)
+    lod, M = 1, 64
+    D0, D1 = fd["g0_dim"], fd["g1_dim"]
+    rng = np.random.default_rng(4)
+    g0 = rng.uniform(-1, 1, (D0, D0, 4)).astype(np.float64)
+    g1 = rng.uniform(-1, 1, (D1, D1, 4)).astype(np.float64)
+    norm_lod = lod / (L.mip_count - 1)
+    x = oracle.draw_features(fd, np.array([13, 40], np.int32), 64, M, norm_lod, g0, g1, dtype=np.float64)
+    for t in (0, 63, 64 * 30 + 17, 64 * 63 + 63):
+        px, py = (t % 64 + 13) % M, (t // 64 + 40) % M
+        uv = np.array([(px + 0.5) / M, (py + 0.5) / M])
+        feats = []
+        xy = np.floor(uv * D0 - 0.5).astype(int)
+        xs = [max(xy[0], 0), min(xy[0] + 1, D0 - 1)]
+        ys = [max(xy[1], 0), min(xy[1] + 1, D0 - 1)]
+        for i in range(2):
+            for j in range(2):
+                feats.extend(g0[ys[i], xs[j]])
+        xy1 = np.floor(uv * D1 - 0.5).astype(int)
+        xs1 = [max(xy1[0], 0), min(xy1[0] + 1, D1 - 1)]
+        ys1 = [max(xy1[1], 0), min(xy1[1] + 1, D1 - 1)]
+        fr = uv * D1 - 0.5 - np.floor(uv * D1 - 0.5)
+        mult = [(1 - fr[0]) * (1 - fr[1]), fr[0] * (1 - fr[1]), (1 - fr[0]) * fr[1], fr[0] * fr[1]]
+        blend = np.zeros(4)
+        for i in range(2):
+            for j in range(2):
+                blend += mult[i * 2 + j] * g1[ys1[i], xs1[j]]
+        feats.extend(blend)
+        cpos = uv * (L.dim >> lod)
+        for P in (8, 4, 2):
+            for a in range(2):
+                for h in range(2):
+                    c = math.floor(cpos[a])
+                    tt = (c + 0.5 + (0 if h == 0 else 0.25 * P)) / P
+                    feats.append(1 - 4 * abs(tt - math.floor(tt) - 0.5))
+        feats.append(norm_lod)
+        assert np.allclose(x[: len(feats), t], feats, atol=1e-12), t
+
+
+def test_gather_scatter_adjoint(oracle):
+    # <dX, J dG> == <J^T dX, dG>: the scatter is the exact adjoint of the
+    # grid part of the gather (FP64, no noise), for both g0 modes and with
+    # wrapping tiles on a coarser mip.
+    for bilinear in (False, True):
+        L, fd = _fd(oracle, bilinear, dim=128)
+        rng = np.random.default_rng(1 + bilinear)
+        M, T = 32, 32
+        origins = np.array([20, 7, 31, 31], np.int32)
+        D0, D1 = fd["g0_dim"], fd["g1_dim"]
+        dg0 = rng.standard_normal((D0, D0, 4))
+        dg1 = rng.standard_normal((D1, D1, 4))
+        zero0, zero1 = np.zeros_like(dg0), np.zeros_like(dg1)
+        jx = (oracle.draw_features(fd, origins, T, M, 0.0, dg0, dg1, dtype=np.float64)
+              - oracle.draw_features(fd, origins, T, M, 0.0, zero0, zero1, dtype=np.float64))
+        dx = rng.standard_normal(jx.shape)
+        s0, s1 = oracle.scatter(fd, origins, T, M, dx, dg0.shape, dg1.shape)
+        lhs = float(np.sum(dx * jx))
+        rhs = float(np.sum(s0 * dg0) + np.sum(s1 * dg1))
+        assert abs(lhs - rhs) <= 1e-10 * abs(lhs), (lhs, rhs)
+
+
+# ---- GEMM / MLP --------------------------------------------------------------------
+@pytest.mark.parametrize("op", [0, 1, 2])
+def test_gemm_matches_numpy(oracle, op):
+    rng = np.random.default_rng(op)
+    n, m, k = 7, 33, 2500
+    a = rng.standard_normal((n, k) if op != 1 else (k, n))
+    b = rng.standard_normal((k, m) if op != 2 else (m, k))
+    c = oracle.gemm(op, n, m, k, a, b, np.float64)
+    want = (a if op != 1 else a.T) @ (b if op != 2 else b.T)
+    assert np.allclose(c, want, rtol=1e-12, atol=1e-11)
+
+
+def _mlp_setup(rng, F=12, H=16, O=3, B=40):
+    x = rng.standard_normal((F, B))
+    yt = rng.uniform(0, 1, (O, B))
+    w0 = rng.standard_normal((H, F)) * 0.5
+    w1 = rng.standard_normal((H, H)) * 0.4
+    wout = rng.standard_normal((16, H)) * 0.4
+    return x, yt, w0, w1, wout
+
+
+@pytest.mark.parametrize("act", [0, 1])
+def test_mlp_gradients_vs_finite_differences(oracle, act):
+    # FP64 central differences of MSE = SSE/(O*B) against the hand-written
+    # backward (A.4-A.6). Entries whose pre-activations sit within the step of
+    # an activation kink are skipped by construction (random data, tiny h).
+    rng = np.random.default_rng(10 + act)
+    F, H, O, B = 12, 16, 3, 40
+    x, yt, w0, w1, wout = _mlp_setup(rng, F, H, O, B)
+
+    def mse(w0_, w1_, wout_, x_):
+        ws = np.zeros(4 * H * B + O * B)
+        sse = oracle.lib().o_mlp_forward_f64(F, H, O, act, B, oracle.ptr(np.ascontiguousarray(x_)),
+                                            oracle.ptr(yt), oracle.ptr(np.ascontiguousarray(w0_)),
+                                            oracle.ptr(np.ascontiguousarray(w1_)),
+                                            oracle.ptr(np.ascontiguousarray(wout_)), oracle.ptr(ws), None, None)
+        return sse / (O * B)
+
+    _, dw0, dw1, dwout, dx = oracle.mlp_fwd_bwd(F, H, O, act, x, yt, w0, w1, wout)
+    h = 1e-6
+    for (arr, grad, name) in ((w0, dw0, "w0"), (w1, dw1, "w1"), (wout[:O], dwout, "wout"), (x, dx, "x")):
+        for _ in range(12):
+            idx = tuple(rng.integers(0, s) for s in grad.shape)
+            args = [w0.copy(), w1.copy(), wout.copy(), x.copy()]
+            which = {"w0": 0, "w1": 1, "wout": 2, "x": 3}[name]
+            args[which][idx] += h
+            fp = mse(*args)
+            args[which][idx] -= 2 * h
+            fm = mse(*args)
+            fd = (fp - fm) / (2 * h)
+            assert abs(fd - grad[idx]) <= 1e-6 * max(1.0, abs(fd)) + 1e-9, (name, idx, fd, grad[idx])
+
+
+def test_end_to_end_latent_gradient_vs_finite_differences(oracle):
+    # Gather -> MLP -> loss -> backward -> scatter, FP64, against central
+    # differences of the MSE with respect to individual latent values.
+    L, fd = _fd(oracle, dim=32)
+    rng = np.random.default_rng(21)
+    M, T = 32, 8
+    origins = np.array([3, 29], np.int32)
+    F, H, O = fd["F"], 16, 3
+    D0, D1 = fd["g0_dim"], fd["g1_dim"]
+    g0 = rng.uniform(-1, 1, (D0, D0, 4))
+    g1 = rng.uniform(-1, 1, (D1, D1, 4))
+    yt = rng.uniform(0, 1, (O, T * T))
+    w0 = rng.standard_normal((H, F)) * 0.3
+    w1 = rng.standard_normal((H, H)) * 0.3
+    wout = rng.standard_normal((16, H)) * 0.3
+
+    def loss(g0_, g1_):
+        x = oracle.draw_features(fd, origins, T, M, 0.5, g0_, g1_, dtype=np.float64)
+        sse, *_ = oracle.mlp_fwd_bwd(F, H, O, 0, x, yt, w0, w1, wout)
+        return sse / (O * T * T)
+
+    x = oracle.draw_features(fd, origins, T, M, 0.5, g0, g1, dtype=np.float64)
+    _, _, _, _, dx = oracle.mlp_fwd_bwd(F, H, O, 0, x, yt, w0, w1, wout)
+    s0, s1 = oracle.scatter(fd, origins, T, M, dx, g0.shape, g1.shape)
+    h = 1e-6
+    touched0 = np.argwhere(np.abs(s0) > 0)
+    touched1 = np.argwhere(np.abs(s1) > 0)
+    for grid, grad, cells in ((0, s0, touched0), (1, s1, touched1)):
+        for idx in cells[rng.choice(len(cells), 8, replace=False)]:
+            idx = tuple(idx)
+            a0, a1 = g0.copy(), g1.copy()
+            (a0 if grid == 0 else a1)[idx] += h
+            fp = loss(a0, a1)
+            (a0 if grid == 0 else a1)[idx] -= 2 * h
+            fm = loss(a0, a1)
+            fdv = (fp - fm) / (2 * h)
+            assert abs(fdv - grad[idx]) <= 1e-6 * abs(fdv) + 1e-10, (grid, idx, fdv, grad[idx])
+
+
+def test_hardgelu_closed_form(oracle):
+    z = np.array([-2.0, -1.5, -1.0, 0.0, 0.75, 1.5, 3.0], np.float32)
+    a = oracle.act_forward(0, z)
+    want = np.where(z < -1.5, 0, np.where(z < 1.5, z / 3 * (z + 1.5), z))
+    assert np.allclose(a, want, rtol=1e-7)
+    g = np.ones_like(z)
+    d = oracle.act_backward(0, z, g)
+    # derivative jumps at +-1.5: -0.5 -> 0 at -1.5 and 1.5 -> 1 at +1.5
+    assert np.allclose(d, [0, -0.5, -1 / 6, 0.5, 1.0, 1.0, 1.0], rtol=1e-6)
+
+
+# ---- optimiser / quantisation ------------------------------------------------------------
+def test_adam_first_step_is_signed_lr(oracle):
+    g = np.array([3e-3, -2e-5, 0.7, -4.0], np.float32)
+    m = np.zeros(4, np.float32)
+    v = np.zeros(4, np.float32)
+    p = np.zeros(4, np.float32)
+    b1 = np.float32(1) / (np.float32(1) - np.float32(0.9))
+    b2 = np.float32(1) / (np.float32(1) - np.float32(0.999))
+    oracle.adam(0.01, 0.9, 0.999, b1, b2, g, m, v, p)
+    # -lr * g / (|g| + 1e-8): the epsilon shows at |g| = 2e-5 (ratio 0.9995)
+    assert np.allclose(p, -0.01 * g / (np.abs(g) + 1e-8), rtol=1e-5)
+    assert np.allclose(p, -0.01 * np.sign(g), rtol=1e-3)
+    # 1 - 0.999f = 0.00099998713 in float32
+    assert np.allclose(m, 0.1 * g, rtol=1e-6) and np.allclose(v, 0.001 * g * g, rtol=2e-5)
+
+
+@pytest.mark.parametrize("bits", [1, 2, 4, 8])
+def test_pack_exhaustive_and_hlsl_unpack(oracle, bits):
+    # Every code once, packed LSB-first; unpacked the way the HLSL decoder
+    # reads g0 ((v << (8-b)) ^ 0x80 as int8, ntc_decode.hlsli:251-253) and g1
+    # ((v - 2^(b-1)) / 2^(b-1), :298-301).
+    half = 2 ** (bits - 1)
+    delta = 1.0 / half
+    codes = np.arange(-half, half)
+    per = 32 // bits
+    vals = np.resize(codes, max(per, codes.size)) * delta
+    vals = np.concatenate([vals, np.zeros((-vals.size) % per)]).astype(np.float32)
+    words = oracle.quantize_pack(bits, vals)
+    for i, val in enumerate(vals):
+        w, s = divmod(i, per)
+        v = (int(words[w]) >> (s * bits)) & ((1 << bits) - 1)
+        assert v == int(round(val * half)) + half
+        g0 = ((v << (8 - bits)) ^ 0x80) & 0xFF
+        g0 = g0 - 256 if g0 >= 128 else g0
+        assert g0 == int(round(val * 128))
+        assert (v - half) / half == val
+
+
+def test_quantize_grid_rounds_half_even_and_clamps(oracle):
+    delta = np.float32(0.25)  # 3 bits
+    x = np.array([0.125, 0.375, -0.125, 2.0, -2.0, 0.8], np.float32)
+    oracle.quantize_grid(3, delta, x)
+    assert np.array_equal(x, np.array([0.0, 0.5, -0.0, 0.75, -1.0, 0.75], np.float32))
+
+
+def test_quantize_weights_closed_form(oracle):
+    w = np.array([[0.5, -1.0, 0.25, 0.0], [0, 0, 0, 0], [127 / 254, 1.0, -0.5, 0.125]], np.float32)
+    q, s = oracle.quantize_weights(w, 2.0)
+    b = q.view(np.uint8).reshape(3, 4).view(np.int8)
+    assert list(b[0]) == [64, -127, 32, 0]  # 63.5 -> 64 (lroundf half away)
+    assert list(b[1]) == [0, 0, 0, 0] and s[1] == 2.0
+    assert s[0] == np.float32(np.float32(1.0) / np.float32(127.0)) * np.float32(2.0)
+    assert b[2][1] == 127
+
+
+# ---- Lanczos -------------------------------------------------------------------------------
+def test_lanczos_weights_symmetric_and_reference_form(oracle):
+    w = oracle.lanczos_weights()
+    assert np.array_equal(w, w[::-1])
+    # the reference evaluates them in float (ntc_kernel.cu:569-579)
+    pi = np.float32(math.pi)
+    for i in range(-6, 6):
+        x = np.float32((0.5 + i) * 0.5)
+        a = np.float32(pi * x)
+        b = np.float32(pi * x / np.float32(3))
+        ref = np.float32(np.sin(a) / a) * np.float32(np.sin(b) / b)
+        assert abs(w[i + 6] - ref) <= 4 * np.spacing(abs(ref)) + 1e-7
+
+
+@pytest.mark.parametrize("value", [0, 1, 128, 200, 255])
+def test_lanczos_constant_image_within_one_lsb(oracle, value):
+    img = np.full((32, 32, 3), value, np.uint8)
+    out = oracle.filter_lanczos(img).astype(int)
+    assert np.all(np.abs(out - value) <= 1)
+
+
+def test_lanczos_matches_separable_float64_filter(oracle):
+    rng = np.random.default_rng(2)
+    img = rng.integers(0, 256, (16, 16, 2)).astype(np.uint8)
+    w = oracle.lanczos_weights().astype(np.float64)
+    dim = 16
+
+    def down(a, axis):
+        a = np.moveaxis(a, axis, 0)
+        out = np.zeros((dim // 2,) + a.shape[1:])
+        for t in range(dim // 2):
+            tot, tw = 0.0, 0.0
+            for i in range(-6, 6):
+                s = 2 * t + 1 + i
+                if 0 <= s < dim:
+                    tot = tot + a[s] * w[i + 6]
+                    tw += w[i + 6]
+            out[t] = tot / tw
+        return np.moveaxis(out, 0, axis)
+
+    ref = down(down(img / 255.0, 1), 0) * 255
+    got = oracle.filter_lanczos(img).astype(np.float64)
+    # the oracle truncates like the reference: result in (ref - 1 - eps, ref + eps]
+    assert np.all(got <= np.clip(ref, 0, 255) + 1e-3) and np.all(got > np.clip(ref, 0, 255) - 1 - 1e-3)
