@@ -71,8 +71,10 @@ __device__ __forceinline__ TexelPos texel_of(int64_t t, const int32_t* origins, 
   const int TT = T * T;
   p.b = (int)(t / TT);
   p.xy = (int)(t - (int64_t)p.b * TT);
-  p.x = (p.xy % T + origins[2 * p.b]) % M;
-  p.y = (p.xy / T + origins[2 * p.b + 1]) % M;
+  // unsigned arithmetic keeps the position inside [0, M) even for an origin
+  // outside [0, M); for valid origins it equals the signed form
+  p.x = (int)(((uint32_t)(p.xy % T) + (uint32_t)origins[2 * p.b]) % (uint32_t)M);
+  p.y = (int)(((uint32_t)(p.xy / T) + (uint32_t)origins[2 * p.b + 1]) % (uint32_t)M);
   return p;
 }
 

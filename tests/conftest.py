@@ -12,6 +12,14 @@ def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs a gfx950 GPU (runs through the C ABI)")
 
 
+@pytest.fixture(autouse=True)
+def _release_device_buffers():
+    yield
+    from tests import gpu_util
+
+    gpu_util.release()
+
+
 @pytest.fixture(scope="session")
 def oracle():
     from oracle import oracle as O

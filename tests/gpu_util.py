@@ -10,8 +10,21 @@ import openntc_amd
 from openntc_amd import _lib as L
 
 
+# Every buffer made here stays alive until release() (called after each test
+# by the autouse fixture in conftest.py, after a device synchronisation), so a
+# call like f(p(dev(x))) never hands a kernel a pointer that Python has
+# already freed.
+_live: list = []
+
+
+def release():
+    if _live:
+        lib().ntc_dev_synchronize()
+        _live.clear()
+
+
 class DevArray:
-    """A device buffer with a numpy shape/dtype, freed on garbage collection."""
+    """A device buffer with a numpy shape/dtype, freed by release()."""
 
     def __init__(self, shape, dtype):
         self.shape = tuple(shape) if isinstance(shape, (tuple, list)) else (shape,)
@@ -20,6 +33,7 @@ class DevArray:
         p = C.c_void_p()
         ok(lib().ntc_dev_alloc(self.nbytes, C.byref(p)), "ntc_dev_alloc")
         self.ptr = p.value
+        _live.append(self)
 
     def __del__(self):
         if getattr(self, "ptr", None):
