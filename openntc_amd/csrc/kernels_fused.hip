@@ -137,10 +137,121 @@ struct Cfg
   static constexpr int STAGE_B = 2 * H * 64 + 16 * 64;
   static constexpr int UNION = (PASS * FP > STAGE_B) ? PASS * FP : STAGE_B;
   static constexpr int LDS = WT + XFN + UNION;
+  // latent-gradient aggregation boxes (floats): a pass is a 16 x BH texel
+  // block; at texel:cell ratios >= 1 (g0) and >= 2 (g1) its footprint fits
+  static constexpr int BH = PASS / 16;
+  static constexpr int W0MAX = 18, V0MAX = BH + 3;
+  static constexpr int W1MAX = 11, V1MAX = BH / 2 + 3;
+  static constexpr int BOX0 = W0MAX * V0MAX * C0;
+  static constexpr int BOX1 = W1MAX * V1MAX * C1;
   static_assert(FP % 32 == 0 && H % 64 == 0 && G % 16 == 0, "tile shapes");
   static_assert(GSLOTS <= G && G <= FP, "grid slots");
-  static_assert(LDS * 2 + 8 * 256 <= 160 * 1024, "LDS budget");
+  static_assert(UNION * 2 >= 8 * 256, "final reduction reuses the union region");
+  static_assert(LDS * 2 + 4 * (BOX0 + BOX1) <= 160 * 1024, "LDS budget");
 };
+
+// Where the texels of a workgroup pass lie. For tiles of side >= 16 a pass
+// is a 16 x BH block of one tile (block-major order), so its latent footprint
+// is compact; smaller tiles are walked linearly.
+struct PassMap
+{
+  bool blocked;
+  int b, bx0, by0;
+};
+
+__device__ __forceinline__ PassMap pass_map(int64_t pass, int T, int pass_texels)
+{
+  PassMap pm;
+  pm.blocked = T >= 16;
+  pm.b = 0;
+  pm.bx0 = 0;
+  pm.by0 = 0;
+  if (pm.blocked)
+  {
+    const int bh = pass_texels / 16;
+    const int per_row = T / 16;
+    const int64_t per_tile = (int64_t)per_row * (T / bh);
+    pm.b = (int)(pass / per_tile);
+    const int q = (int)(pass - (int64_t)pm.b * per_tile);
+    pm.bx0 = (q % per_row) * 16;
+    pm.by0 = (q / per_row) * bh;
+  }
+  return pm;
+}
+
+// Texel tl of the pass: mip position (x, y), unwrapped position (xu, yu) =
+// origin + tile-local coordinate (>= M after a wrap), validity.
+struct PassTexel
+{
+  int x, y, xu, yu;
+  bool valid;
+};
+
+__device__ __forceinline__ PassTexel pass_texel(const PassMap& pm, int64_t pass, int tl, int pass_texels,
+                                                const int32_t* origins, int T, int M, int64_t B)
+{
+  PassTexel p;
+  int b, lx, ly;
+  if (pm.blocked)
+  {
+    b = pm.b;
+    lx = pm.bx0 + (tl & 15);
+    ly = pm.by0 + (tl >> 4);
+    p.valid = true;
+  }
+  else
+  {
+    const int64_t t = pass * pass_texels + tl;
+    p.valid = t < B;
+    const int64_t tt = p.valid ? t : 0;
+    const int TT = T * T;
+    b = (int)(tt / TT);
+    const int xy = (int)(tt - (int64_t)b * TT);
+    lx = xy % T;
+    ly = xy / T;
+  }
+  // unsigned: any origin value maps into [0, M)
+  const uint32_t ox = (uint32_t)origins[2 * b] % (uint32_t)M, oy = (uint32_t)origins[2 * b + 1] % (uint32_t)M;
+  p.xu = (int)(ox + (uint32_t)lx);
+  p.yu = (int)(oy + (uint32_t)ly);
+  p.x = p.xu >= M ? p.xu - M : p.xu;
+  p.y = p.yu >= M ? p.yu - M : p.yu;
+  return p;
+}
+
+// First bilinear corner index along one axis (clamped), as in footprint().
+__device__ __forceinline__ int corner0(int x, int M, int D)
+{
+  const float g = ((float)x + 0.5f) / (float)M * (float)D - 0.5f;
+  return clampi((int)floorf(g), 0, D - 1);
+}
+
+__device__ __forceinline__ int corner1(int x, int M, int D)
+{
+  const float g = ((float)x + 0.5f) / (float)M * (float)D - 0.5f;
+  return clampi((int)floorf(g) + 1, 0, D - 1);
+}
+
+// Aggregation box of one grid for a blocked pass, in unwrapped cell
+// coordinates (cell + D after the texel wrapped past M).
+struct Box
+{
+  int u0, v0, w, h;
+  bool fits;
+};
+
+__device__ __forceinline__ Box make_box(int xu0, int xu1, int yu0, int yu1, int M, int D, int wmax, int vmax)
+{
+  Box bx;
+  auto unwrap0 = [&](int pu) { return pu >= M ? corner0(pu - M, M, D) + D : corner0(pu, M, D); };
+  auto unwrap1 = [&](int pu) { return pu >= M ? corner1(pu - M, M, D) + D : corner1(pu, M, D); };
+  bx.u0 = unwrap0(xu0);
+  bx.v0 = unwrap0(yu0);
+  bx.w = unwrap1(xu1) - bx.u0 + 1;
+  bx.h = unwrap1(yu1) - bx.v0 + 1;
+  bx.fits = bx.w <= wmax && bx.h <= vmax;
+  return bx;
+}
 
 // Gather one texel's feature slots [lo, hi) into the LDS copies of X.
 template <class CF, int C0, int C1, int G0MODE, int PEC, int LODS>
@@ -279,6 +390,8 @@ __global__ __launch_bounds__(kThreads, 1) void k_fused_train(FusedParams P)
   uint16_t* stA = uni;                       // [H][64]  A0 | dZ0
   uint16_t* stB = uni + H * 64;              // [H][64]  dZ1 | A1
   uint16_t* stY = uni + 2 * H * 64;          // [16][64] dY
+  __shared__ float box0[CF::BOX0];           // g0 gradient box [v][u][ch]
+  __shared__ float box1[CF::BOX1];           // g1 gradient box [v][u][ch]
 
   const int tid = threadIdx.x;
   const int wave = tid >> 6, lane = tid & 63;
@@ -306,26 +419,31 @@ __global__ __launch_bounds__(kThreads, 1) void k_fused_train(FusedParams P)
 
   const int64_t npass = (P.B + kPass - 1) / kPass;
   const int T = P.tl.tile_dim, M = P.tl.mip_dim;
+  const int D0 = P.fd.g0_dim, D1 = P.fd.g1_dim;
   for (int64_t pass = blockIdx.x; pass < npass; pass += gridDim.x)
   {
-    const int64_t tb = pass * kPass;
-    __syncthreads();  // previous pass done with the union region; weights visible on the first pass
+    __syncthreads();  // previous pass done with the union region and boxes; weights visible on the first pass
+    const PassMap pm = pass_map(pass, T, kPass);
+    Box bx0 = {0, 0, 0, 0, false}, bx1 = {0, 0, 0, 0, false};
+    if (pm.blocked && P.scatter)
+    {
+      const int xu0 = (int)((uint32_t)P.tl.origins[2 * pm.b] % (uint32_t)M) + pm.bx0;
+      const int yu0 = (int)((uint32_t)P.tl.origins[2 * pm.b + 1] % (uint32_t)M) + pm.by0;
+      bx0 = make_box(xu0, xu0 + 15, yu0, yu0 + CF::BH - 1, M, D0, CF::W0MAX, CF::V0MAX);
+      bx1 = make_box(xu0, xu0 + 15, yu0, yu0 + CF::BH - 1, M, D1, CF::W1MAX, CF::V1MAX);
+      if (bx0.fits)
+        for (int e = tid; e < bx0.w * bx0.h * C0; e += kThreads) box0[e] = 0.0f;
+      if (bx1.fits)
+        for (int e = tid; e < bx1.w * bx1.h * C1; e += kThreads) box1[e] = 0.0f;
+    }
     // ---- P0: gather ----
     {
       // threads [0,128) gather g0 slots, [128,256) the rest; NT passes of 64 texels
       const int part = tid >> 7;
       for (int tl = tid & 127; tl < kPass; tl += 128)
       {
-        const int64_t t = tb + tl;
-        const bool valid = t < P.B;
-        int px = 0, py = 0;
-        if (valid)
-        {
-          const TexelPos pos = texel_of(t, P.tl.origins, T, M);
-          px = pos.x;
-          py = pos.y;
-        }
-        gather_part<CF, C0, C1, G0MODE, PEC, LODS>(P, part, tl, valid, px, py, xnf, xfn);
+        const PassTexel pt = pass_texel(pm, pass, tl, kPass, P.tl.origins, T, M, P.B);
+        gather_part<CF, C0, C1, G0MODE, PEC, LODS>(P, part, tl, pt.valid, pt.x, pt.y, xnf, xfn);
       }
     }
     __syncthreads();
@@ -333,21 +451,17 @@ __global__ __launch_bounds__(kThreads, 1) void k_fused_train(FusedParams P)
     // ---- P1: per-wave forward + backward for 32 texels ----
     int tloc[kNT];
     bool tvalid[kNT];
-    int tx[kNT], ty[kNT];
+    int tx[kNT], ty[kNT], wx[kNT], wy[kNT];
 #pragma unroll
     for (int nt = 0; nt < kNT; ++nt)
     {
       tloc[nt] = wave * 16 * kNT + nt * 16 + n16;
-      const int64_t t = tb + tloc[nt];
-      tvalid[nt] = t < P.B;
-      tx[nt] = 0;
-      ty[nt] = 0;
-      if (tvalid[nt])
-      {
-        const TexelPos pos = texel_of(t, P.tl.origins, T, M);
-        tx[nt] = pos.x;
-        ty[nt] = pos.y;
-      }
+      const PassTexel pt = pass_texel(pm, pass, tloc[nt], kPass, P.tl.origins, T, M, P.B);
+      tvalid[nt] = pt.valid;
+      tx[nt] = pt.x;
+      ty[nt] = pt.y;
+      wx[nt] = pt.xu >= M;  // texel wrapped past the mip edge
+      wy[nt] = pt.yu >= M;
     }
     // layer 0
     f32x4 z0[MT][kNT];
@@ -514,15 +628,36 @@ __global__ __launch_bounds__(kThreads, 1) void k_fused_train(FusedParams P)
           dz0f[j][nt] = frag_pack(v);
         }
     }
-    // dX for the grid slots, scattered into the active level's gradients
+    // dX for the grid slots, scattered into the active level's gradients:
+    // through the LDS boxes when the pass footprint fits, else directly
     if (P.scatter)
     {
 #pragma unroll
       for (int nt = 0; nt < kNT; ++nt)
       {
         if (!tvalid[nt]) continue;
-        const Footprint<float> f0 = footprint<float>(tx[nt], ty[nt], M, P.fd.g0_dim);
-        const Footprint<float> f1 = footprint<float>(tx[nt], ty[nt], M, P.fd.g1_dim);
+        const Footprint<float> f0 = footprint<float>(tx[nt], ty[nt], M, D0);
+        const Footprint<float> f1 = footprint<float>(tx[nt], ty[nt], M, D1);
+        // box-local offsets of the 4 corners (-1: outside, use a global atomic)
+        int o0[4], o1[4];
+        {
+          const int ax0 = corner0(tx[nt], M, D0) + (wx[nt] ? D0 : 0) - bx0.u0;
+          const int ax1 = corner1(tx[nt], M, D0) + (wx[nt] ? D0 : 0) - bx0.u0;
+          const int ay0 = corner0(ty[nt], M, D0) + (wy[nt] ? D0 : 0) - bx0.v0;
+          const int ay1 = corner1(ty[nt], M, D0) + (wy[nt] ? D0 : 0) - bx0.v0;
+          const int bx_0 = corner0(tx[nt], M, D1) + (wx[nt] ? D1 : 0) - bx1.u0;
+          const int bx_1 = corner1(tx[nt], M, D1) + (wx[nt] ? D1 : 0) - bx1.u0;
+          const int by_0 = corner0(ty[nt], M, D1) + (wy[nt] ? D1 : 0) - bx1.v0;
+          const int by_1 = corner1(ty[nt], M, D1) + (wy[nt] ? D1 : 0) - bx1.v0;
+#pragma unroll
+          for (int c = 0; c < 4; ++c)
+          {
+            const int u = (c & 1) ? ax1 : ax0, v = (c >> 1) ? ay1 : ay0;
+            o0[c] = (bx0.fits && (unsigned)u < (unsigned)bx0.w && (unsigned)v < (unsigned)bx0.h) ? (v * bx0.w + u) * C0 : -1;
+            const int u1 = (c & 1) ? bx_1 : bx_0, v1 = (c >> 1) ? by_1 : by_0;
+            o1[c] = (bx1.fits && (unsigned)u1 < (unsigned)bx1.w && (unsigned)v1 < (unsigned)bx1.h) ? (v1 * bx1.w + u1) * C1 : -1;
+          }
+        }
 #pragma unroll
         for (int r = 0; r < GT; ++r)
         {
@@ -539,29 +674,62 @@ __global__ __launch_bounds__(kThreads, 1) void k_fused_train(FusedParams P)
               if (G0MODE == NTC_G0_CORNERS)
               {
                 const int c = slot / C0, ch = slot - (slot / C0) * C0;
-                atomicAdd(&P.dg0[(int64_t)f0.cell[c] * C0 + ch], dx[i]);
+                if (o0[c] >= 0)
+                  atomicAdd(&box0[o0[c] + ch], dx[i]);
+                else
+                  atomicAdd(&P.dg0[(int64_t)f0.cell[c] * C0 + ch], dx[i]);
               }
               else
               {
 #pragma unroll
-                for (int c = 0; c < 4; ++c) atomicAdd(&P.dg0[(int64_t)f0.cell[c] * C0 + slot], f0.w[c] * dx[i]);
+                for (int c = 0; c < 4; ++c)
+                {
+                  if (o0[c] >= 0)
+                    atomicAdd(&box0[o0[c] + slot], f0.w[c] * dx[i]);
+                  else
+                    atomicAdd(&P.dg0[(int64_t)f0.cell[c] * C0 + slot], f0.w[c] * dx[i]);
+                }
               }
             }
             else if (slot < CF::GSLOTS)
             {
               const int ch = slot - CF::G1B;
 #pragma unroll
-              for (int c = 0; c < 4; ++c) atomicAdd(&P.dg1[(int64_t)f1.cell[c] * C1 + ch], f1.w[c] * dx[i]);
+              for (int c = 0; c < 4; ++c)
+              {
+                if (o1[c] >= 0)
+                  atomicAdd(&box1[o1[c] + ch], f1.w[c] * dx[i]);
+                else
+                  atomicAdd(&P.dg1[(int64_t)f1.cell[c] * C1 + ch], f1.w[c] * dx[i]);
+              }
             }
           }
         }
       }
     }
 
-    // ---- P2/P3: decoder weight gradients over the pass (K = texels) ----
-    // Round r stages column tile r of every wave: staged column k = 16*wave + n
-    // holds texel 16*NT*wave + 16*r + n.
-    __syncthreads();  // everyone done reading xnf
+    __syncthreads();  // everyone done reading xnf; boxes complete
+    // one global atomic per touched (cell, channel) of each box
+    if (bx0.fits)
+      for (int e = tid; e < bx0.w * bx0.h * C0; e += kThreads)
+      {
+        const float v = box0[e];
+        if (v == 0.0f) continue;
+        const int ch = e % C0, cell = e / C0;
+        const int u = bx0.u0 + cell % bx0.w, w = bx0.v0 + cell / bx0.w;
+        const int cx = u >= D0 ? u - D0 : u, cy = w >= D0 ? w - D0 : w;
+        atomicAdd(&P.dg0[((int64_t)cy * D0 + cx) * C0 + ch], v);
+      }
+    if (bx1.fits)
+      for (int e = tid; e < bx1.w * bx1.h * C1; e += kThreads)
+      {
+        const float v = box1[e];
+        if (v == 0.0f) continue;
+        const int ch = e % C1, cell = e / C1;
+        const int u = bx1.u0 + cell % bx1.w, w = bx1.v0 + cell / bx1.w;
+        const int cx = u >= D1 ? u - D1 : u, cy = w >= D1 ? w - D1 : w;
+        atomicAdd(&P.dg1[((int64_t)cy * D1 + cx) * C1 + ch], v);
+      }
 #pragma unroll
     for (int rnd = 0; rnd < kNT; ++rnd)
     {
@@ -663,8 +831,9 @@ __global__ __launch_bounds__(kThreads, 1) void k_fused_train(FusedParams P)
 #pragma unroll
     for (int i = 0; i < 4; ++i) pwo[(4 * g + i) * H + (wave * NTO + c) * 16 + n16] = dwo[c][i];
 
-  // SSE of this workgroup
-  __shared__ double red[kThreads];
+  // SSE of this workgroup (the union region is free after the last pass)
+  __syncthreads();
+  double* red = reinterpret_cast<double*>(uni);
   red[tid] = sse;
   __syncthreads();
   for (int w = kThreads / 2; w > 0; w >>= 1)

@@ -85,7 +85,7 @@ def _case(oracle, kind):
 
 @pytest.mark.parametrize("kind", ["c1_bilinear_relu", "c3_bilinear_h128", "c2_bilinear", "ref_bpp_2_25",
                                   "ref_bpp_1_0", "ref_bpp_0_5"])
-@pytest.mark.parametrize("lod,n_tiles", [(0, 2), (1, 1), (2, 3)])
+@pytest.mark.parametrize("lod,n_tiles", [(0, 2), (1, 1), (2, 3), (5, 2)])
 def test_fused_step_vs_fp64_oracle(oracle, kind, lod, n_tiles):
     O = oracle
     Lay, act = _case(O, kind)
