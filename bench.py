@@ -98,6 +98,28 @@ def cpu_baseline(ctx, first_step, n_steps):
                       f"CPU oracle (C, OpenMP, FP32), {dt:.1f} s"}
 
 
+def copy_probe(ntc, mib):
+    """A plain copy of known size: reads and writes mib MiB once each. Its
+    FETCH_SIZE / WRITE_SIZE counter readings calibrate the traffic figures."""
+    import ctypes
+
+    lib = ntc.load()
+    n = mib << 20
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    ntc.check(lib.ntc_dev_alloc(n, ctypes.byref(a)), "alloc")
+    ntc.check(lib.ntc_dev_alloc(n, ctypes.byref(b)), "alloc")
+    for _ in range(3):
+        ntc.check(lib.ntc_dev_copy(b.value, a.value, n), "copy")
+    lib.ntc_dev_synchronize()
+    t0 = time.perf_counter()
+    ntc.check(lib.ntc_dev_copy(b.value, a.value, n), "copy")
+    lib.ntc_dev_synchronize()
+    dt = time.perf_counter() - t0
+    print(json.dumps({"copy_probe_bytes": n, "seconds": dt, "GBps_rw": 2 * n / dt / 1e9}), flush=True)
+    lib.ntc_dev_free(a.value)
+    lib.ntc_dev_free(b.value)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -106,6 +128,8 @@ def main():
     ap.add_argument("--path", choices=["fused", "plain"], default="fused")
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run a device-to-device copy of MIB MiB (HBM counter calibration)")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (every GPU box has one GPU); "
@@ -114,6 +138,9 @@ def main():
 
     import openntc_amd as ntc
 
+    if a.copy_probe:
+        copy_probe(ntc, a.copy_probe)
+        return
     path = ntc.TrainPath.Fused if a.path == "fused" else ntc.TrainPath.Plain
     log(f"setting up {DIM}^2 synthetic set, path={a.path}")
     t_setup = time.perf_counter()

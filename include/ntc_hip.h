@@ -92,6 +92,7 @@ int ntc_dev_free(void* p);
 int ntc_dev_upload(void* dst, const void* src, size_t bytes);
 int ntc_dev_download(void* dst, const void* src, size_t bytes);
 int ntc_dev_synchronize(void);
+int ntc_dev_copy(void* dst, const void* src, size_t bytes); /* device to device */
 /* 0 when a gfx950 device is usable; fills *count with the device count. */
 int ntc_device_check(int32_t* count);
 const char* ntc_status_string(int status);

@@ -164,6 +164,7 @@ SIGNATURES = {
     "ntc_dev_upload": (_I, [_P, _P, _SZ]),
     "ntc_dev_download": (_I, [_P, _P, _SZ]),
     "ntc_dev_synchronize": (_I, []),
+    "ntc_dev_copy": (_I, [_P, _P, _SZ]),
     "ntc_grid_update2_f32": (_I, [_I64, _P, _P, _P, _P, _F, _F, _I64, _P, _P, _P, _P, _F, _F,
                                   _F, _F, _F, _F, _F, _P]),
     "ntc_fused_supported": (_I, [_P, _I, _I]),

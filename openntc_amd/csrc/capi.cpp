@@ -45,6 +45,12 @@ int ntc_dev_download(void* dst, const void* src, size_t bytes)
 
 int ntc_dev_synchronize(void) { return (int)hipDeviceSynchronize(); }
 
+int ntc_dev_copy(void* dst, const void* src, size_t bytes)
+{
+  if (!bytes) return NTC_OK;
+  return (int)hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice);
+}
+
 int ntc_device_check(int32_t* count)
 {
   int n = 0;

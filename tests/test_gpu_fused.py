@@ -7,7 +7,7 @@ Two references: (1) the same step with the kernel's BF16 rounding points
 BF16 where the kernel rounds them, products accumulated in FP64), which the
 kernel must match to 2e-3 relative in norm (FP32 accumulation order);
 (2) the plain FP64 oracle, which bounds the precision loss of BF16: loss within
-1e-2, decoder gradients within 3e-2; latent gradients within 1e-1, because
+1e-2, decoder gradients within 5e-2; latent gradients within 1e-1, because
 activation masks flip for pre-activations within BF16 rounding of zero."""
 import ctypes as C
 import zlib
@@ -22,7 +22,7 @@ pytestmark = pytest.mark.gpu
 
 from tests import gpu_util as G  # noqa: E402
 
-REL_TOL = 3e-2
+REL_TOL = 5e-2
 REL_TOL_LATENT = 1e-1
 REL_TOL_BF16 = 2e-3
 
