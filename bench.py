@@ -20,6 +20,7 @@ from __future__ import annotations
 import argparse
 import json
 import os
+import re
 import sys
 import time
 
@@ -98,6 +99,25 @@ def cpu_baseline(ctx, first_step, n_steps):
                       f"CPU oracle (C, OpenMP, FP32), {dt:.1f} s"}
 
 
+def pmc_traffic(kernel):
+    """HBM bytes per launch of `kernel` as measured with rocprofv3 FETCH_SIZE /
+    WRITE_SIZE (copy-calibrated) and recorded in the committed profile summary;
+    None when the summary holds no row for it."""
+    path = os.path.join(ROOT, "profiles", "grid_update_and_fused_4096_c3_hbm_traffic.md")
+    try:
+        lines = open(path).read().splitlines()
+    except OSError:
+        return None
+    units = {"KB": 1e3, "MB": 1e6, "GB": 1e9}
+    for ln in lines:
+        cols = [c.strip() for c in ln.strip().strip("|").split("|")]
+        if len(cols) >= 6 and cols[0].split(" ")[0] == kernel:
+            m = re.match(r"([0-9.]+) (KB|MB|GB)$", cols[4])
+            if m:
+                return float(m.group(1)) * units[m.group(2)]
+    return None
+
+
 def copy_probe(ntc, mib):
     """A plain copy of known size: reads and writes mib MiB once each. Its
     FETCH_SIZE / WRITE_SIZE counter readings calibrate the traffic figures."""
@@ -126,7 +146,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--path", choices=["fused", "plain"], default="fused")
-    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--cpu-steps", type=int, default=40)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (HBM counter calibration)")
@@ -208,11 +228,13 @@ def main():
         if params and grid_ms >= fused_ms:
             g = kern["grid_update"]
             out["roofline"] = {"bound": "hbm", "kernel": "k_grid_update2", "achieved": g["achieved"],
-                               "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": g["frac"], "traffic": None}
+                               "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": g["frac"],
+                               "traffic": pmc_traffic("k_grid_update2")}
         else:
             f = kern["fused_train_step"]
             out["roofline"] = {"bound": "mfma", "kernel": "k_fused_train", "achieved": f["achieved"],
-                               "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": f["frac"], "traffic": None}
+                               "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": f["frac"],
+                               "traffic": pmc_traffic("k_fused_train")}
 
     if not a.no_cpu_baseline:
         log(f"cpu baseline: {a.cpu_steps} oracle steps")
