@@ -21,8 +21,11 @@
 //     four waves each accumulate a quarter of dW in registers across passes;
 //     the partial sums go to HBM once per workgroup and are reduced in order
 //     by ntc_decoder_update (deterministic);
-//   * latent gradients are added with global float atomics (the reference
-//     uses atomics too, ntc_kernel.cu:261-278).
+//   * latent gradients: dX of the grid slots goes to LDS per texel; each
+//     (cell, channel) of the pass's footprint box then sums its texels'
+//     weighted dX in a fixed order and adds the sum to the grid's gradient
+//     with one global float atomic (the reference adds every texel's
+//     contribution with an atomic, ntc_kernel.cu:261-278).
 // Quantisation noise is the same Philox stream as the plain path.
 #include "ntc_device.h"
 #include "ntc_host.h"
@@ -134,21 +137,58 @@ struct Cfg
   static constexpr int PE_FIRST = PEC == 3 ? 8 : 4;
   static constexpr int WT = H * FP + 2 * H * H + 48 * H + G * H;
   static constexpr int XFN = FP * PASS;
+  static constexpr int XS = FP + 8;  // xnf row stride (elements): spreads texel rows over LDS banks
   static constexpr int STAGE_B = 2 * H * 64 + 16 * 64;
-  static constexpr int UNION = (PASS * FP > STAGE_B) ? PASS * FP : STAGE_B;
-  static constexpr int LDS = WT + XFN + UNION;
-  // latent-gradient aggregation boxes (floats): a pass is a 16 x BH texel
-  // block; at texel:cell ratios >= 1 (g0) and >= 2 (g1) its footprint fits
   static constexpr int BH = PASS / 16;
-  static constexpr int W0MAX = 18, V0MAX = BH + 3;
-  static constexpr int W1MAX = 11, V1MAX = BH / 2 + 3;
-  static constexpr int BOX0 = W0MAX * V0MAX * C0;
-  static constexpr int BOX1 = W1MAX * V1MAX * C1;
+  // Latent gradients of a blocked pass (16 x BH texels) come from the
+  // per-texel dX when the pass footprint spans at most WMAX x VMAX cells of
+  // the grid (texel:cell ratio >= 1 along both axes).
+  static constexpr int WMAX = 18, VMAX = BH + 3;
+  // Staged gather: the footprint cells of both grids, noise added, sit in
+  // LDS as [v][u][CP] floats when the boxes span at most SW x SV cells;
+  // channel quads (4 channels share one Philox draw) are prefetched, PF per
+  // thread, while the previous pass computes its decoder gradients.
+  static constexpr int Q0 = (C0 + 3) / 4, Q1 = (C1 + 3) / 4;
+  static constexpr int CP0 = 4 * Q0, CP1 = 4 * Q1;
+  static constexpr int PF = H >= 128 ? 2 : 3;
+  // dX row stride = 16 mod 64 floats puts the 4 texel rows of one FP32 MFMA
+  // B-operand read in distinct LDS banks
+  static constexpr int GS_BANK = G + (((16 - G) % 64) + 64) % 64;
+  static constexpr int lds_bytes(int sw, int sv, int gs)
+  {
+    const int u = PASS * XS + 2 * sw * sv * (CP0 + CP1);
+    return 2 * (WT + XFN + (u > STAGE_B ? u : STAGE_B)) + 4 * PASS * gs + 4 * 192 + 4 * 128 + 16 * PASS;
+  }
+  static constexpr int LIM = 160 * 1024;
+  static constexpr int SEL = lds_bytes(WMAX, BH + 3, GS_BANK) <= LIM       ? 0
+                             : lds_bytes(WMAX, BH + 3, G + 4) <= LIM       ? 1
+                             : lds_bytes(10, BH / 2 + 3, GS_BANK) <= LIM   ? 2
+                             : lds_bytes(10, BH / 2 + 3, G + 4) <= LIM     ? 3
+                                                                           : 4;
+  static constexpr int SW = SEL <= 1 ? WMAX : (SEL <= 3 ? 10 : 0);
+  static constexpr int SV = SEL <= 1 ? BH + 3 : (SEL <= 3 ? BH / 2 + 3 : 0);
+  static constexpr int GS = (SEL == 0 || SEL == 2) ? GS_BANK : G + 4;  // dX row stride (floats)
+  static constexpr int DXS = PASS * GS;
+  static constexpr int SBOX0 = SW * SV * CP0;  // g1 box offset in the staged boxes (floats)
+  static constexpr int UNION_U = PASS * XS + 2 * SW * SV * (CP0 + CP1);
+  static constexpr int UNION = UNION_U > STAGE_B ? UNION_U : STAGE_B;
+  static constexpr int LDS = WT + XFN + UNION;
   static_assert(FP % 32 == 0 && H % 64 == 0 && G % 16 == 0, "tile shapes");
   static_assert(GSLOTS <= G && G <= FP, "grid slots");
   static_assert(UNION * 2 >= 8 * 256, "final reduction reuses the union region");
-  static_assert(LDS * 2 + 4 * (BOX0 + BOX1) <= 160 * 1024, "LDS budget");
+  static_assert(16 + BH <= 128, "axis table threads");
+  static_assert((PASS * XS * 2) % 16 == 0, "staged boxes 16-byte aligned");
+  static_assert(lds_bytes(SW, SV, GS) <= LIM, "LDS budget");
 };
+
+// Corner tables of one grid for a blocked pass: box-local corner columns
+// and fractions of the pass's 16 texel columns and BH texel rows.
+struct AxisTab
+{
+  float fx[16], fy[16];
+  int8_t cx0[16], cx1[16], cy0[16], cy1[16];
+};
+static_assert(sizeof(AxisTab) == 192, "axis table size in the LDS budget");
 
 // Where the texels of a workgroup pass lie. For tiles of side >= 16 a pass
 // is a 16 x BH block of one tile (block-major order), so its latent footprint
@@ -180,7 +220,9 @@ __device__ __forceinline__ PassMap pass_map(int64_t pass, int T, int pass_texels
 }
 
 // Texel tl of the pass: mip position (x, y), unwrapped position (xu, yu) =
-// origin + tile-local coordinate (>= M after a wrap), validity.
+// origin + tile-local coordinate (>= M after a wrap), validity. orig holds
+// the tile origins reduced mod M (unsigned: any origin value maps into
+// [0, M)).
 struct PassTexel
 {
   int x, y, xu, yu;
@@ -188,7 +230,7 @@ struct PassTexel
 };
 
 __device__ __forceinline__ PassTexel pass_texel(const PassMap& pm, int64_t pass, int tl, int pass_texels,
-                                                const int32_t* origins, int T, int M, int64_t B)
+                                                const uint32_t* orig, int T, int M, int64_t B)
 {
   PassTexel p;
   int b, lx, ly;
@@ -210,30 +252,27 @@ __device__ __forceinline__ PassTexel pass_texel(const PassMap& pm, int64_t pass,
     lx = xy % T;
     ly = xy / T;
   }
-  // unsigned: any origin value maps into [0, M)
-  const uint32_t ox = (uint32_t)origins[2 * b] % (uint32_t)M, oy = (uint32_t)origins[2 * b + 1] % (uint32_t)M;
-  p.xu = (int)(ox + (uint32_t)lx);
-  p.yu = (int)(oy + (uint32_t)ly);
+  p.xu = (int)(orig[2 * b] + (uint32_t)lx);
+  p.yu = (int)(orig[2 * b + 1] + (uint32_t)ly);
   p.x = p.xu >= M ? p.xu - M : p.xu;
   p.y = p.yu >= M ? p.yu - M : p.yu;
   return p;
 }
 
-// First bilinear corner index along one axis (clamped), as in footprint().
-__device__ __forceinline__ int corner0(int x, int M, int D)
+// Bilinear corners of unwrapped texel coordinate pu (>= M after a wrap)
+// along one axis, as unwrapped cell coordinates (cell + D after a wrap)
+// minus o, and the fraction: the arithmetic of footprint().
+__device__ __forceinline__ void axis_corners(int pu, int M, int D, int o, int& c0, int& c1, float& f)
 {
-  const float g = ((float)x + 0.5f) / (float)M * (float)D - 0.5f;
-  return clampi((int)floorf(g), 0, D - 1);
+  const bool wrapped = pu >= M;
+  int x0;
+  footprint_axis<float>(wrapped ? pu - M : pu, M, D, x0, f);
+  const int add = (wrapped ? D : 0) - o;
+  c0 = clampi(x0, 0, D - 1) + add;
+  c1 = clampi(x0 + 1, 0, D - 1) + add;
 }
 
-__device__ __forceinline__ int corner1(int x, int M, int D)
-{
-  const float g = ((float)x + 0.5f) / (float)M * (float)D - 0.5f;
-  return clampi((int)floorf(g) + 1, 0, D - 1);
-}
-
-// Aggregation box of one grid for a blocked pass, in unwrapped cell
-// coordinates (cell + D after the texel wrapped past M).
+// Cell box of one grid covering a blocked pass, in unwrapped cell coordinates.
 struct Box
 {
   int u0, v0, w, h;
@@ -243,17 +282,347 @@ struct Box
 __device__ __forceinline__ Box make_box(int xu0, int xu1, int yu0, int yu1, int M, int D, int wmax, int vmax)
 {
   Box bx;
-  auto unwrap0 = [&](int pu) { return pu >= M ? corner0(pu - M, M, D) + D : corner0(pu, M, D); };
-  auto unwrap1 = [&](int pu) { return pu >= M ? corner1(pu - M, M, D) + D : corner1(pu, M, D); };
-  bx.u0 = unwrap0(xu0);
-  bx.v0 = unwrap0(yu0);
-  bx.w = unwrap1(xu1) - bx.u0 + 1;
-  bx.h = unwrap1(yu1) - bx.v0 + 1;
+  int a, b;
+  float f;
+  axis_corners(xu0, M, D, 0, bx.u0, a, f);
+  axis_corners(xu1, M, D, 0, a, b, f);
+  bx.w = b - bx.u0 + 1;
+  axis_corners(yu0, M, D, 0, bx.v0, a, f);
+  axis_corners(yu1, M, D, 0, a, b, f);
+  bx.h = b - bx.v0 + 1;
   bx.fits = bx.w <= wmax && bx.h <= vmax;
   return bx;
 }
 
-// Gather one texel's feature slots [lo, hi) into the LDS copies of X.
+// Fills the corner tables of one grid; t in [0, 128) is the thread's slot.
+template <int BH>
+__device__ __forceinline__ void fill_axis_tab(AxisTab& tb, const Box& bx, int xu0, int yu0, int M, int D, int t)
+{
+  int c0, c1;
+  float f;
+  if (t < 16)
+  {
+    axis_corners(xu0 + t, M, D, bx.u0, c0, c1, f);
+    tb.cx0[t] = (int8_t)c0;
+    tb.cx1[t] = (int8_t)c1;
+    tb.fx[t] = f;
+  }
+  else if (t < 16 + BH)
+  {
+    const int r = t - 16;
+    axis_corners(yu0 + r, M, D, bx.v0, c0, c1, f);
+    tb.cy0[r] = (int8_t)c0;
+    tb.cy1[r] = (int8_t)c1;
+    tb.fy[r] = f;
+  }
+}
+
+__device__ __forceinline__ f32x4 mfma_f32(float a, float b, const f32x4& c)
+{
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+
+// Gradient of every (cell, channel) of a grid's box as one FP32 MFMA product
+// per 16 cells x 16 channels: S[cell][ch] = sum over the pass's texels t of
+// Wt[cell][t] * dX[t][slot(ch)], where Wt holds the gather's corner weights
+// ((1-fx or fx) * (1-fy or fy) for bilinear grids; 1 for the unweighted
+// corner layout of g0, one product per corner c reading slot c * C + ch).
+// Each sum goes to the grid's gradient with one global atomic.
+// MFMA 16x16x4 operands: lane l gives A[l%16][l/16] and B[l/16][l%16] and
+// receives D[4(l/16)+i][l%16]; K step kk covers texels 4kk..4kk+3, so lane l
+// meets texel column 4(kk%4) + l/16 of texel row kk/4.
+template <int C, bool CORNERS, int GS, int PASS>
+__device__ __forceinline__ void box_gradients(const AxisTab& tb, const Box& bx, const float* dxs, int slot0,
+                                              float* dg, int D, int wave, int lane)
+{
+  constexpr int BH = PASS / 16;
+  constexpr int KS = PASS / 4;
+  constexpr int NTC = (C + 15) / 16;
+  constexpr int NC = CORNERS ? 4 : 1;
+  const int cells = bx.w * bx.h;
+  const int tiles = ((cells + 15) / 16) * NTC;
+  if (wave >= tiles) return;
+  const int q = lane >> 4, n = lane & 15;
+  int ax0[4], ax1[4], ay0[BH], ay1[BH];
+  float fx[4], fy[BH];
+#pragma unroll
+  for (int j = 0; j < 4; ++j)
+  {
+    ax0[j] = tb.cx0[4 * j + q];
+    ax1[j] = tb.cx1[4 * j + q];
+    fx[j] = tb.fx[4 * j + q];
+  }
+#pragma unroll
+  for (int y = 0; y < BH; ++y)
+  {
+    ay0[y] = tb.cy0[y];
+    ay1[y] = tb.cy1[y];
+    fy[y] = tb.fy[y];
+  }
+  for (int tile = wave; tile < tiles; tile += 4)
+  {
+    const int mt = tile / NTC, ct = tile - mt * NTC;
+    const int m = 16 * mt + n;
+    const int u = m % bx.w, v = m / bx.w;  // v >= bx.h past the last cell: no corner matches
+    const int ch = 16 * ct + n;
+    f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int c = 0; c < NC; ++c)
+    {
+      // x and y factors of this lane's cell; one of the two terms is zero
+      // unless both corners clamp onto the same border cell
+      float wx[4], wy[BH];
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        wx[j] = CORNERS ? (((c & 1) ? ax1[j] : ax0[j]) == u ? 1.0f : 0.0f)
+                        : (ax0[j] == u ? 1.0f - fx[j] : 0.0f) + (ax1[j] == u ? fx[j] : 0.0f);
+#pragma unroll
+      for (int y = 0; y < BH; ++y)
+        wy[y] = CORNERS ? (((c >> 1) ? ay1[y] : ay0[y]) == v ? 1.0f : 0.0f)
+                        : (ay0[y] == v ? 1.0f - fy[y] : 0.0f) + (ay1[y] == v ? fy[y] : 0.0f);
+#pragma unroll
+      for (int kk = 0; kk < KS; ++kk)
+      {
+        const float d = ch < C ? dxs[(4 * kk + q) * GS + slot0 + c * C + ch] : 0.0f;
+        acc = mfma_f32(wx[kk & 3] * wy[kk >> 2], d, acc);
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+    {
+      const int cell = 16 * mt + 4 * q + i;
+      if (cell < cells && ch < C)
+      {
+        const int cu = bx.u0 + cell % bx.w, cv = bx.v0 + cell / bx.w;
+        const int cx = cu >= D ? cu - D : cu, cy = cv >= D ? cv - D : cv;
+        atomicAdd(&dg[((int64_t)cy * D + cx) * C + ch], acc[i]);
+      }
+    }
+  }
+}
+
+// Geometry of one workgroup pass: where its texels lie, the cell boxes of
+// both grids and whether its gather is staged through LDS.
+struct PassGeo
+{
+  PassMap pm;
+  int xu0, yu0;  // unwrapped mip position of the block's first texel
+  Box b0, b1;
+  bool staged;
+};
+
+template <class CF>
+__device__ __forceinline__ PassGeo pass_geo(int64_t pass, const uint32_t* orig, int T, int M, int D0, int D1)
+{
+  PassGeo pg;
+  pg.pm = pass_map(pass, T, CF::PASS);
+  pg.xu0 = 0;
+  pg.yu0 = 0;
+  pg.b0 = Box{0, 0, 0, 0, false};
+  pg.b1 = Box{0, 0, 0, 0, false};
+  pg.staged = false;
+  if (pg.pm.blocked)
+  {
+    pg.xu0 = (int)orig[2 * pg.pm.b] + pg.pm.bx0;
+    pg.yu0 = (int)orig[2 * pg.pm.b + 1] + pg.pm.by0;
+    pg.b0 = make_box(pg.xu0, pg.xu0 + 15, pg.yu0, pg.yu0 + CF::BH - 1, M, D0, CF::WMAX, CF::VMAX);
+    pg.b1 = make_box(pg.xu0, pg.xu0 + 15, pg.yu0, pg.yu0 + CF::BH - 1, M, D1, CF::WMAX, CF::VMAX);
+    pg.staged = pg.b0.w <= CF::SW && pg.b0.h <= CF::SV && pg.b1.w <= CF::SW && pg.b1.h <= CF::SV &&
+                pg.b0.w * pg.b0.h * CF::Q0 + pg.b1.w * pg.b1.h * CF::Q1 <= CF::PF * kThreads;
+  }
+  return pg;
+}
+
+// Channel quads of a staged pass's footprint held in registers between the
+// global loads and their noisy copy into LDS.
+template <int PF>
+struct Prefetch
+{
+  float v[PF][4];
+  int dst[PF];   // float offset in the staged boxes; -1: nothing
+  int cell[PF];  // grid cell (noise counter)
+  int gq[PF];    // grid | quad << 1
+};
+
+// Issues the loads of a pass's staged footprint quads (PF per thread) into
+// registers. Every load is unconditional, from a valid address (idle slots
+// read the first quad), so the issue point stays where it is written.
+template <class CF, int C0, int C1>
+__device__ __forceinline__ void prefetch_issue(Prefetch<CF::PF>& pf, const PassGeo& pg, const FusedParams& P, int tid)
+{
+  const int n0 = pg.b0.w * pg.b0.h * CF::Q0;
+  const int n = pg.staged ? n0 + pg.b1.w * pg.b1.h * CF::Q1 : 0;
+#pragma unroll
+  for (int k = 0; k < CF::PF; ++k)
+  {
+    const int e0 = tid + k * kThreads;
+    const bool live = e0 < n;
+    const bool g1 = live && e0 >= n0;
+    const int ee = !live ? 0 : (g1 ? e0 - n0 : e0);
+    const int Q = g1 ? CF::Q1 : CF::Q0, C = g1 ? C1 : C0, CP = g1 ? CF::CP1 : CF::CP0;
+    const int bw = g1 ? pg.b1.w : pg.b0.w, bu0 = g1 ? pg.b1.u0 : pg.b0.u0, bv0 = g1 ? pg.b1.v0 : pg.b0.v0;
+    const int D = g1 ? P.fd.g1_dim : P.fd.g0_dim;
+    const int q = ee % Q, cb = ee / Q;
+    const int bws = bw > 0 ? bw : 1;
+    const int u = cb % bws, v = cb / bws;
+    const int cu = bu0 + u, cv = bv0 + v;
+    const int cx = cu >= D ? cu - D : cu, cy = cv >= D ? cv - D : cv;
+    const int cell = cy * D + cx;
+    const float* src = (g1 ? P.g1 : P.g0) + (int64_t)cell * C + 4 * q;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) pf.v[k][i] = src[(4 * q + i < C) ? i : 0];
+    pf.dst[k] = live ? (g1 ? CF::SBOX0 : 0) + cb * CP + 4 * q : -1;
+    pf.cell[k] = cell;
+    pf.gq[k] = (int)g1 | (q << 1);
+  }
+}
+
+// Quantisation noise added per channel quad as in the per-texel gather
+// (one Philox draw per cell and quad), then stored into the staged boxes.
+template <class CF, int C0, int C1>
+__device__ __forceinline__ void prefetch_commit(const Prefetch<CF::PF>& pf, const FusedParams& P, float* sbox)
+{
+#pragma unroll
+  for (int k = 0; k < CF::PF; ++k)
+  {
+    if (pf.dst[k] < 0) continue;
+    const int grid = pf.gq[k] & 1, q = pf.gq[k] >> 1;
+    const int C = grid ? C1 : C0;
+    const float delta = grid ? P.nz.delta1 : P.nz.delta0;
+    uint4 w = make_uint4(0, 0, 0, 0);
+    if (P.nz.enabled) w = noise_words(P.nz.seed, P.nz.step, grid, P.nz.level, (uint32_t)pf.cell[k], q);
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+    {
+      float v = pf.v[k][i];
+      if (P.nz.enabled) v = v + noise_value(pick(w, i), delta);
+      if (4 * q + i < C) sbox[pf.dst[k] + i] = v;
+    }
+  }
+}
+
+// Issues the loads of the target bytes of this lane's output rows
+// (4(l/16)+i) for its texels of a pass; unconditional, like prefetch_issue.
+template <class CF, int NT>
+__device__ __forceinline__ void targets_issue(uint32_t (&tg)[NT][4], const PassMap& pm, int64_t pass,
+                                              const FusedParams& P, const uint32_t* orig, int wave, int n16, int g)
+{
+  const int M = P.tl.mip_dim;
+#pragma unroll
+  for (int nt = 0; nt < NT; ++nt)
+  {
+    const PassTexel pt = pass_texel(pm, pass, wave * 16 * NT + nt * 16 + n16, CF::PASS, orig, P.tl.tile_dim, M, P.B);
+    const int64_t base = pt.valid ? ((int64_t)pt.y * M + pt.x) * P.O : 0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+    {
+      const int o = 4 * g + i;
+      tg[nt][i] = (uint32_t)P.pkg[base + (o < P.O ? o : 0)];
+    }
+  }
+}
+
+// Target bytes into LDS, one word per (texel, lane group): [PASS][4] words.
+template <int NT>
+__device__ __forceinline__ void targets_commit(const uint32_t (&tg)[NT][4], uint32_t* tgt_s, int wave, int n16, int g)
+{
+#pragma unroll
+  for (int nt = 0; nt < NT; ++nt)
+    tgt_s[(wave * 16 * NT + nt * 16 + n16) * 4 + g] = tg[nt][0] | (tg[nt][1] << 8) | (tg[nt][2] << 16) | (tg[nt][3] << 24);
+}
+
+// Feature slots after the grids: positional encoding, lod, zero padding.
+template <class CF, int PEC, int LODS, class Put>
+__device__ __forceinline__ void gather_rest(const FusedParams& P, int px, int py, Put&& put)
+{
+  constexpr int FP = CF::KF * 32;
+  int slot = CF::GSLOTS;
+#pragma unroll
+  for (int pi = 0; pi < PEC; ++pi)
+  {
+    const int Pd = CF::PE_FIRST >> pi;
+#pragma unroll
+    for (int axis = 0; axis < 2; ++axis)
+#pragma unroll
+      for (int h = 0; h < 2; ++h) put(slot++, tri_wave<float>(axis == 0 ? px : py, Pd, h));
+  }
+  if (LODS) put(slot++, P.tl.norm_lod);
+  for (; slot < FP; ++slot) put(slot, 0.0f);
+}
+
+// Features of a staged pass from the LDS boxes: one item per (channel quad
+// of a grid, texel) plus one per texel for the remaining slots. Corner
+// weights and the fma order are those of the per-texel gather.
+template <class CF, int C0, int C1, int G0MODE, int PEC, int LODS>
+__device__ __forceinline__ void gather_staged(const FusedParams& P, const PassGeo& pg, const AxisTab* tab,
+                                              const float* sbox, int tid, uint16_t* xnf, uint16_t* xfn)
+{
+  constexpr int Q0 = CF::Q0, Q1 = CF::Q1, QI = Q0 + Q1 + 1;
+  const int M = P.tl.mip_dim;
+  for (int item = tid; item < QI * CF::PASS; item += kThreads)
+  {
+    const int qi = item / CF::PASS, t = item % CF::PASS;
+    const int x = t & 15, y = t >> 4;
+    auto put = [&](int slot, float v) {
+      const uint16_t b = f2bf(v);
+      xnf[t * CF::XS + slot] = b;
+      xfn[slot * CF::PASS + t] = b;
+    };
+    if (qi < Q0 + Q1)
+    {
+      const bool g1 = qi >= Q0;
+      const int q = g1 ? qi - Q0 : qi;
+      const AxisTab& tb = tab[g1 ? 1 : 0];
+      const int bw = g1 ? pg.b1.w : pg.b0.w;
+      const int CP = g1 ? CF::CP1 : CF::CP0, C = g1 ? C1 : C0;
+      const float* box = sbox + (g1 ? CF::SBOX0 : 0) + 4 * q;
+      const int ax0 = tb.cx0[x], ax1 = tb.cx1[x], ay0 = tb.cy0[y], ay1 = tb.cy1[y];
+      const float fx = tb.fx[x], fy = tb.fy[y];
+      const float w[4] = {(1.0f - fx) * (1.0f - fy), fx * (1.0f - fy), (1.0f - fx) * fy, fx * fy};
+      const int off[4] = {(ay0 * bw + ax0) * CP, (ay0 * bw + ax1) * CP, (ay1 * bw + ax0) * CP,
+                          (ay1 * bw + ax1) * CP};
+      float4 val[4];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) val[c] = *reinterpret_cast<const float4*>(box + off[c]);
+      if (!g1 && G0MODE == NTC_G0_CORNERS)
+      {
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+        {
+          if (4 * q + 0 < C0) put(c * C0 + 4 * q + 0, val[c].x);
+          if (4 * q + 1 < C0) put(c * C0 + 4 * q + 1, val[c].y);
+          if (4 * q + 2 < C0) put(c * C0 + 4 * q + 2, val[c].z);
+          if (4 * q + 3 < C0) put(c * C0 + 4 * q + 3, val[c].w);
+        }
+      }
+      else
+      {
+        const int base = (g1 ? CF::G1B : 0) + 4 * q;
+        float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+        {
+          acc[0] = fmaf(w[c], val[c].x, acc[0]);
+          acc[1] = fmaf(w[c], val[c].y, acc[1]);
+          acc[2] = fmaf(w[c], val[c].z, acc[2]);
+          acc[3] = fmaf(w[c], val[c].w, acc[3]);
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+          if (4 * q + i < C) put(base + i, acc[i]);
+      }
+    }
+    else
+    {
+      const int xu = pg.xu0 + x, yu = pg.yu0 + y;
+      gather_rest<CF, PEC, LODS>(P, xu >= M ? xu - M : xu, yu >= M ? yu - M : yu, put);
+    }
+  }
+}
+
+// Gather one texel's feature slots into the LDS copies of X directly from the
+// grids (passes whose footprint is not staged): part 0 the g0 slots, part 1
+// g1 and the remaining slots.
 template <class CF, int C0, int C1, int G0MODE, int PEC, int LODS>
 __device__ __forceinline__ void gather_part(const FusedParams& P, int part, int tl, bool valid, int px, int py,
                                             uint16_t* xnf, uint16_t* xfn)
@@ -261,7 +630,7 @@ __device__ __forceinline__ void gather_part(const FusedParams& P, int part, int 
   constexpr int FP = CF::KF * 32;
   auto put = [&](int slot, float v) {
     const uint16_t b = f2bf(v);
-    xnf[tl * FP + slot] = b;
+    xnf[tl * CF::XS + slot] = b;
     xfn[slot * CF::PASS + tl] = b;
   };
   const int M = P.tl.mip_dim;
@@ -361,18 +730,7 @@ __device__ __forceinline__ void gather_part(const FusedParams& P, int part, int 
       }
     }
   }
-  int slot = CF::GSLOTS;
-#pragma unroll
-  for (int pi = 0; pi < PEC; ++pi)
-  {
-    const int Pd = CF::PE_FIRST >> pi;
-#pragma unroll
-    for (int axis = 0; axis < 2; ++axis)
-#pragma unroll
-      for (int h = 0; h < 2; ++h) put(slot++, tri_wave<float>(axis == 0 ? px : py, Pd, h));
-  }
-  if (LODS) put(slot++, P.tl.norm_lod);
-  for (; slot < FP; ++slot) put(slot, 0.0f);
+  gather_rest<CF, PEC, LODS>(P, px, py, put);
 }
 
 template <int H, int FP, int G, int C0, int C1, int G0MODE, int PEC, int LODS, int ACT, int NT>
@@ -386,12 +744,15 @@ __global__ __launch_bounds__(kThreads, 1) void k_fused_train(FusedParams P)
   uint16_t* wt = smem;
   uint16_t* xfn = smem + CF::WT;
   uint16_t* uni = xfn + CF::XFN;
-  uint16_t* xnf = uni;                       // during gather/forward
+  uint16_t* xnf = uni;                       // [PASS][XS] during gather/forward
+  float* sbox = reinterpret_cast<float*>(uni + kPass * CF::XS);  // staged boxes during gather
   uint16_t* stA = uni;                       // [H][64]  A0 | dZ0
   uint16_t* stB = uni + H * 64;              // [H][64]  dZ1 | A1
   uint16_t* stY = uni + 2 * H * 64;          // [16][64] dY
-  __shared__ float box0[CF::BOX0];           // g0 gradient box [v][u][ch]
-  __shared__ float box1[CF::BOX1];           // g1 gradient box [v][u][ch]
+  __shared__ __attribute__((aligned(16))) float dxs[CF::DXS];  // dX of the grid slots [texel][GS]
+  __shared__ AxisTab atab[4];                                  // corner tables of g0, g1; two passes
+  __shared__ uint32_t orig_s[128];                             // tile origins mod M
+  __shared__ uint32_t tgt_s[kPass * 4];                        // target bytes [texel][lane group]
 
   const int tid = threadIdx.x;
   const int wave = tid >> 6, lane = tid & 63;
@@ -420,29 +781,53 @@ __global__ __launch_bounds__(kThreads, 1) void k_fused_train(FusedParams P)
   const int64_t npass = (P.B + kPass - 1) / kPass;
   const int T = P.tl.tile_dim, M = P.tl.mip_dim;
   const int D0 = P.fd.g0_dim, D1 = P.fd.g1_dim;
+  if (tid < 2 * P.tl.n_tiles) orig_s[tid] = (uint32_t)P.tl.origins[tid] % (uint32_t)M;
+  __syncthreads();
+
+  // Corner tables, staged footprint and target bytes of a pass are written
+  // into LDS by the previous pass, after its decoder-gradient rounds (their
+  // loads are issued before the rounds); the first pass's here.
+  auto fill_tables = [&](const PassGeo& pg, AxisTab* tab) {
+    if (pg.pm.blocked)
+    {
+      if (tid < 128)
+      {
+        if (pg.b0.fits) fill_axis_tab<CF::BH>(tab[0], pg.b0, pg.xu0, pg.yu0, M, D0, tid);
+      }
+      else if (pg.b1.fits)
+        fill_axis_tab<CF::BH>(tab[1], pg.b1, pg.xu0, pg.yu0, M, D1, tid - 128);
+    }
+  };
+  PassGeo cur = pass_geo<CF>((int64_t)blockIdx.x, orig_s, T, M, D0, D1);
+  int tb = 0;  // table pair of the current pass
+  {
+    Prefetch<CF::PF> pf;
+    uint32_t tg[kNT][4];
+    prefetch_issue<CF, C0, C1>(pf, cur, P, tid);
+    targets_issue<CF>(tg, cur.pm, (int64_t)blockIdx.x, P, orig_s, wave, n16, g);
+    fill_tables(cur, atab);
+    prefetch_commit<CF, C0, C1>(pf, P, sbox);
+    targets_commit(tg, tgt_s, wave, n16, g);
+  }
+
   for (int64_t pass = blockIdx.x; pass < npass; pass += gridDim.x)
   {
-    __syncthreads();  // previous pass done with the union region and boxes; weights visible on the first pass
-    const PassMap pm = pass_map(pass, T, kPass);
-    Box bx0 = {0, 0, 0, 0, false}, bx1 = {0, 0, 0, 0, false};
-    if (pm.blocked && P.scatter)
-    {
-      const int xu0 = (int)((uint32_t)P.tl.origins[2 * pm.b] % (uint32_t)M) + pm.bx0;
-      const int yu0 = (int)((uint32_t)P.tl.origins[2 * pm.b + 1] % (uint32_t)M) + pm.by0;
-      bx0 = make_box(xu0, xu0 + 15, yu0, yu0 + CF::BH - 1, M, D0, CF::W0MAX, CF::V0MAX);
-      bx1 = make_box(xu0, xu0 + 15, yu0, yu0 + CF::BH - 1, M, D1, CF::W1MAX, CF::V1MAX);
-      if (bx0.fits)
-        for (int e = tid; e < bx0.w * bx0.h * C0; e += kThreads) box0[e] = 0.0f;
-      if (bx1.fits)
-        for (int e = tid; e < bx1.w * bx1.h * C1; e += kThreads) box1[e] = 0.0f;
-    }
+    __syncthreads();  // this pass's tables, staged boxes and targets are in LDS
+    const PassMap pm = cur.pm;
+    const Box bx0 = cur.b0, bx1 = cur.b1;
+    const AxisTab* tab = atab + 2 * tb;
     // ---- P0: gather ----
+    if (cur.staged)
     {
-      // threads [0,128) gather g0 slots, [128,256) the rest; NT passes of 64 texels
+      gather_staged<CF, C0, C1, G0MODE, PEC, LODS>(P, cur, tab, sbox, tid, xnf, xfn);
+    }
+    else
+    {
+      // threads [0,128) gather g0 slots, [128,256) the rest
       const int part = tid >> 7;
       for (int tl = tid & 127; tl < kPass; tl += 128)
       {
-        const PassTexel pt = pass_texel(pm, pass, tl, kPass, P.tl.origins, T, M, P.B);
+        const PassTexel pt = pass_texel(pm, pass, tl, kPass, orig_s, T, M, P.B);
         gather_part<CF, C0, C1, G0MODE, PEC, LODS>(P, part, tl, pt.valid, pt.x, pt.y, xnf, xfn);
       }
     }
@@ -451,17 +836,15 @@ __global__ __launch_bounds__(kThreads, 1) void k_fused_train(FusedParams P)
     // ---- P1: per-wave forward + backward for 32 texels ----
     int tloc[kNT];
     bool tvalid[kNT];
-    int tx[kNT], ty[kNT], wx[kNT], wy[kNT];
+    int tx[kNT], ty[kNT];
 #pragma unroll
     for (int nt = 0; nt < kNT; ++nt)
     {
       tloc[nt] = wave * 16 * kNT + nt * 16 + n16;
-      const PassTexel pt = pass_texel(pm, pass, tloc[nt], kPass, P.tl.origins, T, M, P.B);
+      const PassTexel pt = pass_texel(pm, pass, tloc[nt], kPass, orig_s, T, M, P.B);
       tvalid[nt] = pt.valid;
       tx[nt] = pt.x;
       ty[nt] = pt.y;
-      wx[nt] = pt.xu >= M;  // texel wrapped past the mip edge
-      wy[nt] = pt.yu >= M;
     }
     // layer 0
     f32x4 z0[MT][kNT];
@@ -470,7 +853,7 @@ __global__ __launch_bounds__(kThreads, 1) void k_fused_train(FusedParams P)
 #pragma unroll
       for (int j = 0; j < KF; ++j)
 #pragma unroll
-        for (int nt = 0; nt < kNT; ++nt) xb[j][nt] = frag_lds(&xnf[tloc[nt] * FP + 32 * j + 8 * g]);
+        for (int nt = 0; nt < kNT; ++nt) xb[j][nt] = frag_lds(&xnf[tloc[nt] * CF::XS + 32 * j + 8 * g]);
 #pragma unroll
       for (int r = 0; r < MT; ++r)
       {
@@ -540,7 +923,6 @@ __global__ __launch_bounds__(kThreads, 1) void k_fused_train(FusedParams P)
 #pragma unroll
       for (int j = 0; j < KH; ++j)
         yh = mfma(frag_lds(&wt[TLY.oWo() + n16 * H + 32 * j + 8 * g]), a1f[j][nt], yh);
-      const int64_t base = ((int64_t)ty[nt] * M + tx[nt]) * P.O;
       float v[8];
 #pragma unroll
       for (int i = 0; i < 4; ++i)
@@ -549,7 +931,7 @@ __global__ __launch_bounds__(kThreads, 1) void k_fused_train(FusedParams P)
         float d = 0.0f;
         if (tvalid[nt] && o < P.O)
         {
-          const float yt = (float)P.pkg[base + o] / 255.0f;
+          const float yt = (float)((tgt_s[tloc[nt] * 4 + g] >> (8 * i)) & 0xFFu) / 255.0f;
           const float r = yt - yh[i];
           sse += (double)r * (double)r;
           d = P.scale * r;
@@ -628,8 +1010,9 @@ __global__ __launch_bounds__(kThreads, 1) void k_fused_train(FusedParams P)
           dz0f[j][nt] = frag_pack(v);
         }
     }
-    // dX for the grid slots, scattered into the active level's gradients:
-    // through the LDS boxes when the pass footprint fits, else directly
+    // dX for the grid slots: kept per texel in LDS for the box gradients of
+    // grids whose pass footprint fits a box, else scattered per corner with
+    // global atomics
     if (P.scatter)
     {
 #pragma unroll
@@ -638,26 +1021,7 @@ __global__ __launch_bounds__(kThreads, 1) void k_fused_train(FusedParams P)
         if (!tvalid[nt]) continue;
         const Footprint<float> f0 = footprint<float>(tx[nt], ty[nt], M, D0);
         const Footprint<float> f1 = footprint<float>(tx[nt], ty[nt], M, D1);
-        // box-local offsets of the 4 corners (-1: outside, use a global atomic)
-        int o0[4], o1[4];
-        {
-          const int ax0 = corner0(tx[nt], M, D0) + (wx[nt] ? D0 : 0) - bx0.u0;
-          const int ax1 = corner1(tx[nt], M, D0) + (wx[nt] ? D0 : 0) - bx0.u0;
-          const int ay0 = corner0(ty[nt], M, D0) + (wy[nt] ? D0 : 0) - bx0.v0;
-          const int ay1 = corner1(ty[nt], M, D0) + (wy[nt] ? D0 : 0) - bx0.v0;
-          const int bx_0 = corner0(tx[nt], M, D1) + (wx[nt] ? D1 : 0) - bx1.u0;
-          const int bx_1 = corner1(tx[nt], M, D1) + (wx[nt] ? D1 : 0) - bx1.u0;
-          const int by_0 = corner0(ty[nt], M, D1) + (wy[nt] ? D1 : 0) - bx1.v0;
-          const int by_1 = corner1(ty[nt], M, D1) + (wy[nt] ? D1 : 0) - bx1.v0;
-#pragma unroll
-          for (int c = 0; c < 4; ++c)
-          {
-            const int u = (c & 1) ? ax1 : ax0, v = (c >> 1) ? ay1 : ay0;
-            o0[c] = (bx0.fits && (unsigned)u < (unsigned)bx0.w && (unsigned)v < (unsigned)bx0.h) ? (v * bx0.w + u) * C0 : -1;
-            const int u1 = (c & 1) ? bx_1 : bx_0, v1 = (c >> 1) ? by_1 : by_0;
-            o1[c] = (bx1.fits && (unsigned)u1 < (unsigned)bx1.w && (unsigned)v1 < (unsigned)bx1.h) ? (v1 * bx1.w + u1) * C1 : -1;
-          }
-        }
+        float* drow = dxs + tloc[nt] * CF::GS;
 #pragma unroll
         for (int r = 0; r < GT; ++r)
         {
@@ -671,36 +1035,30 @@ __global__ __launch_bounds__(kThreads, 1) void k_fused_train(FusedParams P)
             const int slot = 16 * r + 4 * g + i;
             if (slot < CF::G1B)
             {
-              if (G0MODE == NTC_G0_CORNERS)
+              if (bx0.fits)
+                drow[slot] = dx[i];
+              else if (G0MODE == NTC_G0_CORNERS)
               {
                 const int c = slot / C0, ch = slot - (slot / C0) * C0;
-                if (o0[c] >= 0)
-                  atomicAdd(&box0[o0[c] + ch], dx[i]);
-                else
-                  atomicAdd(&P.dg0[(int64_t)f0.cell[c] * C0 + ch], dx[i]);
+                // corner c is lane-dependent: select it instead of indexing
+                const int cell = c == 0 ? f0.cell[0] : (c == 1 ? f0.cell[1] : (c == 2 ? f0.cell[2] : f0.cell[3]));
+                atomicAdd(&P.dg0[(int64_t)cell * C0 + ch], dx[i]);
               }
               else
               {
 #pragma unroll
-                for (int c = 0; c < 4; ++c)
-                {
-                  if (o0[c] >= 0)
-                    atomicAdd(&box0[o0[c] + slot], f0.w[c] * dx[i]);
-                  else
-                    atomicAdd(&P.dg0[(int64_t)f0.cell[c] * C0 + slot], f0.w[c] * dx[i]);
-                }
+                for (int c = 0; c < 4; ++c) atomicAdd(&P.dg0[(int64_t)f0.cell[c] * C0 + slot], f0.w[c] * dx[i]);
               }
             }
             else if (slot < CF::GSLOTS)
             {
-              const int ch = slot - CF::G1B;
-#pragma unroll
-              for (int c = 0; c < 4; ++c)
+              if (bx1.fits)
+                drow[slot] = dx[i];
+              else
               {
-                if (o1[c] >= 0)
-                  atomicAdd(&box1[o1[c] + ch], f1.w[c] * dx[i]);
-                else
-                  atomicAdd(&P.dg1[(int64_t)f1.cell[c] * C1 + ch], f1.w[c] * dx[i]);
+                const int ch = slot - CF::G1B;
+#pragma unroll
+                for (int c = 0; c < 4; ++c) atomicAdd(&P.dg1[(int64_t)f1.cell[c] * C1 + ch], f1.w[c] * dx[i]);
               }
             }
           }
@@ -708,28 +1066,16 @@ __global__ __launch_bounds__(kThreads, 1) void k_fused_train(FusedParams P)
       }
     }
 
-    __syncthreads();  // everyone done reading xnf; boxes complete
-    // one global atomic per touched (cell, channel) of each box
-    if (bx0.fits)
-      for (int e = tid; e < bx0.w * bx0.h * C0; e += kThreads)
-      {
-        const float v = box0[e];
-        if (v == 0.0f) continue;
-        const int ch = e % C0, cell = e / C0;
-        const int u = bx0.u0 + cell % bx0.w, w = bx0.v0 + cell / bx0.w;
-        const int cx = u >= D0 ? u - D0 : u, cy = w >= D0 ? w - D0 : w;
-        atomicAdd(&P.dg0[((int64_t)cy * D0 + cx) * C0 + ch], v);
-      }
-    if (bx1.fits)
-      for (int e = tid; e < bx1.w * bx1.h * C1; e += kThreads)
-      {
-        const float v = box1[e];
-        if (v == 0.0f) continue;
-        const int ch = e % C1, cell = e / C1;
-        const int u = bx1.u0 + cell % bx1.w, w = bx1.v0 + cell / bx1.w;
-        const int cx = u >= D1 ? u - D1 : u, cy = w >= D1 ? w - D1 : w;
-        atomicAdd(&P.dg1[((int64_t)cy * D1 + cx) * C1 + ch], v);
-      }
+    __syncthreads();  // everyone done reading xnf; dX rows complete
+    // next pass's footprint and targets: the loads land during the rounds below
+    const int64_t next = pass + gridDim.x;
+    const int64_t pn = next < npass ? next : pass;
+    PassGeo nxt = pass_geo<CF>(pn, orig_s, T, M, D0, D1);
+    nxt.staged = nxt.staged && next < npass;
+    Prefetch<CF::PF> pf;
+    uint32_t tg[kNT][4];
+    prefetch_issue<CF, C0, C1>(pf, nxt, P, tid);
+    targets_issue<CF>(tg, nxt.pm, pn, P, orig_s, wave, n16, g);
 #pragma unroll
     for (int rnd = 0; rnd < kNT; ++rnd)
     {
@@ -801,6 +1147,19 @@ __global__ __launch_bounds__(kThreads, 1) void k_fused_train(FusedParams P)
       }
       __syncthreads();
     }
+    // the rounds ended with a barrier: the union region and the target words
+    // are free for the next pass (the other table pair is unused); then the
+    // latent gradients of this pass (its dX rows are complete since the box
+    // barrier; the next pass rewrites them after its first barrier)
+    fill_tables(nxt, atab + 2 * (tb ^ 1));
+    prefetch_commit<CF, C0, C1>(pf, P, sbox);
+    targets_commit(tg, tgt_s, wave, n16, g);
+    if (P.scatter && bx0.fits)
+      box_gradients<C0, G0MODE == NTC_G0_CORNERS, CF::GS, kPass>(tab[0], bx0, dxs, 0, P.dg0, D0, wave, lane);
+    if (P.scatter && bx1.fits)
+      box_gradients<C1, false, CF::GS, kPass>(tab[1], bx1, dxs, CF::G1B, P.dg1, D1, wave, lane);
+    cur = nxt;
+    tb ^= 1;
   }
 
   // ---- partial decoder gradients of this workgroup: W0[H][F] | W1[H][H] | Wout[16][H]

@@ -89,16 +89,24 @@ struct Footprint
   T w[4];
 };
 
+// One axis of the footprint: unclamped first corner x0 and fraction f.
+template <typename T>
+__device__ __forceinline__ void footprint_axis(int x, int M, int D, int& x0, T& f)
+{
+  const T u = ((T)x + (T)0.5) / (T)M;
+  const T g = u * (T)D - (T)0.5;
+  const T fl = floor(g);
+  x0 = (int)fl;
+  f = g - fl;
+}
+
 template <typename T>
 __device__ __forceinline__ Footprint<T> footprint(int x, int y, int M, int D)
 {
-  const T u = ((T)x + (T)0.5) / (T)M;
-  const T v = ((T)y + (T)0.5) / (T)M;
-  const T gx = u * (T)D - (T)0.5;
-  const T gy = v * (T)D - (T)0.5;
-  const T flx = floor(gx), fly = floor(gy);
-  const int x0 = (int)flx, y0 = (int)fly;
-  const T fx = gx - flx, fy = gy - fly;
+  int x0, y0;
+  T fx, fy;
+  footprint_axis<T>(x, M, D, x0, fx);
+  footprint_axis<T>(y, M, D, y0, fy);
   Footprint<T> f;
   const T one = (T)1;
   f.w[0] = (one - fx) * (one - fy);
