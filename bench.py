@@ -18,6 +18,7 @@ threads) on a bounded sample of steps.
 from __future__ import annotations
 
 import argparse
+import ctypes
 import json
 import os
 import re
@@ -71,6 +72,35 @@ def step_texels(ctx, first, count):
         tot += si.n_tiles * si.tile_dim * si.tile_dim
         infos.append(si)
     return tot, infos
+
+
+def _axis_corners(pu, M, D):
+    """Unwrapped bilinear corner cells of unwrapped texel coordinate pu, in
+    float32 arithmetic as the kernels compute them."""
+    wrapped = pu >= M
+    x = pu - M if wrapped else pu
+    u = (np.float32(x) + np.float32(0.5)) / np.float32(M)
+    g = u * np.float32(D) - np.float32(0.5)
+    x0 = int(np.floor(g))
+    add = D if wrapped else 0
+    return min(max(x0, 0), D - 1) + add, min(max(x0 + 1, 0), D - 1) + add
+
+
+def footprint_cells(si, M, D):
+    """Cells of a D^2 grid inside the union of the step's tile footprints."""
+    mask = np.zeros((D, D), bool)
+    T = si.tile_dim
+    for b in range(si.n_tiles):
+        ox, oy = si.origins[2 * b] % M, si.origins[2 * b + 1] % M
+        rx, ry = [], []
+        for o, r in ((ox, rx), (oy, ry)):
+            a, _ = _axis_corners(o, M, D)
+            _, d = _axis_corners(o + T - 1, M, D)
+            r.extend([a, min(d - a + 1, D)])
+        xs = (rx[0] + np.arange(rx[1])) % D
+        ys = (ry[0] + np.arange(ry[1])) % D
+        mask[np.ix_(ys, xs)] = True
+    return int(mask.sum())
 
 
 def cpu_baseline(ctx, first_step, n_steps):
@@ -205,36 +235,63 @@ def main():
     }
 
     if a.path == "fused":
-        times = (C_float := __import__("ctypes").c_float * 5)()
-        lib.ntc_ctx_kernel_times(ctx._h, times, 5)
-        fused_ms, dec_ms, grid_ms, n_t, n_grid = list(times)
-        # algorithmic work per launch, averaged over the timed steps
-        flops = [6.0 * (F * H + H * H + H * O) * si.n_tiles * si.tile_dim ** 2 for si in infos]
-        params = []
+        times = (ctypes.c_float * 6)()
+        lib.ntc_ctx_kernel_times(ctx._h, times, 6)
+        fused_ms, dec_ms, grid_ms, n_t, n_grid, n_tiles_upd = list(times)
+        tiles_upd = n_grid > 0 and n_tiles_upd == n_grid
+        if n_grid > 0 and 0 < n_tiles_upd < n_grid:
+            raise SystemExit("timed steps mix the two grid-update kernels")
+        # algorithmic work per launch, averaged over the timed steps:
+        #   fused launch: 6 (F H + H H + H O) flop per texel; HBM bytes = the
+        #   footprint cells read once (4 B/param), targets (O B/texel) and the
+        #   decoder-gradient partials;
+        #   grid update: 32 B/param (p, m, v, g read; p, m, v, g = 0 written)
+        #   inside the tile footprints and 24 B/param (g untouched: zero)
+        #   outside them, or 32 B/param over the level (k_grid_update2)
+        n_dec = H * F + H * H + 16 * H
+        flops, fbytes, gbytes = [], [], []
         for si in infos:
-            if si.grid_update:
-                params.append(lay.g0_dim[si.level] ** 2 * lay.g0_channels + lay.g1_dim[si.level] ** 2 * lay.g1_channels)
+            B = si.n_tiles * si.tile_dim ** 2
+            M = DIM >> si.lod
+            D0, D1 = lay.g0_dim[si.level], lay.g1_dim[si.level]
+            pl = D0 * D0 * lay.g0_channels + D1 * D1 * lay.g1_channels
+            pf = footprint_cells(si, M, D0) * lay.g0_channels + footprint_cells(si, M, D1) * lay.g1_channels
+            blocks = min(256, (B + 63) // 64)
+            fb = 4.0 * pf + O * B + 4.0 * blocks * n_dec
+            flops.append(6.0 * (F * H + H * H + H * O) * B)
+            if si.grid_update and tiles_upd:
+                gbytes.append(24.0 * (pl - pf) + 32.0 * pf)
+            elif si.grid_update:
+                gbytes.append(32.0 * pl)
+            fbytes.append(fb)
         fused_tflops = np.mean(flops) / (fused_ms * 1e-3) / 1e12
-        kern = {"fused_train_step": {"ms": fused_ms, "achieved": fused_tflops, "unit": "TFLOP/s",
-                                     "peak": BF16_PEAK_TFLOPS, "frac": fused_tflops / BF16_PEAK_TFLOPS,
-                                     "flop_per_launch": float(np.mean(flops))},
+        fused_gbs = np.mean(fbytes) / (fused_ms * 1e-3) / 1e9
+        kern = {"fused_train_step": {"ms": fused_ms, "tflops": fused_tflops, "mfma_frac": fused_tflops / BF16_PEAK_TFLOPS,
+                                     "GBps": fused_gbs, "hbm_frac": fused_gbs / HBM_PEAK_GBS,
+                                     "flop_per_launch": float(np.mean(flops)),
+                                     "bytes_per_launch": float(np.mean(fbytes))},
                 "decoder_update": {"ms": dec_ms}}
-        if params:
-            grid_bytes = 32.0 * float(np.mean(params))
-            grid_gbs = grid_bytes / (grid_ms * 1e-3) / 1e9
-            kern["grid_update"] = {"ms": grid_ms, "achieved": grid_gbs, "unit": "GB/s", "peak": HBM_PEAK_GBS,
-                                   "frac": grid_gbs / HBM_PEAK_GBS, "bytes_per_launch": grid_bytes}
+        if gbytes:
+            grid_gbs = float(np.mean(gbytes)) / (grid_ms * 1e-3) / 1e9
+            kern["grid_update"] = {"kernel": "k_grid_update_tiles" if tiles_upd else "k_grid_update2",
+                                   "ms": grid_ms, "achieved": grid_gbs, "unit": "GB/s", "peak": HBM_PEAK_GBS,
+                                   "frac": grid_gbs / HBM_PEAK_GBS, "bytes_per_launch": float(np.mean(gbytes))}
         out["kernels"] = kern
-        if params and grid_ms >= fused_ms:
+        if gbytes and grid_ms >= fused_ms:
             g = kern["grid_update"]
-            out["roofline"] = {"bound": "hbm", "kernel": "k_grid_update2", "achieved": g["achieved"],
+            out["roofline"] = {"bound": "hbm", "kernel": g["kernel"], "achieved": g["achieved"],
                                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": g["frac"],
-                               "traffic": pmc_traffic("k_grid_update2")}
+                               "traffic": pmc_traffic(g["kernel"])}
         else:
             f = kern["fused_train_step"]
-            out["roofline"] = {"bound": "mfma", "kernel": "k_fused_train", "achieved": f["achieved"],
-                               "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": f["frac"],
-                               "traffic": pmc_traffic("k_fused_train")}
+            if f["hbm_frac"] >= f["mfma_frac"]:
+                out["roofline"] = {"bound": "hbm", "kernel": "k_fused_train", "achieved": fused_gbs,
+                                   "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": f["hbm_frac"],
+                                   "traffic": pmc_traffic("k_fused_train")}
+            else:
+                out["roofline"] = {"bound": "mfma", "kernel": "k_fused_train", "achieved": fused_tflops,
+                                   "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": f["mfma_frac"],
+                                   "traffic": pmc_traffic("k_fused_train")}
 
     if not a.no_cpu_baseline:
         log(f"cpu baseline: {a.cpu_steps} oracle steps")

@@ -192,6 +192,29 @@ int ntc_quantize_pack_f32(int64_t n, int32_t bits, const float* in,
 /* max_abs (ntc_kernel.cu:511-531): *result = max(*result, max|x|) */
 int ntc_max_abs_f32(int64_t n, const float* x, float* result,
                     hipStream_t stream);
+/* Adam state and constants of the active level's two grids (update_adam +
+ * clamp_grid, ntc_kernel.cu:375-410 and 468-482). */
+typedef struct ntc_grid_adam
+{
+  float* m0; /* g0 moments */
+  float* v0;
+  float* m1; /* g1 moments */
+  float* v1;
+  float lr, beta1, beta2, bias1, bias2;
+  float lo0, hi0; /* clamp ranges of g0 and g1 */
+  float lo1, hi1;
+} ntc_grid_adam;
+
+/* The dense Adam + clamp + gradient reset of ntc_grid_update2_f32 over the
+ * whole level of g0 (fd->g0_dim^2 x g0_channels) and g1, for gradients that
+ * are zero outside the tile footprints (the latent scatter of one step):
+ * cells outside every tile's footprint rectangle take the zero-gradient step
+ * without reading or writing dg, 24 instead of 32 bytes per parameter.
+ * Same result as ntc_grid_update2_f32, bit for bit. Channel counts must be
+ * multiples of 4 and n_tiles <= 16. */
+int ntc_grid_update_tiles_f32(const ntc_feature_desc* fd, const ntc_tiles* tiles,
+                              float* g0, float* dg0, float* g1, float* dg1,
+                              const ntc_grid_adam* adam, hipStream_t stream);
 /* zero-fill by kernel (graph-safe replacement for memset) */
 int ntc_fill_zero(void* p, size_t bytes, hipStream_t stream);
 

@@ -39,6 +39,13 @@ class Noise(C.Structure):
                 ("enabled", C.c_int32), ("delta0", C.c_float), ("delta1", C.c_float)]
 
 
+class GridAdam(C.Structure):
+    _fields_ = [("m0", C.c_void_p), ("v0", C.c_void_p), ("m1", C.c_void_p), ("v1", C.c_void_p),
+                ("lr", C.c_float), ("beta1", C.c_float), ("beta2", C.c_float), ("bias1", C.c_float),
+                ("bias2", C.c_float), ("lo0", C.c_float), ("hi0", C.c_float), ("lo1", C.c_float),
+                ("hi1", C.c_float)]
+
+
 class FusedArgs(C.Structure):
     _fields_ = [("feat", FeatureDesc), ("tiles", Tiles), ("noise", Noise),
                 ("hidden_dim", C.c_int32), ("out_dim", C.c_int32), ("activation", C.c_int32),
@@ -173,6 +180,7 @@ SIGNATURES = {
     "ntc_fused_blocks_for": (_I, [_P, _I, _I, _I64, _I]),
     "ntc_pack_decoder_bf16": (_I, [_P, _I, _I, _P, _P, _P]),
     "ntc_fused_train_step": (_I, [_P, _P]),
+    "ntc_grid_update_tiles_f32": (_I, [_P, _P, _P, _P, _P, _P, _P, _P]),
     "ntc_decoder_update": (_I, [_I, _I, _I, _I, _P, _F, _F, _F, _F, _F, _P, _P, _P, _P, _P, _P, _P]),
 }
 

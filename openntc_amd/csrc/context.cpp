@@ -711,6 +711,13 @@ Result Context::Impl::RunFusedStep(const StepPlan& st, int32_t step_index, hipEv
   a.dw_partials = dw_part.as<float>();
   a.sse_partials = sse_part.as<double>();
   a.blocks = ntc_fused_blocks_for(&fd, H, act, B, max_blocks);
+  // dense Adam + clamp of the level: the gradient is zero outside the tile
+  // footprints, so the update reads and clears it only inside them
+  const ntc_grid_adam adam = {g0m[lvl].as<float>(), g0v[lvl].as<float>(), g1m[lvl].as<float>(),
+                              g1v[lvl].as<float>(), st.lr_grid, 0.9f, 0.999f, st.grid_bias1, st.grid_bias2,
+                              -1.0f, 1.0f - g0_delta, -1.0f, 1.0f - g1_delta};
+  const bool tiles_update = st.grid_update && g0_channels % 4 == 0 && g1_channels % 4 == 0 && st.n_tiles <= 16;
+  if (tiles_update) ++tiles_update_steps;
   if (ev) CTX_HIP(hipEventRecord(ev[0], stream), Result::CudaFailure);
   CTX_NTC(ntc_fused_train_step(&a, stream));
   if (ev) CTX_HIP(hipEventRecord(ev[1], stream), Result::CudaFailure);
@@ -723,11 +730,15 @@ Result Context::Impl::RunFusedStep(const StepPlan& st, int32_t step_index, hipEv
   if (st.grid_update)
   {
     if (ev) CTX_HIP(hipEventRecord(ev[3], stream), Result::CudaFailure);
-    CTX_NTC(ntc_grid_update2_f32(GridElems(0, lvl), dg0[lvl].as<float>(), g0m[lvl].as<float>(), g0v[lvl].as<float>(),
-                                 g0[lvl].as<float>(), -1.0f, 1.0f - g0_delta, GridElems(1, lvl),
-                                 dg1[lvl].as<float>(), g1m[lvl].as<float>(), g1v[lvl].as<float>(),
-                                 g1[lvl].as<float>(), -1.0f, 1.0f - g1_delta, st.lr_grid, 0.9f, 0.999f,
-                                 st.grid_bias1, st.grid_bias2, stream));
+    if (tiles_update)
+      CTX_NTC(ntc_grid_update_tiles_f32(&fd, &a.tiles, g0[lvl].as<float>(), dg0[lvl].as<float>(),
+                                        g1[lvl].as<float>(), dg1[lvl].as<float>(), &adam, stream));
+    else
+      CTX_NTC(ntc_grid_update2_f32(GridElems(0, lvl), dg0[lvl].as<float>(), g0m[lvl].as<float>(),
+                                   g0v[lvl].as<float>(), g0[lvl].as<float>(), -1.0f, 1.0f - g0_delta,
+                                   GridElems(1, lvl), dg1[lvl].as<float>(), g1m[lvl].as<float>(),
+                                   g1v[lvl].as<float>(), g1[lvl].as<float>(), -1.0f, 1.0f - g1_delta, st.lr_grid,
+                                   0.9f, 0.999f, st.grid_bias1, st.grid_bias2, stream));
     if (ev) CTX_HIP(hipEventRecord(ev[4], stream), Result::CudaFailure);
   }
   grid_batch_i[lvl] += 1;
@@ -1201,6 +1212,7 @@ TrainProgress Context::Train(int32_t num_batches)
     }
   }
   const int first = I.batch_i;
+  I.tiles_update_steps = 0;
   for (; I.batch_i < target; ++I.batch_i)
   {
     const StepPlan& st = I.plan[I.batch_i];
@@ -1227,7 +1239,7 @@ TrainProgress Context::Train(int32_t num_batches)
       }
     }
     I.kernel_ms = {(float)(fused / n), (float)(decoder / n), grids ? (float)(grid / grids) : -1.0f, (float)n,
-                   (float)grids};
+                   (float)grids, (float)I.tiles_update_steps};
   }
   tp.phase_ = TrainPhase::TrainInProgress;
   tp.result_ = Result::Success;

@@ -107,8 +107,11 @@ struct Context::Impl
   // When enabled, Train records events around the fused step kernel, the
   // decoder update (+ table pack) and the dense grid update of every step on
   // the context stream; kernel_ms holds {fused, decoder, grid, steps timed,
-  // grid-update launches timed} averaged over the last Train call.
+  // grid-update launches timed, grid updates that skipped the zero gradient
+  // outside the tile footprints} over the last Train call (durations
+  // averaged).
   bool timing = false;
+  int32_t tiles_update_steps = 0;
   std::vector<float> kernel_ms;
   std::vector<hipEvent_t> events;
 

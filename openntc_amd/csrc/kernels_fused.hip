@@ -259,19 +259,6 @@ __device__ __forceinline__ PassTexel pass_texel(const PassMap& pm, int64_t pass,
   return p;
 }
 
-// Bilinear corners of unwrapped texel coordinate pu (>= M after a wrap)
-// along one axis, as unwrapped cell coordinates (cell + D after a wrap)
-// minus o, and the fraction: the arithmetic of footprint().
-__device__ __forceinline__ void axis_corners(int pu, int M, int D, int o, int& c0, int& c1, float& f)
-{
-  const bool wrapped = pu >= M;
-  int x0;
-  footprint_axis<float>(wrapped ? pu - M : pu, M, D, x0, f);
-  const int add = (wrapped ? D : 0) - o;
-  c0 = clampi(x0, 0, D - 1) + add;
-  c1 = clampi(x0 + 1, 0, D - 1) + add;
-}
-
 // Cell box of one grid covering a blocked pass, in unwrapped cell coordinates.
 struct Box
 {

@@ -5,6 +5,8 @@
 //   k_grid_update   update_adam + clamp_grid (ntc_kernel.cu:468-482) + the
 //                   next step's gradient FillZero (libopenntc.cpp:695-696),
 //                   one HBM pass: reads p, m, v, g, writes p, m, v, g = 0.
+//   k_grid_update_tiles  the same for a gradient that is zero outside the
+//                   step's tile footprints: g is read and cleared only inside.
 //   k_quantize      ntc_kernel.cu:448-466 (quantize_grid)
 //   k_pack          ntc_kernel.cu:486-507 (quantize_pack)
 //   k_max_abs       ntc_kernel.cu:511-531 (max_abs)
@@ -18,19 +20,6 @@ namespace
 constexpr int kThreads = 256;
 
 typedef float f4 __attribute__((ext_vector_type(4)));
-
-struct AdamConst
-{
-  float lr, b1, b2, omb1, omb2, bias1, bias2;
-};
-
-// m = b1 m + (1-b1) g ; v = b2 v + ((1-b2) g) g ; p -= (lr (m bias1)) / (sqrt(v bias2) + 1e-8)
-__device__ __forceinline__ void adam_one(const AdamConst& c, float g, float& m, float& v, float& p)
-{
-  m = fmaf(c.b1, m, c.omb1 * g);
-  v = fmaf(c.b2, v, (c.omb2 * g) * g);
-  p = p - (c.lr * (m * c.bias1)) / (__builtin_sqrtf(v * c.bias2) + 1e-8f);
-}
 
 __global__ __launch_bounds__(kThreads) void k_adam(int64_t n, AdamConst c, const float* __restrict__ g,
                                                    float* __restrict__ m, float* __restrict__ v,
@@ -114,6 +103,62 @@ __global__ __launch_bounds__(kThreads) void k_grid_update2(int64_t a4, int64_t n
     __builtin_nontemporal_store(vi, &s.v[k]);
     __builtin_nontemporal_store(pi, &s.p[k]);
     __builtin_nontemporal_store(zero, &s.g[k]);
+  }
+}
+
+// The dense update for a gradient that is zero outside the tile footprints
+// (one step's latent scatter): cells outside every footprint rectangle take
+// the step with g = 0 without touching the gradient buffer. Channel counts
+// are multiples of 4, so a float4 item never straddles two cells.
+struct TileUpdate
+{
+  ntc_tiles tl;
+  int D[2], Q[2];  // grid sides, channel quads
+  int64_t a4, n4;  // float4 items of g0, of both grids
+  GridSeg s[2];
+  AdamConst c;
+};
+
+__global__ __launch_bounds__(kThreads) void k_grid_update_tiles(TileUpdate A)
+{
+  __shared__ int4 rect[2][16];
+  const int tid = threadIdx.x, n = A.tl.n_tiles;
+  const int M = A.tl.mip_dim, T = A.tl.tile_dim;
+  if (tid < 2 * n)
+  {
+    const int grid = tid >= n, b = grid ? tid - n : tid;
+    const int ox = (int)((uint32_t)A.tl.origins[2 * b] % (uint32_t)M);
+    const int oy = (int)((uint32_t)A.tl.origins[2 * b + 1] % (uint32_t)M);
+    rect[grid][b] = tile_rect(ox, oy, T, M, A.D[grid]);
+  }
+  __syncthreads();
+  const f4 zero = {0.f, 0.f, 0.f, 0.f};
+  for (int64_t i = (int64_t)blockIdx.x * kThreads + tid; i < A.n4; i += (int64_t)gridDim.x * kThreads)
+  {
+    const int grid = i >= A.a4;
+    const GridSeg& s = A.s[grid];
+    const int64_t k = grid ? i - A.a4 : i;
+    const int D = A.D[grid];
+    const uint32_t cell = (uint32_t)k / (uint32_t)A.Q[grid];
+    const int y = (int)(cell / (uint32_t)D), x = (int)cell - y * D;
+    const bool hot = in_rects(rect[grid], n, x, y, D);
+    f4 mi = __builtin_nontemporal_load(&s.m[k]);
+    f4 vi = __builtin_nontemporal_load(&s.v[k]);
+    f4 pi = __builtin_nontemporal_load(&s.p[k]);
+    const f4 gi = hot ? __builtin_nontemporal_load(&s.g[k]) : zero;
+#pragma unroll
+    for (int e = 0; e < 4; ++e)
+    {
+      float me = mi[e], ve = vi[e], pe = pi[e];
+      adam_one(A.c, gi[e], me, ve, pe);
+      mi[e] = me;
+      vi[e] = ve;
+      pi[e] = fmaxf(s.lo, fminf(s.hi, pe));
+    }
+    __builtin_nontemporal_store(mi, &s.m[k]);
+    __builtin_nontemporal_store(vi, &s.v[k]);
+    __builtin_nontemporal_store(pi, &s.p[k]);
+    if (hot) __builtin_nontemporal_store(zero, &s.g[k]);
   }
 }
 
@@ -265,6 +310,32 @@ int ntc_grid_update2_f32(int64_t n0, float* grad0, float* m0, float* v0, float* 
   const int64_t want = (n4 + kThreads - 1) / kThreads;
   const unsigned blocks = (unsigned)(want < 2048 ? want : 2048);
   hipLaunchKernelGGL(k_grid_update2, dim3(blocks), dim3(kThreads), 0, stream, n0 / 4, n4, c, s0, s1);
+  return NTC_LAUNCH_STATUS();
+}
+
+int ntc_grid_update_tiles_f32(const ntc_feature_desc* fd, const ntc_tiles* tiles, float* g0, float* dg0,
+                              float* g1, float* dg1, const ntc_grid_adam* adam, hipStream_t stream)
+{
+  NTC_RETURN_IF(!valid_feature_desc(fd) || !valid_tiles(tiles) || !adam, NTC_E_INVALID);
+  NTC_RETURN_IF(tiles->n_tiles > 16 || fd->g0_channels % 4 || fd->g1_channels % 4, NTC_E_INVALID);
+  NTC_RETURN_IF(!g0 || !dg0 || !g1 || !dg1 || !adam->m0 || !adam->v0 || !adam->m1 || !adam->v1, NTC_E_INVALID);
+  const uintptr_t bits = (uintptr_t)g0 | (uintptr_t)dg0 | (uintptr_t)g1 | (uintptr_t)dg1 | (uintptr_t)adam->m0 |
+                         (uintptr_t)adam->v0 | (uintptr_t)adam->m1 | (uintptr_t)adam->v1;
+  NTC_RETURN_IF(bits & 15, NTC_E_INVALID);
+  TileUpdate A;
+  A.tl = *tiles;
+  A.D[0] = fd->g0_dim;
+  A.D[1] = fd->g1_dim;
+  A.Q[0] = fd->g0_channels / 4;
+  A.Q[1] = fd->g1_channels / 4;
+  A.a4 = (int64_t)A.D[0] * A.D[0] * A.Q[0];
+  A.n4 = A.a4 + (int64_t)A.D[1] * A.D[1] * A.Q[1];
+  A.s[0] = GridSeg{(f4*)dg0, (f4*)adam->m0, (f4*)adam->v0, (f4*)g0, adam->lo0, adam->hi0};
+  A.s[1] = GridSeg{(f4*)dg1, (f4*)adam->m1, (f4*)adam->v1, (f4*)g1, adam->lo1, adam->hi1};
+  A.c = make_adam(adam->lr, adam->beta1, adam->beta2, adam->bias1, adam->bias2);
+  const int64_t want = (A.n4 + kThreads - 1) / kThreads;
+  const unsigned blocks = (unsigned)(want < 2048 ? want : 2048);
+  hipLaunchKernelGGL(k_grid_update_tiles, dim3(blocks), dim3(kThreads), 0, stream, A);
   return NTC_LAUNCH_STATUS();
 }
 

@@ -123,6 +123,53 @@ __device__ __forceinline__ Footprint<T> footprint(int x, int y, int M, int D)
   return f;
 }
 
+// Bilinear corners of unwrapped texel coordinate pu (>= M after a wrap)
+// along one axis, as unwrapped cell coordinates (cell + D after a wrap)
+// minus o, and the fraction: the arithmetic of footprint().
+__device__ __forceinline__ void axis_corners(int pu, int M, int D, int o, int& c0, int& c1, float& f)
+{
+  const bool wrapped = pu >= M;
+  int x0;
+  footprint_axis<float>(wrapped ? pu - M : pu, M, D, x0, f);
+  const int add = (wrapped ? D : 0) - o;
+  c0 = clampi(x0, 0, D - 1) + add;
+  c1 = clampi(x0 + 1, 0, D - 1) + add;
+}
+
+// Cell rectangle (x0, len_x, y0, len_y), mod D, covered by the footprints of
+// a tile's texels [ox, ox + T) x [oy, oy + T) (ox, oy in [0, M), T <= M).
+__device__ __forceinline__ int4 tile_rect(int ox, int oy, int T, int M, int D)
+{
+  int a, b, c, d;
+  float f;
+  int4 r;
+  axis_corners(ox, M, D, 0, a, b, f);
+  axis_corners(ox + T - 1, M, D, 0, c, d, f);
+  r.x = a;
+  r.y = min(d - a + 1, D);
+  axis_corners(oy, M, D, 0, a, b, f);
+  axis_corners(oy + T - 1, M, D, 0, c, d, f);
+  r.z = a;
+  r.w = min(d - a + 1, D);
+  return r;
+}
+
+// Whether cell (x, y) lies in one of the first n cell rectangles of a grid
+// of side D.
+__device__ __forceinline__ bool in_rects(const int4* r, int n, int x, int y, int D)
+{
+  bool in = false;
+  for (int b = 0; b < n; ++b)
+  {
+    const int4 q = r[b];
+    int dx = x - q.x, dy = y - q.z;
+    dx += dx < 0 ? D : 0;
+    dy += dy < 0 ? D : 0;
+    in = in || (dx < q.y && dy < q.w);
+  }
+  return in;
+}
+
 // Triangle-wave positional encoding value (ntc_kernel.cu:153-158).
 template <typename T>
 __device__ __forceinline__ T tri_wave(int coord, int P, int h)
@@ -148,5 +195,21 @@ __device__ __forceinline__ T act_bwd(int act, T z, T g)
   if (act == NTC_ACT_HARDGELU)
     return z < (T)-1.5 ? (T)0 : (z < (T)1.5 ? (((T)2 * z + (T)1.5) * g) / (T)3 : g);
   return z > (T)0 ? g : (T)0;
+}
+
+// Adam constants of one update (bias1/bias2 are the reciprocal bias
+// corrections 1 / (1 - beta^t)).
+struct AdamConst
+{
+  float lr, b1, b2, omb1, omb2, bias1, bias2;
+};
+
+// One Adam step (ntc_kernel.cu:375-410):
+// m = b1 m + (1-b1) g ; v = b2 v + ((1-b2) g) g ; p -= (lr (m bias1)) / (sqrt(v bias2) + 1e-8)
+__device__ __forceinline__ void adam_one(const AdamConst& c, float g, float& m, float& v, float& p)
+{
+  m = fmaf(c.b1, m, c.omb1 * g);
+  v = fmaf(c.b2, v, (c.omb2 * g) * g);
+  p = p - (c.lr * (m * c.bias1)) / (__builtin_sqrtf(v * c.bias2) + 1e-8f);
 }
 } // namespace ntc

@@ -213,3 +213,73 @@ def test_fused_context_training_tracks_plain(oracle):
         psnr[path] = ctx.Eval().psnr
     assert psnr[ntc.TrainPath.Fused] > 15.0
     assert abs(psnr[ntc.TrainPath.Fused] - psnr[ntc.TrainPath.Plain]) < 0.5, psnr
+
+
+
+def test_grid_update_tiles_bitexact(oracle):
+    """The footprint-aware dense update (gradient read and cleared only inside
+    the tile footprints) leaves the level exactly as ntc_grid_update2_f32 does
+    for the gradients of a fused scatter: 2048^2 mip, g0 1024^2 and g1 512^2 x
+    16 channels, three tiles (one wrapping both edges, one overlapping it)."""
+    O = oracle
+    Lay = O.layout_custom(4, 4, 16, 16, 2, 128, 1, 1, 2048)
+    act = 1
+    Lay.activation = act
+    lib = G.lib()
+    rng = np.random.default_rng(11)
+    fd = O.feature_desc(Lay, 0)
+    M, T, H = Lay.mip_dim[0], 256, 128
+    origins = np.array([M - 100, M - 37, M - 20, 150, 517, 1203], np.int32)
+    n_tiles = len(origins) // 2
+    fdc = G.feature_desc(fd)
+    shapes = [(fd["g0_dim"], fd["g0_dim"], fd["c0"]), (fd["g1_dim"], fd["g1_dim"], fd["c1"])]
+    init = [(rng.uniform(-1, 1, shp).astype(np.float32), (rng.standard_normal(shp) * 1e-3).astype(np.float32),
+             rng.uniform(0, 1e-6, shp).astype(np.float32)) for shp in shapes]
+    F = Lay.feature_dim
+    dec = np.concatenate([(rng.uniform(-1, 1, H * F) / np.sqrt(F)).astype(np.float32),
+                          (rng.uniform(-1, 1, H * H) / np.sqrt(H)).astype(np.float32),
+                          (rng.uniform(-1, 1, 16 * H) / np.sqrt(H)).astype(np.float32)])
+    d_dec = G.dev(dec)
+    tab = G.empty((lib.ntc_fused_table_elems(G.byref(fdc), H),), np.uint16)
+    G.ok(lib.ntc_pack_decoder_bf16(G.byref(fdc), H, 9, G.p(d_dec), G.p(tab), None))
+    d_pkg = G.dev(rng.integers(0, 256, (M, M, 9)).astype(np.uint8))
+    maxb = C.c_int32()
+    G.ok(lib.ntc_fused_max_blocks(C.byref(maxb)))
+    B = n_tiles * T * T
+    blocks = lib.ntc_fused_blocks_for(G.byref(fdc), H, act, B, maxb.value)
+    part = G.empty((blocks * dec.size,), np.float32)
+    ssep = G.empty((blocks,), np.float64)
+    sa = [[G.dev(x) for x in st] + [G.empty(st[0].shape, np.float32)] for st in init]  # p, m, v, g
+    sb = [[G.dev(x) for x in st] + [G.empty(st[0].shape, np.float32)] for st in init]
+    og = G.dev(origins)
+    a = L.FusedArgs()
+    a.feat = fdc
+    a.tiles = G.tiles(og, n_tiles, T, M, 0.0)
+    a.noise = L.Noise(123, 7, 0, 1, Lay.g0_delta, Lay.g1_delta)
+    a.hidden_dim, a.out_dim, a.activation = H, 9, act
+    a.g0, a.g1, a.weights, a.targets = G.p(sa[0][0]), G.p(sa[1][0]), G.p(tab), G.p(d_pkg)
+    a.loss_scale = -2.0 / (9 * B)
+    a.scatter = 1
+    a.dg0, a.dg1, a.dw_partials, a.sse_partials, a.blocks = G.p(sa[0][3]), G.p(sa[1][3]), G.p(part), G.p(ssep), blocks
+    G.ok(lib.ntc_fused_train_step(G.byref(a), None), "ntc_fused_train_step")
+    grads = [G.host(sa[0][3]), G.host(sa[1][3])]
+    assert np.count_nonzero(grads[0]) > 0 and np.count_nonzero(grads[1]) > 0
+    for k in range(2):
+        G.ok(lib.ntc_dev_upload(G.p(sb[k][3]), grads[k].ctypes.data, grads[k].nbytes), "upload")
+    lr, b1c, b2c = 0.01, 1.0 / (1 - 0.9 ** 3), 1.0 / (1 - 0.999 ** 3)
+    lo0, hi0, lo1, hi1 = -1.0, 1.0 - Lay.g0_delta, -1.0, 1.0 - Lay.g1_delta
+    n = [int(np.prod(s)) for s in shapes]
+    G.ok(lib.ntc_grid_update2_f32(n[0], G.p(sa[0][3]), G.p(sa[0][1]), G.p(sa[0][2]), G.p(sa[0][0]), lo0, hi0,
+                                  n[1], G.p(sa[1][3]), G.p(sa[1][1]), G.p(sa[1][2]), G.p(sa[1][0]), lo1, hi1,
+                                  lr, 0.9, 0.999, b1c, b2c, None))
+    ad = L.GridAdam(G.p(sb[0][1]), G.p(sb[0][2]), G.p(sb[1][1]), G.p(sb[1][2]), lr, 0.9, 0.999, b1c, b2c,
+                    lo0, hi0, lo1, hi1)
+    G.ok(lib.ntc_grid_update_tiles_f32(G.byref(fdc), G.byref(a.tiles), G.p(sb[0][0]), G.p(sb[0][3]),
+                                       G.p(sb[1][0]), G.p(sb[1][3]), G.byref(ad), None))
+    for k in range(2):
+        for j, name in enumerate("pmvg"):
+            got, want = G.host(sb[k][j]), G.host(sa[k][j])
+            assert np.array_equal(got, want), (k, name, int(np.count_nonzero(got != want)))
+        assert not G.host(sb[k][3]).any()
+    # every cell moved
+    assert np.count_nonzero(G.host(sb[0][0]) == init[0][0]) < 0.01 * n[0]
