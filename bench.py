@@ -148,6 +148,51 @@ def pmc_traffic(kernel):
     return None
 
 
+def time_to_psnr(ntc, chunk):
+    """Seconds of training for the fused BF16 path to reach the plain FP32
+    path's final PSNR - 0.05 dB on the same workload and seed. Both paths
+    run the whole 8000-step schedule; the fused one is evaluated every
+    `chunk` steps (evaluation time excluded). Before the lock step (95 %)
+    the evaluated latents are unquantised; the final figures of both paths
+    are after the lock (quantised latents, as exported)."""
+    lib = ntc.load()
+    res = {}
+    for path in ("plain", "fused"):
+        log(f"time-to-psnr: {path} path, {TOTAL_STEPS} steps")
+        ctx, _ = make_context(ntc.TrainPath.Fused if path == "fused" else ntc.TrainPath.Plain)
+        lib.ntc_dev_synchronize()
+        t_train, trace, done = 0.0, [], 0
+        step = chunk if path == "fused" else 1000
+        while done < TOTAL_STEPS:
+            n = min(step, TOTAL_STEPS - done)
+            t0 = time.perf_counter()
+            prog = ctx.Train(n)
+            lib.ntc_dev_synchronize()
+            t_train += time.perf_counter() - t0
+            if prog.result != ntc.Result.Success:
+                raise SystemExit(f"{path} training failed: {prog}")
+            done += n
+            if path == "fused" or done == TOTAL_STEPS:
+                psnr = ctx.Eval().psnr
+                trace.append({"steps": done, "train_s": round(t_train, 4), "psnr": round(psnr, 4)})
+                log(f"  {path} {done} steps {t_train:.2f} s psnr {psnr:.3f}")
+            else:
+                log(f"  {path} {done} steps {t_train:.2f} s")
+        res[path] = {"train_s": t_train, "final_psnr": trace[-1]["psnr"], "trace": trace}
+    target = res["plain"]["final_psnr"] - 0.05
+    hit = next((t for t in res["fused"]["trace"] if t["psnr"] >= target), None)
+    out = {"metric": "seconds to FP32 PSNR (BF16 fused vs FP32 plain), 4096^2 9-ch, 8000-step schedule",
+           "plain_fp32": {"train_s": res["plain"]["train_s"], "final_psnr": res["plain"]["final_psnr"]},
+           "fused_bf16": {"train_s": res["fused"]["train_s"], "final_psnr": res["fused"]["final_psnr"],
+                          "trace": res["fused"]["trace"]},
+           "target_psnr": target,
+           "final_within_0.05dB": abs(res["fused"]["final_psnr"] - res["plain"]["final_psnr"]) <= 0.05,
+           "seconds_to_target": hit["train_s"] if hit else None,
+           "steps_to_target": hit["steps"] if hit else None,
+           "speedup_full_schedule": res["plain"]["train_s"] / res["fused"]["train_s"]}
+    print(json.dumps(out), flush=True)
+
+
 def copy_probe(ntc, mib):
     """A plain copy of known size: reads and writes mib MiB once each. Its
     FETCH_SIZE / WRITE_SIZE counter readings calibrate the traffic figures."""
@@ -178,6 +223,9 @@ def main():
     ap.add_argument("--path", choices=["fused", "plain"], default="fused")
     ap.add_argument("--cpu-steps", type=int, default=40)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--time-to-psnr", type=int, default=0, metavar="CHUNK",
+                    help="only measure the seconds for the fused path to reach the plain path's final PSNR, "
+                         "evaluating every CHUNK steps (runs both paths over the whole schedule)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (HBM counter calibration)")
     a = ap.parse_args()
@@ -190,6 +238,9 @@ def main():
 
     if a.copy_probe:
         copy_probe(ntc, a.copy_probe)
+        return
+    if a.time_to_psnr:
+        time_to_psnr(ntc, a.time_to_psnr)
         return
     path = ntc.TrainPath.Fused if a.path == "fused" else ntc.TrainPath.Plain
     log(f"setting up {DIM}^2 synthetic set, path={a.path}")
