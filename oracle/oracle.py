@@ -423,6 +423,7 @@ class Trainer:
         self.P = P
         self.seed = seed
         self.packages = [np.ascontiguousarray(p) for p in packages]
+        self.max_batch = max_batch
         self.ws = np.zeros(lib().o_step_workspace_elems(C.byref(L), max_batch), np.float32)
         self._g0 = _ptr_array(P.g0)
         self._g1 = _ptr_array(P.g1)
@@ -438,13 +439,22 @@ class Trainer:
             self._g0v, self._g1m, self._g1v, self.P.dec.ctypes.data, self.P.dec_m.ctypes.data,
             self.P.dec_v.ctypes.data, self._pk, self.ws.ctypes.data)
 
+    def _eval_batch(self):
+        # evaluation walks each mip in tiles of min(M, 256)^2, up to 8 per batch
+        T = min(self.L.mip_dim[0], 256)
+        return min(8, (self.L.mip_dim[0] // T) ** 2) * T * T
+
     def eval(self, channel_sse=False):
+        if self.max_batch < self._eval_batch():
+            raise ValueError(f"Trainer workspace holds {self.max_batch} texels; evaluation needs {self._eval_batch()}")
         cs = np.zeros(16, np.float64)
         mse = lib().o_eval_f32(C.byref(self.L), self._g0, self._g1, self.P.dec.ctypes.data,
                                self._pk, self.ws.ctypes.data, cs.ctypes.data)
         return (mse, cs[: self.L.out_dim]) if channel_sse else mse
 
     def calibrate(self):
+        if self.max_batch < self._eval_batch():
+            raise ValueError(f"Trainer workspace holds {self.max_batch} texels; calibration needs {self._eval_batch()}")
         out = np.zeros(2, np.float32)
         lib().o_calibrate_f32(C.byref(self.L), self._g0, self._g1, self.P.dec.ctypes.data,
                               self.ws.ctypes.data, out.ctypes.data)

@@ -140,3 +140,33 @@ def test_state_machine_errors(oracle):
     bad = [(tex[0][0], ntc.Semantic.Albedo, "RG")]
     assert ctx.LoadTextures(bad, 1024) == ntc.Result.InvalidManifest
     assert ctx.LoadManifest("/nonexistent/manifest.json") == ntc.Result.FileNotFound
+
+
+GOLDEN_C1 = __import__("os").path.join(__import__("os").path.dirname(__file__), "golden", "c1_1000_steps.json")
+
+
+@pytest.mark.parametrize("path,tol", [(ntc.TrainPath.Plain, 0.05), (ntc.TrainPath.Fused, 0.05)])
+def test_c1_1000_steps_psnr_matches_cpu_oracle(path, tol):
+    """The 512^2 configuration trained for its whole 1000-step schedule on the
+    GPU reaches the CPU oracle's PSNR (committed fixture from
+    tests/golden/make_c1_psnr.py) within 0.05 dB: the plain FP32 path (same
+    arithmetic up to atomic summation order) and the fused BF16 path. The
+    250- and 500-step checkpoints are reported alongside."""
+    import json
+    gold = json.load(open(GOLDEN_C1))
+    cfg = gold["config"]
+    tex = ntc.synthetic_pbr(cfg["dim"], cfg["seed_set"])
+    ctx = ntc.Context()
+    lay = ntc.LayoutInfo(*cfg["layout"], ntc.FeatureLayout.Bilinear, ntc.Activation.ReLU)
+    assert ctx.InitEx(layout=lay, min_dimension=cfg["dim"]) == ntc.Result.Success
+    assert ctx.LoadTextures(tex, cfg["dim"]) == ntc.Result.Success
+    assert ctx.BeginTrainingEx(cfg["steps"], cfg["tiles"], *cfg["lr"], *cfg["init"], path) == ntc.Result.Success
+    done, got = 0, {}
+    for ck in gold["checkpoints"]:
+        prog = ctx.Train(ck["steps"] - done)
+        assert prog.result == ntc.Result.Success, prog
+        done = ck["steps"]
+        got[done] = ctx.Eval().psnr
+    print({k: (round(v, 4), round(c["psnr"], 4)) for (k, v), c in zip(got.items(), gold["checkpoints"])})
+    final = gold["checkpoints"][-1]
+    assert abs(got[final["steps"]] - final["psnr"]) < tol, (got, final["psnr"])
