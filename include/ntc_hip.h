@@ -269,6 +269,40 @@ int ntc_decoder_update(int32_t hidden_dim, int32_t feature_dim,
                        const double* sse_partials, double* sse_out,
                        hipStream_t stream);
 
+/* ---- evaluation decode at reduced precision (Eval, libopenntc.cpp:972-1053,
+ * at the precisions of EvalEx; INT8 = the deployed HLSL decoder,
+ * openntc-gui/ntc_decode.hlsli:112-115, 400-502) ----------------------------
+ * A decode table holds the weights of one precision (built on the host);
+ * ntc_decode_sse_f32 decodes FP32 features X [F][B] from ntc_draw_features_f32
+ * and adds each channel's squared error against Y* [O][B] to sse[16] (FP64,
+ * fixed order). FP32: one FMA chain per output in ascending order (the
+ * plain forward). BF16: weights, features and hidden activations rounded to
+ * BF16, FP32 accumulation. FP8: OCP E4M3 operands, per-layer scales
+ * (features 1/256, weights max|W|/448, hidden activations max|a|/448 from
+ * max_a0/max_a1), exact products, FP32 accumulation. INT8: int8 features
+ * clamp(round(128 x)), the exported blob's int8 weights and row scales,
+ * int32 dot products, per-layer requantisation with rcp_s_a1/rcp_s_a2. */
+#define NTC_PREC_FP32 0
+#define NTC_PREC_BF16 1
+#define NTC_PREC_FP8 2
+#define NTC_PREC_INT8 3
+size_t ntc_decode_table_bytes(int32_t precision, int32_t feature_dim, int32_t feature_dim_padded,
+                              int32_t hidden_dim);
+/* FP32 / BF16 / FP8 table (host memory) from the FP32 decoder
+ * W0[H][F] | W1[H][H] | Wout[16][H] (host memory); H is 64 or 128. */
+int ntc_build_decode_table_f(int32_t precision, int32_t feature_dim, int32_t hidden_dim, int32_t out_dim,
+                             const float* dec, float max_a0, float max_a1, void* table);
+/* INT8 table (host memory) from the exported decoder blob
+ * W0 i8 [H][Fp] | W1 i8 [H][H] | Wout i8 [16][H] | f32 scales W0s[H] W1s[H] Wouts[16]. */
+int ntc_build_decode_table_int8(int32_t feature_dim, int32_t feature_dim_padded, int32_t hidden_dim,
+                                int32_t out_dim, const uint8_t* blob, float rcp_s_a1, float rcp_s_a2,
+                                void* table);
+size_t ntc_decode_workspace_bytes(int64_t batch);
+/* table: device copy; y_out [O][B] may be NULL. */
+int ntc_decode_sse_f32(int32_t precision, int32_t activation, int32_t feature_dim, int32_t feature_dim_padded,
+                       int32_t hidden_dim, int32_t out_dim, int64_t batch, const void* table, const float* x,
+                       const float* y_target, double* sse, float* y_out, void* workspace, hipStream_t stream);
+
 /* ---- mip targets (prepare_tex / filter_lanczos / package_tex,
  * ntc_kernel.cu:533-686) ---------------------------------------------------- */
 /* out[y][x][i] = tex[y][x][cmap[i]] for i < count of leading valid cmap

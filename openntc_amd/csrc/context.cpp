@@ -380,7 +380,7 @@ void Context::Impl::FreeAll()
 {
   FreeManifestBuffers();
   for (DeviceBuffer* b : {&dec, &dec_m, &dec_v, &dec_grad, &scalars, &x, &dx, &yt, &z0, &a0, &z1, &a1, &yp, &dy,
-                          &da1, &dz1, &da0, &dz0, &gemm_ws, &loss_ws, &chan_ws})
+                          &da1, &dz1, &da0, &dz0, &gemm_ws, &loss_ws, &chan_ws, &decode_table, &decode_ws})
     b->Free();
   plain_batch = 0;
   if (stream) (void)hipStreamDestroy(stream);
@@ -779,6 +779,29 @@ Result Context::Impl::ForwardTiles(int32_t mip, const int32_t* origins, int32_t 
   return Result::Success;
 }
 
+// Features and targets of one eval batch, then the reduced-precision decode
+// with the device table in decode_table, adding per-channel SSE into sse.
+Result Context::Impl::DecodeTiles(int32_t mip, const int32_t* origins, int32_t n_tiles, int32_t T, int32_t precision,
+                                  double* sse)
+{
+  const int64_t B = (int64_t)n_tiles * T * T;
+  Result r = EnsurePlainBuffers(B);
+  if (r != Result::Success) return r;
+  const int lvl = LevelForLod(mip);
+  CTX_HIP(hipMemcpyAsync(eval_origins.ptr, origins, sizeof(int32_t) * 2 * n_tiles, hipMemcpyHostToDevice, stream),
+          Result::CudaFailure);
+  const ntc_feature_desc fd = FeatureDesc(lvl);
+  ntc_tiles tl = {eval_origins.as<int32_t>(), n_tiles, T, mip_dim[mip], (float)mip / (float)(mip_count - 1)};
+  ntc_noise nz = {};
+  CTX_NTC(ntc_draw_features_f32(&fd, &tl, &nz, g0[lvl].as<float>(), g1[lvl].as<float>(), x.as<float>(), stream));
+  CTX_NTC(ntc_draw_targets_f32(&tl, out_dim, package[mip].as<uint8_t>(), yt.as<float>(), stream));
+  CTX_NTC(ntc_decode_sse_f32(precision, ActCode(), feature_dim, feature_dim_padded, hidden_dim, out_dim, B,
+                             decode_table.ptr, x.as<float>(), yt.as<float>(), sse, nullptr, decode_ws.ptr, stream));
+  // the origins buffer is reused by the next batch
+  CTX_HIP(hipStreamSynchronize(stream), Result::CudaFailure);
+  return Result::Success;
+}
+
 // Tile walk of libopenntc.cpp:982-1005: per mip, 256^2 tiles (or one
 // mip-sized tile) in row-major order, at most 8 per batch.
 template <typename Fn>
@@ -806,13 +829,86 @@ static Result WalkEvalTiles(const Context::Impl& I, Fn&& fn)
   return Result::Success;
 }
 
-EvalResults Context::Impl::Evaluate(EvalPrecision p, EvalDetail* detail, Result& err)
+EvalResults Context::Impl::Evaluate(const Context& ctx, EvalPrecision p, EvalDetail* detail, Result& err)
 {
   err = Result::Success;
-  if (p != EvalPrecision::FP32)
+  const int prec = (int)p;
+  if (prec < NTC_PREC_FP32 || prec > NTC_PREC_INT8)
   {
     err = Result::InvalidState;
     return EvalResults{};
+  }
+  if (p != EvalPrecision::FP32)
+  {
+    // decode table of the precision: BF16 / FP8 from the current FP32
+    // decoder (FP8 with the activation ranges of a full forward), INT8 from
+    // the exported blob and calibration (after training completes)
+    const int H = hidden_dim, F = feature_dim, Fp = feature_dim_padded, O = out_dim;
+    if (H != 64 && H != 128)
+    {
+      err = Result::InvalidState;
+      return EvalResults{};
+    }
+    std::vector<uint8_t> table(ntc_decode_table_bytes(prec, F, Fp, H));
+    int st;
+    if (p == EvalPrecision::INT8)
+    {
+      if (phase != TrainPhase::TrainComplete || !ctx.decoder_host_ || !(ctx.caldata_.s_a1_ > 0.0f) ||
+          !(ctx.caldata_.s_a2_ > 0.0f))
+      {
+        err = Result::InvalidState;
+        return EvalResults{};
+      }
+      st = ntc_build_decode_table_int8(F, Fp, H, O, ctx.decoder_host_, 1.0f / ctx.caldata_.s_a1_,
+                                       1.0f / ctx.caldata_.s_a2_, table.data());
+    }
+    else
+    {
+      std::vector<float> w(dec.bytes / sizeof(float));
+      if (CopySync(w.data(), dec.ptr, dec.bytes, hipMemcpyDeviceToHost) != hipSuccess)
+      {
+        err = Result::CudaFailure;
+        return EvalResults{};
+      }
+      float ma0 = 0.0f, ma1 = 0.0f;
+      if (p == EvalPrecision::FP8)
+      {
+        const CalibrationData c = RunCalibration(1.0f, err);
+        if (err != Result::Success) return EvalResults{};
+        ma0 = c.max_abs_a1_;
+        ma1 = c.max_abs_a2_;
+      }
+      st = ntc_build_decode_table_f(prec, F, H, O, w.data(), ma0, ma1, table.data());
+    }
+    if (st != NTC_OK)
+    {
+      err = Result::InvalidState;
+      return EvalResults{};
+    }
+    if (decode_table.bytes < table.size())
+    {
+      decode_table.Free();
+      if (decode_table.Alloc(table.size()) != hipSuccess)
+      {
+        err = Result::AllocationFailure;
+        return EvalResults{};
+      }
+    }
+    const size_t wsb = ntc_decode_workspace_bytes(8 * 65536);
+    if (decode_ws.bytes < wsb)
+    {
+      decode_ws.Free();
+      if (decode_ws.Alloc(wsb) != hipSuccess)
+      {
+        err = Result::AllocationFailure;
+        return EvalResults{};
+      }
+    }
+    if (CopySync(decode_table.ptr, table.data(), table.size(), hipMemcpyHostToDevice) != hipSuccess)
+    {
+      err = Result::CudaFailure;
+      return EvalResults{};
+    }
   }
   double* sse = scalars.as<double>();
   if (ntc_fill_zero(sse, sizeof(double) * 16, stream) != NTC_OK)
@@ -822,16 +918,21 @@ EvalResults Context::Impl::Evaluate(EvalPrecision p, EvalDetail* detail, Result&
   }
   double count = 0.0;
   err = WalkEvalTiles(*this, [&](int mip, const int32_t* origins, int nb, int T) -> Result {
+    const int64_t B = (int64_t)nb * T * T;
+    if (p != EvalPrecision::FP32)
+    {
+      count += (double)B;
+      return DecodeTiles(mip, origins, nb, T, prec, sse);
+    }
     Result r = ForwardTiles(mip, origins, nb, T, true);
     if (r != Result::Success) return r;
-    const int64_t B = (int64_t)nb * T * T;
     CTX_NTC(ntc_channel_sse_f32(out_dim, B, yt.as<float>(), yp.as<float>(), sse, chan_ws.ptr, stream));
     count += (double)B;
     return Result::Success;
   });
   if (err != Result::Success) return EvalResults{};
   double host[16] = {};
-  if (hipMemcpy(host, sse, sizeof(double) * 16, hipMemcpyDeviceToHost) != hipSuccess)
+  if (CopySync(host, sse, sizeof(double) * 16, hipMemcpyDeviceToHost) != hipSuccess)
   {
     err = Result::CudaFailure;
     return EvalResults{};
@@ -869,7 +970,7 @@ CalibrationData Context::Impl::RunCalibration(float headroom, Result& err)
   });
   if (err != Result::Success) return CalibrationData{};
   float hmax[2] = {};
-  if (hipMemcpy(hmax, dmax, sizeof(hmax), hipMemcpyDeviceToHost) != hipSuccess)
+  if (CopySync(hmax, dmax, sizeof(hmax), hipMemcpyDeviceToHost) != hipSuccess)
   {
     err = Result::CudaFailure;
     return CalibrationData{};
@@ -906,7 +1007,7 @@ Result Context::Impl::Finalize(Context& ctx)
 
   const int H = hidden_dim, F = feature_dim, Fp = feature_dim_padded, O = out_dim;
   std::vector<float> w(DecoderElems());
-  CTX_HIP(hipMemcpy(w.data(), dec.ptr, sizeof(float) * w.size(), hipMemcpyDeviceToHost), Result::CudaFailure);
+  CTX_HIP(CopySync(w.data(), dec.ptr, sizeof(float) * w.size(), hipMemcpyDeviceToHost), Result::CudaFailure);
   std::vector<float> w0p((size_t)H * Fp, 0.0f);
   for (int i = 0; i < H; ++i) std::memcpy(&w0p[(size_t)i * Fp], &w[(size_t)i * F], sizeof(float) * F);
   const float* w1 = w.data() + (size_t)H * F;
@@ -1028,7 +1129,7 @@ Result Context::LoadTextures(const TextureInput* textures, int32_t count, int32_
     {
       const size_t n = (size_t)I.mip_dim[m] * I.mip_dim[m];
       mips_host_[i][m] = new uint8_t[n * 4]();
-      if (hipMemcpy(mips_host_[i][m], I.mips[i][m].ptr, n * nc, hipMemcpyDeviceToHost) != hipSuccess)
+      if (I.CopySync(mips_host_[i][m], I.mips[i][m].ptr, n * nc, hipMemcpyDeviceToHost) != hipSuccess)
       {
         I.manifest_loaded = true;
         UnloadManifest();
@@ -1126,7 +1227,7 @@ void Context::BeginTrainingEx(const TrainInfoEx& info)
       DeviceBuffer& m = which == 0 ? I.g0m[l] : I.g1m[l];
       DeviceBuffer& v = which == 0 ? I.g0v[l] : I.g1v[l];
       DeviceBuffer& d = which == 0 ? I.dg0[l] : I.dg1[l];
-      if (hipMemcpy(g.ptr, host.data(), g.bytes, hipMemcpyHostToDevice) != hipSuccess ||
+      if (I.CopySync(g.ptr, host.data(), g.bytes, hipMemcpyHostToDevice) != hipSuccess ||
           ntc_fill_zero(m.ptr, m.bytes, I.stream) != NTC_OK || ntc_fill_zero(v.ptr, v.bytes, I.stream) != NTC_OK ||
           ntc_fill_zero(d.ptr, d.bytes, I.stream) != NTC_OK)
         return fail(Result::CudaFailure);
@@ -1140,7 +1241,7 @@ void Context::BeginTrainingEx(const TrainInfoEx& info)
   bound = 1.0f / std::sqrt((float)H);
   for (int i = 0; i < H * H; ++i) host[k++] = DrawUniformRange(I.gen, -bound, bound);
   for (int i = 0; i < Impl::kOutPadded * H; ++i) host[k++] = DrawUniformRange(I.gen, -bound, bound);
-  if (hipMemcpy(I.dec.ptr, host.data(), I.dec.bytes, hipMemcpyHostToDevice) != hipSuccess ||
+  if (I.CopySync(I.dec.ptr, host.data(), I.dec.bytes, hipMemcpyHostToDevice) != hipSuccess ||
       ntc_fill_zero(I.dec_m.ptr, I.dec_m.bytes, I.stream) != NTC_OK ||
       ntc_fill_zero(I.dec_v.ptr, I.dec_v.bytes, I.stream) != NTC_OK ||
       ntc_fill_zero(I.dec_grad.ptr, I.dec_grad.bytes, I.stream) != NTC_OK)
@@ -1160,7 +1261,7 @@ void Context::BeginTrainingEx(const TrainInfoEx& info)
   std::vector<int32_t> table((size_t)I.batch_count * 16, 0);
   for (int t = 0; t < I.batch_count; ++t) std::memcpy(&table[(size_t)t * 16], I.plan[t].origins, sizeof(int32_t) * 16);
   if (I.origins_table.Alloc(sizeof(int32_t) * table.size()) != hipSuccess) return fail(Result::AllocationFailure);
-  if (hipMemcpy(I.origins_table.ptr, table.data(), I.origins_table.bytes, hipMemcpyHostToDevice) != hipSuccess ||
+  if (I.CopySync(I.origins_table.ptr, table.data(), I.origins_table.bytes, hipMemcpyHostToDevice) != hipSuccess ||
       hipStreamSynchronize(I.stream) != hipSuccess)
     return fail(Result::CudaFailure);
   I.phase = TrainPhase::TrainInProgress;
@@ -1283,7 +1384,7 @@ EvalResults Context::EvalEx(EvalPrecision precision, EvalDetail* o_detail)
     return EvalResults{};
   }
   Result err;
-  EvalResults r = I.Evaluate(precision, o_detail, err);
+  EvalResults r = I.Evaluate(*this, precision, o_detail, err);
   if (err != Result::Success) I.last_error = err;
   return r;
 }

@@ -87,6 +87,7 @@ struct Context::Impl
   // plain path activations (allocated on first use)
   int64_t plain_batch = 0;
   DeviceBuffer x, dx, yt, z0, a0, z1, a1, yp, dy, da1, dz1, da0, dz0, gemm_ws, loss_ws, chan_ws;
+  DeviceBuffer decode_table, decode_ws;  // reduced-precision evaluation
 
   // ---- training state ----
   int32_t batch_count = 0, batch_i = 0, lock_i = 0, grids_per_batch = 1;
@@ -140,10 +141,19 @@ struct Context::Impl
   Result PackDecoderTables();
   Result RunFusedStep(const StepPlan& st, int32_t step_index, hipEvent_t* ev);
   int ActCode() const { return activation == Activation::HardGELU ? NTC_ACT_HARDGELU : NTC_ACT_RELU; }
+  // Blocking copy ordered after all work queued on the context's stream. The
+  // stream is non-blocking, so a plain hipMemcpy (legacy null stream) would
+  // not wait for the kernels that produce or read the buffer.
+  hipError_t CopySync(void* dst, const void* src, size_t bytes, hipMemcpyKind kind)
+  {
+    const hipError_t e = hipMemcpyAsync(dst, src, bytes, kind, stream);
+    return e == hipSuccess ? hipStreamSynchronize(stream) : e;
+  }
   Result LockGrids();
   Result Finalize(Context& ctx);
   Result ForwardTiles(int32_t mip, const int32_t* origins, int32_t n_tiles, int32_t T, bool with_targets);
-  EvalResults Evaluate(EvalPrecision p, EvalDetail* detail, Result& err);
+  EvalResults Evaluate(const Context& ctx, EvalPrecision p, EvalDetail* detail, Result& err);
+  Result DecodeTiles(int32_t mip, const int32_t* origins, int32_t n_tiles, int32_t T, int32_t precision, double* sse);
   CalibrationData RunCalibration(float headroom, Result& err);
 };
 } // namespace openntc

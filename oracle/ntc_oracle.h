@@ -177,6 +177,21 @@ double o_eval_f32(const o_layout* L, float** g0, float** g1, const float* dec,
 void o_calibrate_f32(const o_layout* L, float** g0, float** g1,
                      const float* dec, float* ws, float out[2]);
 
+/* ---- evaluation decode at reduced precision (ntc_oracle_decode.c) ------ */
+float o_bf16_round(float x);
+uint8_t o_e4m3_encode(float x);
+float o_e4m3_decode(uint8_t c);
+int32_t o_clamp_s8(float f);
+/* s = {s_x, s_w0, s_w1, s_wout, s_a0, s_a1} */
+void o_fp8_scales(int32_t F, int32_t H, int32_t O, const float* dec, float max_a0, float max_a1, float s[6]);
+/* precision 0 FP32, 1 BF16, 2 FP8 (s8 = o_fp8_scales), 3 INT8 (blob = exported
+ * decoder, rcp = {rcp_s_a1, rcp_s_a2}); x [F][B] FP32 features -> y [O][B]. */
+void o_decode_f32(int32_t precision, int32_t act, int32_t F, int32_t Fp, int32_t H, int32_t O, int64_t B,
+                  const float* dec, const float* s8, const uint8_t* blob, const float* rcp, const float* x,
+                  float* y);
+void o_decode_f64(int32_t act, int32_t F, int32_t H, int32_t O, int64_t B, const double* dec, const double* x,
+                  double* y);
+
 #ifdef __cplusplus
 }
 #endif

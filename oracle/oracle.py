@@ -123,6 +123,17 @@ def _declare(L):
     L.o_eval_f32.argtypes = [C.POINTER(Layout), _P, _P, _P, _P, _P, _P]
     L.o_eval_f32.restype = _D
     L.o_calibrate_f32.argtypes = [C.POINTER(Layout), _P, _P, _P, _P, _P]
+    L.o_bf16_round.argtypes = [_F]
+    L.o_bf16_round.restype = _F
+    L.o_e4m3_encode.argtypes = [_F]
+    L.o_e4m3_encode.restype = C.c_uint8
+    L.o_e4m3_decode.argtypes = [C.c_uint8]
+    L.o_e4m3_decode.restype = _F
+    L.o_clamp_s8.argtypes = [_F]
+    L.o_clamp_s8.restype = _I
+    L.o_fp8_scales.argtypes = [_I, _I, _I, _P, _F, _F, _P]
+    L.o_decode_f32.argtypes = [_I, _I, _I, _I, _I, _I, _I64, _P, _P, _P, _P, _P, _P]
+    L.o_decode_f64.argtypes = [_I, _I, _I, _I, _I64, _P, _P, _P]
 
 
 def ptr(a: np.ndarray):
@@ -459,3 +470,56 @@ class Trainer:
         lib().o_calibrate_f32(C.byref(self.L), self._g0, self._g1, self.P.dec.ctypes.data,
                               self.ws.ctypes.data, out.ctypes.data)
         return out
+
+
+
+# ---- evaluation decode at reduced precision ---------------------------------------------
+PREC_FP32, PREC_BF16, PREC_FP8, PREC_INT8 = 0, 1, 2, 3
+
+
+def bf16_round(x: float) -> float:
+    return lib().o_bf16_round(float(x))
+
+
+def e4m3_encode(x: float) -> int:
+    return int(lib().o_e4m3_encode(float(x)))
+
+
+def e4m3_decode(c: int) -> float:
+    return lib().o_e4m3_decode(int(c))
+
+
+def clamp_s8(x: float) -> int:
+    return int(lib().o_clamp_s8(float(x)))
+
+
+def fp8_scales(F, H, O, dec, max_a0, max_a1):
+    dec = np.ascontiguousarray(dec, np.float32)
+    s = np.zeros(6, np.float32)
+    lib().o_fp8_scales(F, H, O, ptr(dec), float(max_a0), float(max_a1), ptr(s))
+    return s
+
+
+def decode(precision, act, F, Fp, H, O, x, dec=None, s8=None, blob=None, rcp=None):
+    """Outputs y [O][B] of the decode of FP32 features x [F][B] at the given
+    precision (dec: FP32 W0[H][F] | W1[H][H] | Wout[16][H]; s8: fp8_scales;
+    blob: exported INT8 decoder; rcp: (rcp_s_a1, rcp_s_a2))."""
+    x = np.ascontiguousarray(x, np.float32)
+    B = x.shape[1]
+    y = np.zeros((O, B), np.float32)
+    dec = None if dec is None else np.ascontiguousarray(dec, np.float32)
+    s8 = None if s8 is None else np.ascontiguousarray(s8, np.float32)
+    blob = None if blob is None else np.ascontiguousarray(blob, np.uint8)
+    rc = None if rcp is None else np.ascontiguousarray(rcp, np.float32)
+    lib().o_decode_f32(precision, act, F, Fp, H, O, B, None if dec is None else ptr(dec),
+                       None if s8 is None else ptr(s8), None if blob is None else ptr(blob),
+                       None if rc is None else ptr(rc), ptr(x), ptr(y))
+    return y
+
+
+def decode_f64(act, F, H, O, x, dec):
+    x = np.ascontiguousarray(x, np.float64)
+    dec = np.ascontiguousarray(dec, np.float64)
+    y = np.zeros((O, x.shape[1]), np.float64)
+    lib().o_decode_f64(act, F, H, O, x.shape[1], ptr(dec), ptr(x), ptr(y))
+    return y

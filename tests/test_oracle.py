@@ -416,3 +416,161 @@ def test_lanczos_matches_separable_float64_filter(oracle):
     got = oracle.filter_lanczos(img).astype(np.float64)
     # the oracle truncates like the reference: result in (ref - 1 - eps, ref + eps]
     assert np.all(got <= np.clip(ref, 0, 255) + 1e-3) and np.all(got > np.clip(ref, 0, 255) - 1 - 1e-3)
+
+
+# ---- evaluation decode at reduced precision --------------------------------------------------
+def _e4m3_value(c):
+    """OCP FP8 E4M3 by its definition: bias 7, 3 mantissa bits, subnormals at
+    exponent 0."""
+    s = -1.0 if c & 0x80 else 1.0
+    e, m = (c >> 3) & 15, c & 7
+    return s * (m / 8 * 2.0 ** -6 if e == 0 else (1 + m / 8) * 2.0 ** (e - 7))
+
+
+def test_e4m3_codes_round_trip_and_round_to_nearest_even(oracle):
+    codes = [c for c in range(256) if (c & 0x7F) != 0x7F]
+    vals = {c: _e4m3_value(c) for c in codes}
+    for c in codes:
+        assert oracle.e4m3_decode(c) == vals[c], c
+        assert oracle.e4m3_encode(vals[c]) == c, c
+    assert vals[0x7E] == 448.0 and oracle.e4m3_decode(0x08) == 2.0 ** -6
+    # brute force: nearest representable value, ties to the even code, saturation
+    pos = sorted((vals[c], c) for c in codes if c < 0x80)
+    rng = np.random.default_rng(0)
+    xs = np.concatenate([rng.uniform(-470, 470, 3000), rng.uniform(-0.03, 0.03, 3000),
+                         [(a[0] + b[0]) / 2 for a, b in zip(pos, pos[1:])]]).astype(np.float32)
+    for x in xs:
+        a = min(abs(float(x)), 448.0)
+        best = min(pos, key=lambda vc: (abs(vc[0] - a), vc[1] & 1))
+        want = best[1] | (0x80 if np.signbit(x) else 0)
+        assert oracle.e4m3_encode(float(x)) == want, (x, want, oracle.e4m3_encode(float(x)))
+    assert oracle.e4m3_encode(1e9) == 0x7E and oracle.e4m3_encode(-1e9) == 0xFE
+
+
+def test_bf16_round_is_nearest_even(oracle):
+    rng = np.random.default_rng(1)
+    xs = np.concatenate([rng.standard_normal(2000) * 10 ** rng.uniform(-6, 6, 2000),
+                         [1.0 + 2 ** -8, 1.0 + 3 * 2 ** -8, -(1.0 + 2 ** -8)]]).astype(np.float32)
+    for x in xs:
+        u = int(np.float32(x).view(np.uint32))
+        lo = np.uint32(u & 0xFFFF0000).view(np.float32)
+        hi = np.uint32((u & 0xFFFF0000) + 0x10000).view(np.float32)
+        dlo, dhi = abs(float(x) - float(lo)), abs(float(hi) - float(x))
+        want = lo if dlo < dhi else (hi if dhi < dlo else (lo if ((u >> 16) & 1) == 0 else hi))
+        assert oracle.bf16_round(float(x)) == float(want), x
+
+
+def test_clamp_s8_rounds_half_to_even(oracle):
+    for x, want in ((2.5, 2), (3.5, 4), (-2.5, -2), (-0.5, 0), (127.4, 127), (127.5, 127), (300.0, 127),
+                    (-128.5, -128), (-1000.0, -128), (0.49, 0)):
+        assert oracle.clamp_s8(x) == want, x
+
+
+def _routing_decoder(F, H, O, rng):
+    """Each hidden unit and output reads one input with weight +-2^-k: every
+    chain holds a single product."""
+    w0 = np.zeros((H, F), np.float32)
+    w1 = np.zeros((H, H), np.float32)
+    wo = np.zeros((16, H), np.float32)
+    p0, p1, po = rng.integers(0, F, H), rng.permutation(H), rng.permutation(H)[:O]
+    for h in range(H):
+        w0[h, p0[h]] = rng.choice([1.0, -0.5, 0.25, 2.0])
+        w1[h, p1[h]] = rng.choice([1.0, 0.5, -1.0, 0.125])
+    for o in range(O):
+        wo[o, po[o]] = rng.choice([1.0, -0.25, 0.75])
+    return w0, w1, wo, p0, p1, po
+
+
+@pytest.mark.parametrize("precision", [0, 1, 2])
+def test_decode_routing_network_closed_form(oracle, precision):
+    """Sparse decoders whose chains have one product each: the decode equals
+    the precision's rounding applied at the stated points (computed here with
+    single float32 operations), for ReLU."""
+    F, H, O, B = 40, 64, 9, 257
+    rng = np.random.default_rng(7 + precision)
+    w0, w1, wo, p0, p1, po = _routing_decoder(F, H, O, rng)
+    x = rng.uniform(-1, 1, (F, B)).astype(np.float32)
+    dec = np.concatenate([w0.ravel(), w1.ravel(), wo.ravel()])
+    f32 = np.float32
+    if precision == 0:
+        q = lambda v, r=None: v  # noqa: E731
+    elif precision == 1:
+        q = lambda v, r=None: np.vectorize(oracle.bf16_round, otypes=[np.float32])(v)  # noqa: E731
+    else:
+        q = lambda v, r: np.vectorize(lambda t: oracle.e4m3_decode(oracle.e4m3_encode(float(t))),  # noqa: E731
+                                      otypes=[np.float32])(np.asarray(v, np.float32) * r)
+    relu = lambda v: np.maximum(v, f32(0))  # noqa: E731
+    if precision == 2:
+        s = oracle.fp8_scales(F, H, O, dec, 0.9, 1.1)
+        want_s = [f32(1) / f32(256), np.abs(w0).max() / f32(448), np.abs(w1).max() / f32(448),
+                  np.abs(wo[:O]).max() / f32(448), f32(0.9) / f32(448), f32(1.1) / f32(448)]
+        assert np.array_equal(s, np.array(want_s, np.float32))
+        r = [f32(1) / f32(v) for v in s]
+        xq = q(x, r[0])
+        z0 = (q(w0[np.arange(H), p0], r[1])[:, None] * xq[p0]) * (s[1] * s[0])
+        a0 = q(relu(z0), r[4])
+        z1 = (q(w1[np.arange(H), p1], r[2])[:, None] * a0[p1]) * (s[2] * s[4])
+        a1 = q(relu(z1), r[5])
+        y = (q(wo[np.arange(O), po], r[3])[:, None] * a1[po]) * (s[3] * s[5])
+        got = oracle.decode(precision, 1, F, 48, H, O, x, dec=dec, s8=s)
+    else:
+        xq = q(x)
+        a0 = q(relu(q(w0[np.arange(H), p0])[:, None] * xq[p0]))
+        a1 = q(relu(q(w1[np.arange(H), p1])[:, None] * a0[p1]))
+        y = q(wo[np.arange(O), po])[:, None] * a1[po]
+        got = oracle.decode(precision, 1, F, 48, H, O, x, dec=dec)
+    assert np.array_equal(got, y.astype(np.float32))
+
+
+def _int8_blob(oracle, w0, w1, wo, Fp, s_a1, s_a2):
+    H, F = w0.shape
+    O = wo.shape[0]
+    w0p = np.zeros((H, Fp), np.float32)
+    w0p[:, :F] = w0
+    q0, s0 = oracle.quantize_weights(w0p, 1.0 / 128.0)
+    q1, s1 = oracle.quantize_weights(w1, s_a1)
+    q2, s2 = oracle.quantize_weights(wo, s_a2)
+    wo16 = np.zeros(16 * H, np.uint8)
+    wo16[: q2.nbytes] = q2.view(np.uint8)
+    sc = np.zeros(2 * H + 16, np.float32)
+    sc[:H], sc[H:2 * H], sc[2 * H:2 * H + O] = s0, s1, s2
+    return np.concatenate([q0.view(np.uint8), q1.view(np.uint8), wo16, sc.view(np.uint8)])
+
+
+@pytest.mark.parametrize("act", [0, 1])
+def test_decode_int8_matches_hlsl_restatement(oracle, act):
+    """The INT8 decode against a vectorised restatement of the deployed HLSL
+    decoder (ntc_decode.hlsli:112-115, 400-502): int8 features, int32 dot
+    products, float(acc) * scale, activation, * rcp_s_a, round half to even,
+    clamp; output float(acc) * scale."""
+    F, Fp, H, O, B = 40, 48, 64, 9, 333
+    rng = np.random.default_rng(21 + act)
+    w0 = (rng.uniform(-1, 1, (H, F)) / np.sqrt(F)).astype(np.float32)
+    w1 = (rng.uniform(-1, 1, (H, H)) / np.sqrt(H)).astype(np.float32)
+    wo = (rng.uniform(-1, 1, (O, H)) / np.sqrt(H)).astype(np.float32)
+    x = rng.uniform(-1, 1, (F, B)).astype(np.float32)
+    s_a1, s_a2 = np.float32(1.1 * 1.3 / 127), np.float32(1.1 * 0.9 / 127)
+    blob = _int8_blob(oracle, w0, w1, wo, Fp, s_a1, s_a2)
+    rcp = np.array([np.float32(1) / s_a1, np.float32(1) / s_a2], np.float32)
+    got = oracle.decode(3, act, F, Fp, H, O, x, blob=blob, rcp=rcp)
+    q0 = blob[: H * Fp].view(np.int8).reshape(H, Fp).astype(np.int64)
+    q1 = blob[H * Fp: H * Fp + H * H].view(np.int8).reshape(H, H).astype(np.int64)
+    qo = blob[H * Fp + H * H: H * Fp + H * H + 16 * H].view(np.int8).reshape(16, H)[:O].astype(np.int64)
+    sc = blob[H * Fp + H * H + 16 * H:].view(np.float32)
+    f32 = np.float32
+
+    def act_f(z):
+        if act == 1:
+            return np.maximum(z, f32(0))
+        return np.where(z < f32(-1.5), f32(0), np.where(z < f32(1.5), (z / f32(3)) * (z + f32(1.5)), z)).astype(f32)
+
+    xi = np.zeros((Fp, B), np.int64)
+    xi[:F] = np.clip(np.rint(x * f32(128)), -128, 127)
+    a0 = np.clip(np.rint(act_f((q0 @ xi).astype(f32) * sc[:H, None]) * rcp[0]), -128, 127).astype(np.int64)
+    a1 = np.clip(np.rint(act_f((q1 @ a0).astype(f32) * sc[H:2 * H, None]) * rcp[1]), -128, 127).astype(np.int64)
+    y = (qo @ a1).astype(f32) * sc[2 * H:2 * H + O, None]
+    assert np.array_equal(got, y)
+    # and within the quantisation error of the FP32 decode
+    ref = oracle.decode(0, act, F, Fp, H, O, x, dec=np.concatenate([w0.ravel(), w1.ravel(), np.concatenate(
+        [wo, np.zeros((16 - O, H), np.float32)]).ravel()]))
+    assert np.max(np.abs(got - ref)) < 0.1 * np.max(np.abs(ref))
