@@ -45,23 +45,39 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def make_context(path):
+def make_context(path, dim=DIM, seed=SEED_SET, layout=LAYOUT, steps=TOTAL_STEPS, tiles=TILES):
     import openntc_amd as ntc
 
-    tex = ntc.synthetic_pbr(DIM, SEED_SET)
+    tex = ntc.synthetic_pbr(dim, seed)
     ctx = ntc.Context()
-    lay = ntc.LayoutInfo(LAYOUT["g0_bits"], LAYOUT["g1_bits"], LAYOUT["g0_channels"], LAYOUT["g1_channels"],
-                         LAYOUT["g0_scale"], LAYOUT["hidden"], ntc.FeatureLayout.Bilinear, ntc.Activation.ReLU)
+    lay = ntc.LayoutInfo(layout["g0_bits"], layout["g1_bits"], layout["g0_channels"], layout["g1_channels"],
+                         layout["g0_scale"], layout["hidden"], ntc.FeatureLayout.Bilinear, ntc.Activation.ReLU)
     r = ctx.InitEx(layout=lay, min_dimension=512, mip_min_dim=1, seed=123)
     if r != ntc.Result.Success:
         raise SystemExit(f"InitEx failed: {r.name}")
-    r = ctx.LoadTextures(tex, DIM)
+    r = ctx.LoadTextures(tex, dim)
     if r != ntc.Result.Success:
         raise SystemExit(f"LoadTextures failed: {r.name}")
-    r = ctx.BeginTrainingEx(TOTAL_STEPS, TILES, 1e-2, 1e-3, -0.5, 0.5, path)
+    r = ctx.BeginTrainingEx(steps, tiles, 1e-2, 1e-3, -0.5, 0.5, path)
     if r != ntc.Result.Success:
         raise SystemExit(f"BeginTrainingEx failed: {r.name} ({ctx.GetLastError().name})")
     return ctx, tex
+
+
+# Evaluation workload: the 2048^2 configuration's layout (g0 12 ch at half
+# resolution, 4-bit; g1 16 ch at quarter resolution, 2-bit; 36 -> 64 -> 64 -> 9)
+# trained on a 4096^2 set (4 tiles per step, 4000 steps), then decoded over
+# every mip down to 1x1 (22.37 M texels).
+EVAL_WORKLOAD = dict(dim=4096, seed=20264, steps=4000, tiles=4,
+                     layout=dict(g0_bits=4, g1_bits=2, g0_channels=12, g1_channels=16, g0_scale=2, hidden=64))
+# target channels after the semantic sort: albedo RGB, metallic, normal RGB, AO, roughness
+CHANNEL_GROUPS = {"albedo": (0, 1, 2), "normal": (4, 5, 6), "rough_metal_ao": (3, 7, 8)}
+
+
+def group_psnr(channel_sse, count):
+    """PSNR over each channel group from per-channel FP64 squared errors."""
+    return {g: float(-10.0 * np.log10(sum(channel_sse[c] for c in ch) / (count * len(ch))))
+            for g, ch in CHANNEL_GROUPS.items()}
 
 
 def step_texels(ctx, first, count):
@@ -193,6 +209,51 @@ def time_to_psnr(ntc, chunk):
     print(json.dumps(out), flush=True)
 
 
+def eval_precisions(ntc, reps):
+    """Full-chain evaluation of the trained evaluation workload at FP32, BF16,
+    FP8 E4M3 and INT8 (the exported decoder): per-precision wall time of one
+    Context::EvalEx walk (gather + targets + decode + FP64 SSE over all mips),
+    decode throughput and PSNR overall and per channel group."""
+    lib = ntc.load()
+    w = EVAL_WORKLOAD
+    log(f"eval workload: {w['dim']}^2 set, training {w['steps']} fused steps")
+    ctx, _ = make_context(ntc.TrainPath.Fused, **w)
+    lib.ntc_dev_synchronize()
+    t0 = time.perf_counter()
+    prog = ctx.TrainUntilComplete()
+    lib.ntc_dev_synchronize()
+    t_train = time.perf_counter() - t0
+    if prog.phase != ntc.TrainPhase.TrainComplete:
+        raise SystemExit(f"training failed: {prog}")
+    log(f"trained in {t_train:.2f} s")
+    g0b, g1b, blob, cal = ctx.GetCompressedData()
+    lay = ctx.layout()
+    F, H, O = lay.feature_dim, lay.hidden_dim, lay.out_dim
+    texels = sum(lay.mip_dim[m] ** 2 for m in range(lay.mip_count))
+    flop_per_texel = 2.0 * (F * H + H * H + H * O)
+    res = {}
+    for prec in (ntc.EvalPrecision.FP32, ntc.EvalPrecision.BF16, ntc.EvalPrecision.FP8, ntc.EvalPrecision.INT8):
+        ctx.EvalEx(prec)  # untimed: builds the precision's tables (FP8 runs its calibration walk)
+        lib.ntc_dev_synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            ev = ctx.EvalEx(prec)
+        lib.ntc_dev_synchronize()
+        dt = (time.perf_counter() - t0) / reps
+        res[prec.name] = {"eval_ms": dt * 1e3, "texels_per_s": texels / dt, "psnr": ev.psnr,
+                          "psnr_groups": group_psnr(ev.channel_sse, texels)}
+        log(f"  {prec.name}: {dt * 1e3:.1f} ms per walk, psnr {ev.psnr:.4f}")
+    out = {"metric": "evaluation decode over all mips, 4096^2 9-ch (FP32 / BF16 / FP8 E4M3 / INT8)",
+           "unit": "texels/s", "texels_per_walk": texels, "flop_per_texel": flop_per_texel, "reps": reps,
+           "train_s": t_train,
+           "config": {"workload": "4096^2 9-ch synthetic set; g0 12ch@2048^2 4-bit, g1 16ch@1024^2 2-bit; "
+                                  "MLP 36->64->64->9 ReLU; 4 x 256^2 tiles/step, 4000 fused steps; "
+                                  "decode of every mip down to 1x1"},
+           "export_bytes": {"g0_packed": int(g0b.nbytes), "g1_packed": int(g1b.nbytes), "decoder_blob": int(blob.nbytes)},
+           "precisions": res}
+    print(json.dumps(out), flush=True)
+
+
 def copy_probe(ntc, mib):
     """A plain copy of known size: reads and writes mib MiB once each. Its
     FETCH_SIZE / WRITE_SIZE counter readings calibrate the traffic figures."""
@@ -226,6 +287,8 @@ def main():
     ap.add_argument("--time-to-psnr", type=int, default=0, metavar="CHUNK",
                     help="only measure the seconds for the fused path to reach the plain path's final PSNR, "
                          "evaluating every CHUNK steps (runs both paths over the whole schedule)")
+    ap.add_argument("--eval-precisions", type=int, default=0, metavar="REPS",
+                    help="only train the evaluation workload and time REPS full evaluations per precision")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (HBM counter calibration)")
     a = ap.parse_args()
@@ -241,6 +304,9 @@ def main():
         return
     if a.time_to_psnr:
         time_to_psnr(ntc, a.time_to_psnr)
+        return
+    if a.eval_precisions:
+        eval_precisions(ntc, a.eval_precisions)
         return
     path = ntc.TrainPath.Fused if a.path == "fused" else ntc.TrainPath.Plain
     log(f"setting up {DIM}^2 synthetic set, path={a.path}")

@@ -311,6 +311,7 @@ int ntc_ctx_write_grid(ntc_context* c, int32_t which, int32_t level, const float
   DeviceBuffer* b = grid_buffer(I, which, level);
   if (!b || !b->ptr) return (int)Result::InvalidState;
   if (hipStreamSynchronize(I.stream) != hipSuccess) return (int)Result::CudaFailure;
+  ++I.state_version;
   return hipMemcpy(b->ptr, host, b->bytes, hipMemcpyHostToDevice) == hipSuccess ? 0 : (int)Result::CudaFailure;
 }
 
@@ -343,6 +344,7 @@ int ntc_ctx_write_decoder(ntc_context* c, int32_t which, const float* host)
   DeviceBuffer* b = decoder_buffer(I, which);
   if (!I.initialized || !b || !b->ptr) return (int)Result::InvalidState;
   if (hipStreamSynchronize(I.stream) != hipSuccess) return (int)Result::CudaFailure;
+  ++I.state_version;
   return hipMemcpy(b->ptr, host, b->bytes, hipMemcpyHostToDevice) == hipSuccess ? 0 : (int)Result::CudaFailure;
 }
 

@@ -383,6 +383,7 @@ void Context::Impl::FreeAll()
                           &da1, &dz1, &da0, &dz0, &gemm_ws, &loss_ws, &chan_ws, &decode_table, &decode_ws})
     b->Free();
   plain_batch = 0;
+  decode_table_prec = -1;
   if (stream) (void)hipStreamDestroy(stream);
   stream = nullptr;
 }
@@ -838,7 +839,7 @@ EvalResults Context::Impl::Evaluate(const Context& ctx, EvalPrecision p, EvalDet
     err = Result::InvalidState;
     return EvalResults{};
   }
-  if (p != EvalPrecision::FP32)
+  if (p != EvalPrecision::FP32 && (decode_table_prec != prec || decode_table_version != state_version))
   {
     // decode table of the precision: BF16 / FP8 from the current FP32
     // decoder (FP8 with the activation ranges of a full forward), INT8 from
@@ -909,6 +910,8 @@ EvalResults Context::Impl::Evaluate(const Context& ctx, EvalPrecision p, EvalDet
       err = Result::CudaFailure;
       return EvalResults{};
     }
+    decode_table_prec = prec;
+    decode_table_version = state_version;
   }
   double* sse = scalars.as<double>();
   if (ntc_fill_zero(sse, sizeof(double) * 16, stream) != NTC_OK)
@@ -1264,6 +1267,7 @@ void Context::BeginTrainingEx(const TrainInfoEx& info)
   if (I.CopySync(I.origins_table.ptr, table.data(), I.origins_table.bytes, hipMemcpyHostToDevice) != hipSuccess ||
       hipStreamSynchronize(I.stream) != hipSuccess)
     return fail(Result::CudaFailure);
+  ++I.state_version;
   I.phase = TrainPhase::TrainInProgress;
 }
 
@@ -1314,6 +1318,7 @@ TrainProgress Context::Train(int32_t num_batches)
   }
   const int first = I.batch_i;
   I.tiles_update_steps = 0;
+  ++I.state_version;
   for (; I.batch_i < target; ++I.batch_i)
   {
     const StepPlan& st = I.plan[I.batch_i];

@@ -88,6 +88,10 @@ struct Context::Impl
   int64_t plain_batch = 0;
   DeviceBuffer x, dx, yt, z0, a0, z1, a1, yp, dy, da1, dz1, da0, dz0, gemm_ws, loss_ws, chan_ws;
   DeviceBuffer decode_table, decode_ws;  // reduced-precision evaluation
+  // Any change of latents or weights bumps state_version; decode_table holds
+  // the table of decode_table_prec built at decode_table_version.
+  uint64_t state_version = 0, decode_table_version = ~0ull;
+  int32_t decode_table_prec = -1;
 
   // ---- training state ----
   int32_t batch_count = 0, batch_i = 0, lock_i = 0, grids_per_batch = 1;

@@ -214,33 +214,54 @@ void o_decode_f32(int32_t precision, int32_t act, int32_t F, int32_t Fp, int32_t
   }
 }
 
+/* Blocks of NB texels side by side: each texel's chains keep the ascending k
+ * order above, and the texel loop innermost lets the compiler vectorise the
+ * fma calls across texels. */
+#define O_NB 16
 void o_decode_f64(int32_t act, int32_t F, int32_t H, int32_t O, int64_t B, const double* dec, const double* x,
                   double* y)
 {
   const double* w0 = dec;
   const double* w1 = w0 + (int64_t)H * F;
   const double* wo = w1 + (int64_t)H * H;
+  const int64_t nblk = (B + O_NB - 1) / O_NB;
 #pragma omp parallel for schedule(static)
-  for (int64_t t = 0; t < B; ++t)
+  for (int64_t blk = 0; blk < nblk; ++blk)
   {
-    double a0[256], a1[256];
+    const int64_t t0 = blk * O_NB;
+    const int32_t n = (int32_t)(B - t0 < O_NB ? B - t0 : O_NB);
+    double xs[256][O_NB], a0[256][O_NB], a1[256][O_NB];
+    for (int32_t k = 0; k < F; ++k)
+      for (int32_t j = 0; j < O_NB; ++j) xs[k][j] = j < n ? x[(int64_t)k * B + t0 + j] : 0.0;
     for (int32_t h = 0; h < H; ++h)
     {
-      double acc = 0.0;
-      for (int32_t k = 0; k < F; ++k) acc = fma(w0[(int64_t)h * F + k], x[(int64_t)k * B + t], acc);
-      a0[h] = o_act1d(act, acc);
+      double acc[O_NB] = {0};
+      for (int32_t k = 0; k < F; ++k)
+      {
+        const double w = w0[(int64_t)h * F + k];
+        for (int32_t j = 0; j < O_NB; ++j) acc[j] = fma(w, xs[k][j], acc[j]);
+      }
+      for (int32_t j = 0; j < O_NB; ++j) a0[h][j] = o_act1d(act, acc[j]);
     }
     for (int32_t h = 0; h < H; ++h)
     {
-      double acc = 0.0;
-      for (int32_t k = 0; k < H; ++k) acc = fma(w1[(int64_t)h * H + k], a0[k], acc);
-      a1[h] = o_act1d(act, acc);
+      double acc[O_NB] = {0};
+      for (int32_t k = 0; k < H; ++k)
+      {
+        const double w = w1[(int64_t)h * H + k];
+        for (int32_t j = 0; j < O_NB; ++j) acc[j] = fma(w, a0[k][j], acc[j]);
+      }
+      for (int32_t j = 0; j < O_NB; ++j) a1[h][j] = o_act1d(act, acc[j]);
     }
     for (int32_t o = 0; o < O; ++o)
     {
-      double acc = 0.0;
-      for (int32_t k = 0; k < H; ++k) acc = fma(wo[(int64_t)o * H + k], a1[k], acc);
-      y[(int64_t)o * B + t] = acc;
+      double acc[O_NB] = {0};
+      for (int32_t k = 0; k < H; ++k)
+      {
+        const double w = wo[(int64_t)o * H + k];
+        for (int32_t j = 0; j < O_NB; ++j) acc[j] = fma(w, a1[k][j], acc[j]);
+      }
+      for (int32_t j = 0; j < n; ++j) y[(int64_t)o * B + t0 + j] = acc[j];
     }
   }
 }

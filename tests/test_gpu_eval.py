@@ -63,51 +63,127 @@ def test_decode_kernel_matches_oracle(oracle, precision, act, H, F):
     assert rel < {0: 1e-5, 1: 3e-2, 2: 2e-1, 3: 2e-1}[precision], rel
 
 
+def _walk(oracle, Lay, pk, grids):
+    """The evaluation walk's batches: (features, targets) per mip, as the
+    oracle computes them from the given latents."""
+    for mip in range(Lay.mip_count):
+        M = Lay.mip_dim[mip]
+        T = min(M, 256)
+        per = (M + T - 1) // T
+        lvl = oracle.level_for_lod(Lay, mip)
+        fd = oracle.feature_desc(Lay, lvl)
+        for start in range(0, per * per, 8):
+            tiles = range(start, min(start + 8, per * per))
+            origins = np.array([v for g in tiles for v in ((g % per) * T, (g // per) * T)], np.int32)
+            x = oracle.draw_features(fd, origins, T, M, np.float32(mip / (Lay.mip_count - 1)), *grids[lvl])
+            yt = oracle.draw_targets(origins, T, M, Lay.out_dim, pk(mip))
+            yield x, yt
+
+
+def _oracle_sse(oracle, Lay, pk, grids, dec, precisions, s8=None, blob=None, rcp=None):
+    F, Fp, H, O = Lay.feature_dim, Lay.feature_dim_padded, Lay.hidden_dim, Lay.out_dim
+    sse = {p: np.zeros(O) for p in precisions}
+    count = 0
+    for x, yt in _walk(oracle, Lay, pk, grids):
+        for p in precisions:
+            if p == "f64":
+                y = oracle.decode_f64(Lay.activation, F, H, O, x.astype(np.float64), dec.astype(np.float64))
+                r = yt.astype(np.float64) - y
+            else:
+                y = oracle.decode(p, Lay.activation, F, Fp, H, O, x, dec=dec, s8=s8, blob=blob, rcp=rcp)
+                r = (yt - y).astype(np.float64)
+            sse[p] += np.sum(r * r, axis=1)
+        count += x.shape[1]
+    return sse, count
+
+
+def _fp8_scales(oracle, Lay, pk, grids, dec):
+    P2 = oracle.Params(Lay)
+    for l in range(Lay.level_count):
+        P2.g0[l][...], P2.g1[l][...] = grids[l]
+    P2.dec[...] = dec
+    mx = oracle.Trainer(Lay, P2, pk, max_batch=65536).calibrate()
+    return P2, oracle.fp8_scales(Lay.feature_dim, Lay.hidden_dim, Lay.out_dim, dec, mx[0], mx[1])
+
+
+PRECISIONS = ((0, ntc.EvalPrecision.FP32), (1, ntc.EvalPrecision.BF16), (2, ntc.EvalPrecision.FP8),
+              (3, ntc.EvalPrecision.INT8))
+
+
 def test_context_eval_precisions_match_oracle(oracle):
     """A trained 256^2 context evaluated at every precision over all mips
-    against the oracle's decode of the context's own trained state."""
+    against the oracle's decode of the context's own trained state; then the
+    decoder is changed and the cached reduced-precision tables must follow."""
     from tests.test_gpu_context import _make
     steps = 60
     ctx, Lay, P, sched, pk = _make(oracle, steps, ntc.TrainPath.Plain, grids=1)
     assert ctx.TrainUntilComplete().phase == ntc.TrainPhase.TrainComplete
     g0b, g1b, blob, cal = ctx.GetCompressedData()
     dec = ctx.read_decoder()
-    F, Fp, H, O = Lay.feature_dim, Lay.feature_dim_padded, Lay.hidden_dim, Lay.out_dim
+    O = Lay.out_dim
     grids = [(ctx.read_grid(0, l), ctx.read_grid(1, l)) for l in range(Lay.level_count)]
-    # the oracle's activation ranges of the same state (FP8 scales)
-    P2 = oracle.Params(Lay)
-    for l in range(Lay.level_count):
-        P2.g0[l][...], P2.g1[l][...] = grids[l]
-    P2.dec[...] = dec
-    mx = oracle.Trainer(Lay, P2, pk, max_batch=65536).calibrate()
-    s8 = oracle.fp8_scales(F, H, O, dec, mx[0], mx[1])
+    P2, s8 = _fp8_scales(oracle, Lay, pk, grids, dec)
     rcp = np.array([np.float32(1) / np.float32(cal[2]), np.float32(1) / np.float32(cal[3])], np.float32)
-    sse = {p: np.zeros(O) for p in (0, 1, 2, 3, "f64")}
-    count = 0
-    for mip in range(Lay.mip_count):
-        M = Lay.mip_dim[mip]
-        T = min(M, 256)
-        per = (M + T - 1) // T
-        origins = np.array([v for g in range(per * per) for v in ((g % per) * T, (g // per) * T)], np.int32)
-        lvl = oracle.level_for_lod(Lay, mip)
-        fd = oracle.feature_desc(Lay, lvl)
-        x = oracle.draw_features(fd, origins, T, M, np.float32(mip / (Lay.mip_count - 1)), *grids[lvl])
-        yt = oracle.draw_targets(origins, T, M, O, pk[mip])
-        for p in (0, 1, 2, 3):
-            y = oracle.decode(p, Lay.activation, F, Fp, H, O, x, dec=dec, s8=s8, blob=blob, rcp=rcp)
-            sse[p] += np.sum(((yt - y).astype(np.float64)) ** 2, axis=1)
-        y64 = oracle.decode_f64(Lay.activation, F, H, O, x.astype(np.float64), dec.astype(np.float64))
-        sse["f64"] += np.sum((yt.astype(np.float64) - y64) ** 2, axis=1)
-        count += x.shape[1]
+    sse, count = _oracle_sse(oracle, Lay, lambda m: pk[m], grids, dec, (0, 1, 2, 3, "f64"), s8, blob, rcp)
     psnr64 = -10 * np.log10(sse["f64"].sum() / (count * O))
-    # the oracle's own FP32 eval walk agrees with the per-mip restatement
+    # the oracle's own FP32 eval walk agrees with the per-batch restatement
     _, ocs = oracle.Trainer(Lay, P2, pk, max_batch=65536).eval(channel_sse=True)
     assert np.allclose(ocs, sse[0], rtol=1e-12, atol=0)
-    for p, prec in ((0, ntc.EvalPrecision.FP32), (1, ntc.EvalPrecision.BF16), (2, ntc.EvalPrecision.FP8),
-                    (3, ntc.EvalPrecision.INT8)):
+    got = {}
+    for p, prec in PRECISIONS:
         ev = ctx.EvalEx(prec)
+        got[p] = ev.channel_sse[:O].copy()
         assert np.allclose(ev.channel_sse[:O], sse[p], rtol=1e-10, atol=0), (prec, ev.channel_sse[:O], sse[p])
         want = -10 * np.log10(sse[p].sum() / (count * O))
         assert abs(ev.psnr - want) < 1e-8, (prec, ev.psnr, want)
         print(f"{prec.name}: psnr {ev.psnr:.4f} dB, vs FP64 decode {ev.psnr - psnr64:+.4f} dB")
         assert abs(ev.psnr - psnr64) < {0: 1e-3, 1: 0.2, 2: 1.5, 3: 1.5}[p]
+    # switching precisions rebuilds the tables: a second BF16 walk repeats the first
+    assert np.array_equal(ctx.EvalEx(ntc.EvalPrecision.BF16).channel_sse[:O], got[1])
+    # a new decoder invalidates the cached tables (BF16 and FP8, whose scales
+    # come from a fresh calibration walk)
+    dec2 = (dec * np.float32(0.75)).astype(np.float32)
+    ctx.write_decoder(dec2)
+    _, s8b = _fp8_scales(oracle, Lay, pk, grids, dec2)
+    sse2, _ = _oracle_sse(oracle, Lay, lambda m: pk[m], grids, dec2, (1, 2), s8b)
+    for p, prec in PRECISIONS[1:3]:
+        ev = ctx.EvalEx(prec)
+        assert np.allclose(ev.channel_sse[:O], sse2[p], rtol=1e-10, atol=0), (prec, ev.channel_sse[:O], sse2[p])
+
+
+def test_eval_4096_precisions_vs_fp64_oracle(oracle):
+    """The evaluation workload (the 2048^2 configuration's layout trained on a
+    4096^2 set, every mip down to 1x1, 22.37 M texels): each precision's
+    PSNR per channel group against the FP64 CPU decode of the same trained
+    state, and FP32 per-channel errors equal to it up to FP32 rounding."""
+    import bench
+    w = bench.EVAL_WORKLOAD
+    ctx, _ = bench.make_context(ntc.TrainPath.Fused, **w)
+    assert ctx.TrainUntilComplete().phase == ntc.TrainPhase.TrainComplete
+    ctx.GetCompressedData()
+    lay = ctx.layout()
+    lw = w["layout"]
+    Lay = oracle.layout_custom(lw["g0_bits"], lw["g1_bits"], lw["g0_channels"], lw["g1_channels"], lw["g0_scale"],
+                               lw["hidden"], 1, 1, w["dim"], 9, 1)
+    assert Lay.mip_count == lay.mip_count and Lay.feature_dim == lay.feature_dim
+    O = Lay.out_dim
+    grids = [(ctx.read_grid(0, l), ctx.read_grid(1, l)) for l in range(Lay.level_count)]
+    dec = ctx.read_decoder()
+    sse, count = _oracle_sse(oracle, Lay, ctx.read_package, grids, dec, ("f64",))
+    ref = bench.group_psnr(sse["f64"], count)
+    ref_all = -10 * np.log10(sse["f64"].sum() / (count * O))
+    tol = {0: 1e-3, 1: 0.2, 2: 3.0, 3: 3.0}
+    rows = []
+    for p, prec in PRECISIONS:
+        ev = ctx.EvalEx(prec)
+        rows.append((p, prec.name, ev.psnr, bench.group_psnr(ev.channel_sse, count), ev.channel_sse[:O].copy()))
+    print(f"\nFP64 oracle: psnr {ref_all:.4f} dB " + " ".join(f"{g} {v:.4f}" for g, v in ref.items()))
+    for p, name, psnr, grp, _ in rows:
+        print(f"{name}: psnr {psnr:.4f} dB ({psnr - ref_all:+.4f}) " +
+              " ".join(f"{g} {grp[g]:.4f} ({grp[g] - ref[g]:+.4f})" for g in grp))
+    for p, name, psnr, grp, cs in rows:
+        if p == 0:
+            assert np.allclose(cs, sse["f64"], rtol=1e-4, atol=0)
+        for g in grp:
+            assert abs(grp[g] - ref[g]) < tol[p], (name, g, grp[g], ref[g])
+        assert abs(psnr - ref_all) < tol[p]
