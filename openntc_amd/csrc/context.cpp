@@ -839,17 +839,21 @@ EvalResults Context::Impl::Evaluate(const Context& ctx, EvalPrecision p, EvalDet
     err = Result::InvalidState;
     return EvalResults{};
   }
-  if (p != EvalPrecision::FP32 && (decode_table_prec != prec || decode_table_version != state_version))
+  // The decode kernel (one fused forward per texel) serves every precision
+  // for hidden widths 64 and 128; FP32 at other widths runs the GEMM chain.
+  // Both evaluate the same FMA chains, so FP32 results are identical.
+  const int H = hidden_dim, F = feature_dim, Fp = feature_dim_padded, O = out_dim;
+  const bool table_path = H == 64 || H == 128;
+  if (!table_path && p != EvalPrecision::FP32)
   {
-    // decode table of the precision: BF16 / FP8 from the current FP32
+    err = Result::InvalidState;
+    return EvalResults{};
+  }
+  if (table_path && (decode_table_prec != prec || decode_table_version != state_version))
+  {
+    // decode table of the precision: FP32 / BF16 / FP8 from the current FP32
     // decoder (FP8 with the activation ranges of a full forward), INT8 from
     // the exported blob and calibration (after training completes)
-    const int H = hidden_dim, F = feature_dim, Fp = feature_dim_padded, O = out_dim;
-    if (H != 64 && H != 128)
-    {
-      err = Result::InvalidState;
-      return EvalResults{};
-    }
     std::vector<uint8_t> table(ntc_decode_table_bytes(prec, F, Fp, H));
     int st;
     if (p == EvalPrecision::INT8)
@@ -922,7 +926,7 @@ EvalResults Context::Impl::Evaluate(const Context& ctx, EvalPrecision p, EvalDet
   double count = 0.0;
   err = WalkEvalTiles(*this, [&](int mip, const int32_t* origins, int nb, int T) -> Result {
     const int64_t B = (int64_t)nb * T * T;
-    if (p != EvalPrecision::FP32)
+    if (table_path)
     {
       count += (double)B;
       return DecodeTiles(mip, origins, nb, T, prec, sse);

@@ -227,14 +227,10 @@ __global__ __launch_bounds__(kThreads) void k_decode(DecodeArgs a)
   }
 }
 
-// sse[o] += sum of the block partials in block order
-__global__ void k_decode_sum(int O, int blocks, const double* __restrict__ part, double* __restrict__ sse)
+__global__ __launch_bounds__(256) void k_decode_sum(int blocks, const double* __restrict__ part,
+                                                    double* __restrict__ sse)
 {
-  const int o = threadIdx.x;
-  if (o >= O) return;
-  double s = 0.0;
-  for (int b = 0; b < blocks; ++b) s += part[(int64_t)b * 16 + o];
-  sse[o] += s;
+  sum_channel_partials(blocks, part, sse);
 }
 
 typedef void (*DecodeKernel)(DecodeArgs);
@@ -380,7 +376,7 @@ int ntc_decode_sse_f32(int32_t precision, int32_t activation, int32_t F, int32_t
   a.part = static_cast<double*>(workspace);
   const int blocks = (int)blocks_for(batch, kThreads);
   hipLaunchKernelGGL(k, dim3(blocks), dim3(kThreads), 0, stream, a);
-  hipLaunchKernelGGL(k_decode_sum, dim3(1), dim3(16), 0, stream, O, blocks, a.part, sse);
+  hipLaunchKernelGGL(k_decode_sum, dim3(O), dim3(256), 0, stream, blocks, a.part, sse);
   return NTC_LAUNCH_STATUS();
 }
 } // extern "C"

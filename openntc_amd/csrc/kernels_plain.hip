@@ -338,14 +338,10 @@ __global__ __launch_bounds__(kThreads) void k_channel_sse(int O, int64_t B, cons
   }
 }
 
-__global__ void k_sum_channel_partials(int O, int nblocks, const double* __restrict__ partial,
-                                       double* __restrict__ sse)
+__global__ __launch_bounds__(256) void k_sum_channel_partials(int nblocks, const double* __restrict__ partial,
+                                                              double* __restrict__ sse)
 {
-  const int o = threadIdx.x;
-  if (o >= O) return;
-  double s = 0.0;
-  for (int b = 0; b < nblocks; ++b) s += partial[(int64_t)b * 16 + o];
-  sse[o] += s;
+  sum_channel_partials(nblocks, partial, sse);
 }
 
 // ---- scatter ------------------------------------------------------------------------
@@ -581,7 +577,7 @@ int ntc_channel_sse_f32(int32_t out_dim, int64_t batch, const float* y_target, c
                 NTC_E_INVALID);
   hipLaunchKernelGGL(k_channel_sse, dim3(kLossBlocks), dim3(kThreads), 0, stream, out_dim, batch, y_target,
                      y_pred, (double*)workspace);
-  hipLaunchKernelGGL(k_sum_channel_partials, dim3(1), dim3(64), 0, stream, out_dim, kLossBlocks,
+  hipLaunchKernelGGL(k_sum_channel_partials, dim3(out_dim), dim3(256), 0, stream, kLossBlocks,
                      (const double*)workspace, sse);
   return NTC_LAUNCH_STATUS();
 }

@@ -170,6 +170,26 @@ __device__ __forceinline__ bool in_rects(const int4* r, int n, int x, int y, int
   return in;
 }
 
+// sse[blockIdx.x] += the sum of part[b * 16 + blockIdx.x] over b < n, one
+// 256-thread block per channel: thread i sums b = i, i + 256, ... in order,
+// then a fixed tree over the block. Deterministic for a given n.
+__device__ __forceinline__ void sum_channel_partials(int n, const double* __restrict__ part,
+                                                     double* __restrict__ sse)
+{
+  __shared__ double red[256];
+  const int o = blockIdx.x;
+  double s = 0.0;
+  for (int b = threadIdx.x; b < n; b += 256) s += part[(int64_t)b * 16 + o];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1)
+  {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) sse[o] += red[0];
+}
+
 // Triangle-wave positional encoding value (ntc_kernel.cu:153-158).
 template <typename T>
 __device__ __forceinline__ T tri_wave(int coord, int P, int h)
