@@ -68,10 +68,47 @@ def make_context(path, dim=DIM, seed=SEED_SET, layout=LAYOUT, steps=TOTAL_STEPS,
 # resolution, 4-bit; g1 16 ch at quarter resolution, 2-bit; 36 -> 64 -> 64 -> 9)
 # trained on a 4096^2 set (4 tiles per step, 4000 steps), then decoded over
 # every mip down to 1x1 (22.37 M texels).
-EVAL_WORKLOAD = dict(dim=4096, seed=20264, steps=4000, tiles=4,
-                     layout=dict(g0_bits=4, g1_bits=2, g0_channels=12, g1_channels=16, g0_scale=2, hidden=64))
+C2_LAYOUT = dict(g0_bits=4, g1_bits=2, g0_channels=12, g1_channels=16, g0_scale=2, hidden=64)
+EVAL_WORKLOAD = dict(dim=4096, seed=20264, steps=4000, tiles=4, layout=C2_LAYOUT)
+# The 2048^2 configuration itself: the same layout on a 2048^2 set, 4 tiles
+# (262 144 texels) per step, 4000 steps.
+C2_WORKLOAD = dict(dim=2048, seed=20261, steps=4000, tiles=4, layout=C2_LAYOUT)
 # target channels after the semantic sort: albedo RGB, metallic, normal RGB, AO, roughness
 CHANNEL_GROUPS = {"albedo": (0, 1, 2), "normal": (4, 5, 6), "rough_metal_ao": (3, 7, 8)}
+
+
+# Batched configuration: 16 independent 9-channel sets with sides drawn
+# (seeded) from {256, 512, 1024, 2048}; per set g0 8 ch at half and g1 16 ch at
+# quarter resolution (4-bit), decoder 32 -> 64 -> 64 -> 9 ReLU, one 256^2 tile
+# (65 536 texels) per set and step, 2000 steps, trained together.
+SETS_WORKLOAD = dict(n_sets=16, sides=(256, 512, 1024, 2048), seed=20263, steps=2000,
+                     layout=dict(g0_bits=4, g1_bits=4, g0_channels=8, g1_channels=16, g0_scale=2, hidden=64))
+
+
+def make_sets(n_sets=16, sides=(256, 512, 1024, 2048), seed=20263, steps=2000, layout=None, dims=None):
+    """The batched configuration's contexts, each ready to train on the fused
+    path (own latents, decoder, targets and schedule); sides drawn from
+    `sides` unless `dims` lists them."""
+    import openntc_amd as ntc
+
+    lw = layout or SETS_WORKLOAD["layout"]
+    if dims is None:
+        rng = np.random.default_rng(seed)
+        dims = [int(d) for d in rng.choice(sides, n_sets)]
+    sides = tuple(dims) + tuple(sides)
+    ctxs = []
+    for i, d in enumerate(dims):
+        tex = ntc.synthetic_pbr(d, seed * 100 + i)
+        ctx = ntc.Context()
+        lay = ntc.LayoutInfo(lw["g0_bits"], lw["g1_bits"], lw["g0_channels"], lw["g1_channels"], lw["g0_scale"],
+                             lw["hidden"], ntc.FeatureLayout.Bilinear, ntc.Activation.ReLU)
+        for r in (ctx.InitEx(layout=lay, min_dimension=min(sides), mip_min_dim=1, seed=123 + i),
+                  ctx.LoadTextures(tex, d),
+                  ctx.BeginTrainingEx(steps, 1, 1e-2, 1e-3, -0.5, 0.5, ntc.TrainPath.Fused)):
+            if r != ntc.Result.Success:
+                raise SystemExit(f"set {i} ({d}^2) setup failed: {r.name}")
+        ctxs.append(ctx)
+    return ctxs, dims
 
 
 def group_psnr(channel_sse, count):
@@ -104,8 +141,15 @@ def _axis_corners(pu, M, D):
 
 def footprint_cells(si, M, D):
     """Cells of a D^2 grid inside the union of the step's tile footprints."""
-    mask = np.zeros((D, D), bool)
     T = si.tile_dim
+    if si.n_tiles == 1:
+        n = 1
+        for o in (si.origins[0] % M, si.origins[1] % M):
+            a, _ = _axis_corners(o, M, D)
+            _, d = _axis_corners(o + T - 1, M, D)
+            n *= min(d - a + 1, D)
+        return n
+    mask = np.zeros((D, D), bool)
     for b in range(si.n_tiles):
         ox, oy = si.origins[2 * b] % M, si.origins[2 * b + 1] % M
         rx, ry = [], []
@@ -164,23 +208,26 @@ def pmc_traffic(kernel):
     return None
 
 
-def time_to_psnr(ntc, chunk):
+def time_to_psnr(ntc, chunk, workload=None):
     """Seconds of training for the fused BF16 path to reach the plain FP32
-    path's final PSNR - 0.05 dB on the same workload and seed. Both paths
-    run the whole 8000-step schedule; the fused one is evaluated every
+    path's final PSNR - 0.05 dB on the same workload and seed (default: the
+    4096^2 bench workload). Both paths run the whole schedule; the fused one
+    is evaluated every
     `chunk` steps (evaluation time excluded). Before the lock step (95 %)
     the evaluated latents are unquantised; the final figures of both paths
     are after the lock (quantised latents, as exported)."""
     lib = ntc.load()
+    w = workload or dict(dim=DIM, seed=SEED_SET, steps=TOTAL_STEPS, tiles=TILES, layout=LAYOUT)
+    total = w["steps"]
     res = {}
     for path in ("plain", "fused"):
-        log(f"time-to-psnr: {path} path, {TOTAL_STEPS} steps")
-        ctx, _ = make_context(ntc.TrainPath.Fused if path == "fused" else ntc.TrainPath.Plain)
+        log(f"time-to-psnr: {path} path, {total} steps")
+        ctx, _ = make_context(ntc.TrainPath.Fused if path == "fused" else ntc.TrainPath.Plain, **w)
         lib.ntc_dev_synchronize()
         t_train, trace, done = 0.0, [], 0
         step = chunk if path == "fused" else 1000
-        while done < TOTAL_STEPS:
-            n = min(step, TOTAL_STEPS - done)
+        while done < total:
+            n = min(step, total - done)
             t0 = time.perf_counter()
             prog = ctx.Train(n)
             lib.ntc_dev_synchronize()
@@ -188,7 +235,7 @@ def time_to_psnr(ntc, chunk):
             if prog.result != ntc.Result.Success:
                 raise SystemExit(f"{path} training failed: {prog}")
             done += n
-            if path == "fused" or done == TOTAL_STEPS:
+            if path == "fused" or done == total:
                 psnr = ctx.Eval().psnr
                 trace.append({"steps": done, "train_s": round(t_train, 4), "psnr": round(psnr, 4)})
                 log(f"  {path} {done} steps {t_train:.2f} s psnr {psnr:.3f}")
@@ -197,7 +244,12 @@ def time_to_psnr(ntc, chunk):
         res[path] = {"train_s": t_train, "final_psnr": trace[-1]["psnr"], "trace": trace}
     target = res["plain"]["final_psnr"] - 0.05
     hit = next((t for t in res["fused"]["trace"] if t["psnr"] >= target), None)
-    out = {"metric": "seconds to FP32 PSNR (BF16 fused vs FP32 plain), 4096^2 9-ch, 8000-step schedule",
+    lw = w["layout"]
+    out = {"metric": f"seconds to FP32 PSNR (BF16 fused vs FP32 plain), {w['dim']}^2 9-ch, {total}-step schedule",
+           "config": {"workload": f"{w['dim']}^2 9-ch synthetic set (seed {w['seed']}); g0 {lw['g0_channels']}ch@"
+                                  f"{w['dim'] // lw['g0_scale']}^2 {lw['g0_bits']}-bit, g1 {lw['g1_channels']}ch@"
+                                  f"{w['dim'] // lw['g0_scale'] // 2}^2 {lw['g1_bits']}-bit; hidden {lw['hidden']}; "
+                                  f"{w['tiles']} x 256^2 tiles/step"},
            "plain_fp32": {"train_s": res["plain"]["train_s"], "final_psnr": res["plain"]["final_psnr"]},
            "fused_bf16": {"train_s": res["fused"]["train_s"], "final_psnr": res["fused"]["final_psnr"],
                           "trace": res["fused"]["trace"]},
@@ -254,6 +306,61 @@ def eval_precisions(ntc, reps):
     print(json.dumps(out), flush=True)
 
 
+def sets_bench(ntc, warmup, steps):
+    """The batched configuration: 16 sets trained together, one launch per
+    stage and step replayed from a captured graph, launch shape tuned at
+    creation. Times `steps` schedule steps after `warmup` (the lock step and
+    the completion are outside the timed span)."""
+    w = SETS_WORKLOAD
+    log(f"sets: building {w['n_sets']} sets")
+    ctxs, dims = make_sets(n_sets=w["n_sets"], sides=w["sides"], seed=w["seed"], steps=w["steps"])
+    lib = ntc.load()
+    t0 = time.perf_counter()
+    b = ntc.Batch(ctxs)
+    t_tune = time.perf_counter() - t0
+    st = b.stats()
+    log(f"sets: launch shape pass {st.pass_texels} grid {st.fused_grid} (tuned in {t_tune:.2f} s)")
+    lock = ctxs[0].layout().lock_step
+    steps = min(steps, lock - warmup)
+    if b.train(warmup) != ntc.Result.Success:
+        raise SystemExit("sets warmup failed")
+    lib.ntc_dev_synchronize()
+    t0 = time.perf_counter()
+    if b.train(steps) != ntc.Result.Success:
+        raise SystemExit("sets timed steps failed")
+    lib.ntc_dev_synchronize()
+    dt = time.perf_counter() - t0
+    texels = 0
+    gbytes = 0.0
+    for c, d in zip(ctxs, dims):
+        lay = c.layout()
+        for t in range(warmup, warmup + steps):
+            si = c.step_info(t)
+            B = si.n_tiles * si.tile_dim ** 2
+            texels += B
+            M = d >> si.lod
+            D0, D1 = lay.g0_dim[si.level], lay.g1_dim[si.level]
+            pl = D0 * D0 * lay.g0_channels + D1 * D1 * lay.g1_channels
+            pf = footprint_cells(si, M, D0) * lay.g0_channels + footprint_cells(si, M, D1) * lay.g1_channels
+            # grid update (24 B/param, +8 inside the footprints), targets, footprint gather
+            gbytes += (24.0 * pl + 8.0 * pf if si.grid_update else 0.0) + lay.out_dim * B + 4.0 * pf
+    gbs = gbytes / steps / (dt / steps) / 1e9
+    out = {"metric": "train texel-samples/s, 16 uneven sets in one launch per stage (graph replay)",
+           "value": texels / dt, "unit": "texel-samples/s", "n_gpus": 1, "steps": steps, "warmup": warmup,
+           "ms_per_step": dt * 1e3 / steps, "higher_is_better": True, "dtype": "bf16",
+           "data": "synthetic (seeded 9-channel PBR sets, random-init latents and decoders)",
+           "config": {"workload": "16 sets, sides drawn from {256,512,1024,2048} (seed 20263): " + ",".join(map(str, dims))
+                                  + "; per set g0 8ch@S/2 4-bit, g1 16ch@S/4 4-bit, MLP 32->64->64->9 ReLU, "
+                                    "1 x 256^2 tile per set and step, 2000-step schedule"},
+           "launch": {"pass_texels": st.pass_texels, "fused_grid": st.fused_grid,
+                      "grid_update_grid": st.grid_update_grid, "tune_ms": list(st.tune_ms),
+                      "tune_pass": list(st.tune_pass), "tune_grid": list(st.tune_grid), "tune_s": t_tune},
+           "roofline": {"bound": "hbm", "kernel": "whole step (all stages)", "achieved": gbs, "peak": HBM_PEAK_GBS,
+                        "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS, "bytes_per_step": gbytes / steps}}
+    print(json.dumps(out), flush=True)
+    b.close()
+
+
 def copy_probe(ntc, mib):
     """A plain copy of known size: reads and writes mib MiB once each. Its
     FETCH_SIZE / WRITE_SIZE counter readings calibrate the traffic figures."""
@@ -287,8 +394,12 @@ def main():
     ap.add_argument("--time-to-psnr", type=int, default=0, metavar="CHUNK",
                     help="only measure the seconds for the fused path to reach the plain path's final PSNR, "
                          "evaluating every CHUNK steps (runs both paths over the whole schedule)")
+    ap.add_argument("--workload", choices=["4096", "2048"], default="4096",
+                    help="workload of --time-to-psnr: the 4096^2 bench workload or the 2048^2 configuration")
     ap.add_argument("--eval-precisions", type=int, default=0, metavar="REPS",
                     help="only train the evaluation workload and time REPS full evaluations per precision")
+    ap.add_argument("--sets", action="store_true",
+                    help="only measure the batched 16-set configuration (--steps after --warmup schedule steps)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (HBM counter calibration)")
     a = ap.parse_args()
@@ -303,10 +414,13 @@ def main():
         copy_probe(ntc, a.copy_probe)
         return
     if a.time_to_psnr:
-        time_to_psnr(ntc, a.time_to_psnr)
+        time_to_psnr(ntc, a.time_to_psnr, C2_WORKLOAD if a.workload == "2048" else None)
         return
     if a.eval_precisions:
         eval_precisions(ntc, a.eval_precisions)
+        return
+    if a.sets:
+        sets_bench(ntc, a.warmup, a.steps)
         return
     path = ntc.TrainPath.Fused if a.path == "fused" else ntc.TrainPath.Plain
     log(f"setting up {DIM}^2 synthetic set, path={a.path}")

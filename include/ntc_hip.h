@@ -269,6 +269,76 @@ int ntc_decoder_update(int32_t hidden_dim, int32_t feature_dim,
                        const double* sse_partials, double* sse_out,
                        hipStream_t stream);
 
+/* ---- several independent training sets in one launch per stage -------------
+ * No counterpart in the reference (one Context trains one texture set,
+ * libopenntc.cpp:536-879); these batch the fused step above over sets of one
+ * decoder shape. Per-step parameters live in device tables indexed by a
+ * device-side step counter (*step), so a captured graph of one step replays
+ * every step unchanged; ntc_step_advance increments the counter. */
+typedef struct ntc_dec_set
+{
+  const float* dw_partials;   /* [blocks][decoder elems] of this set */
+  float* grad;
+  float* m;
+  float* v;
+  float* p;
+  const double* sse_partials; /* [blocks] */
+  double* sse_out;            /* may be NULL */
+} ntc_dec_set;
+
+typedef struct ntc_dec_step
+{
+  int32_t blocks;             /* workgroups the set had this step */
+  float lr, beta1, beta2, bias1, bias2;
+} ntc_dec_step;
+
+typedef struct ntc_pack_set
+{
+  const float* dec;
+  uint16_t* table;
+} ntc_pack_set;
+
+/* Size of one opaque per-(step, set) record of the fused launch, and the
+ * record of the arguments a (a->blocks is not used: the set's workgroup
+ * range comes from wg_first). */
+size_t ntc_fused_params_bytes(void);
+int ntc_fused_params_pack(const ntc_fused_args* a, void* record);
+/* Texels per workgroup pass of the compiled multi-set kernels of a shape
+ * (sizes[0..n), n <= 2 returned; 0: none). */
+int ntc_fused_pass_sizes(const ntc_feature_desc* fd, int32_t hidden_dim,
+                         int32_t activation, int32_t* sizes);
+/* params: device records [steps][n_sets]; wg_first: [steps][n_sets + 1]
+ * workgroup offsets, set s of step t taking workgroups
+ * [wg_first[t][s], wg_first[t][s + 1]) (its partial slots 0..count-1);
+ * grid >= wg_first[t][n_sets] for every step. */
+int ntc_fused_train_sets(const ntc_feature_desc* fd, int32_t hidden_dim,
+                         int32_t activation, int32_t pass_texels,
+                         const void* params, const int32_t* wg_first,
+                         int32_t n_sets, const int32_t* step, int32_t grid,
+                         hipStream_t stream);
+/* ntc_decoder_update for every set: sets[n_sets] fixed, steps[steps][n_sets]. */
+int ntc_decoder_update_sets(int32_t hidden_dim, int32_t feature_dim,
+                            int32_t out_dim, const ntc_dec_set* sets,
+                            const ntc_dec_step* steps, int32_t n_sets,
+                            const int32_t* step, hipStream_t stream);
+/* ntc_pack_decoder_bf16 for every set (sets: device array). */
+int ntc_pack_decoder_bf16_sets(const ntc_feature_desc* fd, int32_t hidden_dim,
+                               int32_t out_dim, const ntc_pack_set* sets,
+                               int32_t n_sets, hipStream_t stream);
+/* ntc_grid_update_tiles_f32 for every set: records [steps][n_sets] from
+ * ntc_grid_update_tiles_pack (items: the float4 count of the record, for
+ * sizing block ranges), blk_first [steps][n_sets + 1] block offsets (a set
+ * without a grid update that step gets an empty range). */
+size_t ntc_grid_update_tiles_bytes(void);
+int ntc_grid_update_tiles_pack(const ntc_feature_desc* fd, const ntc_tiles* tiles,
+                               float* g0, float* dg0, float* g1, float* dg1,
+                               const ntc_grid_adam* adam, void* record,
+                               int64_t* items);
+int ntc_grid_update_tiles_sets(const void* records, const int32_t* blk_first,
+                               int32_t n_sets, const int32_t* step,
+                               int32_t grid, hipStream_t stream);
+int ntc_step_advance(int32_t* step, hipStream_t stream);
+
 /* ---- evaluation decode at reduced precision (Eval, libopenntc.cpp:972-1053,
  * at the precisions of EvalEx; INT8 = the deployed HLSL decoder,
  * openntc-gui/ntc_decode.hlsli:112-115, 400-502) ----------------------------
@@ -427,6 +497,36 @@ void* ntc_ctx_stream(ntc_context* ctx);
  * measured with events on the context's stream; -1 when not recorded. */
 int ntc_ctx_kernel_times(ntc_context* ctx, float* out, int32_t n);
 int ntc_ctx_set_timing(ntc_context* ctx, int32_t enabled);
+
+/* ---- several texture sets trained together --------------------------------
+ * No counterpart in the reference (one Context trains one set). The contexts
+ * (each after BeginTrainingEx on the fused path, none trained yet, the same
+ * decoder shape and step count, channel counts multiples of 4, at most 16
+ * tiles per step) keep their own state; ntc_batch_train advances all of them
+ * together, one launch per stage and step, optionally replaying one captured
+ * hipGraph per step, and completes each context (lock, export, calibration)
+ * like Context::Train. While the batch exists the contexts are trained only
+ * through it. pass_texels / fused_grid 0: the fused launch's pass size and
+ * workgroup budget are tuned by timing the first steps (cached per decoder
+ * shape and set sizes for the process). */
+typedef struct ntc_batch ntc_batch;
+
+typedef struct ntc_batch_stats
+{
+  int32_t n_sets, steps, steps_done;
+  int32_t pass_texels, fused_grid, grid_update_grid;
+  int32_t tuned;        /* 1 tuned at creation, 2 from the cache, 0 given */
+  int32_t graph;        /* 1 once a step graph is instantiated */
+  int64_t texels_done;  /* texels of all sets over the steps run */
+  float tune_ms[4];     /* fused launch time per step of each candidate */
+  int32_t tune_pass[4], tune_grid[4];
+} ntc_batch_stats;
+
+int ntc_batch_create(ntc_context* const* sets, int32_t n_sets,
+                     int32_t pass_texels, int32_t fused_grid, ntc_batch** out);
+int ntc_batch_train(ntc_batch* b, int32_t n_steps, int32_t use_graph);
+int ntc_batch_get_stats(const ntc_batch* b, ntc_batch_stats* out);
+void ntc_batch_destroy(ntc_batch* b);
 
 /* ---- synthetic PBR set ----------------------------------------------------
  * Seeded 9-channel set: albedo RGB (value noise through sRGB decode),

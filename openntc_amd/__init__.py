@@ -6,6 +6,6 @@ binds it; it contains no compute of its own.
 """
 from ._lib import load, check, LIB_PATH  # noqa: F401
 from .context import (  # noqa: F401
-    Activation, Context, EvalPrecision, FeatureLayout, LayoutInfo, Profile, Quality, Result,
+    Activation, Batch, Context, EvalPrecision, FeatureLayout, LayoutInfo, Profile, Quality, Result,
     Semantic, TrainPath, TrainPhase, synthetic_pbr,
 )

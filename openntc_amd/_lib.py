@@ -97,6 +97,13 @@ class StepInfo(C.Structure):
                 ("lock", C.c_int32)]
 
 
+class BatchStats(C.Structure):
+    _fields_ = [("n_sets", C.c_int32), ("steps", C.c_int32), ("steps_done", C.c_int32),
+                ("pass_texels", C.c_int32), ("fused_grid", C.c_int32), ("grid_update_grid", C.c_int32),
+                ("tuned", C.c_int32), ("graph", C.c_int32), ("texels_done", C.c_int64),
+                ("tune_ms", C.c_float * 4), ("tune_pass", C.c_int32 * 4), ("tune_grid", C.c_int32 * 4)]
+
+
 _P = C.c_void_p
 _I = C.c_int32
 _I64 = C.c_int64
@@ -187,6 +194,20 @@ SIGNATURES = {
     "ntc_decode_workspace_bytes": (C.c_size_t, [C.c_int64]),
     "ntc_decode_sse_f32": (_I, [_I, _I, _I, _I, _I, _I, C.c_int64, _P, _P, _P, _P, _P, _P, _P]),
     "ntc_decoder_update": (_I, [_I, _I, _I, _I, _P, _F, _F, _F, _F, _F, _P, _P, _P, _P, _P, _P, _P]),
+    "ntc_fused_params_bytes": (C.c_size_t, []),
+    "ntc_fused_params_pack": (_I, [_P, _P]),
+    "ntc_fused_pass_sizes": (_I, [_P, _I, _I, _P]),
+    "ntc_fused_train_sets": (_I, [_P, _I, _I, _I, _P, _P, _I, _P, _I, _P]),
+    "ntc_decoder_update_sets": (_I, [_I, _I, _I, _P, _P, _I, _P, _P]),
+    "ntc_pack_decoder_bf16_sets": (_I, [_P, _I, _I, _P, _I, _P]),
+    "ntc_grid_update_tiles_bytes": (C.c_size_t, []),
+    "ntc_grid_update_tiles_pack": (_I, [_P, _P, _P, _P, _P, _P, _P, _P, _P]),
+    "ntc_grid_update_tiles_sets": (_I, [_P, _P, _I, _P, _I, _P]),
+    "ntc_step_advance": (_I, [_P, _P]),
+    "ntc_batch_create": (_I, [_P, _I, _I, _I, _P]),
+    "ntc_batch_train": (_I, [_P, _I, _I]),
+    "ntc_batch_get_stats": (_I, [_P, _P]),
+    "ntc_batch_destroy": (None, [_P]),
 }
 
 _lib = None

@@ -289,6 +289,39 @@ class Context:
         return self._lib.ntc_ctx_stream(self._h) or 0
 
 
+class Batch:
+    """Several contexts (each after BeginTrainingEx on the fused path, same
+    decoder shape and step count) trained together: one launch per stage and
+    step for all of them, optionally replayed from one captured hipGraph.
+    pass_texels / fused_grid 0: tuned by timing the first steps."""
+
+    def __init__(self, contexts, pass_texels: int = 0, fused_grid: int = 0):
+        self._lib = L.load()
+        self._contexts = list(contexts)  # keeps the contexts alive
+        arr = (C.c_void_p * len(self._contexts))(*[c._h for c in self._contexts])
+        h = C.c_void_p()
+        r = self._lib.ntc_batch_create(arr, len(self._contexts), pass_texels, fused_grid, C.byref(h))
+        if r != 0:
+            raise RuntimeError(f"ntc_batch_create failed: {Result(r).name}")
+        self._h = h.value
+
+    def train(self, n_steps: int, graph: bool = True) -> Result:
+        return Result(self._lib.ntc_batch_train(self._h, n_steps, int(graph)))
+
+    def stats(self) -> L.BatchStats:
+        out = L.BatchStats()
+        L.check(self._lib.ntc_batch_get_stats(self._h, C.byref(out)), "ntc_batch_get_stats")
+        return out
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.ntc_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+
 def synthetic_pbr(dim: int, seed: int):
     """Seeded synthetic PBR set: returns the three RGBA8 images and the texture
     list for LoadTextures (9 channels: albedo RGB, metallic, normal RGB, AO,

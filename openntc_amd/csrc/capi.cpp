@@ -8,11 +8,6 @@
 
 using namespace openntc;
 
-struct ntc_context
-{
-  Context ctx;
-};
-
 extern "C" {
 const char* ntc_version(void) { return "openntc-mi355x 0.1 (gfx950)"; }
 

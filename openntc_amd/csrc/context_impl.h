@@ -161,3 +161,9 @@ struct Context::Impl
   CalibrationData RunCalibration(float headroom, Result& err);
 };
 } // namespace openntc
+
+// The C ABI's context handle (include/ntc_hip.h).
+struct ntc_context
+{
+  openntc::Context ctx;
+};

@@ -27,6 +27,8 @@
 //     with one global float atomic (the reference adds every texel's
 //     contribution with an atomic, ntc_kernel.cu:261-278).
 // Quantisation noise is the same Philox stream as the plain path.
+#include <cstring>
+
 #include "ntc_device.h"
 #include "ntc_host.h"
 
@@ -720,8 +722,10 @@ __device__ __forceinline__ void gather_part(const FusedParams& P, int part, int 
   gather_rest<CF, PEC, LODS>(P, px, py, put);
 }
 
+// One workgroup's share of a step: passes vblk, vblk + vgrid, ... of the
+// batch described by P; its decoder-gradient and SSE partials go to slot vblk.
 template <int H, int FP, int G, int C0, int C1, int G0MODE, int PEC, int LODS, int ACT, int NT>
-__global__ __launch_bounds__(kThreads, 1) void k_fused_train(FusedParams P)
+__device__ __forceinline__ void fused_body(const FusedParams& P, const int vblk, const int vgrid)
 {
   using CF = Cfg<H, FP, G, C0, C1, G0MODE, PEC, LODS, ACT, NT>;
   constexpr int kNT = NT, kPass = CF::PASS;
@@ -785,19 +789,19 @@ __global__ __launch_bounds__(kThreads, 1) void k_fused_train(FusedParams P)
         fill_axis_tab<CF::BH>(tab[1], pg.b1, pg.xu0, pg.yu0, M, D1, tid - 128);
     }
   };
-  PassGeo cur = pass_geo<CF>((int64_t)blockIdx.x, orig_s, T, M, D0, D1);
+  PassGeo cur = pass_geo<CF>((int64_t)vblk, orig_s, T, M, D0, D1);
   int tb = 0;  // table pair of the current pass
   {
     Prefetch<CF::PF> pf;
     uint32_t tg[kNT][4];
     prefetch_issue<CF, C0, C1>(pf, cur, P, tid);
-    targets_issue<CF>(tg, cur.pm, (int64_t)blockIdx.x, P, orig_s, wave, n16, g);
+    targets_issue<CF>(tg, cur.pm, (int64_t)vblk, P, orig_s, wave, n16, g);
     fill_tables(cur, atab);
     prefetch_commit<CF, C0, C1>(pf, P, sbox);
     targets_commit(tg, tgt_s, wave, n16, g);
   }
 
-  for (int64_t pass = blockIdx.x; pass < npass; pass += gridDim.x)
+  for (int64_t pass = vblk; pass < npass; pass += vgrid)
   {
     __syncthreads();  // this pass's tables, staged boxes and targets are in LDS
     const PassMap pm = cur.pm;
@@ -1055,7 +1059,7 @@ __global__ __launch_bounds__(kThreads, 1) void k_fused_train(FusedParams P)
 
     __syncthreads();  // everyone done reading xnf; dX rows complete
     // next pass's footprint and targets: the loads land during the rounds below
-    const int64_t next = pass + gridDim.x;
+    const int64_t next = pass + vgrid;
     const int64_t pn = next < npass ? next : pass;
     PassGeo nxt = pass_geo<CF>(pn, orig_s, T, M, D0, D1);
     nxt.staged = nxt.staged && next < npass;
@@ -1150,7 +1154,7 @@ __global__ __launch_bounds__(kThreads, 1) void k_fused_train(FusedParams P)
   }
 
   // ---- partial decoder gradients of this workgroup: W0[H][F] | W1[H][H] | Wout[16][H]
-  float* part = P.dw_part + (int64_t)blockIdx.x * P.dec_elems;
+  float* part = P.dw_part + (int64_t)vblk * P.dec_elems;
   const int F = P.F;
   float* pw0 = part;
   float* pw1 = part + H * F;
@@ -1187,14 +1191,38 @@ __global__ __launch_bounds__(kThreads, 1) void k_fused_train(FusedParams P)
     if (tid < w) red[tid] += red[tid + w];
     __syncthreads();
   }
-  if (tid == 0) P.sse_part[blockIdx.x] = red[0];
+  if (tid == 0) P.sse_part[vblk] = red[0];
+}
+
+template <int H, int FP, int G, int C0, int C1, int G0MODE, int PEC, int LODS, int ACT, int NT>
+__global__ __launch_bounds__(kThreads, 1) void k_fused_train(FusedParams P)
+{
+  fused_body<H, FP, G, C0, C1, G0MODE, PEC, LODS, ACT, NT>(P, blockIdx.x, gridDim.x);
+}
+
+// Several independent training sets in one launch. For the step held in
+// *step, set s owns workgroups [wg_first[s], wg_first[s + 1]) of the grid
+// (wg_first has n_sets + 1 entries per step; workgroups past the last set's
+// range have no work) and runs its batch with those as its grid.
+template <int H, int FP, int G, int C0, int C1, int G0MODE, int PEC, int LODS, int ACT, int NT>
+__global__ __launch_bounds__(kThreads, 1) void k_fused_train_sets(const FusedParams* __restrict__ params,
+                                                                  const int32_t* __restrict__ wg_first,
+                                                                  int n_sets, const int32_t* __restrict__ step)
+{
+  const int t = read_step(step);
+  const int32_t* first = wg_first + (int64_t)t * (n_sets + 1);
+  const int b = blockIdx.x;
+  if (b >= first[n_sets]) return;
+  int s = 0;
+  while (s + 1 < n_sets && b >= first[s + 1]) ++s;
+  const FusedParams& P = params[(int64_t)t * n_sets + s];
+  fused_body<H, FP, G, C0, C1, G0MODE, PEC, LODS, ACT, NT>(P, b - first[s], first[s + 1] - first[s]);
 }
 
 // ---- decoder table packing (FP32 masters -> permuted BF16 tables) ----------------
-__global__ __launch_bounds__(256) void k_pack_tables(TableLayout L, int F, int O, int gslots,
-                                                     const float* __restrict__ dec, uint16_t* __restrict__ tab)
+__device__ __forceinline__ void pack_table_elem(const TableLayout& L, int F, int O, int gslots,
+                                                const float* __restrict__ dec, uint16_t* __restrict__ tab, int e)
 {
-  const int e = blockIdx.x * 256 + threadIdx.x;
   if (e >= L.total()) return;
   const int H = L.H;
   const float* w0 = dec;
@@ -1235,20 +1263,33 @@ __global__ __launch_bounds__(256) void k_pack_tables(TableLayout L, int F, int O
   tab[e] = f2bf(v);
 }
 
+__global__ __launch_bounds__(256) void k_pack_tables(TableLayout L, int F, int O, int gslots,
+                                                     const float* __restrict__ dec, uint16_t* __restrict__ tab)
+{
+  pack_table_elem(L, F, O, gslots, dec, tab, blockIdx.x * 256 + threadIdx.x);
+}
+
+// The tables of several sets of one shape: set blockIdx.y.
+__global__ __launch_bounds__(256) void k_pack_tables_sets(TableLayout L, int F, int O, int gslots,
+                                                          const ntc_pack_set* __restrict__ sets)
+{
+  const ntc_pack_set& S = sets[blockIdx.y];
+  pack_table_elem(L, F, O, gslots, S.dec, S.table, blockIdx.x * 256 + threadIdx.x);
+}
+
 // ---- decoder update: ordered sum of workgroup partials + Adam ----------------------
 struct DecAdam
 {
   float lr, b1, b2, omb1, omb2, bias1, bias2;
 };
 
-__global__ __launch_bounds__(256) void k_decoder_update(int n_dec, int n_adam_w01, int H, int O, int blocks,
-                                                        const float* __restrict__ part, DecAdam c,
-                                                        float* __restrict__ grad, float* __restrict__ m,
-                                                        float* __restrict__ v, float* __restrict__ p,
-                                                        const double* __restrict__ sse_part,
-                                                        double* __restrict__ sse_out)
+__device__ __forceinline__ void decoder_update_elem(int e, int n_dec, int n_adam_w01, int H, int O, int blocks,
+                                                    const float* __restrict__ part, const DecAdam& c,
+                                                    float* __restrict__ grad, float* __restrict__ m,
+                                                    float* __restrict__ v, float* __restrict__ p,
+                                                    const double* __restrict__ sse_part,
+                                                    double* __restrict__ sse_out)
 {
-  const int e = blockIdx.x * 256 + threadIdx.x;
   if (e == 0 && sse_out)
   {
     double s = 0.0;
@@ -1266,6 +1307,38 @@ __global__ __launch_bounds__(256) void k_decoder_update(int n_dec, int n_adam_w0
   m[e] = mi;
   v[e] = vi;
   p[e] = p[e] - (c.lr * (mi * c.bias1)) / (__builtin_sqrtf(vi * c.bias2) + 1e-8f);
+}
+
+__global__ __launch_bounds__(256) void k_decoder_update(int n_dec, int n_adam_w01, int H, int O, int blocks,
+                                                        const float* __restrict__ part, DecAdam c,
+                                                        float* __restrict__ grad, float* __restrict__ m,
+                                                        float* __restrict__ v, float* __restrict__ p,
+                                                        const double* __restrict__ sse_part,
+                                                        double* __restrict__ sse_out)
+{
+  decoder_update_elem(blockIdx.x * 256 + threadIdx.x, n_dec, n_adam_w01, H, O, blocks, part, c, grad, m, v, p,
+                      sse_part, sse_out);
+}
+
+// The decoder updates of several sets of one shape for the step held in
+// *step: set blockIdx.y, its partial count and Adam scalars from
+// steps[step][set].
+__global__ __launch_bounds__(256) void k_decoder_update_sets(int n_dec, int n_adam_w01, int H, int O,
+                                                             const ntc_dec_set* __restrict__ sets,
+                                                             const ntc_dec_step* __restrict__ steps, int n_sets,
+                                                             const int32_t* __restrict__ step)
+{
+  const int s = blockIdx.y;
+  const ntc_dec_set& S = sets[s];
+  const ntc_dec_step& st = steps[(int64_t)read_step(step) * n_sets + s];
+  const DecAdam c = {st.lr, st.beta1, st.beta2, 1.0f - st.beta1, 1.0f - st.beta2, st.bias1, st.bias2};
+  decoder_update_elem(blockIdx.x * 256 + threadIdx.x, n_dec, n_adam_w01, H, O, st.blocks, S.dw_partials, c,
+                      S.grad, S.m, S.v, S.p, S.sse_partials, S.sse_out);
+}
+
+__global__ void k_step_advance(int32_t* step)
+{
+  if (threadIdx.x == 0) __hip_atomic_fetch_add(step, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // ---- dispatch ---------------------------------------------------------------------------
@@ -1296,6 +1369,7 @@ bool shape_of(const ntc_feature_desc& fd, int H, int act, Shape& s)
 }
 
 typedef void (*FusedKernel)(FusedParams);
+typedef void (*FusedSetsKernel)(const FusedParams*, const int32_t*, int, const int32_t*);
 
 template <int H, int FP, int G, int C0, int C1, int G0MODE, int PEC, int LODS, int ACT>
 constexpr Shape shape_v()
@@ -1307,12 +1381,24 @@ struct Entry
 {
   Shape s;
   FusedKernel k;
-  int pass;  // texels per workgroup pass
+  int pass;                // texels per workgroup pass
+  FusedSetsKernel ks[2];   // multi-set variants: ks[0] with the pass above,
+  int spass[2];            // ks[1] (if any) with a 64-texel pass
 };
 
-#define NTC_FUSED(H, FP, G, C0, C1, MODE, PEC, LODS, ACT)                 \
-  Entry{shape_v<H, FP, G, C0, C1, MODE, PEC, LODS, ACT>(),                 \
-        k_fused_train<H, FP, G, C0, C1, MODE, PEC, LODS, ACT, nt_for(H)>, 64 * nt_for(H)}
+#define NTC_FUSED(H, FP, G, C0, C1, MODE, PEC, LODS, ACT)                                      \
+  Entry{shape_v<H, FP, G, C0, C1, MODE, PEC, LODS, ACT>(),                                      \
+        k_fused_train<H, FP, G, C0, C1, MODE, PEC, LODS, ACT, nt_for(H)>, 64 * nt_for(H),       \
+        {k_fused_train_sets<H, FP, G, C0, C1, MODE, PEC, LODS, ACT, nt_for(H)>, nullptr},       \
+        {64 * nt_for(H), 0}}
+// ... and with a second tile shape (one 16-texel column tile per wave) for
+// the multi-set launch's shape tuning
+#define NTC_FUSED2(H, FP, G, C0, C1, MODE, PEC, LODS, ACT)                                     \
+  Entry{shape_v<H, FP, G, C0, C1, MODE, PEC, LODS, ACT>(),                                      \
+        k_fused_train<H, FP, G, C0, C1, MODE, PEC, LODS, ACT, nt_for(H)>, 64 * nt_for(H),       \
+        {k_fused_train_sets<H, FP, G, C0, C1, MODE, PEC, LODS, ACT, nt_for(H)>,                 \
+         k_fused_train_sets<H, FP, G, C0, C1, MODE, PEC, LODS, ACT, 1>},                        \
+        {64 * nt_for(H), 64}}
 
 // Layouts with a compiled kernel: the reference profiles (corner g0 features,
 // HardGELU) and the measured configurations (bilinear g0, ReLU or HardGELU).
@@ -1321,10 +1407,10 @@ const Entry kEntries[] = {
     NTC_FUSED(64, 96, 80, 12, 20, NTC_G0_CORNERS, 3, 1, NTC_ACT_HARDGELU),  // Bpp_0_5
     NTC_FUSED(64, 96, 64, 12, 10, NTC_G0_CORNERS, 3, 1, NTC_ACT_HARDGELU),  // Bpp_1_0
     NTC_FUSED(64, 96, 80, 16, 12, NTC_G0_CORNERS, 3, 1, NTC_ACT_HARDGELU),  // Bpp_2_25
-    NTC_FUSED(64, 32, 32, 8, 16, NTC_G0_BILINEAR, 2, 0, NTC_ACT_RELU),      // 8+16+8 -> 64
-    NTC_FUSED(64, 64, 32, 12, 16, NTC_G0_BILINEAR, 2, 0, NTC_ACT_RELU),     // 12+16+8 -> 64
+    NTC_FUSED2(64, 32, 32, 8, 16, NTC_G0_BILINEAR, 2, 0, NTC_ACT_RELU),     // 8+16+8 -> 64
+    NTC_FUSED2(64, 64, 32, 12, 16, NTC_G0_BILINEAR, 2, 0, NTC_ACT_RELU),    // 12+16+8 -> 64
     NTC_FUSED(128, 64, 32, 16, 16, NTC_G0_BILINEAR, 2, 0, NTC_ACT_RELU),    // 16+16+8 -> 128
-    NTC_FUSED(64, 32, 32, 8, 16, NTC_G0_BILINEAR, 2, 0, NTC_ACT_HARDGELU),
+    NTC_FUSED2(64, 32, 32, 8, 16, NTC_G0_BILINEAR, 2, 0, NTC_ACT_HARDGELU),
     NTC_FUSED(128, 64, 32, 16, 16, NTC_G0_BILINEAR, 2, 0, NTC_ACT_HARDGELU),
 };
 
@@ -1413,6 +1499,98 @@ int ntc_fused_train_step(const ntc_fused_args* a, hipStream_t stream)
   const int blocks = (int)(npass < a->blocks ? npass : a->blocks);
   NTC_RETURN_IF(blocks != a->blocks, NTC_E_INVALID);  // every partial slot must be written
   hipLaunchKernelGGL(e->k, dim3(blocks), dim3(kThreads), 0, stream, P);
+  return NTC_LAUNCH_STATUS();
+}
+
+size_t ntc_fused_params_bytes(void) { return sizeof(FusedParams); }
+
+int ntc_fused_params_pack(const ntc_fused_args* a, void* record)
+{
+  NTC_RETURN_IF(!a || !record || !valid_feature_desc(&a->feat) || !valid_tiles(&a->tiles), NTC_E_INVALID);
+  NTC_RETURN_IF(!a->g0 || !a->g1 || !a->weights || !a->targets || !a->dw_partials || !a->sse_partials,
+                NTC_E_INVALID);
+  NTC_RETURN_IF(a->scatter && (!a->dg0 || !a->dg1), NTC_E_INVALID);
+  NTC_RETURN_IF(a->out_dim < 1 || a->out_dim > 16, NTC_E_INVALID);
+  NTC_RETURN_IF(!find_entry(a->feat, a->hidden_dim, a->activation), NTC_E_INVALID);
+  FusedParams P = {};
+  P.fd = a->feat;
+  P.tl = a->tiles;
+  P.nz = a->noise;
+  P.g0 = a->g0;
+  P.g1 = a->g1;
+  P.wtab = a->weights;
+  P.pkg = a->targets;
+  P.O = a->out_dim;
+  P.F = a->feat.feature_dim;
+  P.scale = a->loss_scale;
+  P.B = batch_of(a->tiles);
+  P.scatter = a->scatter;
+  P.dg0 = a->dg0;
+  P.dg1 = a->dg1;
+  P.dw_part = a->dw_partials;
+  P.sse_part = a->sse_partials;
+  P.dec_elems = a->hidden_dim * a->feat.feature_dim + a->hidden_dim * a->hidden_dim + 16 * a->hidden_dim;
+  std::memcpy(record, &P, sizeof(P));
+  return NTC_OK;
+}
+
+int ntc_fused_pass_sizes(const ntc_feature_desc* fd, int32_t hidden_dim, int32_t activation, int32_t* sizes)
+{
+  if (!valid_feature_desc(fd) || !sizes) return 0;
+  const Entry* e = find_entry(*fd, hidden_dim, activation);
+  if (!e) return 0;
+  int n = 0;
+  for (int i = 0; i < 2; ++i)
+    if (e->ks[i]) sizes[n++] = e->spass[i];
+  return n;
+}
+
+int ntc_fused_train_sets(const ntc_feature_desc* fd, int32_t hidden_dim, int32_t activation, int32_t pass_texels,
+                         const void* params, const int32_t* wg_first, int32_t n_sets, const int32_t* step,
+                         int32_t grid, hipStream_t stream)
+{
+  NTC_RETURN_IF(!valid_feature_desc(fd) || !params || !wg_first || !step || n_sets < 1 || grid < 1, NTC_E_INVALID);
+  const Entry* e = find_entry(*fd, hidden_dim, activation);
+  NTC_RETURN_IF(!e, NTC_E_INVALID);
+  FusedSetsKernel k = nullptr;
+  for (int i = 0; i < 2; ++i)
+    if (e->ks[i] && e->spass[i] == pass_texels) k = e->ks[i];
+  NTC_RETURN_IF(!k, NTC_E_INVALID);
+  hipLaunchKernelGGL(k, dim3(grid), dim3(kThreads), 0, stream, static_cast<const FusedParams*>(params), wg_first,
+                     n_sets, step);
+  return NTC_LAUNCH_STATUS();
+}
+
+int ntc_decoder_update_sets(int32_t hidden_dim, int32_t feature_dim, int32_t out_dim, const ntc_dec_set* sets,
+                            const ntc_dec_step* steps, int32_t n_sets, const int32_t* step, hipStream_t stream)
+{
+  NTC_RETURN_IF(hidden_dim < 1 || feature_dim < 1 || out_dim < 1 || out_dim > 16 || n_sets < 1, NTC_E_INVALID);
+  NTC_RETURN_IF(!sets || !steps || !step, NTC_E_INVALID);
+  const int H = hidden_dim;
+  const int n_dec = H * feature_dim + H * H + 16 * H;
+  hipLaunchKernelGGL(k_decoder_update_sets, dim3(blocks_for(n_dec, 256), n_sets), dim3(256), 0, stream, n_dec,
+                     H * feature_dim + H * H, H, out_dim, sets, steps, n_sets, step);
+  return NTC_LAUNCH_STATUS();
+}
+
+int ntc_pack_decoder_bf16_sets(const ntc_feature_desc* fd, int32_t hidden_dim, int32_t out_dim,
+                               const ntc_pack_set* sets, int32_t n_sets, hipStream_t stream)
+{
+  NTC_RETURN_IF(!valid_feature_desc(fd) || !sets || n_sets < 1 || out_dim < 1 || out_dim > 16, NTC_E_INVALID);
+  NTC_RETURN_IF(hidden_dim % 64 != 0 || hidden_dim < 64 || hidden_dim > 256, NTC_E_INVALID);
+  Shape s;
+  shape_of(*fd, hidden_dim, 0, s);
+  const TableLayout L{s.H, s.FP, s.G};
+  const int gslots = g1_base(*fd) + fd->g1_channels;
+  hipLaunchKernelGGL(k_pack_tables_sets, dim3(blocks_for(L.total(), 256), n_sets), dim3(256), 0, stream, L,
+                     fd->feature_dim, out_dim, gslots, sets);
+  return NTC_LAUNCH_STATUS();
+}
+
+int ntc_step_advance(int32_t* step, hipStream_t stream)
+{
+  NTC_RETURN_IF(!step, NTC_E_INVALID);
+  hipLaunchKernelGGL(k_step_advance, dim3(1), dim3(64), 0, stream, step);
   return NTC_LAUNCH_STATUS();
 }
 

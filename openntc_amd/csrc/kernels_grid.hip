@@ -10,6 +10,8 @@
 //   k_quantize      ntc_kernel.cu:448-466 (quantize_grid)
 //   k_pack          ntc_kernel.cu:486-507 (quantize_pack)
 //   k_max_abs       ntc_kernel.cu:511-531 (max_abs)
+#include <cstring>
+
 #include "ntc_device.h"
 #include "ntc_host.h"
 
@@ -119,7 +121,9 @@ struct TileUpdate
   AdamConst c;
 };
 
-__global__ __launch_bounds__(kThreads) void k_grid_update_tiles(TileUpdate A)
+// One block's share (items vblk * 256 + tid, stepping vgrid * 256) of the
+// update described by A.
+__device__ __forceinline__ void grid_update_tiles_body(const TileUpdate& A, const int vblk, const int vgrid)
 {
   __shared__ int4 rect[2][16];
   const int tid = threadIdx.x, n = A.tl.n_tiles;
@@ -133,7 +137,7 @@ __global__ __launch_bounds__(kThreads) void k_grid_update_tiles(TileUpdate A)
   }
   __syncthreads();
   const f4 zero = {0.f, 0.f, 0.f, 0.f};
-  for (int64_t i = (int64_t)blockIdx.x * kThreads + tid; i < A.n4; i += (int64_t)gridDim.x * kThreads)
+  for (int64_t i = (int64_t)vblk * kThreads + tid; i < A.n4; i += (int64_t)vgrid * kThreads)
   {
     const int grid = i >= A.a4;
     const GridSeg& s = A.s[grid];
@@ -160,6 +164,27 @@ __global__ __launch_bounds__(kThreads) void k_grid_update_tiles(TileUpdate A)
     __builtin_nontemporal_store(pi, &s.p[k]);
     if (hot) __builtin_nontemporal_store(zero, &s.g[k]);
   }
+}
+
+__global__ __launch_bounds__(kThreads) void k_grid_update_tiles(TileUpdate A)
+{
+  grid_update_tiles_body(A, blockIdx.x, gridDim.x);
+}
+
+// The grid updates of several sets for the step held in *step: set s owns
+// blocks [blk_first[s], blk_first[s + 1]) (blk_first has n_sets + 1 entries
+// per step; a set without a grid update this step owns none).
+__global__ __launch_bounds__(kThreads) void k_grid_update_tiles_sets(const TileUpdate* __restrict__ recs,
+                                                                     const int32_t* __restrict__ blk_first,
+                                                                     int n_sets, const int32_t* __restrict__ step)
+{
+  const int t = read_step(step);
+  const int32_t* first = blk_first + (int64_t)t * (n_sets + 1);
+  const int b = blockIdx.x;
+  if (b >= first[n_sets]) return;
+  int s = 0;
+  while (s + 1 < n_sets && b >= first[s + 1]) ++s;
+  grid_update_tiles_body(recs[(int64_t)t * n_sets + s], b - first[s], first[s + 1] - first[s]);
 }
 
 __global__ __launch_bounds__(kThreads) void k_grid_update_tail(int64_t beg, int64_t n, AdamConst c,
@@ -313,8 +338,8 @@ int ntc_grid_update2_f32(int64_t n0, float* grad0, float* m0, float* v0, float* 
   return NTC_LAUNCH_STATUS();
 }
 
-int ntc_grid_update_tiles_f32(const ntc_feature_desc* fd, const ntc_tiles* tiles, float* g0, float* dg0,
-                              float* g1, float* dg1, const ntc_grid_adam* adam, hipStream_t stream)
+static int tile_update_of(const ntc_feature_desc* fd, const ntc_tiles* tiles, float* g0, float* dg0, float* g1,
+                          float* dg1, const ntc_grid_adam* adam, TileUpdate& A)
 {
   NTC_RETURN_IF(!valid_feature_desc(fd) || !valid_tiles(tiles) || !adam, NTC_E_INVALID);
   NTC_RETURN_IF(tiles->n_tiles > 16 || fd->g0_channels % 4 || fd->g1_channels % 4, NTC_E_INVALID);
@@ -322,7 +347,6 @@ int ntc_grid_update_tiles_f32(const ntc_feature_desc* fd, const ntc_tiles* tiles
   const uintptr_t bits = (uintptr_t)g0 | (uintptr_t)dg0 | (uintptr_t)g1 | (uintptr_t)dg1 | (uintptr_t)adam->m0 |
                          (uintptr_t)adam->v0 | (uintptr_t)adam->m1 | (uintptr_t)adam->v1;
   NTC_RETURN_IF(bits & 15, NTC_E_INVALID);
-  TileUpdate A;
   A.tl = *tiles;
   A.D[0] = fd->g0_dim;
   A.D[1] = fd->g1_dim;
@@ -333,9 +357,41 @@ int ntc_grid_update_tiles_f32(const ntc_feature_desc* fd, const ntc_tiles* tiles
   A.s[0] = GridSeg{(f4*)dg0, (f4*)adam->m0, (f4*)adam->v0, (f4*)g0, adam->lo0, adam->hi0};
   A.s[1] = GridSeg{(f4*)dg1, (f4*)adam->m1, (f4*)adam->v1, (f4*)g1, adam->lo1, adam->hi1};
   A.c = make_adam(adam->lr, adam->beta1, adam->beta2, adam->bias1, adam->bias2);
+  return NTC_OK;
+}
+
+int ntc_grid_update_tiles_f32(const ntc_feature_desc* fd, const ntc_tiles* tiles, float* g0, float* dg0,
+                              float* g1, float* dg1, const ntc_grid_adam* adam, hipStream_t stream)
+{
+  TileUpdate A;
+  const int st = tile_update_of(fd, tiles, g0, dg0, g1, dg1, adam, A);
+  if (st != NTC_OK) return st;
   const int64_t want = (A.n4 + kThreads - 1) / kThreads;
   const unsigned blocks = (unsigned)(want < 2048 ? want : 2048);
   hipLaunchKernelGGL(k_grid_update_tiles, dim3(blocks), dim3(kThreads), 0, stream, A);
+  return NTC_LAUNCH_STATUS();
+}
+
+size_t ntc_grid_update_tiles_bytes(void) { return sizeof(TileUpdate); }
+
+int ntc_grid_update_tiles_pack(const ntc_feature_desc* fd, const ntc_tiles* tiles, float* g0, float* dg0, float* g1,
+                               float* dg1, const ntc_grid_adam* adam, void* record, int64_t* items)
+{
+  NTC_RETURN_IF(!record, NTC_E_INVALID);
+  TileUpdate A;
+  const int st = tile_update_of(fd, tiles, g0, dg0, g1, dg1, adam, A);
+  if (st != NTC_OK) return st;
+  std::memcpy(record, &A, sizeof(A));
+  if (items) *items = A.n4;
+  return NTC_OK;
+}
+
+int ntc_grid_update_tiles_sets(const void* records, const int32_t* blk_first, int32_t n_sets, const int32_t* step,
+                               int32_t grid, hipStream_t stream)
+{
+  NTC_RETURN_IF(!records || !blk_first || !step || n_sets < 1 || grid < 1, NTC_E_INVALID);
+  hipLaunchKernelGGL(k_grid_update_tiles_sets, dim3(grid), dim3(kThreads), 0, stream,
+                     static_cast<const TileUpdate*>(records), blk_first, n_sets, step);
   return NTC_LAUNCH_STATUS();
 }
 

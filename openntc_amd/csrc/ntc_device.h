@@ -190,6 +190,16 @@ __device__ __forceinline__ void sum_channel_partials(int n, const double* __rest
   if (threadIdx.x == 0) sse[o] += red[0];
 }
 
+// The device step counter of the multi-set launches is rewritten between
+// launches; read it with a device-coherent vector load (not through the
+// scalar cache, which later launches are not guaranteed to see refreshed)
+// and broadcast it to the whole wave.
+__device__ __forceinline__ int read_step(const int32_t* step)
+{
+  const int v = __hip_atomic_load(const_cast<int32_t*>(step), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return __builtin_amdgcn_readfirstlane(v);
+}
+
 // Triangle-wave positional encoding value (ntc_kernel.cu:153-158).
 template <typename T>
 __device__ __forceinline__ T tri_wave(int coord, int P, int h)

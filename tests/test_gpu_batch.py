@@ -1,0 +1,138 @@
+"""Several texture sets trained together (ntc_batch_*): every set of a batch
+follows the same fused-path arithmetic as when its Context trains alone, the
+captured-graph replay equals eager launches, and the 16-set configuration
+reaches each set's single-context PSNR."""
+import numpy as np
+import pytest
+
+import openntc_amd as ntc
+
+pytestmark = pytest.mark.gpu
+
+
+def _sets(sides, steps, seed=7):
+    import bench
+    return bench.make_sets(len(sides), tuple(sides), seed, steps, dims=list(sides))
+
+
+def _state(ctx):
+    lay = ctx.layout()
+    return [ctx.read_grid(w, l) for l in range(lay.level_count) for w in (0, 1)] + [ctx.read_decoder()]
+
+
+def _close(a, b, tol):
+    return max(float(np.max(np.abs(x - y))) for x, y in zip(a, b)) < tol
+
+
+def test_batch_step_matches_single_contexts():
+    """One and then six steps of three uneven sets in a batch against the same
+    sets trained alone: the decoder and latents agree to the rounding of the
+    atomic latent-gradient order and of the per-workgroup partial grouping."""
+    import bench
+    sides = (256, 512, 256)
+    batch_ctx, dims = _sets(sides, 40)
+    solo_ctx, _ = _sets(sides, 40)
+    assert dims == list(sides)
+    b = ntc.Batch(batch_ctx, pass_texels=128, fused_grid=256)
+    st = b.stats()
+    assert st.n_sets == 3 and st.pass_texels == 128 and st.tuned == 0
+    for n in (1, 6):
+        assert b.train(n, graph=False) == ntc.Result.Success
+        for c in solo_ctx:
+            assert c.Train(n).result == ntc.Result.Success
+        for cb, cs in zip(batch_ctx, solo_ctx):
+            assert cb.layout().batch_index == cs.layout().batch_index
+            assert _close(_state(cb), _state(cs), 2e-4)
+    b.close()
+
+
+def test_batch_graph_replay_matches_eager_and_completes():
+    """The same batch trained by replaying one captured step graph and by eager
+    launches: before the lock step the two runs agree step for step within the
+    atomic-order rounding; after the lock (which rounds the latents to the
+    4-bit lattice, where a rounding-level difference can move a value by one
+    step) and completion (export included) they agree in PSNR and in all but
+    a few latent codes."""
+    sides = (256, 512, 1024)
+    steps = 60  # lock at step 57
+    a_ctx, _ = _sets(sides, steps, seed=11)
+    e_ctx, _ = _sets(sides, steps, seed=11)
+    ba = ntc.Batch(a_ctx, pass_texels=128, fused_grid=256)
+    be = ntc.Batch(e_ctx, pass_texels=128, fused_grid=256)
+    done = 0
+    for n in (1, 1, 5, 20, 33):
+        assert ba.train(n, graph=True) == ntc.Result.Success
+        assert be.train(n, graph=False) == ntc.Result.Success
+        done += n
+        if done < 57:
+            for ca, ce in zip(a_ctx, e_ctx):
+                assert _close(_state(ca), _state(ce), 5e-4), done
+    assert ba.stats().graph == 1 and ba.stats().steps_done == steps
+    for ca, ce in zip(a_ctx, e_ctx):
+        assert ca.layout().batch_index == steps
+        pa, pe = ca.Eval().psnr, ce.Eval().psnr
+        assert abs(pa - pe) < 0.01, (pa, pe)
+        ga, ge = _state(ca)[:-1], _state(ce)[:-1]
+        differ = sum(int(np.count_nonzero(x != y)) for x, y in zip(ga, ge))
+        assert differ <= 1e-3 * sum(x.size for x in ga), differ
+        # completion ran the export
+        g0a, _, blob_a, _ = ca.GetCompressedData()
+        assert g0a.size > 0 and blob_a.size > 0
+    assert ba.train(1) == ntc.Result.Success  # nothing left: no-op
+
+
+def test_batch_rejects_mixed_shapes_and_trained_sets():
+    import bench
+    ctxs, _ = _sets((256, 256), 10)
+    other, _ = bench.make_sets(1, (256,), 3, 10, dims=[256], layout=dict(g0_bits=4, g1_bits=4, g0_channels=12,
+                                                             g1_channels=16, g0_scale=2, hidden=64))
+    with pytest.raises(RuntimeError):
+        ntc.Batch(ctxs + other, pass_texels=128, fused_grid=256)
+    assert ctxs[0].Train(1).result == ntc.Result.Success
+    with pytest.raises(RuntimeError):
+        ntc.Batch(ctxs, pass_texels=128, fused_grid=256)
+
+
+def test_sets_configuration_psnr_matches_single_contexts():
+    """The 16-set configuration (sides from {256, 512, 1024, 2048}, 2000
+    steps, graph replay) against each set trained alone. With the single-set
+    kernel's pass shape (128 texels) every set's final PSNR is within 0.05 dB
+    of its solo run. The tuned launch shape may use 64-texel passes, whose
+    latent-gradient atomics overlap more between passes; steps that sample a
+    1x1 .. 4x4 mip (a batch of 1-16 texels with the reference's loss scale
+    -2 / (O B)) amplify that summation-order rounding into run-to-run spreads
+    of up to about 0.5 dB on single sets (identical 64-texel runs differ by
+    that much from each other), so the tuned run is held to the mean over the
+    sets (0.1 dB) and to 1 dB per set."""
+    import time
+
+    import bench
+    w = bench.SETS_WORKLOAD
+    kw = dict(n_sets=w["n_sets"], sides=w["sides"], seed=w["seed"], steps=w["steps"])
+    solo, dims = bench.make_sets(**kw)
+    t0 = time.perf_counter()
+    for c in solo:
+        assert c.TrainUntilComplete().phase == ntc.TrainPhase.TrainComplete
+    t_solo = time.perf_counter() - t0
+    psnr_s = np.array([c.Eval().psnr for c in solo])
+    del solo
+    res = {}
+    for label, shape in (("pass 128", (128, 256)), ("tuned", (0, 0))):
+        ctxs, _ = bench.make_sets(**kw)
+        t0 = time.perf_counter()
+        b = ntc.Batch(ctxs, *shape)
+        assert b.train(w["steps"], graph=True) == ntc.Result.Success
+        dt = time.perf_counter() - t0
+        st = b.stats()
+        assert st.steps_done == w["steps"] and st.graph == 1
+        res[label] = (np.array([c.Eval().psnr for c in ctxs]), dt, st.pass_texels, st.fused_grid, list(st.tune_ms))
+        b.close()
+        del b, ctxs
+    print(f"\nsides {dims}\none by one {t_solo:.2f} s")
+    for label, (p, dt, pt, fg, tm) in res.items():
+        print(f"{label}: {dt:.2f} s (pass {pt}, grid {fg}, candidates {tm}); PSNR - alone:",
+              " ".join(f"{x:+.3f}" for x in p - psnr_s))
+    d128 = res["pass 128"][0] - psnr_s
+    assert np.max(np.abs(d128)) < 0.05, d128
+    dt = res["tuned"][0] - psnr_s
+    assert abs(float(np.mean(dt))) < 0.1 and np.max(np.abs(dt)) < 1.0, dt
