@@ -1009,7 +1009,9 @@ __device__ __forceinline__ void fused_body(const FusedParams& P, const int vblk,
 #pragma unroll
       for (int nt = 0; nt < kNT; ++nt)
       {
-        if (!tvalid[nt]) continue;
+        // Every lane runs the MFMAs (lane l supplies row l%16 of the A
+        // operand for the whole wave); only the writes are per texel.
+        const bool tv = tvalid[nt];
         const Footprint<float> f0 = footprint<float>(tx[nt], ty[nt], M, D0);
         const Footprint<float> f1 = footprint<float>(tx[nt], ty[nt], M, D1);
         float* drow = dxs + tloc[nt] * CF::GS;
@@ -1020,6 +1022,7 @@ __device__ __forceinline__ void fused_body(const FusedParams& P, const int vblk,
 #pragma unroll
           for (int j = 0; j < KH; ++j)
             dx = mfma(frag_lds(&wt[TLY.oW0T() + (16 * r + n16) * H + 32 * j + 8 * g]), dz0f[j][nt], dx);
+          if (!tv) continue;
 #pragma unroll
           for (int i = 0; i < 4; ++i)
           {

@@ -95,15 +95,9 @@ def test_batch_rejects_mixed_shapes_and_trained_sets():
 
 def test_sets_configuration_psnr_matches_single_contexts():
     """The 16-set configuration (sides from {256, 512, 1024, 2048}, 2000
-    steps, graph replay) against each set trained alone. With the single-set
-    kernel's pass shape (128 texels) every set's final PSNR is within 0.05 dB
-    of its solo run. The tuned launch shape may use 64-texel passes, whose
-    latent-gradient atomics overlap more between passes; steps that sample a
-    1x1 .. 4x4 mip (a batch of 1-16 texels with the reference's loss scale
-    -2 / (O B)) amplify that summation-order rounding into run-to-run spreads
-    of up to about 0.5 dB on single sets (identical 64-texel runs differ by
-    that much from each other), so the tuned run is held to the mean over the
-    sets (0.1 dB) and to 1 dB per set."""
+    steps, graph replay) against each set trained alone: at the single-set
+    kernel's pass shape (128 texels) and at the tuned launch shape, every
+    set's final PSNR is within 0.05 dB of its solo run."""
     import time
 
     import bench
@@ -132,7 +126,5 @@ def test_sets_configuration_psnr_matches_single_contexts():
     for label, (p, dt, pt, fg, tm) in res.items():
         print(f"{label}: {dt:.2f} s (pass {pt}, grid {fg}, candidates {tm}); PSNR - alone:",
               " ".join(f"{x:+.3f}" for x in p - psnr_s))
-    d128 = res["pass 128"][0] - psnr_s
-    assert np.max(np.abs(d128)) < 0.05, d128
-    dt = res["tuned"][0] - psnr_s
-    assert abs(float(np.mean(dt))) < 0.1 and np.max(np.abs(dt)) < 1.0, dt
+    for label, (p, *_rest) in res.items():
+        assert np.max(np.abs(p - psnr_s)) < 0.05, (label, p - psnr_s)

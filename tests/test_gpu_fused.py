@@ -67,9 +67,9 @@ def bf16_step(x32, yt, w0, w1, wout, O, act, scale):
     return sse, dz0 @ xb.T, dz1 @ a0.T, dy @ a1.T, dx
 
 
-def _case(oracle, kind):
+def _case(oracle, kind, dim=256, mip_min=4):
     if kind == "c1_bilinear_relu":
-        return oracle.layout_custom(4, 4, 8, 16, 2, 64, 1, 1, 256), 1
+        return oracle.layout_custom(4, 4, 8, 16, 2, 64, 1, 1, dim, 9, mip_min), 1
     if kind == "c3_bilinear_h128":
         return oracle.layout_custom(4, 4, 16, 16, 2, 128, 1, 1, 256), 1
     if kind == "c2_bilinear":
@@ -83,14 +83,15 @@ def _case(oracle, kind):
     raise KeyError(kind)
 
 
-def _step_case(oracle, kind, lod, n_tiles):
+def _step_case(oracle, kind, lod, n_tiles, dim=256, mip_min=4):
     """Inputs of one fused step, its FP64-oracle and BF16-rounding references,
     and the device-side arguments (without outputs)."""
     O = oracle
-    Lay, act = _case(O, kind)
+    Lay, act = _case(O, kind, dim, mip_min)
     Lay.activation = act
     lib = G.lib()
-    rng = np.random.default_rng(zlib.crc32(f"{kind}-{lod}".encode()))
+    rng = np.random.default_rng(zlib.crc32(f"{kind}-{lod}-{dim}".encode()) if dim != 256 else
+                                zlib.crc32(f"{kind}-{lod}".encode()))
     level = O.level_for_lod(Lay, lod)
     fd = O.feature_desc(Lay, level)
     M = Lay.mip_dim[lod]
@@ -142,6 +143,7 @@ def _step_case(oracle, kind, lod, n_tiles):
     a.loss_scale = float(scale)
     a.scatter = 1
     return dict(a=a, fdc=fdc, H=H, F=F, Od=Od, act=act, B=B, dec=dec, g0=g0, g1=g1, support=support,
+                fd=fd, origins=origins, T=T, M=M, bdx=np.ascontiguousarray(bdx),
                 ref=(sse, dw0, dw1, dwout, rdg0, rdg1), bref=(bsse, bw0, bw1, bwo, bdg0, bdg1))
 
 
@@ -402,3 +404,122 @@ def test_fused_sets_partials_repeat_bit_for_bit(oracle, kind, lod, n_tiles):
                     scale = float(np.max(np.abs(ref[s][k]))) + 1e-30
                     err = float(np.max(np.abs(got[s][k] - ref[s][k]))) / scale
                     assert err < 1e-5, (pass_texels, rep, s, k, err)
+
+
+def _sets_step_outputs(lib, cs, pass_texels, workgroups=0):
+    """One single-set multi-set launch at a pass size (on `workgroups`
+    workgroups, default as many as passes up to one per CU); returns the
+    reduced decoder gradient and the latent gradients."""
+    rb = lib.ntc_fused_params_bytes()
+    passes = -(-cs["B"] // pass_texels)
+    maxb = C.c_int32()
+    G.ok(lib.ntc_fused_max_blocks(C.byref(maxb)))
+    count = min(passes, workgroups or maxb.value)
+    out = _outputs(cs, count)
+    a = cs["a"]
+    a.dg0, a.dg1, a.dw_partials, a.sse_partials, a.blocks = (G.p(out["dg0"]), G.p(out["dg1"]), G.p(out["part"]),
+                                                             G.p(out["ssep"]), count)
+    rec = np.zeros(rb, np.uint8)
+    G.ok(lib.ntc_fused_params_pack(G.byref(a), rec.ctypes.data))
+    first = np.array([0, count], np.int32)
+    G.ok(lib.ntc_fused_train_sets(G.byref(cs["fdc"]), cs["H"], cs["act"], int(pass_texels), G.p(G.dev(rec)),
+                                  G.p(G.dev(first)), 1, G.p(G.dev(np.zeros(1, np.int32))), count, None))
+    n_dec = cs["dec"].size
+    grad = G.empty((n_dec,), np.float32)
+    m = G.empty((n_dec,), np.float32)
+    v = G.empty((n_dec,), np.float32)
+    sse_out = G.empty((1,), np.float64)
+    G.ok(lib.ntc_decoder_update(cs["H"], cs["F"], cs["Od"], count, G.p(out["part"]), 0.0, 0.9, 0.999, 1.0, 1.0,
+                                G.p(grad), G.p(m), G.p(v), G.p(G.dev(cs["dec"])), G.p(out["ssep"]), G.p(sse_out),
+                                None))
+    return G.host(grad), G.host(out["dg0"]), G.host(out["dg1"]), float(G.host(sse_out)[0])
+
+
+@pytest.mark.parametrize("kind", ["c1_bilinear_relu", "c2_bilinear"])
+@pytest.mark.parametrize("lod,n_tiles", [(0, 2), (1, 1), (2, 3), (3, 4), (4, 5), (5, 2), (6, 7)])
+def test_fused_pass_sizes_agree_elementwise(oracle, kind, lod, n_tiles):
+    """The 64- and 128-texel pass kernels form the same BF16 products per
+    texel and differ only in how the FP32 sums are grouped, so on the same
+    inputs every decoder-gradient and latent-gradient element agrees to FP32
+    rounding (relative to the largest element)."""
+    lib = G.lib()
+    cs = _step_case(oracle, kind, lod, n_tiles)
+    sizes = (C.c_int32 * 2)()
+    assert lib.ntc_fused_pass_sizes(G.byref(cs["fdc"]), cs["H"], cs["act"], sizes) == 2
+    r128 = _sets_step_outputs(lib, cs, 128)
+    r64 = _sets_step_outputs(lib, cs, 64)
+    assert abs(r64[3] - r128[3]) <= 1e-9 * r128[3]
+    for k, name in ((0, "dW"), (1, "dG0"), (2, "dG1")):
+        scale = float(np.max(np.abs(r128[k]))) + 1e-30
+        err = np.abs(r64[k] - r128[k]) / scale
+        worst = int(np.argmax(err))
+        assert float(err.max()) < 1e-5, (name, float(err.max()), worst, float(r64[k].ravel()[worst]),
+                                         float(r128[k].ravel()[worst]))
+
+
+@pytest.mark.parametrize("dim,lod,n_tiles", [(1024, 0, 1), (1024, 0, 3), (1024, 1, 2), (1024, 2, 1), (1024, 3, 2),
+                                             (2048, 0, 1), (2048, 0, 2), (2048, 1, 1), (2048, 2, 2), (2048, 3, 1)])
+def test_fused_pass_sizes_agree_on_large_mips(oracle, dim, lod, n_tiles):
+    """The same element-wise agreement of the 64- and 128-texel kernels on
+    the batched configuration's layout with 256^2 tiles inside larger mips
+    (tiles that stop short of the mip edge as well as wrapping ones), and the
+    64-texel step against both references."""
+    lib = G.lib()
+    cs = _step_case(oracle, "c1_bilinear_relu", lod, n_tiles, dim)
+    r128 = _sets_step_outputs(lib, cs, 128)
+    r64 = _sets_step_outputs(lib, cs, 64)
+    assert abs(r64[3] - r128[3]) <= 1e-9 * r128[3]
+    for k, name in ((0, "dW"), (1, "dG0"), (2, "dG1")):
+        scale = float(np.max(np.abs(r128[k]))) + 1e-30
+        err = np.abs(r64[k] - r128[k]) / scale
+        worst = int(np.argmax(err))
+        assert float(err.max()) < 1e-5, (name, float(err.max()), worst, float(r64[k].ravel()[worst]),
+                                         float(r128[k].ravel()[worst]))
+
+
+@pytest.mark.parametrize("dim,lod,n_tiles", [(256, 0, 1), (1024, 0, 1), (2048, 0, 1), (2048, 1, 2), (256, 3, 4)])
+@pytest.mark.parametrize("workgroups", [16, 5, 1])
+def test_fused_pass_sizes_agree_with_many_passes_per_workgroup(oracle, dim, lod, n_tiles, workgroups):
+    """As above with few workgroups, so that each runs many passes through the
+    cross-pass prefetch pipeline (a set of the batched configuration gets
+    about 16 workgroups of a launch)."""
+    lib = G.lib()
+    cs = _step_case(oracle, "c1_bilinear_relu", lod, n_tiles, dim)
+    r128 = _sets_step_outputs(lib, cs, 128, workgroups)
+    r64 = _sets_step_outputs(lib, cs, 64, workgroups)
+    full = _sets_step_outputs(lib, cs, 128)
+    assert abs(r64[3] - r128[3]) <= 1e-9 * r128[3]
+    for k, name in ((0, "dW"), (1, "dG0"), (2, "dG1")):
+        scale = float(np.max(np.abs(full[k]))) + 1e-30
+        for got, label in ((r64[k], "64"), (r128[k], "128")):
+            err = np.abs(got - full[k]) / scale
+            worst = int(np.argmax(err))
+            assert float(err.max()) < 1e-5, (name, label, float(err.max()), worst, float(got.ravel()[worst]),
+                                             float(full[k].ravel()[worst]))
+
+
+@pytest.mark.parametrize("lod,n_tiles", [(7, 1), (8, 1), (7, 3), (8, 2)])
+def test_fused_smallest_mips(oracle, lod, n_tiles):
+    """Batches of 1-12 texels (the 2x2 and 1x1 mips of a full chain): both
+    pass sizes against both references and against each other."""
+    lib = G.lib()
+    cs = _step_case(oracle, "c1_bilinear_relu", lod, n_tiles, 256, 1)
+    assert cs["B"] == n_tiles * (256 >> lod) ** 2
+    r128 = _sets_step_outputs(lib, cs, 128)
+    r64 = _sets_step_outputs(lib, cs, 64)
+    assert abs(r64[3] - r128[3]) <= 1e-9 * r128[3]
+    for k, name in ((0, "dW"), (1, "dG0"), (2, "dG1")):
+        scale = float(np.max(np.abs(r128[k]))) + 1e-30
+        err = np.abs(r64[k] - r128[k]) / scale
+        assert float(err.max()) < 1e-5, (name, float(err.max()))
+    bsse, bw0, bw1, bwo, bdg0, bdg1 = cs["bref"]
+    H, F, Od = cs["H"], cs["F"], cs["Od"]
+    for r, label in ((r64, "64"), (r128, "128")):
+        g = r[0]
+        gw0, gw1, gwo = g[: H * F].reshape(H, F), g[H * F: H * F + H * H].reshape(H, H), g[H * F + H * H:].reshape(16, H)
+        assert abs(r[3] - bsse) <= 1e-4 * bsse, label
+        for got, want, name in ((gw0, bw0, "dW0"), (gw1, bw1, "dW1"), (gwo[:Od], bwo, "dWout"), (r[1], bdg0, "dG0"),
+                                (r[2], bdg1, "dG1")):
+            assert rel(got, want) < REL_TOL_BF16, (label, name, rel(got, want))
+        for got, sup, name in ((r[1], cs["support"][0], "dG0"), (r[2], cs["support"][1], "dG1")):
+            assert not got[~sup].any(), (label, name)
