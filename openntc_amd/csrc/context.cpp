@@ -344,9 +344,6 @@ Result Context::Impl::InitDevice()
   int count = 0;
   if (hipGetDeviceCount(&count) != hipSuccess || count < 1) return Result::CudaFailure;
   CTX_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), Result::CudaFailure);
-  CTX_HIP(hipStreamCreateWithFlags(&side_stream, hipStreamNonBlocking), Result::CudaFailure);
-  CTX_HIP(hipEventCreateWithFlags(&fork_ev, hipEventDisableTiming), Result::CudaFailure);
-  CTX_HIP(hipEventCreateWithFlags(&join_ev, hipEventDisableTiming), Result::CudaFailure);
   const size_t dbytes = sizeof(float) * (size_t)DecoderElems();
   CTX_HIP(dec.Alloc(dbytes), Result::AllocationFailure);
   CTX_HIP(dec_m.Alloc(dbytes), Result::AllocationFailure);
@@ -388,11 +385,6 @@ void Context::Impl::FreeAll()
   plain_batch = 0;
   decode_table_prec = -1;
   if (stream) (void)hipStreamDestroy(stream);
-  if (side_stream) (void)hipStreamDestroy(side_stream);
-  if (fork_ev) (void)hipEventDestroy(fork_ev);
-  if (join_ev) (void)hipEventDestroy(join_ev);
-  side_stream = nullptr;
-  fork_ev = join_ev = nullptr;
   stream = nullptr;
 }
 
@@ -678,10 +670,10 @@ Result Context::Impl::EnsureFusedBuffers()
   return Result::Success;
 }
 
-Result Context::Impl::PackDecoderTables(hipStream_t s)
+Result Context::Impl::PackDecoderTables()
 {
   const ntc_feature_desc fd = FeatureDesc(0);
-  CTX_NTC(ntc_pack_decoder_bf16(&fd, hidden_dim, out_dim, dec.as<float>(), wtab.as<uint16_t>(), s));
+  CTX_NTC(ntc_pack_decoder_bf16(&fd, hidden_dim, out_dim, dec.as<float>(), wtab.as<uint16_t>(), stream));
   return Result::Success;
 }
 
@@ -730,17 +722,12 @@ Result Context::Impl::RunFusedStep(const StepPlan& st, int32_t step_index, hipEv
   if (ev) CTX_HIP(hipEventRecord(ev[0], stream), Result::CudaFailure);
   CTX_NTC(ntc_fused_train_step(&a, stream));
   if (ev) CTX_HIP(hipEventRecord(ev[1], stream), Result::CudaFailure);
-  // decoder update + table refresh on the side stream, concurrent with the
-  // grid update below (both read only what the fused step wrote)
-  CTX_HIP(hipEventRecord(fork_ev, stream), Result::CudaFailure);
-  CTX_HIP(hipStreamWaitEvent(side_stream, fork_ev, 0), Result::CudaFailure);
   CTX_NTC(ntc_decoder_update(H, feature_dim, O, a.blocks, dw_part.as<float>(), st.lr_decoder, 0.9f, 0.999f, st.bias1,
                              st.bias2, dec_grad.as<float>(), dec_m.as<float>(), dec_v.as<float>(), dec.as<float>(),
-                             sse_part.as<double>(), scalars.as<double>() + 16, side_stream));
-  Result r = PackDecoderTables(side_stream);
+                             sse_part.as<double>(), scalars.as<double>() + 16, stream));
+  Result r = PackDecoderTables();
   if (r != Result::Success) return r;
-  if (ev) CTX_HIP(hipEventRecord(ev[2], side_stream), Result::CudaFailure);
-  CTX_HIP(hipEventRecord(join_ev, side_stream), Result::CudaFailure);
+  if (ev) CTX_HIP(hipEventRecord(ev[2], stream), Result::CudaFailure);
   if (st.grid_update)
   {
     if (ev) CTX_HIP(hipEventRecord(ev[3], stream), Result::CudaFailure);
@@ -755,8 +742,6 @@ Result Context::Impl::RunFusedStep(const StepPlan& st, int32_t step_index, hipEv
                                    0.9f, 0.999f, st.grid_bias1, st.grid_bias2, stream));
     if (ev) CTX_HIP(hipEventRecord(ev[4], stream), Result::CudaFailure);
   }
-  // the next step (and anything else on stream) sees the new decoder
-  CTX_HIP(hipStreamWaitEvent(stream, join_ev, 0), Result::CudaFailure);
   grid_batch_i[lvl] += 1;
   return Result::Success;
 }
@@ -1275,7 +1260,7 @@ void Context::BeginTrainingEx(const TrainInfoEx& info)
     if (!ntc_fused_supported(&fd, I.hidden_dim, I.ActCode())) return fail(Result::InvalidState);
     Result r = I.EnsureFusedBuffers();
     if (r != Result::Success) return fail(r);
-    r = I.PackDecoderTables(I.stream);
+    r = I.PackDecoderTables();
     if (r != Result::Success) return fail(r);
   }
 

@@ -74,11 +74,6 @@ struct Context::Impl
 
   // ---- device state ----
   hipStream_t stream = nullptr;
-  // The fused path's decoder update and table refresh depend only on the
-  // fused step, so they run on side_stream while the grid update runs on
-  // stream; join_ev makes the next step wait for both.
-  hipStream_t side_stream = nullptr;
-  hipEvent_t fork_ev = nullptr, join_ev = nullptr;
   DeviceBuffer g0[kMaxLevels], g1[kMaxLevels];
   DeviceBuffer g0m[kMaxLevels], g0v[kMaxLevels], g1m[kMaxLevels], g1v[kMaxLevels];
   DeviceBuffer dg0[kMaxLevels], dg1[kMaxLevels];
@@ -147,7 +142,7 @@ struct Context::Impl
   void BuildPlan(int32_t steps);
   Result RunPlainStep(const StepPlan& st, int32_t step_index);
   Result EnsureFusedBuffers();
-  Result PackDecoderTables(hipStream_t s);
+  Result PackDecoderTables();
   Result RunFusedStep(const StepPlan& st, int32_t step_index, hipEvent_t* ev);
   int ActCode() const { return activation == Activation::HardGELU ? NTC_ACT_HARDGELU : NTC_ACT_RELU; }
   // Blocking copy ordered after all work queued on the context's stream. The

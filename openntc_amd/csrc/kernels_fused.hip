@@ -1302,8 +1302,19 @@ __device__ __forceinline__ void decoder_update_elem(int e, int n_dec, int n_adam
   if (e >= n_dec) return;
   // Wout rows >= O are outside the optimised set (libopenntc.cpp:722-732)
   if (e >= n_adam_w01 && (e - n_adam_w01) / H >= O) return;
+  // partials summed in block order; the loads of 8 partials are in flight
+  // at once (the adds stay sequential)
   float gsum = 0.0f;
-  for (int b = 0; b < blocks; ++b) gsum = gsum + part[(int64_t)b * n_dec + e];
+  int b = 0;
+  for (; b + 8 <= blocks; b += 8)
+  {
+    float v[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) v[k] = part[(int64_t)(b + k) * n_dec + e];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) gsum = gsum + v[k];
+  }
+  for (; b < blocks; ++b) gsum = gsum + part[(int64_t)b * n_dec + e];
   grad[e] = gsum;
   const float mi = fmaf(c.b1, m[e], c.omb1 * gsum);
   const float vi = fmaf(c.b2, v[e], (c.omb2 * gsum) * gsum);

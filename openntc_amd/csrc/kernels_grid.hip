@@ -137,56 +137,32 @@ __device__ __forceinline__ void grid_update_tiles_body(const TileUpdate& A, cons
   }
   __syncthreads();
   const f4 zero = {0.f, 0.f, 0.f, 0.f};
-  const int64_t stride = (int64_t)vgrid * kThreads;
-  // two items per thread and iteration, all loads issued before the
-  // arithmetic; a second item past the end re-reads the first and stores
-  // nothing
-  for (int64_t i0 = (int64_t)vblk * kThreads + tid; i0 < A.n4; i0 += 2 * stride)
+  for (int64_t i = (int64_t)vblk * kThreads + tid; i < A.n4; i += (int64_t)vgrid * kThreads)
   {
-    f4 mi[2], vi[2], pi[2], gi[2];
-    bool hot[2], live[2];
-    int gsel[2];
-    int64_t kk[2];
+    const int grid = i >= A.a4;
+    const GridSeg& s = A.s[grid];
+    const int64_t k = grid ? i - A.a4 : i;
+    const int D = A.D[grid];
+    const uint32_t cell = (uint32_t)k / (uint32_t)A.Q[grid];
+    const int y = (int)(cell / (uint32_t)D), x = (int)cell - y * D;
+    const bool hot = in_rects(rect[grid], n, x, y, D);
+    f4 mi = __builtin_nontemporal_load(&s.m[k]);
+    f4 vi = __builtin_nontemporal_load(&s.v[k]);
+    f4 pi = __builtin_nontemporal_load(&s.p[k]);
+    const f4 gi = hot ? __builtin_nontemporal_load(&s.g[k]) : zero;
 #pragma unroll
-    for (int u = 0; u < 2; ++u)
+    for (int e = 0; e < 4; ++e)
     {
-      const int64_t iu = i0 + u * stride;
-      live[u] = iu < A.n4;
-      const int64_t i = live[u] ? iu : i0;
-      const int grid = i >= A.a4;
-      const GridSeg& s = A.s[grid];
-      const int64_t k = grid ? i - A.a4 : i;
-      const int D = A.D[grid];
-      const uint32_t cell = (uint32_t)k / (uint32_t)A.Q[grid];
-      const int y = (int)(cell / (uint32_t)D), x = (int)cell - y * D;
-      hot[u] = in_rects(rect[grid], n, x, y, D);
-      gsel[u] = grid;
-      kk[u] = k;
-      mi[u] = __builtin_nontemporal_load(&s.m[k]);
-      vi[u] = __builtin_nontemporal_load(&s.v[k]);
-      pi[u] = __builtin_nontemporal_load(&s.p[k]);
-      gi[u] = hot[u] ? __builtin_nontemporal_load(&s.g[k]) : zero;
+      float me = mi[e], ve = vi[e], pe = pi[e];
+      adam_one(A.c, gi[e], me, ve, pe);
+      mi[e] = me;
+      vi[e] = ve;
+      pi[e] = fmaxf(s.lo, fminf(s.hi, pe));
     }
-#pragma unroll
-    for (int u = 0; u < 2; ++u)
-    {
-      const GridSeg& s = A.s[gsel[u]];
-#pragma unroll
-      for (int e = 0; e < 4; ++e)
-      {
-        float me = mi[u][e], ve = vi[u][e], pe = pi[u][e];
-        adam_one(A.c, gi[u][e], me, ve, pe);
-        mi[u][e] = me;
-        vi[u][e] = ve;
-        pi[u][e] = fmaxf(s.lo, fminf(s.hi, pe));
-      }
-      if (!live[u]) continue;
-      const int64_t k = kk[u];
-      __builtin_nontemporal_store(mi[u], &s.m[k]);
-      __builtin_nontemporal_store(vi[u], &s.v[k]);
-      __builtin_nontemporal_store(pi[u], &s.p[k]);
-      if (hot[u]) __builtin_nontemporal_store(zero, &s.g[k]);
-    }
+    __builtin_nontemporal_store(mi, &s.m[k]);
+    __builtin_nontemporal_store(vi, &s.v[k]);
+    __builtin_nontemporal_store(pi, &s.p[k]);
+    if (hot) __builtin_nontemporal_store(zero, &s.g[k]);
   }
 }
 
