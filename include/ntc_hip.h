@@ -528,6 +528,27 @@ int ntc_batch_train(ntc_batch* b, int32_t n_steps, int32_t use_graph);
 int ntc_batch_get_stats(const ntc_batch* b, ntc_batch_stats* out);
 void ntc_batch_destroy(ntc_batch* b);
 
+/* ---- texture manifest front end (Context::LoadManifest, libopenntc.cpp:
+ * 1103-1207, 1296-1313), GPU-free parts -------------------------------------
+ * ntc_manifest_parse checks a manifest with the reference's rules and returns
+ * its sources in training order (stable sort by semantic), then decodes every
+ * image; the status is what LoadManifest returns before any GPU work.
+ * ntc_image_decode_rgba8 decodes one image (PNG, TGA, binary PNM) to RGBA8
+ * (rgba may be NULL to query the size). */
+typedef struct ntc_manifest_source
+{
+  char name[64];
+  char path[256];
+  char channels[8];
+  int32_t semantic;
+  int32_t is_srgb, vertical_flip;
+} ntc_manifest_source;
+
+int ntc_manifest_parse(const char* path, int32_t min_dimension, int32_t* dim,
+                       int32_t* count, ntc_manifest_source* sources /* 16 */);
+int ntc_image_decode_rgba8(const char* path, int32_t* width, int32_t* height,
+                           uint8_t* rgba, size_t capacity);
+
 /* ---- synthetic PBR set ----------------------------------------------------
  * Seeded 9-channel set: albedo RGB (value noise through sRGB decode),
  * normal RGB (3-octave value-noise height field), roughness / metallic / AO

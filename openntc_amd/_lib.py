@@ -97,6 +97,11 @@ class StepInfo(C.Structure):
                 ("lock", C.c_int32)]
 
 
+class ManifestSource(C.Structure):
+    _fields_ = [("name", C.c_char * 64), ("path", C.c_char * 256), ("channels", C.c_char * 8),
+                ("semantic", C.c_int32), ("is_srgb", C.c_int32), ("vertical_flip", C.c_int32)]
+
+
 class BatchStats(C.Structure):
     _fields_ = [("n_sets", C.c_int32), ("steps", C.c_int32), ("steps_done", C.c_int32),
                 ("pass_texels", C.c_int32), ("fused_grid", C.c_int32), ("grid_update_grid", C.c_int32),
@@ -208,6 +213,8 @@ SIGNATURES = {
     "ntc_batch_train": (_I, [_P, _I, _I]),
     "ntc_batch_get_stats": (_I, [_P, _P]),
     "ntc_batch_destroy": (None, [_P]),
+    "ntc_manifest_parse": (_I, [C.c_char_p, _I, _P, _P, _P]),
+    "ntc_image_decode_rgba8": (_I, [C.c_char_p, _P, _P, _P, _SZ]),
 }
 
 _lib = None

@@ -21,7 +21,7 @@ LIB = os.path.join(HERE, "libopenntc_amd.so")
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 
 HIP_SOURCES = ["kernels_plain.hip", "kernels_grid.hip", "kernels_tex.hip", "kernels_fused.hip", "kernels_eval.hip"]
-CXX_SOURCES = ["context.cpp", "runtime.cpp", "capi.cpp", "synthetic.cpp", "batch.cpp"]
+CXX_SOURCES = ["context.cpp", "runtime.cpp", "capi.cpp", "synthetic.cpp", "batch.cpp", "manifest.cpp"]
 
 INCLUDES = ["-I", os.path.join(ROOT, "include"), "-I", CSRC]
 
@@ -58,7 +58,7 @@ def build(jobs: int = 8, verbose: bool = True) -> str:
         objs = list(ex.map(_compile, sources))
     tmp = LIB + ".tmp"
     cmd = [os.path.join(ROCM, "bin", "hipcc"), "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp,
-           *objs, "-lpthread"]
+           *objs, "-lpthread", "-lz"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")

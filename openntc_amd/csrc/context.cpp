@@ -1149,18 +1149,6 @@ Result Context::LoadTextures(const TextureInput* textures, int32_t count, int32_
   return Result::Success;
 }
 
-// The manifest front end (JSON + image decode, libopenntc.cpp:1125-1207,
-// 1296-1313) is not part of this build: the image decoder the reference
-// vendors is third-party. In-memory textures go through LoadTextures.
-Result Context::LoadManifest(const std::string& filepath)
-{
-  UnloadManifest();
-  FILE* f = fopen(filepath.c_str(), "rb");
-  if (!f) return Result::FileNotFound;
-  fclose(f);
-  return Result::InvalidManifest;
-}
-
 void Context::UnloadManifest()
 {
   Impl& I = *impl_;

@@ -193,3 +193,35 @@ def test_c2_2048_fused_bf16_reaches_plain_fp32_psnr():
     print(res)
     assert abs(res["fused_a"][0] - res["plain"][0]) < 0.05, res
     assert abs(res["fused_a"][0] - res["fused_b"][0]) < 0.01, res
+
+
+def test_load_manifest_matches_load_textures(tmp_path):
+    """Context::LoadManifest on a JSON manifest of PNG / TGA files builds the
+    same mip targets as LoadTextures of the same pixels (bit for bit), the
+    same layout, and trains."""
+    from tests.test_manifest import _manifest, write_png, write_tga
+    dim = 256
+    tex = ntc.synthetic_pbr(dim, 77)
+    albedo, normal, orm = tex[0][0], tex[1][0], tex[2][0]
+    write_png(tmp_path / "albedo.png", albedo[..., :3], 2, 8)
+    write_png(tmp_path / "normal.png", normal[..., :3], 2, 8, interlace=1)
+    write_tga(tmp_path / "orm.tga", orm, rle=True)
+    m = _manifest(tmp_path, [
+        {"fileName": "orm.tga", "semantics": {"AO": "R", "Roughness": "G", "Metallic": "B"}},
+        {"fileName": "albedo.png", "semantics": {"Albedo": "RGB"}},
+        {"fileName": "normal.png", "semantics": {"Normal": "RGB"}}], width=dim, height=dim)
+    lay = ntc.LayoutInfo(*LAYOUT_ARGS, ntc.FeatureLayout(1), ntc.Activation(1))
+    a, b = ntc.Context(), ntc.Context()
+    for c in (a, b):
+        assert c.InitEx(layout=lay, min_dimension=64) == ntc.Result.Success
+    assert a.LoadManifest(str(m)) == ntc.Result.Success
+    assert b.LoadTextures(tex, dim) == ntc.Result.Success
+    la, lb = a.layout(), b.layout()
+    assert (la.mip_count, la.out_dim, la.feature_dim) == (lb.mip_count, lb.out_dim, lb.feature_dim)
+    for mip in range(la.mip_count):
+        assert np.array_equal(a.read_package(mip), b.read_package(mip)), mip
+    assert a.BeginTrainingEx(20, 1, 1e-2, 1e-3, -0.5, 0.5, ntc.TrainPath.Fused) == ntc.Result.Success
+    assert a.TrainUntilComplete().phase == ntc.TrainPhase.TrainComplete
+    assert np.isfinite(a.Eval().psnr)
+    # errors come back as the reference's codes
+    assert a.LoadManifest(str(tmp_path / "nothere.json")) == ntc.Result.FileNotFound
