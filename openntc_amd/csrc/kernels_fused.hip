@@ -71,6 +71,21 @@ __device__ __forceinline__ uint16_t frag_elem(const bf16x8& f, int e)
   return (uint16_t)((e & 1) ? (w >> 16) : (w & 0xFFFFu));
 }
 
+// What the activation backward of hidden unit 16 r + 4 (l/16) + i (accumulator
+// layout) reads. HardGELU needs the FP32 pre-activation itself; ReLU only its
+// sign, and the BF16 activation of the same unit (fragment r/2, element
+// 4 (r%2) + i) is positive exactly when the pre-activation is (bar positive
+// values below 2^-133, which round to zero), so for ReLU the FP32
+// pre-activations need not stay live through the backward.
+template <int ACT, int MT, int KH, int NT>
+__device__ __forceinline__ float pre_act(const f32x4 (&z)[MT][NT], const bf16x8 (&af)[KH][NT], int r, int nt, int i)
+{
+  if constexpr (ACT == NTC_ACT_RELU)
+    return __uint_as_float((uint32_t)frag_elem(af[r >> 1][nt], 4 * (r & 1) + i) << 16);
+  else
+    return z[r][nt][i];
+}
+
 __device__ __forceinline__ f32x4 mfma(const bf16x8& a, const bf16x8& b, const f32x4& c)
 {
   return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
@@ -946,7 +961,7 @@ __device__ __forceinline__ void fused_body(const FusedParams& P, const int vblk,
         {
           const f32x4 da = mfma(a, dyf[nt], f32x4{0, 0, 0, 0});
 #pragma unroll
-          for (int i = 0; i < 4; ++i) dz1[r][nt][i] = act_bwd<float>(ACT, z1[r][nt][i], da[i]);
+          for (int i = 0; i < 4; ++i) dz1[r][nt][i] = act_bwd<float>(ACT, pre_act<ACT>(z1, a1f, r, nt, i), da[i]);
         }
       }
 #pragma unroll
@@ -971,20 +986,20 @@ __device__ __forceinline__ void fused_body(const FusedParams& P, const int vblk,
 #pragma unroll
       for (int r = 0; r < MT; ++r)
       {
-        f32x4 da[kNT];
 #pragma unroll
-        for (int nt = 0; nt < kNT; ++nt) da[nt] = f32x4{0, 0, 0, 0};
+        for (int nt = 0; nt < kNT; ++nt) dz0[r][nt] = f32x4{0, 0, 0, 0};
 #pragma unroll
         for (int j = 0; j < KH; ++j)
         {
           const bf16x8 a = frag_lds(&wt[TLY.oW1T() + (16 * r + n16) * H + 32 * j + 8 * g]);
 #pragma unroll
-          for (int nt = 0; nt < kNT; ++nt) da[nt] = mfma(a, dz1f[j][nt], da[nt]);
+          for (int nt = 0; nt < kNT; ++nt) dz0[r][nt] = mfma(a, dz1f[j][nt], dz0[r][nt]);
         }
 #pragma unroll
         for (int nt = 0; nt < kNT; ++nt)
 #pragma unroll
-          for (int i = 0; i < 4; ++i) dz0[r][nt][i] = act_bwd<float>(ACT, z0[r][nt][i], da[nt][i]);
+          for (int i = 0; i < 4; ++i)
+            dz0[r][nt][i] = act_bwd<float>(ACT, pre_act<ACT>(z0, a0f, r, nt, i), dz0[r][nt][i]);
       }
 #pragma unroll
       for (int j = 0; j < KH; ++j)

