@@ -215,6 +215,20 @@ typedef struct ntc_grid_adam
 int ntc_grid_update_tiles_f32(const ntc_feature_desc* fd, const ntc_tiles* tiles,
                               float* g0, float* dg0, float* g1, float* dg1,
                               const ntc_grid_adam* adam, hipStream_t stream);
+/* One part of ntc_grid_update_tiles_f32 (together the two parts give its
+ * result bit for bit): NTC_TILES_COLD, the cells outside every footprint
+ * rectangle (dg not touched; this part reads and writes nothing that the
+ * same step's training kernel reads or writes, so it may run on another
+ * stream beside it), or NTC_TILES_HOT, the cells inside (dg read and
+ * cleared). blocks: launch width, <= 0 for the default. Grid sides must be
+ * powers of two and channel counts at most 16. No counterpart in the
+ * reference, whose update is one dense pass (ntc_kernel.cu:375-410). */
+#define NTC_TILES_COLD 1
+#define NTC_TILES_HOT 2
+int ntc_grid_update_tiles_part_f32(const ntc_feature_desc* fd, const ntc_tiles* tiles,
+                                   float* g0, float* dg0, float* g1, float* dg1,
+                                   const ntc_grid_adam* adam, int32_t part,
+                                   int32_t blocks, hipStream_t stream);
 /* zero-fill by kernel (graph-safe replacement for memset) */
 int ntc_fill_zero(void* p, size_t bytes, hipStream_t stream);
 
@@ -257,6 +271,8 @@ int ntc_fused_blocks_for(const ntc_feature_desc* fd, int32_t hidden_dim,
 int ntc_pack_decoder_bf16(const ntc_feature_desc* fd, int32_t hidden_dim,
                           int32_t out_dim, const float* dec, uint16_t* table,
                           hipStream_t stream);
+/* tiles.tile_dim must be a power of two (NTC_E_INVALID otherwise), as every
+ * mip side of a Context is. */
 int ntc_fused_train_step(const ntc_fused_args* args, hipStream_t stream);
 /* Sums the workgroup partials in order into grad, then Adam on W0 | W1 |
  * Wout rows < O (update_adam x3, libopenntc.cpp:722-754); sse_out (may be
@@ -300,7 +316,7 @@ typedef struct ntc_pack_set
 
 /* Size of one opaque per-(step, set) record of the fused launch, and the
  * record of the arguments a (a->blocks is not used: the set's workgroup
- * range comes from wg_first). */
+ * range comes from wg_first; tile_dim a power of two as above). */
 size_t ntc_fused_params_bytes(void);
 int ntc_fused_params_pack(const ntc_fused_args* a, void* record);
 /* Texels per workgroup pass of the compiled multi-set kernels of a shape
@@ -493,8 +509,11 @@ int ntc_ctx_get_compressed(ntc_context* ctx, uint8_t* g0, size_t* g0_size,
 int ntc_ctx_dump(ntc_context* ctx, const char* path);
 /* Device pointer to the context's stream (for timing with events). */
 void* ntc_ctx_stream(ntc_context* ctx);
-/* Average duration (ms) of the dominant kernels over the last Train call,
- * measured with events on the context's stream; -1 when not recorded. */
+/* Average durations over the last Train call with timing on, measured with
+ * events: {fused kernel, decoder update + table pack, grid update, steps
+ * timed, grid updates timed, grid updates that skipped the zero gradient
+ * outside the tile footprints, cold part, hot part (ms; the split update),
+ * steps with the split update}; -1 when not recorded. */
 int ntc_ctx_kernel_times(ntc_context* ctx, float* out, int32_t n);
 int ntc_ctx_set_timing(ntc_context* ctx, int32_t enabled);
 

@@ -193,6 +193,7 @@ SIGNATURES = {
     "ntc_pack_decoder_bf16": (_I, [_P, _I, _I, _P, _P, _P]),
     "ntc_fused_train_step": (_I, [_P, _P]),
     "ntc_grid_update_tiles_f32": (_I, [_P, _P, _P, _P, _P, _P, _P, _P]),
+    "ntc_grid_update_tiles_part_f32": (_I, [_P, _P, _P, _P, _P, _P, _P, _I, _I, _P]),
     "ntc_decode_table_bytes": (C.c_size_t, [_I, _I, _I, _I]),
     "ntc_build_decode_table_f": (_I, [_I, _I, _I, _I, _P, _F, _F, _P]),
     "ntc_build_decode_table_int8": (_I, [_I, _I, _I, _I, _P, _F, _F, _P]),

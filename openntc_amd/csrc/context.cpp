@@ -344,6 +344,9 @@ Result Context::Impl::InitDevice()
   int count = 0;
   if (hipGetDeviceCount(&count) != hipSuccess || count < 1) return Result::CudaFailure;
   CTX_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), Result::CudaFailure);
+  CTX_HIP(hipStreamCreateWithFlags(&side, hipStreamNonBlocking), Result::CudaFailure);
+  CTX_HIP(hipEventCreateWithFlags(&fork_ev, hipEventDisableTiming), Result::CudaFailure);
+  CTX_HIP(hipEventCreateWithFlags(&join_ev, hipEventDisableTiming), Result::CudaFailure);
   const size_t dbytes = sizeof(float) * (size_t)DecoderElems();
   CTX_HIP(dec.Alloc(dbytes), Result::AllocationFailure);
   CTX_HIP(dec_m.Alloc(dbytes), Result::AllocationFailure);
@@ -385,7 +388,11 @@ void Context::Impl::FreeAll()
   plain_batch = 0;
   decode_table_prec = -1;
   if (stream) (void)hipStreamDestroy(stream);
-  stream = nullptr;
+  if (side) (void)hipStreamDestroy(side);
+  if (fork_ev) (void)hipEventDestroy(fork_ev);
+  if (join_ev) (void)hipEventDestroy(join_ev);
+  stream = side = nullptr;
+  fork_ev = join_ev = nullptr;
 }
 
 Result Context::Impl::EnsurePlainBuffers(int64_t batch)
@@ -681,6 +688,13 @@ Result Context::Impl::PackDecoderTables()
 // loss, backward and latent scatter; the ordered decoder-gradient reduction
 // with Adam; the BF16 table refresh; one dense update of both grids of the
 // level (Adam + clamp + gradient reset). ev: 6 events or null.
+//
+// When the level update can skip the zero gradient outside the tile
+// footprints, its cold part (those cells: no gradient access, nothing the
+// fused kernel reads or writes) runs on the side stream beside the fused
+// kernel, which then leaves cold_reserve CUs to it; the hot part (inside the
+// footprints) follows the fused kernel on the stream, and the stream waits
+// for the cold part before anything after the step.
 Result Context::Impl::RunFusedStep(const StepPlan& st, int32_t step_index, hipEvent_t* ev)
 {
   const int lvl = st.level, T = st.tile_dim, M = mip_dim[st.lod];
@@ -711,24 +725,48 @@ Result Context::Impl::RunFusedStep(const StepPlan& st, int32_t step_index, hipEv
   a.dg1 = dg1[lvl].as<float>();
   a.dw_partials = dw_part.as<float>();
   a.sse_partials = sse_part.as<double>();
-  a.blocks = ntc_fused_blocks_for(&fd, H, act, B, max_blocks);
+  const bool tiles_update = st.grid_update && g0_channels % 4 == 0 && g1_channels % 4 == 0 && st.n_tiles <= 16;
+  const bool overlap = tiles_update && cold_reserve > 0 && cold_reserve < max_blocks;
+  a.blocks = ntc_fused_blocks_for(&fd, H, act, B, overlap ? max_blocks - cold_reserve : max_blocks);
   // dense Adam + clamp of the level: the gradient is zero outside the tile
   // footprints, so the update reads and clears it only inside them
   const ntc_grid_adam adam = {g0m[lvl].as<float>(), g0v[lvl].as<float>(), g1m[lvl].as<float>(),
                               g1v[lvl].as<float>(), st.lr_grid, 0.9f, 0.999f, st.grid_bias1, st.grid_bias2,
                               -1.0f, 1.0f - g0_delta, -1.0f, 1.0f - g1_delta};
-  const bool tiles_update = st.grid_update && g0_channels % 4 == 0 && g1_channels % 4 == 0 && st.n_tiles <= 16;
   if (tiles_update) ++tiles_update_steps;
+  if (overlap) CTX_HIP(hipEventRecord(fork_ev, stream), Result::CudaFailure);
   if (ev) CTX_HIP(hipEventRecord(ev[0], stream), Result::CudaFailure);
   CTX_NTC(ntc_fused_train_step(&a, stream));
   if (ev) CTX_HIP(hipEventRecord(ev[1], stream), Result::CudaFailure);
+  if (overlap)
+  {
+    CTX_HIP(hipStreamWaitEvent(side, fork_ev, 0), Result::CudaFailure);
+    if (ev) CTX_HIP(hipEventRecord(ev[3], side), Result::CudaFailure);
+    CTX_NTC(ntc_grid_update_tiles_part_f32(&fd, &a.tiles, g0[lvl].as<float>(), dg0[lvl].as<float>(),
+                                           g1[lvl].as<float>(), dg1[lvl].as<float>(), &adam, NTC_TILES_COLD, 0,
+                                           side));
+    if (ev) CTX_HIP(hipEventRecord(ev[4], side), Result::CudaFailure);
+    CTX_HIP(hipEventRecord(join_ev, side), Result::CudaFailure);
+  }
   CTX_NTC(ntc_decoder_update(H, feature_dim, O, a.blocks, dw_part.as<float>(), st.lr_decoder, 0.9f, 0.999f, st.bias1,
                              st.bias2, dec_grad.as<float>(), dec_m.as<float>(), dec_v.as<float>(), dec.as<float>(),
                              sse_part.as<double>(), scalars.as<double>() + 16, stream));
   Result r = PackDecoderTables();
   if (r != Result::Success) return r;
   if (ev) CTX_HIP(hipEventRecord(ev[2], stream), Result::CudaFailure);
-  if (st.grid_update)
+  if (overlap)
+  {
+    CTX_NTC(ntc_grid_update_tiles_part_f32(&fd, &a.tiles, g0[lvl].as<float>(), dg0[lvl].as<float>(),
+                                           g1[lvl].as<float>(), dg1[lvl].as<float>(), &adam, NTC_TILES_HOT, 0,
+                                           stream));
+    if (ev)
+    {
+      CTX_HIP(hipEventRecord(ev[5], stream), Result::CudaFailure);
+      ev_overlap[(size_t)(ev - events.data()) / 6] = 1;
+    }
+    CTX_HIP(hipStreamWaitEvent(stream, join_ev, 0), Result::CudaFailure);
+  }
+  else if (st.grid_update)
   {
     if (ev) CTX_HIP(hipEventRecord(ev[3], stream), Result::CudaFailure);
     if (tiles_update)
@@ -1310,6 +1348,7 @@ TrainProgress Context::Train(int32_t num_batches)
   }
   const int first = I.batch_i;
   I.tiles_update_steps = 0;
+  if (timed) I.ev_overlap.assign((size_t)n, 0);
   ++I.state_version;
   for (; I.batch_i < target; ++I.batch_i)
   {
@@ -1322,22 +1361,37 @@ TrainProgress Context::Train(int32_t num_batches)
   if (hipStreamSynchronize(I.stream) != hipSuccess) return error(Result::CudaFailure);
   if (timed)
   {
-    double fused = 0, decoder = 0, grid = 0;
-    int grids = 0;
+    double fused = 0, decoder = 0, grid = 0, cold = 0, hot = 0;
+    int grids = 0, split = 0;
     for (int s = 0; s < n; ++s)
     {
       hipEvent_t* ev = &I.events[(size_t)s * 6];
-      float ms = 0;
+      float ms = 0, ms2 = 0;
       if (hipEventElapsedTime(&ms, ev[0], ev[1]) == hipSuccess) fused += ms;
       if (hipEventElapsedTime(&ms, ev[1], ev[2]) == hipSuccess) decoder += ms;
-      if (I.plan[first + s].grid_update && hipEventElapsedTime(&ms, ev[3], ev[4]) == hipSuccess)
+      if (!I.plan[first + s].grid_update) continue;
+      if (I.ev_overlap[(size_t)s])
+      {
+        // cold part (beside the fused kernel) + hot part
+        if (hipEventElapsedTime(&ms, ev[3], ev[4]) == hipSuccess &&
+            hipEventElapsedTime(&ms2, ev[2], ev[5]) == hipSuccess)
+        {
+          cold += ms;
+          hot += ms2;
+          grid += ms + ms2;
+          ++grids;
+          ++split;
+        }
+      }
+      else if (hipEventElapsedTime(&ms, ev[3], ev[4]) == hipSuccess)
       {
         grid += ms;
         ++grids;
       }
     }
     I.kernel_ms = {(float)(fused / n), (float)(decoder / n), grids ? (float)(grid / grids) : -1.0f, (float)n,
-                   (float)grids, (float)I.tiles_update_steps};
+                   (float)grids, (float)I.tiles_update_steps, split ? (float)(cold / split) : -1.0f,
+                   split ? (float)(hot / split) : -1.0f, (float)split};
   }
   tp.phase_ = TrainPhase::TrainInProgress;
   tp.result_ = Result::Success;

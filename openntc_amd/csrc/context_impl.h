@@ -60,6 +60,10 @@ struct Context::Impl
   Activation activation = Activation::HardGELU;
   int32_t feature_dim = 0, feature_dim_padded = 0, out_dim = 0;
   static constexpr int32_t kOutPadded = 16;
+  // Measured at the 4096^2 workload (8 tiles, H = 128) with reserves of 0,
+  // 16, 32, 64, 80 and 96 CUs: 64 balances the fused kernel on the other
+  // CUs against the cold update on these (0.91 -> 0.80 ms per step).
+  static constexpr int32_t kColdReserve = 64;
   int32_t min_dimension = Context::kMinDimension;
   int32_t mip_min_dim = 4;
   uint32_t seed = 123;
@@ -74,6 +78,11 @@ struct Context::Impl
 
   // ---- device state ----
   hipStream_t stream = nullptr;
+  // The cold part of a step's grid update (cells outside the step's tile
+  // footprints) runs on side beside the step's fused kernel; fork and join
+  // order it after the previous work on stream and before the next.
+  hipStream_t side = nullptr;
+  hipEvent_t fork_ev = nullptr, join_ev = nullptr;
   DeviceBuffer g0[kMaxLevels], g1[kMaxLevels];
   DeviceBuffer g0m[kMaxLevels], g0v[kMaxLevels], g1m[kMaxLevels], g1v[kMaxLevels];
   DeviceBuffer dg0[kMaxLevels], dg1[kMaxLevels];
@@ -107,18 +116,24 @@ struct Context::Impl
   DeviceBuffer dw_part;       // [blocks][decoder elems]
   DeviceBuffer sse_part;      // [blocks]
   int32_t max_blocks = 0;
+  // CUs left free of fused workgroups for the concurrent cold grid update
+  // (the fused workgroups fill a CU each)
+  int32_t cold_reserve = kColdReserve;
 
   // ---- timing (optional) ----
   // When enabled, Train records events around the fused step kernel, the
   // decoder update (+ table pack) and the dense grid update of every step on
   // the context stream; kernel_ms holds {fused, decoder, grid, steps timed,
   // grid-update launches timed, grid updates that skipped the zero gradient
-  // outside the tile footprints} over the last Train call (durations
-  // averaged).
+  // outside the tile footprints, cold part, hot part, steps with the split
+  // update} over the last Train call (durations averaged; with the split
+  // update the grid figure is cold + hot, the cold part running beside the
+  // fused kernel).
   bool timing = false;
   int32_t tiles_update_steps = 0;
   std::vector<float> kernel_ms;
   std::vector<hipEvent_t> events;
+  std::vector<uint8_t> ev_overlap;  // per timed step: grid update split in cold + hot
 
   // ---- helpers ----
   int64_t DecoderElems() const

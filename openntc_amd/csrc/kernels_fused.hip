@@ -196,6 +196,8 @@ struct Cfg
   static_assert(16 + BH <= 128, "axis table threads");
   static_assert((PASS * XS * 2) % 16 == 0, "staged boxes 16-byte aligned");
   static_assert(lds_bytes(SW, SV, GS) <= LIM, "LDS budget");
+  // box cell indices (below WMAX * VMAX + 16) divided by box widths (at most WMAX)
+  static_assert((WMAX * VMAX + 16) * WMAX < 65536, "small_div range");
 };
 
 // Corner tables of one grid for a blocked pass: box-local corner columns
@@ -210,6 +212,16 @@ static_assert(sizeof(AxisTab) == 192, "axis table size in the LDS budget");
 // Where the texels of a workgroup pass lie. For tiles of side >= 16 a pass
 // is a 16 x BH block of one tile (block-major order), so its latent footprint
 // is compact; smaller tiles are walked linearly.
+// log2 of a power of two
+__device__ __forceinline__ int ilog2(int v) { return 31 - __builtin_clz((unsigned)v); }
+
+// Division by a small runtime divisor d as a multiply and shift:
+// q = (n * ceil(2^16 / d)) >> 16 equals n / d whenever n * d < 2^16
+// (the rounding-up error of the reciprocal, below d / 2^16 per unit of n,
+// then stays below the distance 1/d from n/d to the next integer).
+__device__ __forceinline__ uint32_t small_recip(int d) { return (65535u + (uint32_t)d) / (uint32_t)d; }
+__device__ __forceinline__ int small_div(int n, uint32_t recip) { return (int)(((uint32_t)n * recip) >> 16); }
+
 struct PassMap
 {
   bool blocked;
@@ -225,13 +237,14 @@ __device__ __forceinline__ PassMap pass_map(int64_t pass, int T, int pass_texels
   pm.by0 = 0;
   if (pm.blocked)
   {
-    const int bh = pass_texels / 16;
-    const int per_row = T / 16;
-    const int64_t per_tile = (int64_t)per_row * (T / bh);
-    pm.b = (int)(pass / per_tile);
-    const int q = (int)(pass - (int64_t)pm.b * per_tile);
-    pm.bx0 = (q % per_row) * 16;
-    pm.by0 = (q / per_row) * bh;
+    // T and the block height pass_texels / 16 are powers of two: a tile holds
+    // T / 16 blocks per row and T / bh block rows
+    const int lt = ilog2(T), lbh = ilog2(pass_texels) - 4;
+    const int lrow = lt - 4, ltile = lrow + lt - lbh;
+    pm.b = (int)(pass >> ltile);
+    const int q = (int)(pass & (((int64_t)1 << ltile) - 1));
+    pm.bx0 = (q & ((1 << lrow) - 1)) << 4;
+    pm.by0 = (q >> lrow) << lbh;
   }
   return pm;
 }
@@ -263,11 +276,11 @@ __device__ __forceinline__ PassTexel pass_texel(const PassMap& pm, int64_t pass,
     const int64_t t = pass * pass_texels + tl;
     p.valid = t < B;
     const int64_t tt = p.valid ? t : 0;
-    const int TT = T * T;
-    b = (int)(tt / TT);
-    const int xy = (int)(tt - (int64_t)b * TT);
-    lx = xy % T;
-    ly = xy / T;
+    const int lt = ilog2(T);  // T is a power of two
+    b = (int)(tt >> (2 * lt));
+    const int xy = (int)(tt & (((int64_t)1 << (2 * lt)) - 1));
+    lx = xy & (T - 1);
+    ly = xy >> lt;
   }
   p.xu = (int)(orig[2 * b] + (uint32_t)lx);
   p.yu = (int)(orig[2 * b + 1] + (uint32_t)ly);
@@ -280,6 +293,7 @@ __device__ __forceinline__ PassTexel pass_texel(const PassMap& pm, int64_t pass,
 struct Box
 {
   int u0, v0, w, h;
+  uint32_t wr;  // small_recip(w)
   bool fits;
 };
 
@@ -295,6 +309,7 @@ __device__ __forceinline__ Box make_box(int xu0, int xu1, int yu0, int yu1, int 
   axis_corners(yu1, M, D, 0, a, b, f);
   bx.h = b - bx.v0 + 1;
   bx.fits = bx.w <= wmax && bx.h <= vmax;
+  bx.wr = small_recip(bx.w);
   return bx;
 }
 
@@ -367,7 +382,7 @@ __device__ __forceinline__ void box_gradients(const AxisTab& tb, const Box& bx, 
   {
     const int mt = tile / NTC, ct = tile - mt * NTC;
     const int m = 16 * mt + n;
-    const int u = m % bx.w, v = m / bx.w;  // v >= bx.h past the last cell: no corner matches
+    const int v = small_div(m, bx.wr), u = m - v * bx.w;  // v >= bx.h past the last cell: no corner matches
     const int ch = 16 * ct + n;
     f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
@@ -397,7 +412,8 @@ __device__ __forceinline__ void box_gradients(const AxisTab& tb, const Box& bx, 
       const int cell = 16 * mt + 4 * q + i;
       if (cell < cells && ch < C)
       {
-        const int cu = bx.u0 + cell % bx.w, cv = bx.v0 + cell / bx.w;
+        const int rv = small_div(cell, bx.wr);
+        const int cu = bx.u0 + cell - rv * bx.w, cv = bx.v0 + rv;
         const int cx = cu >= D ? cu - D : cu, cy = cv >= D ? cv - D : cv;
         atomicAdd(&dg[((int64_t)cy * D + cx) * C + ch], acc[i]);
       }
@@ -422,8 +438,8 @@ __device__ __forceinline__ PassGeo pass_geo(int64_t pass, const uint32_t* orig, 
   pg.pm = pass_map(pass, T, CF::PASS);
   pg.xu0 = 0;
   pg.yu0 = 0;
-  pg.b0 = Box{0, 0, 0, 0, false};
-  pg.b1 = Box{0, 0, 0, 0, false};
+  pg.b0 = Box{0, 0, 0, 0, 65536u, false};
+  pg.b1 = Box{0, 0, 0, 0, 65536u, false};
   pg.staged = false;
   if (pg.pm.blocked)
   {
@@ -465,10 +481,10 @@ __device__ __forceinline__ void prefetch_issue(Prefetch<CF::PF>& pf, const PassG
     const int ee = !live ? 0 : (g1 ? e0 - n0 : e0);
     const int Q = g1 ? CF::Q1 : CF::Q0, C = g1 ? C1 : C0, CP = g1 ? CF::CP1 : CF::CP0;
     const int bw = g1 ? pg.b1.w : pg.b0.w, bu0 = g1 ? pg.b1.u0 : pg.b0.u0, bv0 = g1 ? pg.b1.v0 : pg.b0.v0;
+    const uint32_t bwr = g1 ? pg.b1.wr : pg.b0.wr;
     const int D = g1 ? P.fd.g1_dim : P.fd.g0_dim;
     const int q = ee % Q, cb = ee / Q;
-    const int bws = bw > 0 ? bw : 1;
-    const int u = cb % bws, v = cb / bws;
+    const int v = small_div(cb, bwr), u = cb - v * bw;
     const int cu = bu0 + u, cv = bv0 + v;
     const int cx = cu >= D ? cu - D : cu, cy = cv >= D ? cv - D : cv;
     const int cell = cy * D + cx;
@@ -1443,6 +1459,9 @@ const Entry kEntries[] = {
     NTC_FUSED(128, 64, 32, 16, 16, NTC_G0_BILINEAR, 2, 0, NTC_ACT_HARDGELU),
 };
 
+// The fused kernels map passes to texels with shifts: tile sides are powers of two.
+bool pow2_tiles(const ntc_tiles& t) { return (t.tile_dim & (t.tile_dim - 1)) == 0; }
+
 bool same(const Shape& a, const Shape& b)
 {
   return a.H == b.H && a.FP == b.FP && a.G == b.G && a.C0 == b.C0 && a.C1 == b.C1 && a.G0MODE == b.G0MODE &&
@@ -1500,6 +1519,7 @@ int ntc_pack_decoder_bf16(const ntc_feature_desc* fd, int32_t hidden_dim, int32_
 int ntc_fused_train_step(const ntc_fused_args* a, hipStream_t stream)
 {
   NTC_RETURN_IF(!a || !valid_feature_desc(&a->feat) || !valid_tiles(&a->tiles), NTC_E_INVALID);
+  NTC_RETURN_IF(!pow2_tiles(a->tiles), NTC_E_INVALID);
   NTC_RETURN_IF(!a->g0 || !a->g1 || !a->weights || !a->targets || !a->dw_partials || !a->sse_partials,
                 NTC_E_INVALID);
   NTC_RETURN_IF(a->scatter && (!a->dg0 || !a->dg1), NTC_E_INVALID);
@@ -1536,6 +1556,7 @@ size_t ntc_fused_params_bytes(void) { return sizeof(FusedParams); }
 int ntc_fused_params_pack(const ntc_fused_args* a, void* record)
 {
   NTC_RETURN_IF(!a || !record || !valid_feature_desc(&a->feat) || !valid_tiles(&a->tiles), NTC_E_INVALID);
+  NTC_RETURN_IF(!pow2_tiles(a->tiles), NTC_E_INVALID);
   NTC_RETURN_IF(!a->g0 || !a->g1 || !a->weights || !a->targets || !a->dw_partials || !a->sse_partials,
                 NTC_E_INVALID);
   NTC_RETURN_IF(a->scatter && (!a->dg0 || !a->dg1), NTC_E_INVALID);

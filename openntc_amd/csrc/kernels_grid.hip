@@ -7,6 +7,11 @@
 //                   one HBM pass: reads p, m, v, g, writes p, m, v, g = 0.
 //   k_grid_update_tiles  the same for a gradient that is zero outside the
 //                   step's tile footprints: g is read and cleared only inside.
+//   k_grid_update_cold / k_grid_update_hot  the two halves of that update:
+//                   the cells outside every footprint rectangle (no gradient
+//                   access; disjoint from what the step's training kernel
+//                   reads or writes, so it may run beside it) and the cells
+//                   inside (each once, enumerated per rectangle).
 //   k_quantize      ntc_kernel.cu:448-466 (quantize_grid)
 //   k_pack          ntc_kernel.cu:486-507 (quantize_pack)
 //   k_max_abs       ntc_kernel.cu:511-531 (max_abs)
@@ -169,6 +174,93 @@ __device__ __forceinline__ void grid_update_tiles_body(const TileUpdate& A, cons
 __global__ __launch_bounds__(kThreads) void k_grid_update_tiles(TileUpdate A)
 {
   grid_update_tiles_body(A, blockIdx.x, gridDim.x);
+}
+
+// Adam (g = 0 or the gradient item gi) + clamp of one float4 item.
+__device__ __forceinline__ void tile_item_update(const TileUpdate& A, const GridSeg& s, int64_t k, const f4 gi)
+{
+  f4 mi = __builtin_nontemporal_load(&s.m[k]);
+  f4 vi = __builtin_nontemporal_load(&s.v[k]);
+  f4 pi = __builtin_nontemporal_load(&s.p[k]);
+#pragma unroll
+  for (int e = 0; e < 4; ++e)
+  {
+    float me = mi[e], ve = vi[e], pe = pi[e];
+    adam_one(A.c, gi[e], me, ve, pe);
+    mi[e] = me;
+    vi[e] = ve;
+    pi[e] = fmaxf(s.lo, fminf(s.hi, pe));
+  }
+  __builtin_nontemporal_store(mi, &s.m[k]);
+  __builtin_nontemporal_store(vi, &s.v[k]);
+  __builtin_nontemporal_store(pi, &s.p[k]);
+}
+
+__device__ __forceinline__ void load_rects(const TileUpdate& A, int4 (&rect)[2][16])
+{
+  const int tid = threadIdx.x, n = A.tl.n_tiles, M = A.tl.mip_dim, T = A.tl.tile_dim;
+  if (tid < 2 * n)
+  {
+    const int grid = tid >= n, b = grid ? tid - n : tid;
+    const int ox = (int)((uint32_t)A.tl.origins[2 * b] % (uint32_t)M);
+    const int oy = (int)((uint32_t)A.tl.origins[2 * b + 1] % (uint32_t)M);
+    rect[grid][b] = tile_rect(ox, oy, T, M, A.D[grid]);
+  }
+  __syncthreads();
+}
+
+// The items of grid_update_tiles_body outside every rectangle: the same
+// arithmetic with g = 0; the gradient buffer is not touched. Grid sides are
+// powers of two and the item -> cell division by Q <= 4 is a multiply-high
+// by ceil(2^32 / Q) (exact below 2^31 items).
+__global__ __launch_bounds__(kThreads) void k_grid_update_cold(TileUpdate A)
+{
+  __shared__ int4 rect[2][16];
+  load_rects(A, rect);
+  const f4 zero = {0.f, 0.f, 0.f, 0.f};
+  const uint32_t qr[2] = {0xFFFFFFFFu / (uint32_t)A.Q[0] + 1u, 0xFFFFFFFFu / (uint32_t)A.Q[1] + 1u};
+  const int lg[2] = {31 - __builtin_clz((unsigned)A.D[0]), 31 - __builtin_clz((unsigned)A.D[1])};
+  for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < A.n4; i += (int64_t)gridDim.x * kThreads)
+  {
+    const int grid = i >= A.a4;
+    const int64_t k = grid ? i - A.a4 : i;
+    const int D = A.D[grid];
+    // Q == 1: qr wraps to 0 and the cell is k itself
+    const uint32_t cell = A.Q[grid] == 1 ? (uint32_t)k : __umulhi((uint32_t)k, qr[grid]);
+    const int y = (int)(cell >> lg[grid]), x = (int)(cell & (uint32_t)(D - 1));
+    if (in_rects(rect[grid], A.tl.n_tiles, x, y, D)) continue;
+    tile_item_update(A, A.s[grid], k, zero);
+  }
+}
+
+// The items inside the rectangles: blockIdx.y = grid * 16 + rectangle b;
+// a cell also inside an earlier rectangle of its grid is left to that one,
+// so every item is updated once.
+__global__ __launch_bounds__(kThreads) void k_grid_update_hot(TileUpdate A)
+{
+  __shared__ int4 rect[2][16];
+  load_rects(A, rect);
+  const int grid = blockIdx.y >> 4, b = blockIdx.y & 15;
+  if (b >= A.tl.n_tiles) return;
+  const int4 r = rect[grid][b];
+  const int D = A.D[grid], Q = A.Q[grid];
+  const GridSeg& s = A.s[grid];
+  const int64_t items = (int64_t)r.y * r.w * Q;
+  const f4 zero = {0.f, 0.f, 0.f, 0.f};
+  for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < items; i += (int64_t)gridDim.x * kThreads)
+  {
+    const int q = (int)(i % Q);
+    const int c = (int)(i / Q);
+    const int iy = c / r.y, ix = c - iy * r.y;
+    int x = r.x + ix, y = r.z + iy;
+    x -= x >= D ? D : 0;
+    y -= y >= D ? D : 0;
+    if (in_rects(rect[grid], b, x, y, D)) continue;
+    const int64_t k = ((int64_t)y * D + x) * Q + q;
+    const f4 gi = __builtin_nontemporal_load(&s.g[k]);
+    tile_item_update(A, s, k, gi);
+    __builtin_nontemporal_store(zero, &s.g[k]);
+  }
 }
 
 // The grid updates of several sets for the step held in *step: set s owns
@@ -369,6 +461,44 @@ int ntc_grid_update_tiles_f32(const ntc_feature_desc* fd, const ntc_tiles* tiles
   const int64_t want = (A.n4 + kThreads - 1) / kThreads;
   const unsigned blocks = (unsigned)(want < 2048 ? want : 2048);
   hipLaunchKernelGGL(k_grid_update_tiles, dim3(blocks), dim3(kThreads), 0, stream, A);
+  return NTC_LAUNCH_STATUS();
+}
+
+int ntc_grid_update_tiles_part_f32(const ntc_feature_desc* fd, const ntc_tiles* tiles, float* g0, float* dg0,
+                                   float* g1, float* dg1, const ntc_grid_adam* adam, int32_t part, int32_t blocks,
+                                   hipStream_t stream)
+{
+  NTC_RETURN_IF(part != NTC_TILES_COLD && part != NTC_TILES_HOT, NTC_E_INVALID);
+  TileUpdate A;
+  const int st = tile_update_of(fd, tiles, g0, dg0, g1, dg1, adam, A);
+  if (st != NTC_OK) return st;
+  // the cold kernel maps items to cells with shifts (power-of-two sides) and
+  // a multiply-high by 1/Q (Q <= 4 channel quads)
+  for (int grid = 0; grid < 2; ++grid)
+    NTC_RETURN_IF((A.D[grid] & (A.D[grid] - 1)) || A.Q[grid] < 1 || A.Q[grid] > 4, NTC_E_INVALID);
+  if (part == NTC_TILES_COLD)
+  {
+    const int64_t want = (A.n4 + kThreads - 1) / kThreads;
+    const int64_t cap = blocks > 0 ? blocks : 2048;
+    hipLaunchKernelGGL(k_grid_update_cold, dim3((unsigned)(want < cap ? want : cap)), dim3(kThreads), 0, stream, A);
+  }
+  else
+  {
+    // rectangle side bound: the corners of T texels at D / M cells per texel
+    const int M = tiles->mip_dim, T = tiles->tile_dim;
+    int64_t most = 0;
+    for (int grid = 0; grid < 2; ++grid)
+    {
+      const int64_t D = A.D[grid];
+      int64_t w = ((int64_t)(T - 1) * D + M - 1) / M + 3;
+      w = w < D ? w : D;
+      const int64_t items = w * w * A.Q[grid];
+      most = items > most ? items : most;
+    }
+    const int64_t want = (most + kThreads - 1) / kThreads;
+    const int64_t cap = blocks > 0 ? blocks : 64;
+    hipLaunchKernelGGL(k_grid_update_hot, dim3((unsigned)(want < cap ? want : cap), 32), dim3(kThreads), 0, stream, A);
+  }
   return NTC_LAUNCH_STATUS();
 }
 

@@ -354,6 +354,67 @@ def test_grid_update_tiles_bitexact(oracle):
     assert np.count_nonzero(G.host(sb[0][0]) == init[0][0]) < 0.01 * n[0]
 
 
+# (g0 side, g1 side, g0 channels, g1 channels, mip side, tile side, origins)
+_PART_CASES = {
+    # three tiles, one wrapping both edges, one overlapping it
+    "wrap_overlap": (1024, 512, 16, 16, 2048, 256, [2048 - 100, 2048 - 37, 2048 - 20, 150, 517, 1203]),
+    # a coarse mip: 4 cells per texel, each rectangle half the grid wide; 3
+    # and 2 channel quads
+    "coarse_q3_q2": (256, 128, 12, 8, 64, 32, [10, 50, 33, 7]),
+    # 16 tiles (the most) at random origins, one channel quad in g0
+    "sixteen_q1": (512, 256, 4, 16, 2048, 256, list(np.random.default_rng(5).integers(0, 2048, 32))),
+    # small level sides, one tile wrapping
+    "tiny": (32, 16, 8, 4, 16, 8, [0, 0, 4, 2, 13, 9]),
+}
+
+
+@pytest.mark.parametrize("case", list(_PART_CASES))
+def test_grid_update_parts_match_tiles_update(case):
+    """The cold part (cells outside every tile rectangle, gradient untouched)
+    followed by the hot part (cells inside, gradient read and cleared) leave
+    p, m, v and the gradient exactly as the one-pass footprint-aware update
+    ntc_grid_update_tiles_f32 does, for a dense random gradient (both read it
+    only inside the rectangles); and the cold part alone changes nothing
+    inside the rectangles."""
+    D0, D1, C0, C1, M, T, org = _PART_CASES[case]
+    lib = G.lib()
+    rng = np.random.default_rng(3)
+    fdc = L.FeatureDesc(D0, D1, C0, C1, 1, 4, 2, 0, 128)
+    shapes = [(D0, D0, C0), (D1, D1, C1)]
+    init = [[rng.uniform(-1, 1, shp).astype(np.float32), (rng.standard_normal(shp) * 1e-3).astype(np.float32),
+             rng.uniform(0, 1e-6, shp).astype(np.float32), (rng.standard_normal(shp) * 1e-3).astype(np.float32)]
+            for shp in shapes]  # p, m, v, g
+    origins = np.array(org, np.int32)
+    og = G.dev(origins)
+    tl = G.tiles(og, len(origins) // 2, T, M, 0.0)
+    lr, b1c, b2c = 0.01, 1.0 / (1 - 0.9 ** 3), 1.0 / (1 - 0.999 ** 3)
+    bounds = (-1.0, 1.0 - 2.0 / 16, -1.0, 1.0 - 2.0 / 16)
+    runs = {}
+    for label in ("one_pass", "parts", "cold_only"):
+        st = [[G.dev(x) for x in grid] for grid in init]
+        ad = L.GridAdam(G.p(st[0][1]), G.p(st[0][2]), G.p(st[1][1]), G.p(st[1][2]), lr, 0.9, 0.999, b1c, b2c,
+                        *bounds)
+        args = (G.byref(fdc), G.byref(tl), G.p(st[0][0]), G.p(st[0][3]), G.p(st[1][0]), G.p(st[1][3]), G.byref(ad))
+        if label == "one_pass":
+            G.ok(lib.ntc_grid_update_tiles_f32(*args, None), "tiles update")
+        else:
+            G.ok(lib.ntc_grid_update_tiles_part_f32(*args, 1, 0, None), "cold part")
+            if label == "parts":
+                G.ok(lib.ntc_grid_update_tiles_part_f32(*args, 2, 0, None), "hot part")
+        runs[label] = [[G.host(x) for x in grid] for grid in st]
+    for k in range(2):
+        for j, name in enumerate("pmvg"):
+            got, want = runs["parts"][k][j], runs["one_pass"][k][j]
+            assert np.array_equal(got, want), (case, k, name, int(np.count_nonzero(got != want)))
+        # inside the rectangles the cold part left everything as it was; the
+        # one-pass update cleared the gradient there and only there
+        hot = runs["one_pass"][k][3] != init[k][3]
+        assert hot.any() and not hot.all()
+        for j in range(4):
+            assert np.array_equal(runs["cold_only"][k][j][hot], init[k][j][hot])
+        assert np.array_equal(runs["cold_only"][k][3], init[k][3])
+
+
 @pytest.mark.parametrize("kind,lod,n_tiles", [("c1_bilinear_relu", 0, 8), ("c1_bilinear_relu", 1, 4),
                                               ("c2_bilinear", 0, 8), ("c1_bilinear_relu", 2, 8)])
 def test_fused_sets_partials_repeat_bit_for_bit(oracle, kind, lod, n_tiles):
