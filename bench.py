@@ -208,19 +208,35 @@ def pmc_traffic(kernel):
     return None
 
 
-def time_to_psnr(ntc, chunk, workload=None):
+def time_to_psnr(ntc, chunk, workload=None, only=None):
     """Seconds of training for the fused BF16 path to reach the plain FP32
     path's final PSNR - 0.05 dB on the same workload and seed (default: the
     4096^2 bench workload). Both paths run the whole schedule; the fused one
     is evaluated every
     `chunk` steps (evaluation time excluded). Before the lock step (95 %)
     the evaluated latents are unquantised; the final figures of both paths
-    are after the lock (quantised latents, as exported)."""
-    lib = ntc.load()
+    are after the lock (quantised latents, as exported).
+
+    Each path trains in a child process of its own (only = that path), as a
+    training run normally owns its process: in one process, the fused path
+    trained after a full plain-path schedule measured 1.09 instead of 0.79 ms
+    per step (same clocks and temperatures), which a fresh process avoids."""
     w = workload or dict(dim=DIM, seed=SEED_SET, steps=TOTAL_STEPS, tiles=TILES, layout=LAYOUT)
     total = w["steps"]
+    if only is None:
+        import subprocess
+        res = {}
+        for path in ("plain", "fused"):
+            cmd = [sys.executable, os.path.abspath(__file__), "--time-to-psnr", str(chunk), "--ttp-path", path,
+                   "--workload", "2048" if workload is C2_WORKLOAD else "4096"]
+            r = subprocess.run(cmd, stdout=subprocess.PIPE, text=True)
+            if r.returncode != 0:
+                raise SystemExit(f"time-to-psnr {path} run failed ({r.returncode})")
+            res[path] = json.loads(r.stdout.strip().splitlines()[-1])
+        return time_to_psnr_report(res, w, total)
+    lib = ntc.load()
     res = {}
-    for path in ("plain", "fused"):
+    for path in (only,):
         log(f"time-to-psnr: {path} path, {total} steps")
         ctx, _ = make_context(ntc.TrainPath.Fused if path == "fused" else ntc.TrainPath.Plain, **w)
         lib.ntc_dev_synchronize()
@@ -242,6 +258,10 @@ def time_to_psnr(ntc, chunk, workload=None):
             else:
                 log(f"  {path} {done} steps {t_train:.2f} s")
         res[path] = {"train_s": t_train, "final_psnr": trace[-1]["psnr"], "trace": trace}
+    print(json.dumps(res[only]), flush=True)
+
+
+def time_to_psnr_report(res, w, total):
     target = res["plain"]["final_psnr"] - 0.05
     hit = next((t for t in res["fused"]["trace"] if t["psnr"] >= target), None)
     lw = w["layout"]
@@ -394,6 +414,7 @@ def main():
     ap.add_argument("--time-to-psnr", type=int, default=0, metavar="CHUNK",
                     help="only measure the seconds for the fused path to reach the plain path's final PSNR, "
                          "evaluating every CHUNK steps (runs both paths over the whole schedule)")
+    ap.add_argument("--ttp-path", choices=["fused", "plain"], default=None, help=argparse.SUPPRESS)
     ap.add_argument("--workload", choices=["4096", "2048"], default="4096",
                     help="workload of --time-to-psnr: the 4096^2 bench workload or the 2048^2 configuration")
     ap.add_argument("--eval-precisions", type=int, default=0, metavar="REPS",
@@ -414,7 +435,7 @@ def main():
         copy_probe(ntc, a.copy_probe)
         return
     if a.time_to_psnr:
-        time_to_psnr(ntc, a.time_to_psnr, C2_WORKLOAD if a.workload == "2048" else None)
+        time_to_psnr(ntc, a.time_to_psnr, C2_WORKLOAD if a.workload == "2048" else None, a.ttp_path)
         return
     if a.eval_precisions:
         eval_precisions(ntc, a.eval_precisions)
