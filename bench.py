@@ -246,6 +246,7 @@ def time_to_psnr(ntc, chunk, workload=None, only=None):
             n = min(step, total - done)
             t0 = time.perf_counter()
             prog = ctx.Train(n)
+            lib.ntc_ctx_flush_grids(ctx._h)  # deferred grid steps counted as training
             lib.ntc_dev_synchronize()
             t_train += time.perf_counter() - t0
             if prog.result != ntc.Result.Success:
@@ -406,8 +407,8 @@ def copy_probe(ntc, mib):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--path", choices=["fused", "plain"], default="fused")
     ap.add_argument("--cpu-steps", type=int, default=40)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -458,6 +459,10 @@ def main():
     lib.ntc_dev_synchronize()
     t0 = time.perf_counter()
     prog = ctx.Train(a.steps)
+    # the timed work ends with every latent cell current: the Adam steps the
+    # fused path defers (zero-gradient steps outside the tile footprints) are
+    # applied inside the timed region
+    lib.ntc_ctx_flush_grids(ctx._h)
     lib.ntc_dev_synchronize()
     dt = time.perf_counter() - t0
     if prog.result != ntc.Result.Success:
@@ -489,40 +494,47 @@ def main():
     if a.path == "fused":
         times = (ctypes.c_float * 9)()
         lib.ntc_ctx_kernel_times(ctx._h, times, 9)
-        fused_ms, dec_ms, grid_ms, n_t, n_grid, n_tiles_upd, cold_ms, hot_ms, n_split = list(times)
+        fused_ms, dec_ms, grid_ms, n_t, n_grid, n_tiles_upd, catch_ms, hot_ms, n_lazy = list(times)
         tiles_upd = n_grid > 0 and n_tiles_upd == n_grid
-        split = n_grid > 0 and n_split == n_grid
+        lazy = n_grid > 0 and n_lazy == n_grid
         if n_grid > 0 and 0 < n_tiles_upd < n_grid:
             raise SystemExit("timed steps mix the two grid-update kernels")
-        if n_grid > 0 and 0 < n_split < n_grid:
-            raise SystemExit("timed steps mix the split and one-pass grid updates")
+        if n_grid > 0 and 0 < n_lazy < n_grid:
+            raise SystemExit("timed steps mix the deferred and one-pass grid updates")
         # algorithmic work per launch, averaged over the timed steps:
         #   fused launch: 6 (F H + H H + H O) flop per texel; HBM bytes = the
         #   footprint cells read once (4 B/param), targets (O B/texel) and the
         #   decoder-gradient partials;
         #   grid update: 32 B/param (p, m, v, g read; p, m, v, g = 0 written)
         #   inside the tile footprints and 24 B/param (g untouched: zero)
-        #   outside them, or 32 B/param over the level (k_grid_update2); split
-        #   into the cold part (outside, beside the fused kernel) and the hot
-        #   part (inside) when the context overlaps them
+        #   outside them, or 32 B/param over the level (k_grid_update2); with
+        #   the zero-gradient steps deferred (the default), the catch-up reads
+        #   and writes p, m, v of the footprint cells (24 B/param) and the hot
+        #   part 32 B/param there; the deferred steps of the other cells cost
+        #   their catch-up later, when a footprint or a flush reaches them
         n_dec = H * F + H * H + 16 * H
-        flops, fbytes, gbytes, cbytes, hbytes, blks = [], [], [], [], [], []
+        flops, fbytes, gbytes, cbytes, hbytes, blks, dense = [], [], [], [], [], [], []
         for si in infos:
             B = si.n_tiles * si.tile_dim ** 2
             M = DIM >> si.lod
             D0, D1 = lay.g0_dim[si.level], lay.g1_dim[si.level]
             pl = D0 * D0 * lay.g0_channels + D1 * D1 * lay.g1_channels
             pf = footprint_cells(si, M, D0) * lay.g0_channels + footprint_cells(si, M, D1) * lay.g1_channels
-            blocks = min(256 - (64 if split and si.grid_update else 0), (B + 63) // 64)
+            blocks = min(256, (B + 63) // 64)
             fb = 4.0 * pf + O * B + 4.0 * blocks * n_dec
             flops.append(6.0 * (F * H + H * H + H * O) * B)
             blks.append(blocks)
-            if si.grid_update and tiles_upd:
-                gbytes.append(24.0 * (pl - pf) + 32.0 * pf)
-                cbytes.append(24.0 * (pl - pf))
+            if si.grid_update and lazy:
+                gbytes.append(24.0 * pf + 32.0 * pf)
+                cbytes.append(24.0 * pf)
                 hbytes.append(32.0 * pf)
+                dense.append(24.0 * (pl - pf) + 32.0 * pf)
+            elif si.grid_update and tiles_upd:
+                gbytes.append(24.0 * (pl - pf) + 32.0 * pf)
+                dense.append(gbytes[-1])
             elif si.grid_update:
                 gbytes.append(32.0 * pl)
+                dense.append(gbytes[-1])
             fbytes.append(fb)
         fused_tflops = np.mean(flops) / (fused_ms * 1e-3) / 1e12
         fused_gbs = np.mean(fbytes) / (fused_ms * 1e-3) / 1e9
@@ -532,16 +544,15 @@ def main():
                                      "bytes_per_launch": float(np.mean(fbytes)),
                                      "workgroups": float(np.mean(blks))},
                 "decoder_update": {"ms": dec_ms}}
-        if gbytes and split:
-            cold_gbs = float(np.mean(cbytes)) / (cold_ms * 1e-3) / 1e9
+        if gbytes and lazy:
+            c_gbs = float(np.mean(cbytes)) / (catch_ms * 1e-3) / 1e9
             hot_gbs = float(np.mean(hbytes)) / (hot_ms * 1e-3) / 1e9
-            kern["grid_update"] = {"kernel": "k_grid_update_cold + k_grid_update_hot", "ms": grid_ms,
+            kern["grid_update"] = {"kernel": "k_grid_lazy_catchup_rects + k_grid_lazy_hot", "ms": grid_ms,
                                    "bytes_per_launch": float(np.mean(gbytes)),
-                                   "cold": {"kernel": "k_grid_update_cold", "ms": cold_ms, "achieved": cold_gbs,
-                                            "unit": "GB/s", "frac": cold_gbs / HBM_PEAK_GBS,
-                                            "bytes_per_launch": float(np.mean(cbytes)),
-                                            "beside": "k_fused_train (64 CUs left to it)"},
-                                   "hot": {"kernel": "k_grid_update_hot", "ms": hot_ms, "achieved": hot_gbs,
+                                   "catch_up": {"kernel": "k_grid_lazy_catchup_rects", "ms": catch_ms,
+                                                "achieved": c_gbs, "unit": "GB/s", "frac": c_gbs / HBM_PEAK_GBS,
+                                                "bytes_per_launch": float(np.mean(cbytes))},
+                                   "hot": {"kernel": "k_grid_lazy_hot", "ms": hot_ms, "achieved": hot_gbs,
                                            "unit": "GB/s", "frac": hot_gbs / HBM_PEAK_GBS,
                                            "bytes_per_launch": float(np.mean(hbytes))}}
         elif gbytes:
@@ -549,20 +560,17 @@ def main():
             kern["grid_update"] = {"kernel": "k_grid_update_tiles" if tiles_upd else "k_grid_update2",
                                    "ms": grid_ms, "achieved": grid_gbs, "unit": "GB/s", "peak": HBM_PEAK_GBS,
                                    "frac": grid_gbs / HBM_PEAK_GBS, "bytes_per_launch": float(np.mean(gbytes))}
-        # the whole step against HBM: every algorithmic byte of the step over
-        # the measured step time
-        step_bytes = float(np.mean(fbytes)) + (float(np.sum(gbytes)) / len(infos) if gbytes else 0.0)
-        kern["step"] = {"ms": dt * 1e3 / a.steps, "bytes": step_bytes,
+        # the whole step against HBM: the bytes an eager dense update of the
+        # level plus the fused launch would move, over the measured step time
+        # (with deferred steps the grid part is not moved every step: this is
+        # the equivalent rate, not measured traffic)
+        step_bytes = float(np.mean(fbytes)) + (float(np.sum(dense)) / len(infos) if dense else 0.0)
+        kern["step"] = {"ms": dt * 1e3 / a.steps, "dense_equivalent_bytes": step_bytes,
                         "achieved": step_bytes / (dt / a.steps) / 1e9, "unit": "GB/s",
                         "frac": step_bytes / (dt / a.steps) / 1e9 / HBM_PEAK_GBS}
         out["kernels"] = kern
         # the roofline names the longest-running kernel of the step
-        if gbytes and split and cold_ms >= fused_ms:
-            c = kern["grid_update"]["cold"]
-            out["roofline"] = {"bound": "hbm", "kernel": "k_grid_update_cold", "achieved": c["achieved"],
-                               "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": c["frac"],
-                               "traffic": pmc_traffic("k_grid_update_cold")}
-        elif gbytes and not split and grid_ms >= fused_ms:
+        if gbytes and not lazy and grid_ms >= fused_ms:
             g = kern["grid_update"]
             out["roofline"] = {"bound": "hbm", "kernel": g["kernel"], "achieved": g["achieved"],
                                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": g["frac"],

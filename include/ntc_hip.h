@@ -229,6 +229,33 @@ int ntc_grid_update_tiles_part_f32(const ntc_feature_desc* fd, const ntc_tiles* 
                                    float* g0, float* dg0, float* g1, float* dg1,
                                    const ntc_grid_adam* adam, int32_t part,
                                    int32_t blocks, hipStream_t stream);
+/* The tiles update with the zero-gradient steps of cells outside the
+ * footprints deferred (no counterpart in the reference; its update is one
+ * dense pass, ntc_kernel.cu:375-410, whose results these reproduce bit for
+ * bit). last0/last1 (device, one int per g0 / g1 cell of the level) count the
+ * level's Adam steps applied to each cell; steps (device) holds
+ * {lr, bias1, bias2} of the level's step s at [3 s]; adam supplies m, v,
+ * beta1, beta2 and the clamp bounds (and, for the hot part, this step's lr
+ * and bias factors).
+ * Catch-up: applies the zero-gradient steps last .. upto - 1 (in order) to
+ * the cells of the footprint rectangles of tiles, or, with tiles null, to
+ * every cell of both grids (last is not changed: after an every-cell
+ * catch-up set it to upto with ntc_fill_i32).
+ * Hot: the level's step `step` on the cells of the rectangles (gradient read
+ * and cleared, as NTC_TILES_HOT), which must be current (caught up to step),
+ * then last = step + 1 there. Grid sides powers of two, channel counts
+ * multiples of 4 up to 16, n_tiles <= 16. */
+int ntc_grid_lazy_catchup_f32(const ntc_feature_desc* fd, const ntc_tiles* tiles,
+                              float* g0, float* g1, const ntc_grid_adam* adam,
+                              const float* steps, int32_t upto,
+                              const int32_t* last0, const int32_t* last1,
+                              hipStream_t stream);
+int ntc_grid_lazy_hot_f32(const ntc_feature_desc* fd, const ntc_tiles* tiles,
+                          float* g0, float* dg0, float* g1, float* dg1,
+                          const ntc_grid_adam* adam, int32_t step, int32_t* last0,
+                          int32_t* last1, hipStream_t stream);
+/* out[i] = value, i < n (kernel fill) */
+int ntc_fill_i32(int64_t n, int32_t value, int32_t* out, hipStream_t stream);
 /* zero-fill by kernel (graph-safe replacement for memset) */
 int ntc_fill_zero(void* p, size_t bytes, hipStream_t stream);
 
@@ -507,6 +534,13 @@ int ntc_ctx_get_compressed(ntc_context* ctx, uint8_t* g0, size_t* g0_size,
                            uint8_t* g1, size_t* g1_size, uint8_t* decoder,
                            size_t* decoder_size, float* caldata4);
 int ntc_ctx_dump(ntc_context* ctx, const char* path);
+/* Applies the latent-grid Adam steps the fused path has deferred (the
+ * zero-gradient steps of cells outside the recent tile footprints) to every
+ * cell and waits for it. Reads and writes of the grids, Eval and the lock do
+ * this by themselves; a caller timing training includes it so that all of the
+ * work done is counted. No counterpart in the reference (it updates every
+ * cell every step). */
+int ntc_ctx_flush_grids(ntc_context* ctx);
 /* Device pointer to the context's stream (for timing with events). */
 void* ntc_ctx_stream(ntc_context* ctx);
 /* Average durations over the last Train call with timing on, measured with

@@ -176,6 +176,7 @@ SIGNATURES = {
     "ntc_ctx_dump": (_I, [_P, C.c_char_p]),
     "ntc_ctx_stream": (_P, [_P]),
     "ntc_ctx_kernel_times": (_I, [_P, _P, _I]),
+    "ntc_ctx_flush_grids": (_I, [_P]),
     "ntc_ctx_set_timing": (_I, [_P, _I]),
     "ntc_synthetic_pbr": (_I, [_I, _U, _P, _P, _P]),
     "ntc_dev_alloc": (_I, [_SZ, _P]),
@@ -194,6 +195,9 @@ SIGNATURES = {
     "ntc_fused_train_step": (_I, [_P, _P]),
     "ntc_grid_update_tiles_f32": (_I, [_P, _P, _P, _P, _P, _P, _P, _P]),
     "ntc_grid_update_tiles_part_f32": (_I, [_P, _P, _P, _P, _P, _P, _P, _I, _I, _P]),
+    "ntc_grid_lazy_catchup_f32": (_I, [_P, _P, _P, _P, _P, _P, _I, _P, _P, _P]),
+    "ntc_grid_lazy_hot_f32": (_I, [_P, _P, _P, _P, _P, _P, _P, _I, _P, _P, _P]),
+    "ntc_fill_i32": (_I, [C.c_int64, _I, _P, _P]),
     "ntc_decode_table_bytes": (C.c_size_t, [_I, _I, _I, _I]),
     "ntc_build_decode_table_f": (_I, [_I, _I, _I, _I, _P, _F, _F, _P]),
     "ntc_build_decode_table_int8": (_I, [_I, _I, _I, _I, _P, _F, _F, _P]),

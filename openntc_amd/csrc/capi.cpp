@@ -294,6 +294,8 @@ int ntc_ctx_read_grid(ntc_context* c, int32_t which, int32_t level, float* host)
   if (!I.manifest_loaded) return (int)Result::InvalidState;
   DeviceBuffer* b = grid_buffer(I, which, level);
   if (!b || !b->ptr) return (int)Result::InvalidState;
+  // every cell current: the deferred zero-gradient steps applied
+  if (I.FlushLazyGrids() != Result::Success) return (int)Result::CudaFailure;
   if (hipStreamSynchronize(I.stream) != hipSuccess) return (int)Result::CudaFailure;
   return hipMemcpy(host, b->ptr, b->bytes, hipMemcpyDeviceToHost) == hipSuccess ? 0 : (int)Result::CudaFailure;
 }
@@ -305,6 +307,8 @@ int ntc_ctx_write_grid(ntc_context* c, int32_t which, int32_t level, const float
   if (!I.manifest_loaded) return (int)Result::InvalidState;
   DeviceBuffer* b = grid_buffer(I, which, level);
   if (!b || !b->ptr) return (int)Result::InvalidState;
+  // every cell current: the deferred zero-gradient steps applied
+  if (I.FlushLazyGrids() != Result::Success) return (int)Result::CudaFailure;
   if (hipStreamSynchronize(I.stream) != hipSuccess) return (int)Result::CudaFailure;
   ++I.state_version;
   return hipMemcpy(b->ptr, host, b->bytes, hipMemcpyHostToDevice) == hipSuccess ? 0 : (int)Result::CudaFailure;
@@ -395,6 +399,16 @@ int ntc_ctx_kernel_times(ntc_context* c, float* out, int32_t n)
   if (!c || !out) return (int)Result::InvalidState;
   const auto& v = c->ctx.GetImpl()->kernel_ms;
   for (int i = 0; i < n; ++i) out[i] = i < (int)v.size() ? v[i] : -1.0f;
+  return 0;
+}
+
+int ntc_ctx_flush_grids(ntc_context* c)
+{
+  if (!c) return (int)Result::InvalidState;
+  Context::Impl& I = *c->ctx.GetImpl();
+  if (!I.manifest_loaded) return (int)Result::InvalidState;
+  if (I.FlushLazyGrids() != Result::Success || hipStreamSynchronize(I.stream) != hipSuccess)
+    return (int)Result::CudaFailure;
   return 0;
 }
 

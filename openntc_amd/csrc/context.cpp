@@ -344,9 +344,6 @@ Result Context::Impl::InitDevice()
   int count = 0;
   if (hipGetDeviceCount(&count) != hipSuccess || count < 1) return Result::CudaFailure;
   CTX_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), Result::CudaFailure);
-  CTX_HIP(hipStreamCreateWithFlags(&side, hipStreamNonBlocking), Result::CudaFailure);
-  CTX_HIP(hipEventCreateWithFlags(&fork_ev, hipEventDisableTiming), Result::CudaFailure);
-  CTX_HIP(hipEventCreateWithFlags(&join_ev, hipEventDisableTiming), Result::CudaFailure);
   const size_t dbytes = sizeof(float) * (size_t)DecoderElems();
   CTX_HIP(dec.Alloc(dbytes), Result::AllocationFailure);
   CTX_HIP(dec_m.Alloc(dbytes), Result::AllocationFailure);
@@ -377,6 +374,15 @@ void Context::Impl::FreeManifestBuffers()
   origins_table.Free();
   eval_origins.Free();
   pack_scratch.Free();
+  for (int l = 0; l < kMaxLevels; ++l)
+  {
+    lazy_last[l][0].Free();
+    lazy_last[l][1].Free();
+    lazy_steps[l].Free();
+    lazy_done[l] = 0;
+    lazy_behind[l] = false;
+  }
+  lazy = false;
 }
 
 void Context::Impl::FreeAll()
@@ -388,11 +394,7 @@ void Context::Impl::FreeAll()
   plain_batch = 0;
   decode_table_prec = -1;
   if (stream) (void)hipStreamDestroy(stream);
-  if (side) (void)hipStreamDestroy(side);
-  if (fork_ev) (void)hipEventDestroy(fork_ev);
-  if (join_ev) (void)hipEventDestroy(join_ev);
-  stream = side = nullptr;
-  fork_ev = join_ev = nullptr;
+  stream = nullptr;
 }
 
 Result Context::Impl::EnsurePlainBuffers(int64_t batch)
@@ -578,10 +580,77 @@ void Context::Impl::BuildPlan(int32_t steps)
 // libopenntc.cpp:601-619
 Result Context::Impl::LockGrids()
 {
+  Result r = FlushLazyGrids();
+  if (r != Result::Success) return r;
   for (int l = 0; l < level_count; ++l)
   {
     CTX_NTC(ntc_quantize_grid_f32(GridElems(0, l), g0_bits, g0_delta, g0[l].as<float>(), stream));
     CTX_NTC(ntc_quantize_grid_f32(GridElems(1, l), g1_bits, g1_delta, g1[l].as<float>(), stream));
+  }
+  return Result::Success;
+}
+
+// The fused path defers the zero-gradient Adam steps of latent cells outside
+// a step's tile footprints (every level cell takes a step each time its level
+// is active, libopenntc.cpp:721-778): a cell counts the level steps applied
+// to it, a step first catches up the cells its footprints cover, and the
+// remaining cells catch up before anything reads the grids. The per-step
+// arithmetic and its order are those of the eager update, so the results are
+// the same bit for bit. Needs the tile-aware update's shapes (channel counts
+// multiples of 4 up to 16, at most 16 tiles).
+Result Context::Impl::SetupLazyGrids()
+{
+  lazy = path == TrainPath::Fused && g0_channels % 4 == 0 && g1_channels % 4 == 0 && g0_channels <= 16 &&
+         g1_channels <= 16 && grids_per_batch <= 16;
+  for (int l = 0; l < kMaxLevels; ++l)
+  {
+    lazy_done[l] = 0;
+    lazy_behind[l] = false;
+  }
+  if (!lazy) return Result::Success;
+  std::vector<float> tab[kMaxLevels];
+  for (const StepPlan& st : plan)
+    if (st.grid_update)
+    {
+      tab[st.level].push_back(st.lr_grid);
+      tab[st.level].push_back(st.grid_bias1);
+      tab[st.level].push_back(st.grid_bias2);
+    }
+  for (int l = 0; l < level_count; ++l)
+  {
+    for (int which = 0; which < 2; ++which)
+    {
+      const int D = which == 0 ? g0_dim[l] : g1_dim[l];
+      DeviceBuffer& b = lazy_last[l][which];
+      if (b.bytes < sizeof(int32_t) * (size_t)D * D) CTX_HIP(b.Alloc(sizeof(int32_t) * (size_t)D * D), Result::AllocationFailure);
+      CTX_NTC(ntc_fill_i32((int64_t)D * D, 0, b.as<int32_t>(), stream));
+    }
+    if (tab[l].empty()) tab[l].assign(3, 0.0f);
+    const size_t bytes = sizeof(float) * tab[l].size();
+    if (lazy_steps[l].bytes < bytes) CTX_HIP(lazy_steps[l].Alloc(bytes), Result::AllocationFailure);
+    CTX_HIP(CopySync(lazy_steps[l].ptr, tab[l].data(), bytes, hipMemcpyHostToDevice), Result::CudaFailure);
+  }
+  return Result::Success;
+}
+
+Result Context::Impl::FlushLazyGrids()
+{
+  if (!lazy) return Result::Success;
+  for (int l = 0; l < level_count; ++l)
+  {
+    if (!lazy_behind[l]) continue;
+    const ntc_feature_desc fd = FeatureDesc(l);
+    const ntc_grid_adam adam = {g0m[l].as<float>(), g0v[l].as<float>(), g1m[l].as<float>(), g1v[l].as<float>(),
+                                0.0f, 0.9f, 0.999f, 1.0f, 1.0f, -1.0f, 1.0f - g0_delta, -1.0f, 1.0f - g1_delta};
+    CTX_NTC(ntc_grid_lazy_catchup_f32(&fd, nullptr, g0[l].as<float>(), g1[l].as<float>(), &adam,
+                                      lazy_steps[l].as<float>(), lazy_done[l], lazy_last[l][0].as<int32_t>(),
+                                      lazy_last[l][1].as<int32_t>(), stream));
+    for (int which = 0; which < 2; ++which)
+    {
+      const int D = which == 0 ? g0_dim[l] : g1_dim[l];
+      CTX_NTC(ntc_fill_i32((int64_t)D * D, lazy_done[l], lazy_last[l][which].as<int32_t>(), stream));
+    }
+    lazy_behind[l] = false;
   }
   return Result::Success;
 }
@@ -686,15 +755,11 @@ Result Context::Impl::PackDecoderTables()
 
 // One step of the fused path: one persistent kernel for gather, forward,
 // loss, backward and latent scatter; the ordered decoder-gradient reduction
-// with Adam; the BF16 table refresh; one dense update of both grids of the
-// level (Adam + clamp + gradient reset). ev: 6 events or null.
-//
-// When the level update can skip the zero gradient outside the tile
-// footprints, its cold part (those cells: no gradient access, nothing the
-// fused kernel reads or writes) runs on the side stream beside the fused
-// kernel, which then leaves cold_reserve CUs to it; the hot part (inside the
-// footprints) follows the fused kernel on the stream, and the stream waits
-// for the cold part before anything after the step.
+// with Adam; the BF16 table refresh; the dense update of both grids of the
+// level (Adam + clamp + gradient reset): with deferred zero-gradient steps,
+// the catch-up of the footprint cells before the fused kernel reads them and
+// the gradient step there after it; else one pass over the level.
+// ev: kStepEvents events or null.
 Result Context::Impl::RunFusedStep(const StepPlan& st, int32_t step_index, hipEvent_t* ev)
 {
   const int lvl = st.level, T = st.tile_dim, M = mip_dim[st.lod];
@@ -726,45 +791,43 @@ Result Context::Impl::RunFusedStep(const StepPlan& st, int32_t step_index, hipEv
   a.dw_partials = dw_part.as<float>();
   a.sse_partials = sse_part.as<double>();
   const bool tiles_update = st.grid_update && g0_channels % 4 == 0 && g1_channels % 4 == 0 && st.n_tiles <= 16;
-  const bool overlap = tiles_update && cold_reserve > 0 && cold_reserve < max_blocks;
-  a.blocks = ntc_fused_blocks_for(&fd, H, act, B, overlap ? max_blocks - cold_reserve : max_blocks);
+  const bool deferred = lazy && st.grid_update;
+  a.blocks = ntc_fused_blocks_for(&fd, H, act, B, max_blocks);
   // dense Adam + clamp of the level: the gradient is zero outside the tile
   // footprints, so the update reads and clears it only inside them
   const ntc_grid_adam adam = {g0m[lvl].as<float>(), g0v[lvl].as<float>(), g1m[lvl].as<float>(),
                               g1v[lvl].as<float>(), st.lr_grid, 0.9f, 0.999f, st.grid_bias1, st.grid_bias2,
                               -1.0f, 1.0f - g0_delta, -1.0f, 1.0f - g1_delta};
   if (tiles_update) ++tiles_update_steps;
-  if (overlap) CTX_HIP(hipEventRecord(fork_ev, stream), Result::CudaFailure);
+  if (deferred)
+  {
+    if (ev) CTX_HIP(hipEventRecord(ev[3], stream), Result::CudaFailure);
+    CTX_NTC(ntc_grid_lazy_catchup_f32(&fd, &a.tiles, g0[lvl].as<float>(), g1[lvl].as<float>(), &adam,
+                                      lazy_steps[lvl].as<float>(), lazy_done[lvl], lazy_last[lvl][0].as<int32_t>(),
+                                      lazy_last[lvl][1].as<int32_t>(), stream));
+    if (ev) CTX_HIP(hipEventRecord(ev[4], stream), Result::CudaFailure);
+  }
   if (ev) CTX_HIP(hipEventRecord(ev[0], stream), Result::CudaFailure);
   CTX_NTC(ntc_fused_train_step(&a, stream));
   if (ev) CTX_HIP(hipEventRecord(ev[1], stream), Result::CudaFailure);
-  if (overlap)
-  {
-    CTX_HIP(hipStreamWaitEvent(side, fork_ev, 0), Result::CudaFailure);
-    if (ev) CTX_HIP(hipEventRecord(ev[3], side), Result::CudaFailure);
-    CTX_NTC(ntc_grid_update_tiles_part_f32(&fd, &a.tiles, g0[lvl].as<float>(), dg0[lvl].as<float>(),
-                                           g1[lvl].as<float>(), dg1[lvl].as<float>(), &adam, NTC_TILES_COLD, 0,
-                                           side));
-    if (ev) CTX_HIP(hipEventRecord(ev[4], side), Result::CudaFailure);
-    CTX_HIP(hipEventRecord(join_ev, side), Result::CudaFailure);
-  }
   CTX_NTC(ntc_decoder_update(H, feature_dim, O, a.blocks, dw_part.as<float>(), st.lr_decoder, 0.9f, 0.999f, st.bias1,
                              st.bias2, dec_grad.as<float>(), dec_m.as<float>(), dec_v.as<float>(), dec.as<float>(),
                              sse_part.as<double>(), scalars.as<double>() + 16, stream));
   Result r = PackDecoderTables();
   if (r != Result::Success) return r;
   if (ev) CTX_HIP(hipEventRecord(ev[2], stream), Result::CudaFailure);
-  if (overlap)
+  if (deferred)
   {
-    CTX_NTC(ntc_grid_update_tiles_part_f32(&fd, &a.tiles, g0[lvl].as<float>(), dg0[lvl].as<float>(),
-                                           g1[lvl].as<float>(), dg1[lvl].as<float>(), &adam, NTC_TILES_HOT, 0,
-                                           stream));
+    CTX_NTC(ntc_grid_lazy_hot_f32(&fd, &a.tiles, g0[lvl].as<float>(), dg0[lvl].as<float>(), g1[lvl].as<float>(),
+                                  dg1[lvl].as<float>(), &adam, lazy_done[lvl], lazy_last[lvl][0].as<int32_t>(),
+                                  lazy_last[lvl][1].as<int32_t>(), stream));
     if (ev)
     {
       CTX_HIP(hipEventRecord(ev[5], stream), Result::CudaFailure);
-      ev_overlap[(size_t)(ev - events.data()) / 6] = 1;
+      ev_lazy[(size_t)(ev - events.data()) / kStepEvents] = 1;
     }
-    CTX_HIP(hipStreamWaitEvent(stream, join_ev, 0), Result::CudaFailure);
+    lazy_done[lvl] += 1;
+    lazy_behind[lvl] = true;
   }
   else if (st.grid_update)
   {
@@ -877,6 +940,7 @@ EvalResults Context::Impl::Evaluate(const Context& ctx, EvalPrecision p, EvalDet
     err = Result::InvalidState;
     return EvalResults{};
   }
+  if ((err = FlushLazyGrids()) != Result::Success) return EvalResults{};
   // The decode kernel (one fused forward per texel) serves every precision
   // for hidden widths 64 and 128; FP32 at other widths runs the GEMM chain.
   // Both evaluate the same FMA chains, so FP32 results are identical.
@@ -999,7 +1063,7 @@ EvalResults Context::Impl::Evaluate(const Context& ctx, EvalPrecision p, EvalDet
 CalibrationData Context::Impl::RunCalibration(float headroom, Result& err)
 {
   float* dmax = reinterpret_cast<float*>(scalars.as<double>() + 20);
-  err = Result::Success;
+  if ((err = FlushLazyGrids()) != Result::Success) return CalibrationData{};
   if (hipMemsetAsync(dmax, 0, sizeof(float) * 2, stream) != hipSuccess)
   {
     err = Result::CudaFailure;
@@ -1297,6 +1361,7 @@ void Context::BeginTrainingEx(const TrainInfoEx& info)
   if (I.CopySync(I.origins_table.ptr, table.data(), I.origins_table.bytes, hipMemcpyHostToDevice) != hipSuccess ||
       hipStreamSynchronize(I.stream) != hipSuccess)
     return fail(Result::CudaFailure);
+  if (Result r = I.SetupLazyGrids(); r != Result::Success) return fail(r);
   ++I.state_version;
   I.phase = TrainPhase::TrainInProgress;
 }
@@ -1348,7 +1413,7 @@ TrainProgress Context::Train(int32_t num_batches)
   }
   const int first = I.batch_i;
   I.tiles_update_steps = 0;
-  if (timed) I.ev_overlap.assign((size_t)n, 0);
+  if (timed) I.ev_lazy.assign((size_t)n, 0);
   ++I.state_version;
   for (; I.batch_i < target; ++I.batch_i)
   {
@@ -1370,9 +1435,9 @@ TrainProgress Context::Train(int32_t num_batches)
       if (hipEventElapsedTime(&ms, ev[0], ev[1]) == hipSuccess) fused += ms;
       if (hipEventElapsedTime(&ms, ev[1], ev[2]) == hipSuccess) decoder += ms;
       if (!I.plan[first + s].grid_update) continue;
-      if (I.ev_overlap[(size_t)s])
+      if (I.ev_lazy[(size_t)s])
       {
-        // cold part (beside the fused kernel) + hot part
+        // catch-up (before the fused kernel) + hot part
         if (hipEventElapsedTime(&ms, ev[3], ev[4]) == hipSuccess &&
             hipEventElapsedTime(&ms2, ev[2], ev[5]) == hipSuccess)
         {

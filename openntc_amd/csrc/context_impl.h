@@ -60,10 +60,9 @@ struct Context::Impl
   Activation activation = Activation::HardGELU;
   int32_t feature_dim = 0, feature_dim_padded = 0, out_dim = 0;
   static constexpr int32_t kOutPadded = 16;
-  // Measured at the 4096^2 workload (8 tiles, H = 128) with reserves of 0,
-  // 16, 32, 64, 80 and 96 CUs: 64 balances the fused kernel on the other
-  // CUs against the cold update on these (0.91 -> 0.80 ms per step).
-  static constexpr int32_t kColdReserve = 64;
+  // timing events per fused step: fused kernel start/end, after the decoder
+  // update + table pack, grid catch-up start/end, end of the grid update
+  static constexpr int32_t kStepEvents = 6;
   int32_t min_dimension = Context::kMinDimension;
   int32_t mip_min_dim = 4;
   uint32_t seed = 123;
@@ -78,11 +77,6 @@ struct Context::Impl
 
   // ---- device state ----
   hipStream_t stream = nullptr;
-  // The cold part of a step's grid update (cells outside the step's tile
-  // footprints) runs on side beside the step's fused kernel; fork and join
-  // order it after the previous work on stream and before the next.
-  hipStream_t side = nullptr;
-  hipEvent_t fork_ev = nullptr, join_ev = nullptr;
   DeviceBuffer g0[kMaxLevels], g1[kMaxLevels];
   DeviceBuffer g0m[kMaxLevels], g0v[kMaxLevels], g1m[kMaxLevels], g1v[kMaxLevels];
   DeviceBuffer dg0[kMaxLevels], dg1[kMaxLevels];
@@ -116,24 +110,29 @@ struct Context::Impl
   DeviceBuffer dw_part;       // [blocks][decoder elems]
   DeviceBuffer sse_part;      // [blocks]
   int32_t max_blocks = 0;
-  // CUs left free of fused workgroups for the concurrent cold grid update
-  // (the fused workgroups fill a CU each)
-  int32_t cold_reserve = kColdReserve;
+  // Deferred zero-gradient grid steps (fused path): per level, the count of
+  // the level's Adam steps applied to every g0 / g1 cell, {lr, bias1, bias2}
+  // of each of the level's steps, the steps issued so far and whether some
+  // cell is behind them.
+  bool lazy = false;
+  DeviceBuffer lazy_last[kMaxLevels][2];
+  DeviceBuffer lazy_steps[kMaxLevels];
+  int32_t lazy_done[kMaxLevels] = {};
+  bool lazy_behind[kMaxLevels] = {};
 
   // ---- timing (optional) ----
   // When enabled, Train records events around the fused step kernel, the
   // decoder update (+ table pack) and the dense grid update of every step on
   // the context stream; kernel_ms holds {fused, decoder, grid, steps timed,
   // grid-update launches timed, grid updates that skipped the zero gradient
-  // outside the tile footprints, cold part, hot part, steps with the split
-  // update} over the last Train call (durations averaged; with the split
-  // update the grid figure is cold + hot, the cold part running beside the
-  // fused kernel).
+  // outside the tile footprints, catch-up, hot part, steps with the deferred
+  // update} over the last Train call (durations averaged; with the deferred
+  // update the grid figure is catch-up + hot).
   bool timing = false;
   int32_t tiles_update_steps = 0;
   std::vector<float> kernel_ms;
   std::vector<hipEvent_t> events;
-  std::vector<uint8_t> ev_overlap;  // per timed step: grid update split in cold + hot
+  std::vector<uint8_t> ev_lazy;  // per timed step: deferred grid update (catch-up + hot)
 
   // ---- helpers ----
   int64_t DecoderElems() const
@@ -169,6 +168,10 @@ struct Context::Impl
     return e == hipSuccess ? hipStreamSynchronize(stream) : e;
   }
   Result LockGrids();
+  // Deferred grid steps: set-up at BeginTraining, and the catch-up of every
+  // cell of every level before anything reads or writes the grids.
+  Result SetupLazyGrids();
+  Result FlushLazyGrids();
   Result Finalize(Context& ctx);
   Result ForwardTiles(int32_t mip, const int32_t* origins, int32_t n_tiles, int32_t T, bool with_targets);
   EvalResults Evaluate(const Context& ctx, EvalPrecision p, EvalDetail* detail, Result& err);

@@ -7,6 +7,11 @@
 //                   one HBM pass: reads p, m, v, g, writes p, m, v, g = 0.
 //   k_grid_update_tiles  the same for a gradient that is zero outside the
 //                   step's tile footprints: g is read and cleared only inside.
+//   k_grid_lazy_catchup / k_grid_lazy_hot  the same update with the zero-
+//                   gradient steps of cells outside the footprints deferred:
+//                   each cell counts the steps applied to it and catches up
+//                   (the same per-step arithmetic, in order) before anything
+//                   reads it.
 //   k_grid_update_cold / k_grid_update_hot  the two halves of that update:
 //                   the cells outside every footprint rectangle (no gradient
 //                   access; disjoint from what the step's training kernel
@@ -231,6 +236,176 @@ __global__ __launch_bounds__(kThreads) void k_grid_update_cold(TileUpdate A)
     if (in_rects(rect[grid], A.tl.n_tiles, x, y, D)) continue;
     tile_item_update(A, A.s[grid], k, zero);
   }
+}
+
+// ---- deferred zero-gradient steps ----
+//
+// last[grid][cell] counts the level's Adam steps applied to the cell (its
+// p, m, v for every channel); tab[3 o + {0, 1, 2}] = lr, bias1, bias2 of the
+// level's step o. A zero-gradient step is a function of (p, m, v) and the
+// step's constants only, so applying steps last .. upto - 1 late, in order,
+// gives the bits the eager update gives.
+struct LazyArgs
+{
+  TileUpdate A;  // grids, Adam state, footprint tiles, clamp bounds; A.c: this step's constants (hot)
+  const float* tab;
+  float b1, b2;
+  int32_t* last[2];
+  int32_t upto;  // catch-up: apply steps last .. upto - 1; hot: this step's index
+};
+
+// Smallest v over the wave (every lane active), in a scalar register.
+__device__ __forceinline__ int wave_min(int v)
+{
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = min(v, __shfl_xor(v, off));
+  return __builtin_amdgcn_readfirstlane(v);
+}
+
+// Catch-up of one float4 item per lane to step L.upto (from: the item's
+// cell counter; live: the lane holds an item). Called by every lane of the
+// wave: the step loop runs from the wave's smallest counter with the step
+// constants as wave-uniform loads (the next step's issued before the current
+// one is applied), and a lane applies the steps from its own counter on.
+__device__ __forceinline__ void lazy_run(const LazyArgs& L, const GridSeg& s, int64_t k, int from, bool live)
+{
+  const bool mine = live && from < L.upto;
+  const int lo = wave_min(mine ? from : L.upto);
+  if (lo >= L.upto) return;
+  f4 mi = {0.f, 0.f, 0.f, 0.f}, vi = mi, pi = mi;
+  if (mine)
+  {
+    mi = __builtin_nontemporal_load(&s.m[k]);
+    vi = __builtin_nontemporal_load(&s.v[k]);
+    pi = __builtin_nontemporal_load(&s.p[k]);
+  }
+  AdamConst c;
+  c.b1 = L.b1;
+  c.b2 = L.b2;
+  c.omb1 = 1.0f - L.b1;  // as make_adam
+  c.omb2 = 1.0f - L.b2;
+  float lr = L.tab[3 * lo], bias1 = L.tab[3 * lo + 1], bias2 = L.tab[3 * lo + 2];
+  for (int o = lo; o < L.upto; ++o)
+  {
+    const int on = o + 1 < L.upto ? o + 1 : o;
+    const float nlr = L.tab[3 * on], nb1 = L.tab[3 * on + 1], nb2 = L.tab[3 * on + 2];
+    c.lr = lr;
+    c.bias1 = bias1;
+    c.bias2 = bias2;
+    if (mine && o >= from)
+    {
+#pragma unroll
+      for (int e = 0; e < 4; ++e)
+      {
+        float me = mi[e], ve = vi[e], pe = pi[e];
+        adam_one(c, 0.0f, me, ve, pe);
+        mi[e] = me;
+        vi[e] = ve;
+        pi[e] = fmaxf(s.lo, fminf(s.hi, pe));
+      }
+    }
+    lr = nlr;
+    bias1 = nb1;
+    bias2 = nb2;
+  }
+  if (mine)
+  {
+    __builtin_nontemporal_store(mi, &s.m[k]);
+    __builtin_nontemporal_store(vi, &s.v[k]);
+    __builtin_nontemporal_store(pi, &s.p[k]);
+  }
+}
+
+// Catch-up of the cells inside the step's footprint rectangles (blockIdx.y =
+// grid * 16 + rectangle, cells of earlier rectangles skipped) to step upto.
+__global__ __launch_bounds__(kThreads) void k_grid_lazy_catchup_rects(LazyArgs L)
+{
+  __shared__ int4 rect[2][16];
+  const TileUpdate& A = L.A;
+  load_rects(A, rect);
+  const int grid = blockIdx.y >> 4, b = blockIdx.y & 15;
+  if (b >= A.tl.n_tiles) return;
+  const int4 r = rect[grid][b];
+  const int D = A.D[grid], Q = A.Q[grid];
+  const int64_t items = (int64_t)r.y * r.w * Q;
+  // uniform trip count: every lane of a wave takes part in lazy_run
+  for (int64_t i0 = (int64_t)blockIdx.x * kThreads; i0 < items; i0 += (int64_t)gridDim.x * kThreads)
+  {
+    const int64_t i = i0 + threadIdx.x;
+    bool live = i < items;
+    int64_t k = 0;
+    int from = L.upto;
+    if (live)
+    {
+      const int q = (int)(i % Q);
+      const int c = (int)(i / Q);
+      const int iy = c / r.y, ix = c - iy * r.y;
+      int x = r.x + ix, y = r.z + iy;
+      x -= x >= D ? D : 0;
+      y -= y >= D ? D : 0;
+      live = !in_rects(rect[grid], b, x, y, D);
+      const int64_t cell = (int64_t)y * D + x;
+      k = cell * Q + q;
+      if (live) from = L.last[grid][cell];
+    }
+    lazy_run(L, A.s[grid], k, from, live);
+  }
+}
+
+// Catch-up of every cell of both grids to step upto.
+__global__ __launch_bounds__(kThreads) void k_grid_lazy_catchup_all(LazyArgs L)
+{
+  const TileUpdate& A = L.A;
+  // uniform trip count: every lane of a wave takes part in lazy_run (the
+  // grid, g0 or g1, is per lane)
+  for (int64_t i0 = (int64_t)blockIdx.x * kThreads; i0 < A.n4; i0 += (int64_t)gridDim.x * kThreads)
+  {
+    const int64_t i = i0 + threadIdx.x;
+    const bool live = i < A.n4;
+    const int grid = live && i >= A.a4;
+    const int64_t k = !live ? 0 : (grid ? i - A.a4 : i);
+    const int from = live ? L.last[grid][k / A.Q[grid]] : L.upto;
+    lazy_run(L, A.s[grid], k, from, live);
+  }
+}
+
+// The step with index upto on the cells inside the rectangles (as
+// k_grid_update_hot, whose cells the catch-up made current), then
+// last = upto + 1 for them.
+__global__ __launch_bounds__(kThreads) void k_grid_lazy_hot(LazyArgs L)
+{
+  __shared__ int4 rect[2][16];
+  const TileUpdate& A = L.A;
+  load_rects(A, rect);
+  const int grid = blockIdx.y >> 4, b = blockIdx.y & 15;
+  if (b >= A.tl.n_tiles) return;
+  const int4 r = rect[grid][b];
+  const int D = A.D[grid], Q = A.Q[grid];
+  const GridSeg& s = A.s[grid];
+  const int64_t items = (int64_t)r.y * r.w * Q;
+  const f4 zero = {0.f, 0.f, 0.f, 0.f};
+  for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < items; i += (int64_t)gridDim.x * kThreads)
+  {
+    const int q = (int)(i % Q);
+    const int c = (int)(i / Q);
+    const int iy = c / r.y, ix = c - iy * r.y;
+    int x = r.x + ix, y = r.z + iy;
+    x -= x >= D ? D : 0;
+    y -= y >= D ? D : 0;
+    if (in_rects(rect[grid], b, x, y, D)) continue;
+    const int64_t cell = (int64_t)y * D + x;
+    const int64_t k = cell * Q + q;
+    const f4 gi = __builtin_nontemporal_load(&s.g[k]);
+    tile_item_update(A, s, k, gi);
+    __builtin_nontemporal_store(zero, &s.g[k]);
+    if (q == 0) L.last[grid][cell] = L.upto + 1;
+  }
+}
+
+__global__ __launch_bounds__(kThreads) void k_fill_i32(int64_t n, int32_t value, int32_t* __restrict__ out)
+{
+  for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads)
+    out[i] = value;
 }
 
 // The items inside the rectangles: blockIdx.y = grid * 16 + rectangle b;
@@ -464,6 +639,23 @@ int ntc_grid_update_tiles_f32(const ntc_feature_desc* fd, const ntc_tiles* tiles
   return NTC_LAUNCH_STATUS();
 }
 
+static int rect_blocks(const ntc_tiles* tiles, const TileUpdate& A, int64_t cap)
+{
+  // rectangle side bound: the corners of T texels at D / M cells per texel
+  const int M = tiles->mip_dim, T = tiles->tile_dim;
+  int64_t most = 0;
+  for (int grid = 0; grid < 2; ++grid)
+  {
+    const int64_t D = A.D[grid];
+    int64_t w = ((int64_t)(T - 1) * D + M - 1) / M + 3;
+    w = w < D ? w : D;
+    const int64_t items = w * w * A.Q[grid];
+    most = items > most ? items : most;
+  }
+  const int64_t want = (most + kThreads - 1) / kThreads;
+  return (int)(want < cap ? want : cap);
+}
+
 int ntc_grid_update_tiles_part_f32(const ntc_feature_desc* fd, const ntc_tiles* tiles, float* g0, float* dg0,
                                    float* g1, float* dg1, const ntc_grid_adam* adam, int32_t part, int32_t blocks,
                                    hipStream_t stream)
@@ -483,22 +675,72 @@ int ntc_grid_update_tiles_part_f32(const ntc_feature_desc* fd, const ntc_tiles* 
     hipLaunchKernelGGL(k_grid_update_cold, dim3((unsigned)(want < cap ? want : cap)), dim3(kThreads), 0, stream, A);
   }
   else
+    hipLaunchKernelGGL(k_grid_update_hot, dim3(rect_blocks(tiles, A, blocks > 0 ? blocks : 64), 32), dim3(kThreads), 0,
+                       stream, A);
+  return NTC_LAUNCH_STATUS();
+}
+
+static int lazy_args_of(const ntc_feature_desc* fd, const ntc_tiles* tiles, float* g0, float* dg0, float* g1,
+                        float* dg1, const ntc_grid_adam* adam, const float* tab, int32_t upto, int32_t* last0,
+                        int32_t* last1, LazyArgs& L)
+{
+  NTC_RETURN_IF(!tab || !last0 || !last1 || upto < 0, NTC_E_INVALID);
+  // the every-cell catch-up reads no tile: a one-texel placeholder passes the
+  // checks (its origin pointer is never dereferenced)
+  static const int32_t kNoOrigin[2] = {0, 0};
+  ntc_tiles all = {kNoOrigin, 1, 1, 1, 0.0f};
+  const int st = tile_update_of(fd, tiles ? tiles : &all, g0, dg0, g1, dg1, adam, L.A);
+  if (st != NTC_OK) return st;
+  L.tab = tab;
+  L.b1 = adam->beta1;
+  L.b2 = adam->beta2;
+  L.last[0] = last0;
+  L.last[1] = last1;
+  L.upto = upto;
+  return NTC_OK;
+}
+
+int ntc_grid_lazy_catchup_f32(const ntc_feature_desc* fd, const ntc_tiles* tiles, float* g0, float* g1,
+                              const ntc_grid_adam* adam, const float* steps, int32_t upto, const int32_t* last0,
+                              const int32_t* last1, hipStream_t stream)
+{
+  LazyArgs L;
+  // the catch-up touches no gradient: any non-null pointer passes the checks
+  const int st = lazy_args_of(fd, tiles, g0, g0, g1, g1, adam, steps, upto, const_cast<int32_t*>(last0),
+                              const_cast<int32_t*>(last1), L);
+  if (st != NTC_OK) return st;
+  if (upto == 0) return NTC_OK;
+  if (tiles)
+    hipLaunchKernelGGL(k_grid_lazy_catchup_rects, dim3(rect_blocks(tiles, L.A, 64), 32), dim3(kThreads), 0, stream,
+                       L);
+  else
   {
-    // rectangle side bound: the corners of T texels at D / M cells per texel
-    const int M = tiles->mip_dim, T = tiles->tile_dim;
-    int64_t most = 0;
-    for (int grid = 0; grid < 2; ++grid)
-    {
-      const int64_t D = A.D[grid];
-      int64_t w = ((int64_t)(T - 1) * D + M - 1) / M + 3;
-      w = w < D ? w : D;
-      const int64_t items = w * w * A.Q[grid];
-      most = items > most ? items : most;
-    }
-    const int64_t want = (most + kThreads - 1) / kThreads;
-    const int64_t cap = blocks > 0 ? blocks : 64;
-    hipLaunchKernelGGL(k_grid_update_hot, dim3((unsigned)(want < cap ? want : cap), 32), dim3(kThreads), 0, stream, A);
+    const int64_t want = (L.A.n4 + kThreads - 1) / kThreads;
+    hipLaunchKernelGGL(k_grid_lazy_catchup_all, dim3((unsigned)(want < 2048 ? want : 2048)), dim3(kThreads), 0,
+                       stream, L);
   }
+  return NTC_LAUNCH_STATUS();
+}
+
+int ntc_grid_lazy_hot_f32(const ntc_feature_desc* fd, const ntc_tiles* tiles, float* g0, float* dg0, float* g1,
+                          float* dg1, const ntc_grid_adam* adam, int32_t step, int32_t* last0, int32_t* last1,
+                          hipStream_t stream)
+{
+  NTC_RETURN_IF(!tiles, NTC_E_INVALID);
+  LazyArgs L;
+  static const float kUnused[3] = {0, 0, 0};
+  const int st = lazy_args_of(fd, tiles, g0, dg0, g1, dg1, adam, kUnused, step, last0, last1, L);
+  if (st != NTC_OK) return st;
+  hipLaunchKernelGGL(k_grid_lazy_hot, dim3(rect_blocks(tiles, L.A, 64), 32), dim3(kThreads), 0, stream, L);
+  return NTC_LAUNCH_STATUS();
+}
+
+int ntc_fill_i32(int64_t n, int32_t value, int32_t* out, hipStream_t stream)
+{
+  NTC_RETURN_IF(n < 1 || !out, NTC_E_INVALID);
+  const int64_t want = (n + kThreads - 1) / kThreads;
+  hipLaunchKernelGGL(k_fill_i32, dim3((unsigned)(want < 2048 ? want : 2048)), dim3(kThreads), 0, stream, n, value,
+                     out);
   return NTC_LAUNCH_STATUS();
 }
 

@@ -415,6 +415,75 @@ def test_grid_update_parts_match_tiles_update(case):
         assert np.array_equal(runs["cold_only"][k][3], init[k][3])
 
 
+@pytest.mark.parametrize("case", list(_PART_CASES))
+def test_grid_lazy_matches_eager(case):
+    """Twelve steps with fresh tile origins, gradients and Adam constants per
+    step: the one-pass footprint-aware update of every step (eager) against
+    the deferred form (catch-up of the step's footprint cells, the hot step
+    there, the other cells' zero-gradient steps applied only at a flush).
+    After a flush at step 7 and at the end, p, m, v and the gradient agree
+    bit for bit, and the cell counters hold the step count."""
+    D0, D1, C0, C1, M, T, org = _PART_CASES[case]
+    n_tiles, K, flush_at = len(org) // 2, 12, 7
+    lib = G.lib()
+    rng = np.random.default_rng(17)
+    fdc = L.FeatureDesc(D0, D1, C0, C1, 1, 4, 2, 0, 128)
+    shapes = [(D0, D0, C0), (D1, D1, C1)]
+    init = [[rng.uniform(-1, 1, shp).astype(np.float32), (rng.standard_normal(shp) * 1e-3).astype(np.float32),
+             rng.uniform(0, 1e-6, shp).astype(np.float32), np.zeros(shp, np.float32)] for shp in shapes]
+    consts = np.stack([rng.uniform(0.002, 0.01, K), 1.0 / (1 - 0.9 ** np.arange(1, K + 1)),
+                       1.0 / (1 - 0.999 ** np.arange(1, K + 1))], 1).astype(np.float32)
+    bounds = (-1.0, 1.0 - 2.0 / 16, -1.0, 1.0 - 2.0 / 16)
+    eager = [[G.dev(x) for x in grid] for grid in init]
+    lazy = [[G.dev(x) for x in grid] for grid in init]
+    last = [G.dev(np.zeros(D0 * D0, np.int32)), G.dev(np.zeros(D1 * D1, np.int32))]
+    tab = G.dev(consts.ravel())
+
+    def adam(st, k):
+        lr, b1c, b2c = (float(x) for x in consts[k])
+        return L.GridAdam(G.p(st[0][1]), G.p(st[0][2]), G.p(st[1][1]), G.p(st[1][2]), lr, 0.9, 0.999, b1c, b2c,
+                          *bounds)
+
+    def flush(upto):
+        ad = adam(lazy, 0)
+        G.ok(lib.ntc_grid_lazy_catchup_f32(G.byref(fdc), None, G.p(lazy[0][0]), G.p(lazy[1][0]), G.byref(ad),
+                                           G.p(tab), upto, G.p(last[0]), G.p(last[1]), None), "flush")
+        for k in range(2):
+            G.ok(lib.ntc_fill_i32((D0, D1)[k] ** 2, upto, G.p(last[k]), None), "mark")
+
+    def compare(step):
+        for k in range(2):
+            for j, name in enumerate("pmvg"):
+                got, want = G.host(lazy[k][j]), G.host(eager[k][j])
+                assert np.array_equal(got, want), (case, step, k, name, int(np.count_nonzero(got != want)))
+            assert (G.host(last[k]) == step).all()
+
+    for k in range(K):
+        origins = rng.integers(0, M, 2 * n_tiles).astype(np.int32)
+        og = G.dev(origins)
+        tl = G.tiles(og, n_tiles, T, M, 0.0)
+        grads = [(rng.standard_normal(shp) * 1e-3).astype(np.float32) for shp in shapes]
+        for st in (eager, lazy):
+            for g in range(2):
+                G.ok(lib.ntc_dev_upload(G.p(st[g][3]), grads[g].ctypes.data, grads[g].nbytes), "upload")
+        ad = adam(eager, k)
+        G.ok(lib.ntc_grid_update_tiles_f32(G.byref(fdc), G.byref(tl), G.p(eager[0][0]), G.p(eager[0][3]),
+                                           G.p(eager[1][0]), G.p(eager[1][3]), G.byref(ad), None), "eager")
+        ad = adam(lazy, k)
+        G.ok(lib.ntc_grid_lazy_catchup_f32(G.byref(fdc), G.byref(tl), G.p(lazy[0][0]), G.p(lazy[1][0]),
+                                           G.byref(ad), G.p(tab), k, G.p(last[0]), G.p(last[1]), None), "catch-up")
+        G.ok(lib.ntc_grid_lazy_hot_f32(G.byref(fdc), G.byref(tl), G.p(lazy[0][0]), G.p(lazy[0][3]),
+                                       G.p(lazy[1][0]), G.p(lazy[1][3]), G.byref(ad), k, G.p(last[0]),
+                                       G.p(last[1]), None), "hot")
+        if k + 1 == flush_at:
+            flush(k + 1)
+            compare(k + 1)
+    # before the final flush some cells are behind
+    assert any((G.host(last[g]) < K).any() for g in range(2))
+    flush(K)
+    compare(K)
+
+
 @pytest.mark.parametrize("kind,lod,n_tiles", [("c1_bilinear_relu", 0, 8), ("c1_bilinear_relu", 1, 4),
                                               ("c2_bilinear", 0, 8), ("c1_bilinear_relu", 2, 8)])
 def test_fused_sets_partials_repeat_bit_for_bit(oracle, kind, lod, n_tiles):
