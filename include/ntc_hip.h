@@ -238,15 +238,19 @@ int ntc_grid_update_tiles_part_f32(const ntc_feature_desc* fd, const ntc_tiles* 
  * beta1, beta2 and the clamp bounds (and, for the hot part, this step's lr
  * and bias factors).
  * Catch-up: applies the zero-gradient steps last .. upto - 1 (in order) to
- * the cells of the footprint rectangles of tiles, or, with tiles null, to
- * every cell of both grids (last is not changed: after an every-cell
- * catch-up set it to upto with ntc_fill_i32).
+ * the cells of the footprint rectangles of tiles, leaving out those in the
+ * rectangles of skip (null: none; the previous step's tiles on the same
+ * level, whose hot part brings those cells current, so that this catch-up
+ * can run beside it), or, with tiles null, to every cell of both grids (last
+ * is not changed: after an every-cell catch-up set it to upto with
+ * ntc_fill_i32).
  * Hot: the level's step `step` on the cells of the rectangles (gradient read
  * and cleared, as NTC_TILES_HOT), which must be current (caught up to step),
  * then last = step + 1 there. Grid sides powers of two, channel counts
  * multiples of 4 up to 16, n_tiles <= 16. */
 int ntc_grid_lazy_catchup_f32(const ntc_feature_desc* fd, const ntc_tiles* tiles,
-                              float* g0, float* g1, const ntc_grid_adam* adam,
+                              const ntc_tiles* skip, float* g0, float* g1,
+                              const ntc_grid_adam* adam,
                               const float* steps, int32_t upto,
                               const int32_t* last0, const int32_t* last1,
                               hipStream_t stream);

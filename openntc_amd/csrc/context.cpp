@@ -344,6 +344,9 @@ Result Context::Impl::InitDevice()
   int count = 0;
   if (hipGetDeviceCount(&count) != hipSuccess || count < 1) return Result::CudaFailure;
   CTX_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), Result::CudaFailure);
+  CTX_HIP(hipStreamCreateWithFlags(&side, hipStreamNonBlocking), Result::CudaFailure);
+  CTX_HIP(hipEventCreateWithFlags(&fused_ev, hipEventDisableTiming), Result::CudaFailure);
+  CTX_HIP(hipEventCreateWithFlags(&catch_ev, hipEventDisableTiming), Result::CudaFailure);
   const size_t dbytes = sizeof(float) * (size_t)DecoderElems();
   CTX_HIP(dec.Alloc(dbytes), Result::AllocationFailure);
   CTX_HIP(dec_m.Alloc(dbytes), Result::AllocationFailure);
@@ -394,7 +397,11 @@ void Context::Impl::FreeAll()
   plain_batch = 0;
   decode_table_prec = -1;
   if (stream) (void)hipStreamDestroy(stream);
-  stream = nullptr;
+  if (side) (void)hipStreamDestroy(side);
+  if (fused_ev) (void)hipEventDestroy(fused_ev);
+  if (catch_ev) (void)hipEventDestroy(catch_ev);
+  stream = side = nullptr;
+  fused_ev = catch_ev = nullptr;
 }
 
 Result Context::Impl::EnsurePlainBuffers(int64_t batch)
@@ -607,6 +614,7 @@ Result Context::Impl::SetupLazyGrids()
     lazy_done[l] = 0;
     lazy_behind[l] = false;
   }
+  caught_step = -1;
   if (!lazy) return Result::Success;
   std::vector<float> tab[kMaxLevels];
   for (const StepPlan& st : plan)
@@ -642,7 +650,7 @@ Result Context::Impl::FlushLazyGrids()
     const ntc_feature_desc fd = FeatureDesc(l);
     const ntc_grid_adam adam = {g0m[l].as<float>(), g0v[l].as<float>(), g1m[l].as<float>(), g1v[l].as<float>(),
                                 0.0f, 0.9f, 0.999f, 1.0f, 1.0f, -1.0f, 1.0f - g0_delta, -1.0f, 1.0f - g1_delta};
-    CTX_NTC(ntc_grid_lazy_catchup_f32(&fd, nullptr, g0[l].as<float>(), g1[l].as<float>(), &adam,
+    CTX_NTC(ntc_grid_lazy_catchup_f32(&fd, nullptr, nullptr, g0[l].as<float>(), g1[l].as<float>(), &adam,
                                       lazy_steps[l].as<float>(), lazy_done[l], lazy_last[l][0].as<int32_t>(),
                                       lazy_last[l][1].as<int32_t>(), stream));
     for (int which = 0; which < 2; ++which)
@@ -799,17 +807,47 @@ Result Context::Impl::RunFusedStep(const StepPlan& st, int32_t step_index, hipEv
                               g1v[lvl].as<float>(), st.lr_grid, 0.9f, 0.999f, st.grid_bias1, st.grid_bias2,
                               -1.0f, 1.0f - g0_delta, -1.0f, 1.0f - g1_delta};
   if (tiles_update) ++tiles_update_steps;
-  if (deferred)
+  if (deferred && caught_step == step_index)
+    CTX_HIP(hipStreamWaitEvent(stream, catch_ev, 0), Result::CudaFailure);  // issued during the last step
+  else if (deferred)
   {
     if (ev) CTX_HIP(hipEventRecord(ev[3], stream), Result::CudaFailure);
-    CTX_NTC(ntc_grid_lazy_catchup_f32(&fd, &a.tiles, g0[lvl].as<float>(), g1[lvl].as<float>(), &adam,
+    CTX_NTC(ntc_grid_lazy_catchup_f32(&fd, &a.tiles, nullptr, g0[lvl].as<float>(), g1[lvl].as<float>(), &adam,
                                       lazy_steps[lvl].as<float>(), lazy_done[lvl], lazy_last[lvl][0].as<int32_t>(),
                                       lazy_last[lvl][1].as<int32_t>(), stream));
     if (ev) CTX_HIP(hipEventRecord(ev[4], stream), Result::CudaFailure);
   }
+  caught_step = -1;
   if (ev) CTX_HIP(hipEventRecord(ev[0], stream), Result::CudaFailure);
   CTX_NTC(ntc_fused_train_step(&a, stream));
   if (ev) CTX_HIP(hipEventRecord(ev[1], stream), Result::CudaFailure);
+  // the next step's catch-up, beside the rest of this step: it leaves out
+  // this step's footprint cells when it is on the same level (the hot part
+  // below brings those current) and reads nothing else this step writes
+  const int nx = step_index + 1;
+  if (deferred && nx < train_end && plan[nx].grid_update && !plan[nx].lock)
+  {
+    const StepPlan& sn = plan[nx];
+    const int ln = sn.level;
+    const ntc_feature_desc fdn = FeatureDesc(ln);
+    const ntc_tiles tn = {origins_table.as<int32_t>() + (int64_t)nx * 16, sn.n_tiles, sn.tile_dim,
+                          mip_dim[sn.lod], (float)sn.lod / (float)(mip_count - 1)};
+    const ntc_grid_adam adn = {g0m[ln].as<float>(), g0v[ln].as<float>(), g1m[ln].as<float>(), g1v[ln].as<float>(),
+                               sn.lr_grid, 0.9f, 0.999f, sn.grid_bias1, sn.grid_bias2,
+                               -1.0f, 1.0f - g0_delta, -1.0f, 1.0f - g1_delta};
+    // its step count: this step's hot part counts one more on the same level
+    const int32_t upto = lazy_done[ln] + (ln == lvl ? 1 : 0);
+    CTX_HIP(hipEventRecord(fused_ev, stream), Result::CudaFailure);
+    CTX_HIP(hipStreamWaitEvent(side, fused_ev, 0), Result::CudaFailure);
+    hipEvent_t* evn = ev ? ev + kStepEvents : nullptr;
+    if (evn) CTX_HIP(hipEventRecord(evn[3], side), Result::CudaFailure);
+    CTX_NTC(ntc_grid_lazy_catchup_f32(&fdn, &tn, ln == lvl ? &a.tiles : nullptr, g0[ln].as<float>(),
+                                      g1[ln].as<float>(), &adn, lazy_steps[ln].as<float>(), upto,
+                                      lazy_last[ln][0].as<int32_t>(), lazy_last[ln][1].as<int32_t>(), side));
+    if (evn) CTX_HIP(hipEventRecord(evn[4], side), Result::CudaFailure);
+    CTX_HIP(hipEventRecord(catch_ev, side), Result::CudaFailure);
+    caught_step = nx;
+  }
   CTX_NTC(ntc_decoder_update(H, feature_dim, O, a.blocks, dw_part.as<float>(), st.lr_decoder, 0.9f, 0.999f, st.bias1,
                              st.bias2, dec_grad.as<float>(), dec_m.as<float>(), dec_v.as<float>(), dec.as<float>(),
                              sse_part.as<double>(), scalars.as<double>() + 16, stream));
@@ -1413,6 +1451,8 @@ TrainProgress Context::Train(int32_t num_batches)
   }
   const int first = I.batch_i;
   I.tiles_update_steps = 0;
+  I.train_end = target;
+  I.caught_step = -1;
   if (timed) I.ev_lazy.assign((size_t)n, 0);
   ++I.state_version;
   for (; I.batch_i < target; ++I.batch_i)

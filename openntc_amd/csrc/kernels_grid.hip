@@ -252,6 +252,8 @@ struct LazyArgs
   float b1, b2;
   int32_t* last[2];
   int32_t upto;  // catch-up: apply steps last .. upto - 1; hot: this step's index
+  ntc_tiles skip;  // catch-up of rectangles: cells in these tiles' rectangles are left out
+  int32_t n_skip;  // 0: none
 };
 
 // Smallest v over the wave (every lane active), in a scalar register.
@@ -294,11 +296,15 @@ __device__ __forceinline__ void lazy_run(const LazyArgs& L, const GridSeg& s, in
     c.bias2 = bias2;
     if (mine && o >= from)
     {
+      // adam_one with g = 0, its gradient terms (1-b1) 0 and ((1-b2) 0) 0
+      // formed once (wave-uniform) instead of per element: the same values
+      const float zm = c.omb1 * 0.0f, zv = (c.omb2 * 0.0f) * 0.0f;
 #pragma unroll
       for (int e = 0; e < 4; ++e)
       {
-        float me = mi[e], ve = vi[e], pe = pi[e];
-        adam_one(c, 0.0f, me, ve, pe);
+        const float me = fmaf(c.b1, mi[e], zm);
+        const float ve = fmaf(c.b2, vi[e], zv);
+        const float pe = pi[e] - (c.lr * (me * c.bias1)) / (__builtin_sqrtf(ve * c.bias2) + 1e-8f);
         mi[e] = me;
         vi[e] = ve;
         pi[e] = fmaxf(s.lo, fminf(s.hi, pe));
@@ -320,9 +326,17 @@ __device__ __forceinline__ void lazy_run(const LazyArgs& L, const GridSeg& s, in
 // grid * 16 + rectangle, cells of earlier rectangles skipped) to step upto.
 __global__ __launch_bounds__(kThreads) void k_grid_lazy_catchup_rects(LazyArgs L)
 {
-  __shared__ int4 rect[2][16];
+  __shared__ int4 rect[2][16], srect[2][16];
   const TileUpdate& A = L.A;
-  load_rects(A, rect);
+  if (threadIdx.x < 2 * L.n_skip)
+  {
+    const int n = L.n_skip, M = L.skip.mip_dim;
+    const int g = threadIdx.x >= n, b = g ? threadIdx.x - n : threadIdx.x;
+    const int ox = (int)((uint32_t)L.skip.origins[2 * b] % (uint32_t)M);
+    const int oy = (int)((uint32_t)L.skip.origins[2 * b + 1] % (uint32_t)M);
+    srect[g][b] = tile_rect(ox, oy, L.skip.tile_dim, M, A.D[g]);
+  }
+  load_rects(A, rect);  // ends with a barrier
   const int grid = blockIdx.y >> 4, b = blockIdx.y & 15;
   if (b >= A.tl.n_tiles) return;
   const int4 r = rect[grid][b];
@@ -343,7 +357,7 @@ __global__ __launch_bounds__(kThreads) void k_grid_lazy_catchup_rects(LazyArgs L
       int x = r.x + ix, y = r.z + iy;
       x -= x >= D ? D : 0;
       y -= y >= D ? D : 0;
-      live = !in_rects(rect[grid], b, x, y, D);
+      live = !in_rects(rect[grid], b, x, y, D) && !in_rects(srect[grid], L.n_skip, x, y, D);
       const int64_t cell = (int64_t)y * D + x;
       k = cell * Q + q;
       if (live) from = L.last[grid][cell];
@@ -697,18 +711,26 @@ static int lazy_args_of(const ntc_feature_desc* fd, const ntc_tiles* tiles, floa
   L.last[0] = last0;
   L.last[1] = last1;
   L.upto = upto;
+  L.skip = ntc_tiles{nullptr, 0, 1, 1, 0.0f};
+  L.n_skip = 0;
   return NTC_OK;
 }
 
-int ntc_grid_lazy_catchup_f32(const ntc_feature_desc* fd, const ntc_tiles* tiles, float* g0, float* g1,
-                              const ntc_grid_adam* adam, const float* steps, int32_t upto, const int32_t* last0,
-                              const int32_t* last1, hipStream_t stream)
+int ntc_grid_lazy_catchup_f32(const ntc_feature_desc* fd, const ntc_tiles* tiles, const ntc_tiles* skip,
+                              float* g0, float* g1, const ntc_grid_adam* adam, const float* steps, int32_t upto,
+                              const int32_t* last0, const int32_t* last1, hipStream_t stream)
 {
   LazyArgs L;
   // the catch-up touches no gradient: any non-null pointer passes the checks
   const int st = lazy_args_of(fd, tiles, g0, g0, g1, g1, adam, steps, upto, const_cast<int32_t*>(last0),
                               const_cast<int32_t*>(last1), L);
   if (st != NTC_OK) return st;
+  if (skip)
+  {
+    NTC_RETURN_IF(!tiles || !valid_tiles(skip) || skip->n_tiles > 16, NTC_E_INVALID);
+    L.skip = *skip;
+    L.n_skip = skip->n_tiles;
+  }
   if (upto == 0) return NTC_OK;
   if (tiles)
     hipLaunchKernelGGL(k_grid_lazy_catchup_rects, dim3(rect_blocks(tiles, L.A, 64), 32), dim3(kThreads), 0, stream,

@@ -415,14 +415,17 @@ def test_grid_update_parts_match_tiles_update(case):
         assert np.array_equal(runs["cold_only"][k][3], init[k][3])
 
 
+@pytest.mark.parametrize("pipelined", [False, True])
 @pytest.mark.parametrize("case", list(_PART_CASES))
-def test_grid_lazy_matches_eager(case):
+def test_grid_lazy_matches_eager(case, pipelined):
     """Twelve steps with fresh tile origins, gradients and Adam constants per
     step: the one-pass footprint-aware update of every step (eager) against
     the deferred form (catch-up of the step's footprint cells, the hot step
     there, the other cells' zero-gradient steps applied only at a flush).
-    After a flush at step 7 and at the end, p, m, v and the gradient agree
-    bit for bit, and the cell counters hold the step count."""
+    Pipelined: the catch-up of step k + 1 leaves out step k's rectangles and
+    runs before step k's hot part (as it may beside it); never across a
+    flush. After a flush at step 7 and at the end, p, m, v and the gradient
+    agree bit for bit, and the cell counters hold the step count."""
     D0, D1, C0, C1, M, T, org = _PART_CASES[case]
     n_tiles, K, flush_at = len(org) // 2, 12, 7
     lib = G.lib()
@@ -446,7 +449,7 @@ def test_grid_lazy_matches_eager(case):
 
     def flush(upto):
         ad = adam(lazy, 0)
-        G.ok(lib.ntc_grid_lazy_catchup_f32(G.byref(fdc), None, G.p(lazy[0][0]), G.p(lazy[1][0]), G.byref(ad),
+        G.ok(lib.ntc_grid_lazy_catchup_f32(G.byref(fdc), None, None, G.p(lazy[0][0]), G.p(lazy[1][0]), G.byref(ad),
                                            G.p(tab), upto, G.p(last[0]), G.p(last[1]), None), "flush")
         for k in range(2):
             G.ok(lib.ntc_fill_i32((D0, D1)[k] ** 2, upto, G.p(last[k]), None), "mark")
@@ -458,21 +461,31 @@ def test_grid_lazy_matches_eager(case):
                 assert np.array_equal(got, want), (case, step, k, name, int(np.count_nonzero(got != want)))
             assert (G.host(last[k]) == step).all()
 
+    ogs = [G.dev(rng.integers(0, M, 2 * n_tiles).astype(np.int32)) for _ in range(K)]
+    tls = [G.tiles(og, n_tiles, T, M, 0.0) for og in ogs]
+
+    def catch_up(k, skip):
+        ad = adam(lazy, 0)
+        G.ok(lib.ntc_grid_lazy_catchup_f32(G.byref(fdc), G.byref(tls[k]), G.byref(skip) if skip else None,
+                                           G.p(lazy[0][0]), G.p(lazy[1][0]), G.byref(ad), G.p(tab), k,
+                                           G.p(last[0]), G.p(last[1]), None), "catch-up")
+
+    issued = -1
     for k in range(K):
-        origins = rng.integers(0, M, 2 * n_tiles).astype(np.int32)
-        og = G.dev(origins)
-        tl = G.tiles(og, n_tiles, T, M, 0.0)
         grads = [(rng.standard_normal(shp) * 1e-3).astype(np.float32) for shp in shapes]
         for st in (eager, lazy):
             for g in range(2):
                 G.ok(lib.ntc_dev_upload(G.p(st[g][3]), grads[g].ctypes.data, grads[g].nbytes), "upload")
         ad = adam(eager, k)
-        G.ok(lib.ntc_grid_update_tiles_f32(G.byref(fdc), G.byref(tl), G.p(eager[0][0]), G.p(eager[0][3]),
+        G.ok(lib.ntc_grid_update_tiles_f32(G.byref(fdc), G.byref(tls[k]), G.p(eager[0][0]), G.p(eager[0][3]),
                                            G.p(eager[1][0]), G.p(eager[1][3]), G.byref(ad), None), "eager")
+        if issued != k:
+            catch_up(k, None)
+        if pipelined and k + 1 < K and k + 1 != flush_at:
+            catch_up(k + 1, tls[k])
+            issued = k + 1
         ad = adam(lazy, k)
-        G.ok(lib.ntc_grid_lazy_catchup_f32(G.byref(fdc), G.byref(tl), G.p(lazy[0][0]), G.p(lazy[1][0]),
-                                           G.byref(ad), G.p(tab), k, G.p(last[0]), G.p(last[1]), None), "catch-up")
-        G.ok(lib.ntc_grid_lazy_hot_f32(G.byref(fdc), G.byref(tl), G.p(lazy[0][0]), G.p(lazy[0][3]),
+        G.ok(lib.ntc_grid_lazy_hot_f32(G.byref(fdc), G.byref(tls[k]), G.p(lazy[0][0]), G.p(lazy[0][3]),
                                        G.p(lazy[1][0]), G.p(lazy[1][3]), G.byref(ad), k, G.p(last[0]),
                                        G.p(last[1]), None), "hot")
         if k + 1 == flush_at:
