@@ -607,8 +607,16 @@ Result Context::Impl::LockGrids()
 // multiples of 4 up to 16, at most 16 tiles).
 Result Context::Impl::SetupLazyGrids()
 {
+  // Worth it when a step's tiles cover a small part of the texture: a
+  // step's footprint cells are walked twice (catch-up, gradient step) instead
+  // of once, and every owed step costs its arithmetic when applied. Taken
+  // when the tiles of a mip-0 step cover less than 1/8 of it (4096^2 with
+  // 8 tiles: 3 %; a 256^2 set, whose one tile covers it all, keeps the
+  // eager one-pass update).
+  const int64_t T0 = std::min(mip_dim[0], 256);
   lazy = path == TrainPath::Fused && g0_channels % 4 == 0 && g1_channels % 4 == 0 && g0_channels <= 16 &&
-         g1_channels <= 16 && grids_per_batch <= 16;
+         g1_channels <= 16 && grids_per_batch <= 16 &&
+         8 * (int64_t)grids_per_batch * T0 * T0 < (int64_t)mip_dim[0] * mip_dim[0];
   for (int l = 0; l < kMaxLevels; ++l)
   {
     lazy_done[l] = 0;

@@ -324,7 +324,9 @@ __device__ __forceinline__ void lazy_run(const LazyArgs& L, const GridSeg& s, in
 
 // Catch-up of the cells inside the step's footprint rectangles (blockIdx.y =
 // grid * 16 + rectangle, cells of earlier rectangles skipped) to step upto.
-__global__ __launch_bounds__(kThreads) void k_grid_lazy_catchup_rects(LazyArgs L)
+// Rectangle slot `slot` (grid * n_tiles + rectangle) of a rectangle walk,
+// block bx of nbx on it.
+__device__ __forceinline__ void lazy_catchup_rects_body(const LazyArgs& L, int bx, int nbx, int slot)
 {
   __shared__ int4 rect[2][16], srect[2][16];
   const TileUpdate& A = L.A;
@@ -337,13 +339,13 @@ __global__ __launch_bounds__(kThreads) void k_grid_lazy_catchup_rects(LazyArgs L
     srect[g][b] = tile_rect(ox, oy, L.skip.tile_dim, M, A.D[g]);
   }
   load_rects(A, rect);  // ends with a barrier
-  const int grid = blockIdx.y >> 4, b = blockIdx.y & 15;
-  if (b >= A.tl.n_tiles) return;
+  const int nt = A.tl.n_tiles;
+  const int grid = slot >= nt, b = grid ? slot - nt : slot;
   const int4 r = rect[grid][b];
   const int D = A.D[grid], Q = A.Q[grid];
   const int64_t items = (int64_t)r.y * r.w * Q;
   // uniform trip count: every lane of a wave takes part in lazy_run
-  for (int64_t i0 = (int64_t)blockIdx.x * kThreads; i0 < items; i0 += (int64_t)gridDim.x * kThreads)
+  for (int64_t i0 = (int64_t)bx * kThreads; i0 < items; i0 += (int64_t)nbx * kThreads)
   {
     const int64_t i = i0 + threadIdx.x;
     bool live = i < items;
@@ -366,6 +368,14 @@ __global__ __launch_bounds__(kThreads) void k_grid_lazy_catchup_rects(LazyArgs L
   }
 }
 
+// Catch-up of the cells inside the step's footprint rectangles to step upto:
+// blockIdx.y = rectangle slot, cells of earlier rectangles left out.
+__global__ __launch_bounds__(kThreads) void k_grid_lazy_catchup_rects(LazyArgs L)
+{
+  if ((int)blockIdx.y >= 2 * L.A.tl.n_tiles) return;
+  lazy_catchup_rects_body(L, blockIdx.x, gridDim.x, blockIdx.y);
+}
+
 // Catch-up of every cell of both grids to step upto.
 __global__ __launch_bounds__(kThreads) void k_grid_lazy_catchup_all(LazyArgs L)
 {
@@ -386,19 +396,19 @@ __global__ __launch_bounds__(kThreads) void k_grid_lazy_catchup_all(LazyArgs L)
 // The step with index upto on the cells inside the rectangles (as
 // k_grid_update_hot, whose cells the catch-up made current), then
 // last = upto + 1 for them.
-__global__ __launch_bounds__(kThreads) void k_grid_lazy_hot(LazyArgs L)
+__device__ __forceinline__ void lazy_hot_body(const LazyArgs& L, int bx, int nbx, int slot)
 {
   __shared__ int4 rect[2][16];
   const TileUpdate& A = L.A;
   load_rects(A, rect);
-  const int grid = blockIdx.y >> 4, b = blockIdx.y & 15;
-  if (b >= A.tl.n_tiles) return;
+  const int nt = A.tl.n_tiles;
+  const int grid = slot >= nt, b = grid ? slot - nt : slot;
   const int4 r = rect[grid][b];
   const int D = A.D[grid], Q = A.Q[grid];
   const GridSeg& s = A.s[grid];
   const int64_t items = (int64_t)r.y * r.w * Q;
   const f4 zero = {0.f, 0.f, 0.f, 0.f};
-  for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < items; i += (int64_t)gridDim.x * kThreads)
+  for (int64_t i = (int64_t)bx * kThreads + threadIdx.x; i < items; i += (int64_t)nbx * kThreads)
   {
     const int q = (int)(i % Q);
     const int c = (int)(i / Q);
@@ -414,6 +424,15 @@ __global__ __launch_bounds__(kThreads) void k_grid_lazy_hot(LazyArgs L)
     __builtin_nontemporal_store(zero, &s.g[k]);
     if (q == 0) L.last[grid][cell] = L.upto + 1;
   }
+}
+
+// The step with index upto on the cells inside the rectangles (as
+// k_grid_update_hot, whose cells the catch-up made current), then
+// last = upto + 1 for them; blockIdx.y = rectangle slot.
+__global__ __launch_bounds__(kThreads) void k_grid_lazy_hot(LazyArgs L)
+{
+  if ((int)blockIdx.y >= 2 * L.A.tl.n_tiles) return;
+  lazy_hot_body(L, blockIdx.x, gridDim.x, blockIdx.y);
 }
 
 __global__ __launch_bounds__(kThreads) void k_fill_i32(int64_t n, int32_t value, int32_t* __restrict__ out)
