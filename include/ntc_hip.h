@@ -590,7 +590,7 @@ void ntc_batch_destroy(ntc_batch* b);
  * ntc_manifest_parse checks a manifest with the reference's rules and returns
  * its sources in training order (stable sort by semantic), then decodes every
  * image; the status is what LoadManifest returns before any GPU work.
- * ntc_image_decode_rgba8 decodes one image (PNG, TGA, binary PNM) to RGBA8
+ * ntc_image_decode_rgba8 decodes one image (JPEG, PNG, BMP, TGA, binary PNM) to RGBA8
  * (rgba may be NULL to query the size). */
 typedef struct ntc_manifest_source
 {

@@ -22,6 +22,7 @@
 #include <vector>
 
 #include "context_impl.h"
+#include "images.h"
 #include "json.h"
 
 using namespace openntc;
@@ -357,7 +358,10 @@ bool DecodeImage(const std::string& path, std::vector<uint8_t>& rgba, int& w, in
 {
   std::vector<uint8_t> f;
   if (!ReadFile(path, f)) return false;
+  // the order stb_image tries formats in (those read here)
+  if (img::DecodeJpeg(f, rgba, w, h)) return true;
   if (DecodePng(f, rgba, w, h)) return true;
+  if (img::DecodeBmp(f, rgba, w, h)) return true;
   if (DecodePnm(f, rgba, w, h)) return true;
   return DecodeTga(f, rgba, w, h);
 }
