@@ -752,7 +752,7 @@ int ntc_grid_lazy_catchup_f32(const ntc_feature_desc* fd, const ntc_tiles* tiles
   }
   if (upto == 0) return NTC_OK;
   if (tiles)
-    hipLaunchKernelGGL(k_grid_lazy_catchup_rects, dim3(rect_blocks(tiles, L.A, 64), 32), dim3(kThreads), 0, stream,
+    hipLaunchKernelGGL(k_grid_lazy_catchup_rects, dim3(rect_blocks(tiles, L.A, 256), 32), dim3(kThreads), 0, stream,
                        L);
   else
   {
@@ -772,7 +772,7 @@ int ntc_grid_lazy_hot_f32(const ntc_feature_desc* fd, const ntc_tiles* tiles, fl
   static const float kUnused[3] = {0, 0, 0};
   const int st = lazy_args_of(fd, tiles, g0, dg0, g1, dg1, adam, kUnused, step, last0, last1, L);
   if (st != NTC_OK) return st;
-  hipLaunchKernelGGL(k_grid_lazy_hot, dim3(rect_blocks(tiles, L.A, 64), 32), dim3(kThreads), 0, stream, L);
+  hipLaunchKernelGGL(k_grid_lazy_hot, dim3(rect_blocks(tiles, L.A, 256), 32), dim3(kThreads), 0, stream, L);
   return NTC_LAUNCH_STATUS();
 }
 
