@@ -119,12 +119,13 @@ struct Context::Impl
   DeviceBuffer lazy_steps[kMaxLevels];
   int32_t lazy_done[kMaxLevels] = {};
   bool lazy_behind[kMaxLevels] = {};
-  // The next step's catch-up runs on side once the fused kernel is done
-  // (beside the decoder update, table pack and hot part, whose cells it
-  // leaves out); caught_step: the step it was issued for (-1: none), only
-  // ever the next step of the same Train call. train_end: that call's end.
+  // The next step's catch-up runs on side once ready_ev (after the step's
+  // fused kernel) has fired, beside the step's decoder update, table pack and
+  // hot part, whose cells it leaves out; caught_step: the step it was issued
+  // for (-1: none), only ever the next step of the same Train call.
+  // train_end: that call's end.
   hipStream_t side = nullptr;
-  hipEvent_t fused_ev = nullptr, catch_ev = nullptr;
+  hipEvent_t ready_ev = nullptr, catch_ev = nullptr;
   int32_t caught_step = -1, train_end = 0;
 
   // ---- timing (optional) ----

@@ -286,11 +286,15 @@ __device__ __forceinline__ void lazy_run(const LazyArgs& L, const GridSeg& s, in
   c.b2 = L.b2;
   c.omb1 = 1.0f - L.b1;  // as make_adam
   c.omb2 = 1.0f - L.b2;
-  float lr = L.tab[3 * lo], bias1 = L.tab[3 * lo + 1], bias2 = L.tab[3 * lo + 2];
+  // the step table is written before training and only read here: through
+  // the constant address space the wave-uniform reads are scalar loads
+  typedef __attribute__((address_space(4))) const float ConstFloat;
+  ConstFloat* tab = (ConstFloat*)L.tab;
+  float lr = tab[3 * lo], bias1 = tab[3 * lo + 1], bias2 = tab[3 * lo + 2];
   for (int o = lo; o < L.upto; ++o)
   {
     const int on = o + 1 < L.upto ? o + 1 : o;
-    const float nlr = L.tab[3 * on], nb1 = L.tab[3 * on + 1], nb2 = L.tab[3 * on + 2];
+    const float nlr = tab[3 * on], nb1 = tab[3 * on + 1], nb2 = tab[3 * on + 2];
     c.lr = lr;
     c.bias1 = bias1;
     c.bias2 = bias2;
