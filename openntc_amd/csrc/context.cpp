@@ -14,6 +14,7 @@
 //    written to noise buffers each step;
 //  - eval sums squared errors in FP64 on the device and reads them back once.
 #include <algorithm>
+#include <array>
 #include <bit>
 #include <cmath>
 #include <cstring>
@@ -931,24 +932,20 @@ Result Context::Impl::ForwardTiles(int32_t mip, const int32_t* origins, int32_t 
 
 // Features and targets of one eval batch, then the reduced-precision decode
 // with the device table in decode_table, adding per-channel SSE into sse.
-Result Context::Impl::DecodeTiles(int32_t mip, const int32_t* origins, int32_t n_tiles, int32_t T, int32_t precision,
-                                  double* sse)
+Result Context::Impl::DecodeTiles(int32_t mip, const int32_t* dev_origins, int32_t n_tiles, int32_t T,
+                                  int32_t precision, double* sse)
 {
   const int64_t B = (int64_t)n_tiles * T * T;
   Result r = EnsurePlainBuffers(B);
   if (r != Result::Success) return r;
   const int lvl = LevelForLod(mip);
-  CTX_HIP(hipMemcpyAsync(eval_origins.ptr, origins, sizeof(int32_t) * 2 * n_tiles, hipMemcpyHostToDevice, stream),
-          Result::CudaFailure);
   const ntc_feature_desc fd = FeatureDesc(lvl);
-  ntc_tiles tl = {eval_origins.as<int32_t>(), n_tiles, T, mip_dim[mip], (float)mip / (float)(mip_count - 1)};
+  ntc_tiles tl = {dev_origins, n_tiles, T, mip_dim[mip], (float)mip / (float)(mip_count - 1)};
   ntc_noise nz = {};
   CTX_NTC(ntc_draw_features_f32(&fd, &tl, &nz, g0[lvl].as<float>(), g1[lvl].as<float>(), x.as<float>(), stream));
   CTX_NTC(ntc_draw_targets_f32(&tl, out_dim, package[mip].as<uint8_t>(), yt.as<float>(), stream));
   CTX_NTC(ntc_decode_sse_f32(precision, ActCode(), feature_dim, feature_dim_padded, hidden_dim, out_dim, B,
                              decode_table.ptr, x.as<float>(), yt.as<float>(), sse, nullptr, decode_ws.ptr, stream));
-  // the origins buffer is reused by the next batch
-  CTX_HIP(hipStreamSynchronize(stream), Result::CudaFailure);
   return Result::Success;
 }
 
@@ -1074,19 +1071,47 @@ EvalResults Context::Impl::Evaluate(const Context& ctx, EvalPrecision p, EvalDet
     return EvalResults{};
   }
   double count = 0.0;
-  err = WalkEvalTiles(*this, [&](int mip, const int32_t* origins, int nb, int T) -> Result {
-    const int64_t B = (int64_t)nb * T * T;
-    if (table_path)
+  if (table_path)
+  {
+    // every batch's tile origins uploaded once, then the batches back to back
+    // with no host synchronisation between them
+    std::vector<int32_t> all;
+    std::vector<std::array<int32_t, 4>> batches;  // mip, first origin int, tiles, T
+    int64_t most = 0;
+    WalkEvalTiles(*this, [&](int mip, const int32_t* origins, int nb, int T) -> Result {
+      batches.push_back({mip, (int32_t)all.size(), nb, T});
+      all.insert(all.end(), origins, origins + 2 * nb);
+      most = std::max<int64_t>(most, (int64_t)nb * T * T);
+      return Result::Success;
+    });
+    if ((err = EnsurePlainBuffers(most)) != Result::Success) return EvalResults{};
+    if (eval_origins.bytes < sizeof(int32_t) * all.size() &&
+        eval_origins.Alloc(sizeof(int32_t) * all.size()) != hipSuccess)
     {
-      count += (double)B;
-      return DecodeTiles(mip, origins, nb, T, prec, sse);
+      err = Result::AllocationFailure;
+      return EvalResults{};
     }
-    Result r = ForwardTiles(mip, origins, nb, T, true);
-    if (r != Result::Success) return r;
-    CTX_NTC(ntc_channel_sse_f32(out_dim, B, yt.as<float>(), yp.as<float>(), sse, chan_ws.ptr, stream));
-    count += (double)B;
-    return Result::Success;
-  });
+    if (CopySync(eval_origins.ptr, all.data(), sizeof(int32_t) * all.size(), hipMemcpyHostToDevice) != hipSuccess)
+    {
+      err = Result::CudaFailure;
+      return EvalResults{};
+    }
+    for (const auto& b : batches)
+    {
+      if ((err = DecodeTiles(b[0], eval_origins.as<int32_t>() + b[1], b[2], b[3], prec, sse)) != Result::Success)
+        return EvalResults{};
+      count += (double)b[2] * b[3] * b[3];
+    }
+  }
+  else
+    err = WalkEvalTiles(*this, [&](int mip, const int32_t* origins, int nb, int T) -> Result {
+      const int64_t B = (int64_t)nb * T * T;
+      Result r = ForwardTiles(mip, origins, nb, T, true);
+      if (r != Result::Success) return r;
+      CTX_NTC(ntc_channel_sse_f32(out_dim, B, yt.as<float>(), yp.as<float>(), sse, chan_ws.ptr, stream));
+      count += (double)B;
+      return Result::Success;
+    });
   if (err != Result::Success) return EvalResults{};
   double host[16] = {};
   if (CopySync(host, sse, sizeof(double) * 16, hipMemcpyDeviceToHost) != hipSuccess)

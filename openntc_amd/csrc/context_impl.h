@@ -183,7 +183,9 @@ struct Context::Impl
   Result Finalize(Context& ctx);
   Result ForwardTiles(int32_t mip, const int32_t* origins, int32_t n_tiles, int32_t T, bool with_targets);
   EvalResults Evaluate(const Context& ctx, EvalPrecision p, EvalDetail* detail, Result& err);
-  Result DecodeTiles(int32_t mip, const int32_t* origins, int32_t n_tiles, int32_t T, int32_t precision, double* sse);
+  // one eval batch, its tile origins already in device memory (dev_origins)
+  Result DecodeTiles(int32_t mip, const int32_t* dev_origins, int32_t n_tiles, int32_t T, int32_t precision,
+                     double* sse);
   CalibrationData RunCalibration(float headroom, Result& err);
 };
 } // namespace openntc
