@@ -419,6 +419,16 @@ size_t ntc_decode_workspace_bytes(int64_t batch);
 int ntc_decode_sse_f32(int32_t precision, int32_t activation, int32_t feature_dim, int32_t feature_dim_padded,
                        int32_t hidden_dim, int32_t out_dim, int64_t batch, const void* table, const float* x,
                        const float* y_target, double* sse, float* y_out, void* workspace, hipStream_t stream);
+/* The same for one batch of tiles whose features and targets the decode
+ * kernel forms itself (the gather of ntc_draw_features_f32 with noise off and
+ * the targets of ntc_draw_targets_f32, bit for bit), so they never go through
+ * device memory: g0/g1 the level's grids, package the mip's targets
+ * [M][M][O]; workspace: ntc_decode_workspace_bytes(batch). */
+int ntc_decode_tiles_sse_f32(int32_t precision, int32_t activation, const ntc_feature_desc* fd,
+                             const ntc_tiles* tiles, const float* g0, const float* g1,
+                             const uint8_t* package, int32_t feature_dim_padded, int32_t hidden_dim,
+                             int32_t out_dim, const void* table, double* sse, void* workspace,
+                             hipStream_t stream);
 
 /* ---- mip targets (prepare_tex / filter_lanczos / package_tex,
  * ntc_kernel.cu:533-686) ---------------------------------------------------- */

@@ -202,6 +202,7 @@ SIGNATURES = {
     "ntc_build_decode_table_f": (_I, [_I, _I, _I, _I, _P, _F, _F, _P]),
     "ntc_build_decode_table_int8": (_I, [_I, _I, _I, _I, _P, _F, _F, _P]),
     "ntc_decode_workspace_bytes": (C.c_size_t, [C.c_int64]),
+    "ntc_decode_tiles_sse_f32": (_I, [_I, _I, _P, _P, _P, _P, _P, _I, _I, _I, _P, _P, _P, _P]),
     "ntc_decode_sse_f32": (_I, [_I, _I, _I, _I, _I, _I, C.c_int64, _P, _P, _P, _P, _P, _P, _P]),
     "ntc_decoder_update": (_I, [_I, _I, _I, _I, _P, _F, _F, _F, _F, _F, _P, _P, _P, _P, _P, _P, _P]),
     "ntc_fused_params_bytes": (C.c_size_t, []),

@@ -935,17 +935,13 @@ Result Context::Impl::ForwardTiles(int32_t mip, const int32_t* origins, int32_t 
 Result Context::Impl::DecodeTiles(int32_t mip, const int32_t* dev_origins, int32_t n_tiles, int32_t T,
                                   int32_t precision, double* sse)
 {
-  const int64_t B = (int64_t)n_tiles * T * T;
-  Result r = EnsurePlainBuffers(B);
-  if (r != Result::Success) return r;
   const int lvl = LevelForLod(mip);
   const ntc_feature_desc fd = FeatureDesc(lvl);
   ntc_tiles tl = {dev_origins, n_tiles, T, mip_dim[mip], (float)mip / (float)(mip_count - 1)};
-  ntc_noise nz = {};
-  CTX_NTC(ntc_draw_features_f32(&fd, &tl, &nz, g0[lvl].as<float>(), g1[lvl].as<float>(), x.as<float>(), stream));
-  CTX_NTC(ntc_draw_targets_f32(&tl, out_dim, package[mip].as<uint8_t>(), yt.as<float>(), stream));
-  CTX_NTC(ntc_decode_sse_f32(precision, ActCode(), feature_dim, feature_dim_padded, hidden_dim, out_dim, B,
-                             decode_table.ptr, x.as<float>(), yt.as<float>(), sse, nullptr, decode_ws.ptr, stream));
+  // features and targets formed inside the decode kernel
+  CTX_NTC(ntc_decode_tiles_sse_f32(precision, ActCode(), &fd, &tl, g0[lvl].as<float>(), g1[lvl].as<float>(),
+                                   package[mip].as<uint8_t>(), feature_dim_padded, hidden_dim, out_dim,
+                                   decode_table.ptr, sse, decode_ws.ptr, stream));
   return Result::Success;
 }
 
@@ -1077,14 +1073,11 @@ EvalResults Context::Impl::Evaluate(const Context& ctx, EvalPrecision p, EvalDet
     // with no host synchronisation between them
     std::vector<int32_t> all;
     std::vector<std::array<int32_t, 4>> batches;  // mip, first origin int, tiles, T
-    int64_t most = 0;
     WalkEvalTiles(*this, [&](int mip, const int32_t* origins, int nb, int T) -> Result {
       batches.push_back({mip, (int32_t)all.size(), nb, T});
       all.insert(all.end(), origins, origins + 2 * nb);
-      most = std::max<int64_t>(most, (int64_t)nb * T * T);
       return Result::Success;
     });
-    if ((err = EnsurePlainBuffers(most)) != Result::Success) return EvalResults{};
     if (eval_origins.bytes < sizeof(int32_t) * all.size() &&
         eval_origins.Alloc(sizeof(int32_t) * all.size()) != hipSuccess)
     {

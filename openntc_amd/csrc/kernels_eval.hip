@@ -97,21 +97,40 @@ struct DecodeArgs
   int act, F, Fp, O;
   int64_t B;
   const float* table;
-  const float* x;   // [F][B]
-  const float* yt;  // [O][B]
+  const float* x;   // [F][B] (GATHER: unused)
+  const float* yt;  // [O][B] (GATHER: unused)
   float* y;         // [O][B] or null
   double* part;     // [blocks][16]
+  // GATHER: the batch's features and targets formed here (noise off)
+  ntc_feature_desc fd;
+  ntc_tiles tl;
+  const float* g0;
+  const float* g1;
+  const uint8_t* pkg;  // the mip's targets [M][M][O]
 };
 
-template <int PREC, int H>
+// GATHER: each thread forms its texel's features (the gather of
+// k_draw_features, the same function) into its column of the dynamic LDS
+// array xs[F][256] and its targets from the package bytes (as
+// k_draw_targets), so the batch's features never go through HBM; else X and
+// the targets come from HBM.
+template <int PREC, int H, bool GATHER>
 __global__ __launch_bounds__(kThreads) void k_decode(DecodeArgs a)
 {
   __shared__ double red[kThreads];
+  extern __shared__ float xs[];
   const int tid = threadIdx.x;
   const int64_t t = (int64_t)blockIdx.x * kThreads + tid;
   const bool live = t < a.B;
   const int64_t tc = live ? t : 0;
   const float* __restrict__ hdr = a.table;
+  if (GATHER)
+  {
+    const ntc_noise nz = {};
+    gather_texel<float>(a.fd, a.tl, nz, a.g0, a.g1, tc, [&](int slot, float v) { xs[slot * kThreads + tid] = v; });
+  }
+  // feature k of this thread's texel
+  auto xin = [&](int k) { return GATHER ? xs[k * kThreads + tid] : a.x[(int64_t)k * a.B + tc]; };
   float yv[16];
   if (PREC == NTC_PREC_INT8)
   {
@@ -130,7 +149,7 @@ __global__ __launch_bounds__(kThreads) void k_decode(DecodeArgs a)
       for (int c = 0; c < 4; ++c)
       {
         const int k = 4 * k4 + c;
-        const int v = k < a.F ? clamp_s8(a.x[(int64_t)k * a.B + tc] * 128.0f) : 0;
+        const int v = k < a.F ? clamp_s8(xin(k) * 128.0f) : 0;
         xp |= ((uint32_t)v & 0xFFu) << (8 * c);
       }
 #pragma unroll
@@ -176,7 +195,7 @@ __global__ __launch_bounds__(kThreads) void k_decode(DecodeArgs a)
     for (int h = 0; h < H; ++h) z[h] = 0.0f;
     for (int k = 0; k < a.F; ++k)
     {
-      float xv = a.x[(int64_t)k * a.B + tc];
+      float xv = xin(k);
       xv = PREC == NTC_PREC_BF16 ? bf16_round(xv) : (PREC == NTC_PREC_FP8 ? fp8_q(xv, rx) : xv);
 #pragma unroll
       for (int h = 0; h < H; ++h) z[h] = fmaf(w0t[k * H + h], xv, z[h]);
@@ -205,6 +224,12 @@ __global__ __launch_bounds__(kThreads) void k_decode(DecodeArgs a)
       for (int o = 0; o < 16; ++o) yv[o] = yv[o] * sco;
     }
   }
+  int64_t tbase = 0;
+  if (GATHER)
+  {
+    const TexelPos p = texel_of(tc, a.tl.origins, a.tl.tile_dim, a.tl.mip_dim);
+    tbase = ((int64_t)p.y * a.tl.mip_dim + p.x) * a.O;
+  }
   // outputs and the per-channel squared error (fixed-order block sums)
   for (int o = 0; o < a.O; ++o)
   {
@@ -212,7 +237,8 @@ __global__ __launch_bounds__(kThreads) void k_decode(DecodeArgs a)
     if (live)
     {
       if (a.y) a.y[(int64_t)o * a.B + t] = yv[o];
-      const float r = a.yt[(int64_t)o * a.B + t] - yv[o];
+      const float target = GATHER ? (float)a.pkg[tbase + o] / 255.0f : a.yt[(int64_t)o * a.B + t];
+      const float r = target - yv[o];
       e = (double)r * (double)r;
     }
     red[tid] = e;
@@ -235,15 +261,15 @@ __global__ __launch_bounds__(256) void k_decode_sum(int blocks, const double* __
 
 typedef void (*DecodeKernel)(DecodeArgs);
 
-template <int H>
+template <int H, bool GATHER = false>
 DecodeKernel decode_kernel(int prec)
 {
   switch (prec)
   {
-  case NTC_PREC_FP32: return k_decode<NTC_PREC_FP32, H>;
-  case NTC_PREC_BF16: return k_decode<NTC_PREC_BF16, H>;
-  case NTC_PREC_FP8: return k_decode<NTC_PREC_FP8, H>;
-  case NTC_PREC_INT8: return k_decode<NTC_PREC_INT8, H>;
+  case NTC_PREC_FP32: return k_decode<NTC_PREC_FP32, H, GATHER>;
+  case NTC_PREC_BF16: return k_decode<NTC_PREC_BF16, H, GATHER>;
+  case NTC_PREC_FP8: return k_decode<NTC_PREC_FP8, H, GATHER>;
+  case NTC_PREC_INT8: return k_decode<NTC_PREC_INT8, H, GATHER>;
   default: return nullptr;
   }
 }
@@ -376,6 +402,39 @@ int ntc_decode_sse_f32(int32_t precision, int32_t activation, int32_t F, int32_t
   a.part = static_cast<double*>(workspace);
   const int blocks = (int)blocks_for(batch, kThreads);
   hipLaunchKernelGGL(k, dim3(blocks), dim3(kThreads), 0, stream, a);
+  hipLaunchKernelGGL(k_decode_sum, dim3(O), dim3(256), 0, stream, blocks, a.part, sse);
+  return NTC_LAUNCH_STATUS();
+}
+
+int ntc_decode_tiles_sse_f32(int32_t precision, int32_t activation, const ntc_feature_desc* fd,
+                             const ntc_tiles* tiles, const float* g0, const float* g1, const uint8_t* package,
+                             int32_t Fp, int32_t H, int32_t O, const void* table, double* sse, void* workspace,
+                             hipStream_t stream)
+{
+  NTC_RETURN_IF(!valid_feature_desc(fd) || !valid_tiles(tiles), NTC_E_INVALID);
+  NTC_RETURN_IF(!g0 || !g1 || !package || !table || !sse || !workspace, NTC_E_INVALID);
+  const int F = fd->feature_dim;
+  NTC_RETURN_IF(O < 1 || O > 16 || (H != 64 && H != 128), NTC_E_INVALID);
+  NTC_RETURN_IF(precision == NTC_PREC_INT8 && (Fp < F || Fp % 16), NTC_E_INVALID);
+  const DecodeKernel k = H == 64 ? decode_kernel<64, true>(precision) : decode_kernel<128, true>(precision);
+  NTC_RETURN_IF(!k, NTC_E_INVALID);
+  const int64_t batch = batch_of(*tiles);
+  DecodeArgs a = {};
+  a.act = activation;
+  a.F = F;
+  a.Fp = Fp;
+  a.O = O;
+  a.B = batch;
+  a.table = static_cast<const float*>(table);
+  a.part = static_cast<double*>(workspace);
+  a.fd = *fd;
+  a.tl = *tiles;
+  a.g0 = g0;
+  a.g1 = g1;
+  a.pkg = package;
+  const int blocks = (int)blocks_for(batch, kThreads);
+  const size_t lds = sizeof(float) * (size_t)F * kThreads;  // xs[F][256]
+  hipLaunchKernelGGL(k, dim3(blocks), dim3(kThreads), lds, stream, a);
   hipLaunchKernelGGL(k_decode_sum, dim3(O), dim3(256), 0, stream, blocks, a.part, sse);
   return NTC_LAUNCH_STATUS();
 }

@@ -242,4 +242,102 @@ __device__ __forceinline__ void adam_one(const AdamConst& c, float g, float& m, 
   v = fmaf(c.b2, v, (c.omb2 * g) * g);
   p = p - (c.lr * (m * c.bias1)) / (__builtin_sqrtf(v * c.bias2) + 1e-8f);
 }
+// ---- gather (per texel) ------------------------------------------------------
+// The features of texel t of a batch of tiles (ntc_kernel.cu:75-214):
+// put(slot, value) for every slot < fd.feature_dim. Noise words are drawn per
+// (corner, group of 4 channels); g1 (and g0 in bilinear mode) accumulate
+// sum_c w_c*(G+N) with FMA in corner order.
+template <typename T, class Put>
+__device__ __forceinline__ void gather_texel(const ntc_feature_desc& fd, const ntc_tiles& tl, const ntc_noise& nz,
+                                             const T* __restrict__ g0, const T* __restrict__ g1, int64_t t, Put&& put)
+{
+  const TexelPos p = texel_of(t, tl.origins, tl.tile_dim, tl.mip_dim);
+  const Footprint<T> f0 = footprint<T>(p.x, p.y, tl.mip_dim, fd.g0_dim);
+  const Footprint<T> f1 = footprint<T>(p.x, p.y, tl.mip_dim, fd.g1_dim);
+  const int c0 = fd.g0_channels, c1 = fd.g1_channels;
+
+  // g0
+  if (fd.g0_mode == NTC_G0_CORNERS)
+  {
+    for (int c = 0; c < 4; ++c)
+    {
+      for (int q = 0; q * 4 < c0; ++q)
+      {
+        uint4 w = make_uint4(0, 0, 0, 0);
+        if (nz.enabled) w = noise_words(nz.seed, nz.step, 0, nz.level, (uint32_t)f0.cell[c], q);
+        for (int ch = 4 * q; ch < c0 && ch < 4 * q + 4; ++ch)
+        {
+          T v = g0[(int64_t)f0.cell[c] * c0 + ch];
+          if (nz.enabled) v = v + (T)noise_value(pick(w, ch & 3), nz.delta0);
+          put(c * c0 + ch, v);
+        }
+      }
+    }
+  }
+  else
+  {
+    for (int q = 0; q * 4 < c0; ++q)
+    {
+      uint4 w[4];
+#pragma unroll
+      for (int c = 0; c < 4; ++c)
+        w[c] = nz.enabled ? noise_words(nz.seed, nz.step, 0, nz.level, (uint32_t)f0.cell[c], q)
+                          : make_uint4(0, 0, 0, 0);
+      for (int ch = 4 * q; ch < c0 && ch < 4 * q + 4; ++ch)
+      {
+        T acc = (T)0;
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+        {
+          T v = g0[(int64_t)f0.cell[c] * c0 + ch];
+          if (nz.enabled) v = v + (T)noise_value(pick(w[c], ch & 3), nz.delta0);
+          acc = fma(f0.w[c], v, acc);
+        }
+        put(ch, acc);
+      }
+    }
+  }
+  int slot = fd.g0_mode == NTC_G0_CORNERS ? 4 * c0 : c0;
+
+  // g1, always bilinear
+  for (int q = 0; q * 4 < c1; ++q)
+  {
+    uint4 w[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+      w[c] = nz.enabled ? noise_words(nz.seed, nz.step, 1, nz.level, (uint32_t)f1.cell[c], q)
+                        : make_uint4(0, 0, 0, 0);
+    for (int ch = 4 * q; ch < c1 && ch < 4 * q + 4; ++ch)
+    {
+      T acc = (T)0;
+#pragma unroll
+      for (int c = 0; c < 4; ++c)
+      {
+        T v = g1[(int64_t)f1.cell[c] * c1 + ch];
+        if (nz.enabled) v = v + (T)noise_value(pick(w[c], ch & 3), nz.delta1);
+        acc = fma(f1.w[c], v, acc);
+      }
+      put(slot + ch, acc);
+    }
+  }
+  slot += c1;
+
+  for (int pi = 0; pi < fd.pe_count; ++pi)
+  {
+    const int P = fd.pe_first_period >> pi;
+    for (int axis = 0; axis < 2; ++axis)
+      for (int h = 0; h < 2; ++h)
+      {
+        put(slot, tri_wave<T>(axis == 0 ? p.x : p.y, P, h));
+        ++slot;
+      }
+  }
+  if (fd.lod_slot)
+  {
+    put(slot, (T)tl.norm_lod);
+    ++slot;
+  }
+  for (; slot < fd.feature_dim; ++slot) put(slot, (T)0);
+}
+
 } // namespace ntc

@@ -187,3 +187,55 @@ def test_eval_4096_precisions_vs_fp64_oracle(oracle):
         for g in grp:
             assert abs(grp[g] - ref[g]) < tol[p], (name, g, grp[g], ref[g])
         assert abs(psnr - ref_all) < tol[p]
+
+
+@pytest.mark.parametrize("kind", ["c2_bilinear", "c3_h128", "ref_corners_hardgelu"])
+@pytest.mark.parametrize("precision", [0, 1, 2, 3])
+def test_decode_tiles_matches_gather_then_decode(oracle, kind, precision):
+    """The decode kernel that forms each batch's features and targets itself
+    (ntc_decode_tiles_sse_f32, as Context's evaluation walk runs it) against
+    the separate gather, targets and decode kernels: per-channel squared
+    errors equal bit for bit, for three tiles of 256^2 on a 512^2 mip (one
+    wrapping both edges) and one 8^2 tile of an 8^2 mip."""
+    O = 9
+    args = {"c2_bilinear": (4, 2, 12, 16, 2, 64, 1, 1), "c3_h128": (4, 4, 16, 16, 2, 128, 1, 1),
+            "ref_corners_hardgelu": (4, 4, 16, 12, 2, 64, 0, 0)}[kind]
+    Lay = oracle.layout_custom(*args, 512)
+    F, Fp, H, act = Lay.feature_dim, Lay.feature_dim_padded, Lay.hidden_dim, args[7]
+    lib = G.lib()
+    rng = np.random.default_rng(7 + precision)
+    w0, w1, wo, dec = _decoder(rng, F, H, O)
+    table = np.zeros(lib.ntc_decode_table_bytes(precision, F, Fp, H), np.uint8)
+    if precision == 3:
+        s_a1, s_a2 = np.float32(1.1 * 1.2 / 127), np.float32(1.1 * 0.8 / 127)
+        blob = _int8_blob(oracle, w0, w1, wo[:O], Fp, s_a1, s_a2)
+        G.ok(lib.ntc_build_decode_table_int8(F, Fp, H, O, blob.ctypes.data, float(np.float32(1) / s_a1),
+                                             float(np.float32(1) / s_a2), table.ctypes.data))
+    else:
+        G.ok(lib.ntc_build_decode_table_f(precision, F, H, O, dec.ctypes.data, 1.7, 2.3, table.ctypes.data))
+    d_table = G.dev(table)
+    for M, T, origins in ((512, 256, [512 - 100, 512 - 37, 0, 0, 200, 311]), (8, 8, [3, 5])):
+        lvl = oracle.level_for_lod(Lay, {512: 0, 8: 6}[M])
+        fd = oracle.feature_desc(Lay, lvl)
+        fdc = G.feature_desc(fd)
+        g0 = G.dev(rng.uniform(-1, 1, fd["g0_dim"] ** 2 * fd["c0"]).astype(np.float32))
+        g1 = G.dev(rng.uniform(-1, 1, fd["g1_dim"] ** 2 * fd["c1"]).astype(np.float32))
+        pkg = G.dev(rng.integers(0, 256, M * M * O).astype(np.uint8))
+        og = G.dev(np.array(origins, np.int32))
+        n = len(origins) // 2
+        tl = G.tiles(og, n, T, M, 0.25)
+        B = n * T * T
+        x = G.empty((F, B), np.float32)
+        yt = G.empty((O, B), np.float32)
+        nz = L.Noise(0, 0, 0, 0, 0.0, 0.0)
+        G.ok(lib.ntc_draw_features_f32(G.byref(fdc), G.byref(tl), G.byref(nz), G.p(g0), G.p(g1), G.p(x), None))
+        G.ok(lib.ntc_draw_targets_f32(G.byref(tl), O, G.p(pkg), G.p(yt), None))
+        ws = G.empty((lib.ntc_decode_workspace_bytes(B),), np.uint8)
+        sse_a = G.dev(np.zeros(16, np.float64))
+        sse_b = G.dev(np.zeros(16, np.float64))
+        G.ok(lib.ntc_decode_sse_f32(precision, act, F, Fp, H, O, B, G.p(d_table), G.p(x), G.p(yt), G.p(sse_a),
+                                    None, G.p(ws), None), "decode")
+        G.ok(lib.ntc_decode_tiles_sse_f32(precision, act, G.byref(fdc), G.byref(tl), G.p(g0), G.p(g1), G.p(pkg),
+                                          Fp, H, O, G.p(d_table), G.p(sse_b), G.p(ws), None), "decode tiles")
+        a, b = G.host(sse_a), G.host(sse_b)
+        assert a[:O].min() > 0 and np.array_equal(a, b), (kind, precision, M, a[:O], b[:O])
