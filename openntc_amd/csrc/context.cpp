@@ -1472,8 +1472,10 @@ TrainProgress Context::Train(int32_t num_batches)
     const size_t need = (size_t)n * Impl::kStepEvents;
     while (I.events.size() < need)
     {
+      // timing only: no system-scope fence (its L2 write-back between the
+      // step's kernels would be timed with them and slow the next kernel)
       hipEvent_t e;
-      if (hipEventCreate(&e) != hipSuccess) return error(Result::CudaFailure);
+      if (hipEventCreateWithFlags(&e, hipEventDisableSystemFence) != hipSuccess) return error(Result::CudaFailure);
       I.events.push_back(e);
     }
   }

@@ -55,33 +55,22 @@ __host__ __device__ inline float bf16_round(float x)
   return f_from(u & 0xFFFF0000u);
 }
 
-// OCP FP8 E4M3 (bias 7, largest finite 448 = 0x7E, 0x7F NaN, no infinities):
-// round to nearest even, saturating. Normal range: round the float's
-// mantissa to 3 bits on its bits (a carry moves into the exponent); below
-// 2^-6: multiples of 2^-9.
-__host__ __device__ inline uint32_t e4m3_encode(float x)
+// OCP FP8 E4M3 (bias 7, largest finite 448, 0x7F NaN, no infinities) quantise-dequantise of
+// v * r: round to nearest even, saturating (NaN saturates to 448 with its sign). Normal range:
+// the float's mantissa rounded to 3 bits on its bits (a carry moves into the exponent; the clamp
+// to 448 keeps the result finite); below 2^-6: multiples of 2^-9. Formed on the float directly,
+// without an E4M3 code in between: equal, on every float bit pattern, to encoding the value to
+// its E4M3 byte and decoding it again.
+__host__ __device__ inline float fp8_q(float v, float r)
 {
-  const uint32_t u = f_bits(x);
-  const uint32_t sign = (u >> 24) & 0x80u;
+  const uint32_t u = f_bits(v * r);
   float a = f_from(u & 0x7FFFFFFFu);
-  if (!(a < 448.0f)) a = 448.0f;
-  if (a < 0x1p-6f) return sign | (uint32_t)rintf(a * 512.0f);
-  uint32_t v = f_bits(a);
-  v += 0x7FFFFu + ((v >> 20) & 1u);
-  const int e = (int)((v >> 23) & 0xFFu) - 127;
-  const uint32_t m = (v >> 20) & 7u;
-  if (e > 8 || (e == 8 && m == 7u)) return sign | 0x7Eu;
-  return sign | ((uint32_t)(e + 7) << 3) | m;
+  a = a < 448.0f ? a : 448.0f;
+  uint32_t n = f_bits(a);
+  n = (n + 0x7FFFFu + ((n >> 20) & 1u)) & 0xFFF00000u;
+  const float q = a < 0x1p-6f ? rintf(a * 512.0f) * 0x1p-9f : f_from(n);
+  return f_from(f_bits(q) | (u & 0x80000000u));
 }
-
-__host__ __device__ inline float e4m3_decode(uint32_t c)
-{
-  const int e = (int)((c >> 3) & 15u), m = (int)(c & 7u);
-  const float v = e == 0 ? (float)m * 0x1p-9f : ldexpf((float)(8 + m), e - 10);
-  return (c & 0x80u) ? -v : v;
-}
-
-__host__ __device__ inline float fp8_q(float v, float r) { return e4m3_decode(e4m3_encode(v * r)); }
 
 __host__ __device__ inline int clamp_s8(float f)
 {
