@@ -411,6 +411,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--path", choices=["fused", "plain"], default="fused")
     ap.add_argument("--cpu-steps", type=int, default=40)
+    ap.add_argument("--detail-steps", type=int, default=100,
+                    help="steps after the timed region over which the step's other kernels are timed")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--time-to-psnr", type=int, default=0, metavar="CHUNK",
                     help="only measure the seconds for the fused path to reach the plain path's final PSNR, "
@@ -445,6 +447,10 @@ def main():
         sets_bench(ntc, a.warmup, a.steps)
         return
     path = ntc.TrainPath.Fused if a.path == "fused" else ntc.TrainPath.Plain
+    if a.warmup + a.steps + (a.detail_steps if a.path == "fused" else 0) > TOTAL_STEPS:
+        raise SystemExit(f"warmup + steps + detail steps exceed the {TOTAL_STEPS}-step schedule")
+    if a.path == "fused" and a.detail_steps < 1:
+        raise SystemExit("--detail-steps must be at least 1")
     log(f"setting up {DIM}^2 synthetic set, path={a.path}")
     t_setup = time.perf_counter()
     ctx, _ = make_context(path)
@@ -455,7 +461,10 @@ def main():
     if prog.result != ntc.Result.Success:
         raise SystemExit(f"warmup failed: {prog}")
     log(f"warmup {a.warmup} steps done")
-    lib.ntc_ctx_set_timing(ctx._h, 1)
+    # events around the fused kernel only inside the timed region: an event
+    # between two of the step's kernels delays the second by a few
+    # microseconds; the other kernels are timed over a sample after it
+    lib.ntc_ctx_set_timing(ctx._h, 2)
     lib.ntc_dev_synchronize()
     t0 = time.perf_counter()
     prog = ctx.Train(a.steps)
@@ -494,7 +503,18 @@ def main():
     if a.path == "fused":
         times = (ctypes.c_float * 9)()
         lib.ntc_ctx_kernel_times(ctx._h, times, 9)
-        fused_ms, dec_ms, grid_ms, n_t, n_grid, n_tiles_upd, catch_ms, hot_ms, n_lazy = list(times)
+        fused_ms = float(times[0])
+        # the step's other kernels: an event around each, over the next
+        # `detail_steps` schedule steps (after the timed region)
+        lib.ntc_ctx_set_timing(ctx._h, 1)
+        prog = ctx.Train(a.detail_steps)
+        lib.ntc_ctx_flush_grids(ctx._h)
+        lib.ntc_dev_synchronize()
+        if prog.result != ntc.Result.Success:
+            raise SystemExit(f"kernel-time sample failed: {prog}")
+        lib.ntc_ctx_kernel_times(ctx._h, times, 9)
+        _, dec_ms, grid_ms, n_t, n_grid, n_tiles_upd, catch_ms, hot_ms, n_lazy = list(times)
+        _, dinfos = step_texels(ctx, a.warmup + a.steps, a.detail_steps)
         tiles_upd = n_grid > 0 and n_tiles_upd == n_grid
         lazy = n_grid > 0 and n_lazy == n_grid
         if n_grid > 0 and 0 < n_tiles_upd < n_grid:
@@ -512,30 +532,41 @@ def main():
         #   and writes p, m, v of the footprint cells (24 B/param) and the hot
         #   part 32 B/param there; the deferred steps of the other cells cost
         #   their catch-up later, when a footprint or a flush reaches them
+        #   (fused figures over the timed steps, grid figures over the sample)
         n_dec = H * F + H * H + 16 * H
-        flops, fbytes, gbytes, cbytes, hbytes, blks, dense = [], [], [], [], [], [], []
-        for si in infos:
-            B = si.n_tiles * si.tile_dim ** 2
+
+        def cells(si):
             M = DIM >> si.lod
             D0, D1 = lay.g0_dim[si.level], lay.g1_dim[si.level]
             pl = D0 * D0 * lay.g0_channels + D1 * D1 * lay.g1_channels
             pf = footprint_cells(si, M, D0) * lay.g0_channels + footprint_cells(si, M, D1) * lay.g1_channels
+            return pl, pf
+
+        def grid_bytes(si, pl, pf):
+            if lazy:
+                return 24.0 * pf + 32.0 * pf, 24.0 * (pl - pf) + 32.0 * pf
+            if tiles_upd:
+                return 24.0 * (pl - pf) + 32.0 * pf, 24.0 * (pl - pf) + 32.0 * pf
+            return 32.0 * pl, 32.0 * pl
+
+        flops, fbytes, gbytes, cbytes, hbytes, blks, dense = [], [], [], [], [], [], []
+        for si in infos:
+            B = si.n_tiles * si.tile_dim ** 2
+            pl, pf = cells(si)
             blocks = min(256, (B + 63) // 64)
-            fb = 4.0 * pf + O * B + 4.0 * blocks * n_dec
             flops.append(6.0 * (F * H + H * H + H * O) * B)
             blks.append(blocks)
-            if si.grid_update and lazy:
-                gbytes.append(24.0 * pf + 32.0 * pf)
+            fbytes.append(4.0 * pf + O * B + 4.0 * blocks * n_dec)
+            if si.grid_update:
+                dense.append(grid_bytes(si, pl, pf)[1])
+        for si in dinfos:
+            if not si.grid_update:
+                continue
+            pl, pf = cells(si)
+            gbytes.append(grid_bytes(si, pl, pf)[0])
+            if lazy:
                 cbytes.append(24.0 * pf)
                 hbytes.append(32.0 * pf)
-                dense.append(24.0 * (pl - pf) + 32.0 * pf)
-            elif si.grid_update and tiles_upd:
-                gbytes.append(24.0 * (pl - pf) + 32.0 * pf)
-                dense.append(gbytes[-1])
-            elif si.grid_update:
-                gbytes.append(32.0 * pl)
-                dense.append(gbytes[-1])
-            fbytes.append(fb)
         fused_tflops = np.mean(flops) / (fused_ms * 1e-3) / 1e12
         fused_gbs = np.mean(fbytes) / (fused_ms * 1e-3) / 1e9
         kern = {"fused_train_step": {"ms": fused_ms, "tflops": fused_tflops, "mfma_frac": fused_tflops / BF16_PEAK_TFLOPS,
@@ -543,7 +574,9 @@ def main():
                                      "flop_per_launch": float(np.mean(flops)),
                                      "bytes_per_launch": float(np.mean(fbytes)),
                                      "workgroups": float(np.mean(blks))},
-                "decoder_update": {"ms": dec_ms}}
+                "decoder_update": {"ms": dec_ms},
+                "sample": f"fused kernel: the {a.steps} timed steps; decoder update and grid kernels: the "
+                          f"{a.detail_steps} steps after them, an event around each kernel"}
         if gbytes and lazy:
             c_gbs = float(np.mean(cbytes)) / (catch_ms * 1e-3) / 1e9
             hot_gbs = float(np.mean(hbytes)) / (hot_ms * 1e-3) / 1e9

@@ -315,6 +315,18 @@ int ntc_decoder_update(int32_t hidden_dim, int32_t feature_dim,
                        float* m, float* v, float* p,
                        const double* sse_partials, double* sse_out,
                        hipStream_t stream);
+/* ntc_decoder_update that also brings the BF16 tables of
+ * ntc_pack_decoder_bf16 (packed from p before, same fd / hidden_dim / out_dim)
+ * up to date with the updated parameters: the table refresh of the fused step
+ * without a separate pass over the tables. Bit-identical to
+ * ntc_decoder_update followed by ntc_pack_decoder_bf16. */
+int ntc_decoder_update_bf16(const ntc_feature_desc* fd, int32_t hidden_dim,
+                            int32_t out_dim, int32_t blocks,
+                            const float* dw_partials, float lr, float beta1,
+                            float beta2, float bias1, float bias2, float* grad,
+                            float* m, float* v, float* p, uint16_t* table,
+                            const double* sse_partials, double* sse_out,
+                            hipStream_t stream);
 
 /* ---- several independent training sets in one launch per stage -------------
  * No counterpart in the reference (one Context trains one texture set,
@@ -563,6 +575,9 @@ void* ntc_ctx_stream(ntc_context* ctx);
  * outside the tile footprints, cold part, hot part (ms; the split update),
  * steps with the split update}; -1 when not recorded. */
 int ntc_ctx_kernel_times(ntc_context* ctx, float* out, int32_t n);
+/* enabled: 0 off; 1 events around every kernel of each fused step; 2 around
+ * the fused kernel only (fewer events between the step's kernels; the other
+ * figures of ntc_ctx_kernel_times are then -1). Other values: InvalidState. */
 int ntc_ctx_set_timing(ntc_context* ctx, int32_t enabled);
 
 /* ---- several texture sets trained together --------------------------------

@@ -205,6 +205,7 @@ SIGNATURES = {
     "ntc_decode_tiles_sse_f32": (_I, [_I, _I, _P, _P, _P, _P, _P, _I, _I, _I, _P, _P, _P, _P]),
     "ntc_decode_sse_f32": (_I, [_I, _I, _I, _I, _I, _I, C.c_int64, _P, _P, _P, _P, _P, _P, _P]),
     "ntc_decoder_update": (_I, [_I, _I, _I, _I, _P, _F, _F, _F, _F, _F, _P, _P, _P, _P, _P, _P, _P]),
+    "ntc_decoder_update_bf16": (_I, [_P, _I, _I, _I, _P, _F, _F, _F, _F, _F, _P, _P, _P, _P, _P, _P, _P, _P]),
     "ntc_fused_params_bytes": (C.c_size_t, []),
     "ntc_fused_params_pack": (_I, [_P, _P]),
     "ntc_fused_pass_sizes": (_I, [_P, _I, _I, _P]),

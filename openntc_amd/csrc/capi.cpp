@@ -415,7 +415,8 @@ int ntc_ctx_flush_grids(ntc_context* c)
 int ntc_ctx_set_timing(ntc_context* c, int32_t enabled)
 {
   if (!c) return (int)Result::InvalidState;
-  c->ctx.GetImpl()->timing = enabled != 0;
+  if (enabled < 0 || enabled > 2) return (int)Result::InvalidState;
+  c->ctx.GetImpl()->timing = enabled;
   return 0;
 }
 } // extern "C"

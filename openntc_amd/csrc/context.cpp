@@ -346,8 +346,12 @@ Result Context::Impl::InitDevice()
   if (hipGetDeviceCount(&count) != hipSuccess || count < 1) return Result::CudaFailure;
   CTX_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), Result::CudaFailure);
   CTX_HIP(hipStreamCreateWithFlags(&side, hipStreamNonBlocking), Result::CudaFailure);
-  CTX_HIP(hipEventCreateWithFlags(&ready_ev, hipEventDisableTiming), Result::CudaFailure);
-  CTX_HIP(hipEventCreateWithFlags(&catch_ev, hipEventDisableTiming), Result::CudaFailure);
+  // ordering between the two streams of this device only: no system-scope
+  // fence (device memory stays visible within the device)
+  CTX_HIP(hipEventCreateWithFlags(&ready_ev, hipEventDisableTiming | hipEventDisableSystemFence),
+          Result::CudaFailure);
+  CTX_HIP(hipEventCreateWithFlags(&catch_ev, hipEventDisableTiming | hipEventDisableSystemFence),
+          Result::CudaFailure);
   const size_t dbytes = sizeof(float) * (size_t)DecoderElems();
   CTX_HIP(dec.Alloc(dbytes), Result::AllocationFailure);
   CTX_HIP(dec_m.Alloc(dbytes), Result::AllocationFailure);
@@ -789,6 +793,8 @@ Result Context::Impl::RunFusedStep(const StepPlan& st, int32_t step_index, hipEv
   }
   const ntc_feature_desc fd = FeatureDesc(lvl);
   const int act = ActCode();
+  // ev[0], ev[1]: the fused kernel; the others only with every event recorded
+  hipEvent_t* evd = timing == 1 ? ev : nullptr;
   ntc_fused_args a = {};
   a.feat = fd;
   a.tiles = {origins_table.as<int32_t>() + (int64_t)step_index * 16, st.n_tiles, T, M,
@@ -820,11 +826,11 @@ Result Context::Impl::RunFusedStep(const StepPlan& st, int32_t step_index, hipEv
     CTX_HIP(hipStreamWaitEvent(stream, catch_ev, 0), Result::CudaFailure);  // issued during the last step
   else if (deferred)
   {
-    if (ev) CTX_HIP(hipEventRecord(ev[3], stream), Result::CudaFailure);
+    if (evd) CTX_HIP(hipEventRecord(evd[3], stream), Result::CudaFailure);
     CTX_NTC(ntc_grid_lazy_catchup_f32(&fd, &a.tiles, nullptr, g0[lvl].as<float>(), g1[lvl].as<float>(), &adam,
                                       lazy_steps[lvl].as<float>(), lazy_done[lvl], lazy_last[lvl][0].as<int32_t>(),
                                       lazy_last[lvl][1].as<int32_t>(), stream));
-    if (ev) CTX_HIP(hipEventRecord(ev[4], stream), Result::CudaFailure);
+    if (evd) CTX_HIP(hipEventRecord(evd[4], stream), Result::CudaFailure);
   }
   caught_step = -1;
   if (ev) CTX_HIP(hipEventRecord(ev[0], stream), Result::CudaFailure);
@@ -850,7 +856,7 @@ Result Context::Impl::RunFusedStep(const StepPlan& st, int32_t step_index, hipEv
     const int32_t upto = lazy_done[ln] + (ln == lvl ? 1 : 0);
     CTX_HIP(hipEventRecord(ready_ev, stream), Result::CudaFailure);
     CTX_HIP(hipStreamWaitEvent(side, ready_ev, 0), Result::CudaFailure);
-    hipEvent_t* evn = ev ? ev + kStepEvents : nullptr;
+    hipEvent_t* evn = evd ? evd + kStepEvents : nullptr;
     if (evn) CTX_HIP(hipEventRecord(evn[3], side), Result::CudaFailure);
     CTX_NTC(ntc_grid_lazy_catchup_f32(&fdn, &tn, ln == lvl ? &a.tiles : nullptr, g0[ln].as<float>(),
                                       g1[ln].as<float>(), &adn, lazy_steps[ln].as<float>(), upto,
@@ -859,28 +865,30 @@ Result Context::Impl::RunFusedStep(const StepPlan& st, int32_t step_index, hipEv
     CTX_HIP(hipEventRecord(catch_ev, side), Result::CudaFailure);
     caught_step = nx;
   }
-  CTX_NTC(ntc_decoder_update(H, feature_dim, O, a.blocks, dw_part.as<float>(), st.lr_decoder, 0.9f, 0.999f, st.bias1,
-                             st.bias2, dec_grad.as<float>(), dec_m.as<float>(), dec_v.as<float>(), dec.as<float>(),
-                             sse_part.as<double>(), scalars.as<double>() + 16, stream));
-  Result r = PackDecoderTables();
-  if (r != Result::Success) return r;
-  if (ev) CTX_HIP(hipEventRecord(ev[2], stream), Result::CudaFailure);
+  // the decoder update refreshes the BF16 tables it changes (packed whole at
+  // the start of training)
+  const ntc_feature_desc fd0 = FeatureDesc(0);
+  CTX_NTC(ntc_decoder_update_bf16(&fd0, H, O, a.blocks, dw_part.as<float>(), st.lr_decoder, 0.9f, 0.999f, st.bias1,
+                                  st.bias2, dec_grad.as<float>(), dec_m.as<float>(), dec_v.as<float>(),
+                                  dec.as<float>(), wtab.as<uint16_t>(), sse_part.as<double>(),
+                                  scalars.as<double>() + 16, stream));
+  if (evd) CTX_HIP(hipEventRecord(evd[2], stream), Result::CudaFailure);
   if (deferred)
   {
     CTX_NTC(ntc_grid_lazy_hot_f32(&fd, &a.tiles, g0[lvl].as<float>(), dg0[lvl].as<float>(), g1[lvl].as<float>(),
                                   dg1[lvl].as<float>(), &adam, lazy_done[lvl], lazy_last[lvl][0].as<int32_t>(),
                                   lazy_last[lvl][1].as<int32_t>(), stream));
-    if (ev)
+    if (evd)
     {
-      CTX_HIP(hipEventRecord(ev[5], stream), Result::CudaFailure);
-      ev_lazy[(size_t)(ev - events.data()) / kStepEvents] = 1;
+      CTX_HIP(hipEventRecord(evd[5], stream), Result::CudaFailure);
+      ev_lazy[(size_t)(evd - events.data()) / kStepEvents] = 1;
     }
     lazy_done[lvl] += 1;
     lazy_behind[lvl] = true;
   }
   else if (st.grid_update)
   {
-    if (ev) CTX_HIP(hipEventRecord(ev[3], stream), Result::CudaFailure);
+    if (evd) CTX_HIP(hipEventRecord(evd[3], stream), Result::CudaFailure);
     if (tiles_update)
       CTX_NTC(ntc_grid_update_tiles_f32(&fd, &a.tiles, g0[lvl].as<float>(), dg0[lvl].as<float>(),
                                         g1[lvl].as<float>(), dg1[lvl].as<float>(), &adam, stream));
@@ -890,7 +898,7 @@ Result Context::Impl::RunFusedStep(const StepPlan& st, int32_t step_index, hipEv
                                    GridElems(1, lvl), dg1[lvl].as<float>(), g1m[lvl].as<float>(),
                                    g1v[lvl].as<float>(), g1[lvl].as<float>(), -1.0f, 1.0f - g1_delta, st.lr_grid,
                                    0.9f, 0.999f, st.grid_bias1, st.grid_bias2, stream));
-    if (ev) CTX_HIP(hipEventRecord(ev[4], stream), Result::CudaFailure);
+    if (evd) CTX_HIP(hipEventRecord(evd[4], stream), Result::CudaFailure);
   }
   grid_batch_i[lvl] += 1;
   return Result::Success;
@@ -1503,6 +1511,7 @@ TrainProgress Context::Train(int32_t num_batches)
       hipEvent_t* ev = &I.events[(size_t)s * Impl::kStepEvents];
       float ms = 0, ms2 = 0;
       if (hipEventElapsedTime(&ms, ev[0], ev[1]) == hipSuccess) fused += ms;
+      if (I.timing != 1) continue;
       if (hipEventElapsedTime(&ms, ev[1], ev[2]) == hipSuccess) decoder += ms;
       if (!I.plan[first + s].grid_update) continue;
       if (I.ev_lazy[(size_t)s])
@@ -1524,7 +1533,8 @@ TrainProgress Context::Train(int32_t num_batches)
         ++grids;
       }
     }
-    I.kernel_ms = {(float)(fused / n), (float)(decoder / n), grids ? (float)(grid / grids) : -1.0f, (float)n,
+    I.kernel_ms = {(float)(fused / n), I.timing == 1 ? (float)(decoder / n) : -1.0f,
+                   grids ? (float)(grid / grids) : -1.0f, (float)n,
                    (float)grids, (float)I.tiles_update_steps, deferred ? (float)(catchup / deferred) : -1.0f,
                    deferred ? (float)(hot / deferred) : -1.0f, (float)deferred};
   }

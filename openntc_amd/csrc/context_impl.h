@@ -135,8 +135,10 @@ struct Context::Impl
   // grid-update launches timed, grid updates that skipped the zero gradient
   // outside the tile footprints, catch-up, hot part, steps with the deferred
   // update} over the last Train call (durations averaged; with the deferred
-  // update the grid figure is catch-up + hot).
-  bool timing = false;
+  // update the grid figure is catch-up + hot). Level 2 records only the fused
+  // kernel's pair (each event recorded between the step's kernels delays the
+  // next one by a few microseconds); the other figures are then -1.
+  int32_t timing = 0;
   int32_t tiles_update_steps = 0;
   std::vector<float> kernel_ms;
   std::vector<hipEvent_t> events;

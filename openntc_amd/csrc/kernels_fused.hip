@@ -41,9 +41,9 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int kThreads = 256;
 
-__device__ __forceinline__ uint16_t f2bf(float f)
+__host__ __device__ __forceinline__ uint16_t f2bf(float f)
 {
-  uint32_t u = __float_as_uint(f);
+  uint32_t u = __builtin_bit_cast(uint32_t, f);
   u += 0x7FFFu + ((u >> 16) & 1u);  // round to nearest even
   return (uint16_t)(u >> 16);
 }
@@ -54,13 +54,48 @@ __device__ __forceinline__ bf16x8 frag_lds(const uint16_t* p)
   return __builtin_bit_cast(bf16x8, v);
 }
 
+// body(i, fragment at addr(i)) for i = 0 .. N-1, each fragment read from LDS
+// RING uses ahead: with one wave per SIMD nothing else hides a read's latency,
+// and the compiler left alone reads each fragment just before its MFMA.
+// The layers walk their K chunks outermost (i = j * MT + r): consecutive MFMAs
+// write different accumulators, and each accumulator still sums its K chunks
+// in order.
+template <int N, int RING, class Addr, class Body>
+__device__ __forceinline__ void lds_ring(Addr addr, Body body)
+{
+  constexpr int R = RING < N ? RING : N;
+  bf16x8 ring[R];
+#pragma unroll
+  for (int q = 0; q < R; ++q) ring[q] = frag_lds(addr(q));
+#pragma unroll
+  for (int i = 0; i < N; ++i)
+  {
+    const bf16x8 a = ring[i % R];
+    if (i + R < N) ring[i % R] = frag_lds(addr(i + R));
+    body(i, a);
+  }
+}
+
+typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
+typedef float f32x2_t __attribute__((ext_vector_type(2)));
+
+// lo, hi to BF16 (round to nearest even), packed lo | hi << 16: one
+// v_cvt_pk_bf16_f32, the bits f2bf gives for every finite value.
+__device__ __forceinline__ uint32_t pk_bf16(float lo, float hi)
+{
+  const f32x2_t v = {lo, hi};
+  return __builtin_bit_cast(uint32_t, __builtin_convertvector(v, bf16x2_t));
+}
+
+__device__ __forceinline__ uint16_t bf16_dev(float f) { return (uint16_t)pk_bf16(f, 0.0f); }
+
 __device__ __forceinline__ bf16x8 frag_pack(const float* v)
 {
   uint4 u;
-  u.x = (uint32_t)f2bf(v[0]) | ((uint32_t)f2bf(v[1]) << 16);
-  u.y = (uint32_t)f2bf(v[2]) | ((uint32_t)f2bf(v[3]) << 16);
-  u.z = (uint32_t)f2bf(v[4]) | ((uint32_t)f2bf(v[5]) << 16);
-  u.w = (uint32_t)f2bf(v[6]) | ((uint32_t)f2bf(v[7]) << 16);
+  u.x = pk_bf16(v[0], v[1]);
+  u.y = pk_bf16(v[2], v[3]);
+  u.z = pk_bf16(v[4], v[5]);
+  u.w = pk_bf16(v[6], v[7]);
   return __builtin_bit_cast(bf16x8, u);
 }
 
@@ -101,6 +136,13 @@ __host__ __device__ __forceinline__ int kperm_pos(int pos)
 {
   const int j = pos >> 5, r = pos & 31;
   return kperm(j, r >> 3, r & 7);
+}
+
+// Inverse of kperm_pos: the permuted column that holds index k.
+__host__ __device__ __forceinline__ int kperm_col(int k)
+{
+  const int r = k & 31, q = r & 15;
+  return (k & ~31) + 8 * (q >> 2) + (q & 3) + 4 * (r >> 4);
 }
 
 // Weight table layout (elements, BF16), identical in HBM and LDS.
@@ -584,7 +626,7 @@ __device__ __forceinline__ void gather_staged(const FusedParams& P, const PassGe
     const int qi = item / CF::PASS, t = item % CF::PASS;
     const int x = t & 15, y = t >> 4;
     auto put = [&](int slot, float v) {
-      const uint16_t b = f2bf(v);
+      const uint16_t b = bf16_dev(v);
       xnf[t * CF::XS + slot] = b;
       xfn[slot * CF::PASS + t] = b;
     };
@@ -649,7 +691,7 @@ __device__ __forceinline__ void gather_part(const FusedParams& P, int part, int 
 {
   constexpr int FP = CF::KF * 32;
   auto put = [&](int slot, float v) {
-    const uint16_t b = f2bf(v);
+    const uint16_t b = bf16_dev(v);
     xnf[tl * CF::XS + slot] = b;
     xfn[slot * CF::PASS + tl] = b;
   };
@@ -761,6 +803,7 @@ __device__ __forceinline__ void fused_body(const FusedParams& P, const int vblk,
   using CF = Cfg<H, FP, G, C0, C1, G0MODE, PEC, LODS, ACT, NT>;
   constexpr int kNT = NT, kPass = CF::PASS;
   constexpr int MT = CF::MT, KF = CF::KF, KH = CF::KH, GT = CF::GT, MTW = CF::MTW, NTO = CF::NTO;
+  constexpr int kRing = 4;  // weight fragments read ahead of their MFMA
   constexpr TableLayout TLY = {H, FP, G};
   __shared__ __attribute__((aligned(16))) uint16_t smem[CF::LDS];
   uint16_t* wt = smem;
@@ -876,19 +919,16 @@ __device__ __forceinline__ void fused_body(const FusedParams& P, const int vblk,
       for (int j = 0; j < KF; ++j)
 #pragma unroll
         for (int nt = 0; nt < kNT; ++nt) xb[j][nt] = frag_lds(&xnf[tloc[nt] * CF::XS + 32 * j + 8 * g]);
+      lds_ring<MT * KF, kRing>(
+          [&](int i) { return &wt[TLY.oW0() + (16 * (i % MT) + n16) * FP + 32 * (i / MT) + 8 * g]; },
+          [&](int i, const bf16x8 a) {
+            const int r = i % MT, j = i / MT;
+            if (j == 0)
 #pragma unroll
-      for (int r = 0; r < MT; ++r)
-      {
+              for (int nt = 0; nt < kNT; ++nt) z0[r][nt] = f32x4{0, 0, 0, 0};
 #pragma unroll
-        for (int nt = 0; nt < kNT; ++nt) z0[r][nt] = f32x4{0, 0, 0, 0};
-#pragma unroll
-        for (int j = 0; j < KF; ++j)
-        {
-          const bf16x8 a = frag_lds(&wt[TLY.oW0() + (16 * r + n16) * FP + 32 * j + 8 * g]);
-#pragma unroll
-          for (int nt = 0; nt < kNT; ++nt) z0[r][nt] = mfma(a, xb[j][nt], z0[r][nt]);
-        }
-      }
+            for (int nt = 0; nt < kNT; ++nt) z0[r][nt] = mfma(a, xb[j][nt], z0[r][nt]);
+          });
     }
     bf16x8 a0f[KH][kNT];
 #pragma unroll
@@ -907,19 +947,16 @@ __device__ __forceinline__ void fused_body(const FusedParams& P, const int vblk,
       }
     // layer 1
     f32x4 z1[MT][kNT];
+    lds_ring<MT * KH, kRing>(
+        [&](int i) { return &wt[TLY.oW1() + (16 * (i % MT) + n16) * H + 32 * (i / MT) + 8 * g]; },
+        [&](int i, const bf16x8 a) {
+          const int r = i % MT, j = i / MT;
+          if (j == 0)
 #pragma unroll
-    for (int r = 0; r < MT; ++r)
-    {
+            for (int nt = 0; nt < kNT; ++nt) z1[r][nt] = f32x4{0, 0, 0, 0};
 #pragma unroll
-      for (int nt = 0; nt < kNT; ++nt) z1[r][nt] = f32x4{0, 0, 0, 0};
-#pragma unroll
-      for (int j = 0; j < KH; ++j)
-      {
-        const bf16x8 a = frag_lds(&wt[TLY.oW1() + (16 * r + n16) * H + 32 * j + 8 * g]);
-#pragma unroll
-        for (int nt = 0; nt < kNT; ++nt) z1[r][nt] = mfma(a, a0f[j][nt], z1[r][nt]);
-      }
-    }
+          for (int nt = 0; nt < kNT; ++nt) z1[r][nt] = mfma(a, a0f[j][nt], z1[r][nt]);
+        });
     bf16x8 a1f[KH][kNT];
 #pragma unroll
     for (int j = 0; j < KH; ++j)
@@ -942,9 +979,8 @@ __device__ __forceinline__ void fused_body(const FusedParams& P, const int vblk,
     for (int nt = 0; nt < kNT; ++nt)
     {
       f32x4 yh = {0, 0, 0, 0};
-#pragma unroll
-      for (int j = 0; j < KH; ++j)
-        yh = mfma(frag_lds(&wt[TLY.oWo() + n16 * H + 32 * j + 8 * g]), a1f[j][nt], yh);
+      lds_ring<KH, kRing>([&](int j) { return &wt[TLY.oWo() + n16 * H + 32 * j + 8 * g]; },
+                          [&](int j, const bf16x8 a) { yh = mfma(a, a1f[j][nt], yh); });
       float v[8];
 #pragma unroll
       for (int i = 0; i < 4; ++i)
@@ -968,18 +1004,17 @@ __device__ __forceinline__ void fused_body(const FusedParams& P, const int vblk,
     bf16x8 dz1f[KH][kNT];
     {
       f32x4 dz1[MT][kNT];
+      lds_ring<MT, kRing>([&](int r) { return &wt[TLY.oWoT() + (16 * r + n16) * 32 + 8 * g]; },
+                          [&](int r, const bf16x8 a) {
 #pragma unroll
-      for (int r = 0; r < MT; ++r)
-      {
-        const bf16x8 a = frag_lds(&wt[TLY.oWoT() + (16 * r + n16) * 32 + 8 * g]);
+                            for (int nt = 0; nt < kNT; ++nt)
+                            {
+                              const f32x4 da = mfma(a, dyf[nt], f32x4{0, 0, 0, 0});
 #pragma unroll
-        for (int nt = 0; nt < kNT; ++nt)
-        {
-          const f32x4 da = mfma(a, dyf[nt], f32x4{0, 0, 0, 0});
-#pragma unroll
-          for (int i = 0; i < 4; ++i) dz1[r][nt][i] = act_bwd<float>(ACT, pre_act<ACT>(z1, a1f, r, nt, i), da[i]);
-        }
-      }
+                              for (int i = 0; i < 4; ++i)
+                                dz1[r][nt][i] = act_bwd<float>(ACT, pre_act<ACT>(z1, a1f, r, nt, i), da[i]);
+                            }
+                          });
 #pragma unroll
       for (int j = 0; j < KH; ++j)
 #pragma unroll
@@ -999,24 +1034,22 @@ __device__ __forceinline__ void fused_body(const FusedParams& P, const int vblk,
     bf16x8 dz0f[KH][kNT];
     {
       f32x4 dz0[MT][kNT];
+      lds_ring<MT * KH, kRing>(
+          [&](int i) { return &wt[TLY.oW1T() + (16 * (i % MT) + n16) * H + 32 * (i / MT) + 8 * g]; },
+          [&](int i, const bf16x8 a) {
+            const int r = i % MT, j = i / MT;
+            if (j == 0)
 #pragma unroll
-      for (int r = 0; r < MT; ++r)
-      {
+              for (int nt = 0; nt < kNT; ++nt) dz0[r][nt] = f32x4{0, 0, 0, 0};
 #pragma unroll
-        for (int nt = 0; nt < kNT; ++nt) dz0[r][nt] = f32x4{0, 0, 0, 0};
+            for (int nt = 0; nt < kNT; ++nt) dz0[r][nt] = mfma(a, dz1f[j][nt], dz0[r][nt]);
+            if (j == KH - 1)
 #pragma unroll
-        for (int j = 0; j < KH; ++j)
-        {
-          const bf16x8 a = frag_lds(&wt[TLY.oW1T() + (16 * r + n16) * H + 32 * j + 8 * g]);
+              for (int nt = 0; nt < kNT; ++nt)
 #pragma unroll
-          for (int nt = 0; nt < kNT; ++nt) dz0[r][nt] = mfma(a, dz1f[j][nt], dz0[r][nt]);
-        }
-#pragma unroll
-        for (int nt = 0; nt < kNT; ++nt)
-#pragma unroll
-          for (int i = 0; i < 4; ++i)
-            dz0[r][nt][i] = act_bwd<float>(ACT, pre_act<ACT>(z0, a0f, r, nt, i), dz0[r][nt][i]);
-      }
+                for (int e = 0; e < 4; ++e)
+                  dz0[r][nt][e] = act_bwd<float>(ACT, pre_act<ACT>(z0, a0f, r, nt, e), dz0[r][nt][e]);
+          });
 #pragma unroll
       for (int j = 0; j < KH; ++j)
 #pragma unroll
@@ -1046,14 +1079,14 @@ __device__ __forceinline__ void fused_body(const FusedParams& P, const int vblk,
         const Footprint<float> f0 = footprint<float>(tx[nt], ty[nt], M, D0);
         const Footprint<float> f1 = footprint<float>(tx[nt], ty[nt], M, D1);
         float* drow = dxs + tloc[nt] * CF::GS;
-#pragma unroll
-        for (int r = 0; r < GT; ++r)
-        {
-          f32x4 dx = {0, 0, 0, 0};
-#pragma unroll
-          for (int j = 0; j < KH; ++j)
-            dx = mfma(frag_lds(&wt[TLY.oW0T() + (16 * r + n16) * H + 32 * j + 8 * g]), dz0f[j][nt], dx);
-          if (!tv) continue;
+        f32x4 dx = {0, 0, 0, 0};
+        lds_ring<GT * KH, kRing>(
+            [&](int q) { return &wt[TLY.oW0T() + (16 * (q / KH) + n16) * H + 32 * (q % KH) + 8 * g]; },
+            [&](int q, const bf16x8 a) {
+          const int r = q / KH, j = q % KH;
+          if (j == 0) dx = f32x4{0, 0, 0, 0};
+          dx = mfma(a, dz0f[j][nt], dx);
+          if (j != KH - 1 || !tv) return;
 #pragma unroll
           for (int i = 0; i < 4; ++i)
           {
@@ -1087,7 +1120,7 @@ __device__ __forceinline__ void fused_body(const FusedParams& P, const int vblk,
               }
             }
           }
-        }
+            });
       }
     }
 
@@ -1116,18 +1149,15 @@ __device__ __forceinline__ void fused_body(const FusedParams& P, const int vblk,
         }
       __syncthreads();
       // dW1[h_out][h_in] += dZ1 A0^T, rows of this wave
-#pragma unroll
-      for (int mt = 0; mt < MTW; ++mt)
       {
-        const int row = CF::RW * wave + 16 * mt + n16;
-#pragma unroll
-        for (int kc = 0; kc < 2; ++kc)
-        {
-          const bf16x8 a = frag_lds(&stB[row * 64 + 32 * kc + 8 * g]);
-#pragma unroll
-          for (int c = 0; c < MT; ++c)
-            dw1[mt][c] = mfma(a, frag_lds(&stA[(16 * c + n16) * 64 + 32 * kc + 8 * g]), dw1[mt][c]);
-        }
+        bf16x8 a;
+        lds_ring<MTW * 2 * MT, kRing>(
+            [&](int i) { return &stA[(16 * (i % MT) + n16) * 64 + 32 * ((i / MT) % 2) + 8 * g]; },
+            [&](int i, const bf16x8 b) {
+              const int mt = i / (2 * MT), kc = (i / MT) % 2, c = i % MT;
+              if (c == 0) a = frag_lds(&stB[(CF::RW * wave + 16 * mt + n16) * 64 + 32 * kc + 8 * g]);
+              dw1[mt][c] = mfma(a, b, dw1[mt][c]);
+            });
       }
       __syncthreads();
 #pragma unroll
@@ -1140,23 +1170,23 @@ __device__ __forceinline__ void fused_body(const FusedParams& P, const int vblk,
           stB[h * 64 + col] = frag_elem(a1f[j][rnd], e);
         }
 #pragma unroll
-      for (int i = 0; i < 4; ++i) stY[(4 * g + i) * 64 + col] = f2bf(dy[rnd][i]);
+      for (int i = 0; i < 4; ++i) stY[(4 * g + i) * 64 + col] = bf16_dev(dy[rnd][i]);
       __syncthreads();
       // dW0[h][f] += dZ0 X^T, X^T read from xfn in the staged texel order
-#pragma unroll
-      for (int mt = 0; mt < MTW; ++mt)
       {
-        const int row = CF::RW * wave + 16 * mt + n16;
-#pragma unroll
-        for (int kc = 0; kc < 2; ++kc)
-        {
+        constexpr int NC = FP / 16;
+        auto xcol = [&](int kc) {
           const int kk = 32 * kc + 8 * g;
-          const int xcol = (kk >> 4) * 16 * kNT + 16 * rnd + (kk & 15);
-          const bf16x8 a = frag_lds(&stA[row * 64 + kk]);
-#pragma unroll
-          for (int c = 0; c < FP / 16; ++c)
-            dw0[mt][c] = mfma(a, frag_lds(&xfn[(16 * c + n16) * kPass + xcol]), dw0[mt][c]);
-        }
+          return (kk >> 4) * 16 * kNT + 16 * rnd + (kk & 15);
+        };
+        bf16x8 a;
+        lds_ring<MTW * 2 * NC, kRing>(
+            [&](int i) { return &xfn[(16 * (i % NC) + n16) * kPass + xcol((i / NC) % 2)]; },
+            [&](int i, const bf16x8 b) {
+              const int mt = i / (2 * NC), kc = (i / NC) % 2, c = i % NC;
+              if (c == 0) a = frag_lds(&stA[(CF::RW * wave + 16 * mt + n16) * 64 + 32 * kc + 8 * g]);
+              dw0[mt][c] = mfma(a, b, dw0[mt][c]);
+            });
       }
       // dWout[o][h] += dY A1^T, column tiles of this wave
 #pragma unroll
@@ -1254,7 +1284,7 @@ __global__ __launch_bounds__(kThreads, 1) void k_fused_train_sets(const FusedPar
 }
 
 // ---- decoder table packing (FP32 masters -> permuted BF16 tables) ----------------
-__device__ __forceinline__ void pack_table_elem(const TableLayout& L, int F, int O, int gslots,
+__host__ __device__ __forceinline__ void pack_table_elem(const TableLayout& L, int F, int O, int gslots,
                                                 const float* __restrict__ dec, uint16_t* __restrict__ tab, int e)
 {
   if (e >= L.total()) return;
@@ -1297,6 +1327,34 @@ __device__ __forceinline__ void pack_table_elem(const TableLayout& L, int F, int
   tab[e] = f2bf(v);
 }
 
+// The table entries that pack_table_elem fills from decoder parameter e
+// (W0[H][F] | W1[H][H] | Wout[16][H], row-major; a Wout row below O), set to
+// b. The other entries (padding, Wout rows from O on) hold zeros that no
+// update changes.
+__host__ __device__ __forceinline__ void table_scatter(const TableLayout& L, int F, int gslots, int e, uint16_t b,
+                                                       uint16_t* __restrict__ tab)
+{
+  const int H = L.H;
+  if (e < H * F)
+  {
+    const int m = e / F, c = e % F;
+    tab[L.oW0() + m * L.FP + c] = b;
+    if (c < gslots) tab[L.oW0T() + c * H + kperm_col(m)] = b;
+  }
+  else if (e < H * F + H * H)
+  {
+    const int i = e - H * F, m = i / H, k = i % H;
+    tab[L.oW1() + m * H + kperm_col(k)] = b;
+    tab[L.oW1T() + k * H + kperm_col(m)] = b;
+  }
+  else
+  {
+    const int i = e - H * F - H * H, o = i / H, h = i % H;
+    tab[L.oWo() + o * H + kperm_col(h)] = b;
+    tab[L.oWoT() + h * 32 + 8 * (o >> 2) + (o & 3)] = b;
+  }
+}
+
 __global__ __launch_bounds__(256) void k_pack_tables(TableLayout L, int F, int O, int gslots,
                                                      const float* __restrict__ dec, uint16_t* __restrict__ tab)
 {
@@ -1322,28 +1380,39 @@ __device__ __forceinline__ void decoder_update_elem(int e, int n_dec, int n_adam
                                                     float* __restrict__ grad, float* __restrict__ m,
                                                     float* __restrict__ v, float* __restrict__ p,
                                                     const double* __restrict__ sse_part,
-                                                    double* __restrict__ sse_out)
+                                                    double* __restrict__ sse_out, const TableLayout& L = {},
+                                                    int F = 0, int gslots = 0, uint16_t* __restrict__ tab = nullptr)
 {
   if (e == 0 && sse_out)
   {
+    // in block order, 16 loads in flight
     double s = 0.0;
-    for (int b = 0; b < blocks; ++b) s += sse_part[b];
+    int b = 0;
+    for (; b + 16 <= blocks; b += 16)
+    {
+      double w[16];
+#pragma unroll
+      for (int k = 0; k < 16; ++k) w[k] = sse_part[b + k];
+#pragma unroll
+      for (int k = 0; k < 16; ++k) s += w[k];
+    }
+    for (; b < blocks; ++b) s += sse_part[b];
     *sse_out = s;
   }
   if (e >= n_dec) return;
   // Wout rows >= O are outside the optimised set (libopenntc.cpp:722-732)
   if (e >= n_adam_w01 && (e - n_adam_w01) / H >= O) return;
-  // partials summed in block order; the loads of 8 partials are in flight
+  // partials summed in block order; the loads of 32 partials are in flight
   // at once (the adds stay sequential)
   float gsum = 0.0f;
   int b = 0;
-  for (; b + 8 <= blocks; b += 8)
+  for (; b + 32 <= blocks; b += 32)
   {
-    float v[8];
+    float v[32];
 #pragma unroll
-    for (int k = 0; k < 8; ++k) v[k] = part[(int64_t)(b + k) * n_dec + e];
+    for (int k = 0; k < 32; ++k) v[k] = part[(int64_t)(b + k) * n_dec + e];
 #pragma unroll
-    for (int k = 0; k < 8; ++k) gsum = gsum + v[k];
+    for (int k = 0; k < 32; ++k) gsum = gsum + v[k];
   }
   for (; b < blocks; ++b) gsum = gsum + part[(int64_t)b * n_dec + e];
   grad[e] = gsum;
@@ -1351,24 +1420,28 @@ __device__ __forceinline__ void decoder_update_elem(int e, int n_dec, int n_adam
   const float vi = fmaf(c.b2, v[e], (c.omb2 * gsum) * gsum);
   m[e] = mi;
   v[e] = vi;
-  p[e] = p[e] - (c.lr * (mi * c.bias1)) / (__builtin_sqrtf(vi * c.bias2) + 1e-8f);
+  const float pn = p[e] - (c.lr * (mi * c.bias1)) / (__builtin_sqrtf(vi * c.bias2) + 1e-8f);
+  p[e] = pn;
+  // the BF16 tables of the fused kernel follow the parameter (no separate pack)
+  if (tab) table_scatter(L, F, gslots, e, f2bf(pn), tab);
 }
 
-__global__ __launch_bounds__(256) void k_decoder_update(int n_dec, int n_adam_w01, int H, int O, int blocks,
+__global__ __launch_bounds__(64) void k_decoder_update(int n_dec, int n_adam_w01, int H, int O, int blocks,
                                                         const float* __restrict__ part, DecAdam c,
                                                         float* __restrict__ grad, float* __restrict__ m,
                                                         float* __restrict__ v, float* __restrict__ p,
                                                         const double* __restrict__ sse_part,
-                                                        double* __restrict__ sse_out)
+                                                        double* __restrict__ sse_out, TableLayout L, int F,
+                                                        int gslots, uint16_t* __restrict__ tab)
 {
-  decoder_update_elem(blockIdx.x * 256 + threadIdx.x, n_dec, n_adam_w01, H, O, blocks, part, c, grad, m, v, p,
-                      sse_part, sse_out);
+  decoder_update_elem(blockIdx.x * 64 + threadIdx.x, n_dec, n_adam_w01, H, O, blocks, part, c, grad, m, v, p,
+                      sse_part, sse_out, L, F, gslots, tab);
 }
 
 // The decoder updates of several sets of one shape for the step held in
 // *step: set blockIdx.y, its partial count and Adam scalars from
 // steps[step][set].
-__global__ __launch_bounds__(256) void k_decoder_update_sets(int n_dec, int n_adam_w01, int H, int O,
+__global__ __launch_bounds__(64) void k_decoder_update_sets(int n_dec, int n_adam_w01, int H, int O,
                                                              const ntc_dec_set* __restrict__ sets,
                                                              const ntc_dec_step* __restrict__ steps, int n_sets,
                                                              const int32_t* __restrict__ step)
@@ -1377,7 +1450,7 @@ __global__ __launch_bounds__(256) void k_decoder_update_sets(int n_dec, int n_ad
   const ntc_dec_set& S = sets[s];
   const ntc_dec_step& st = steps[(int64_t)read_step(step) * n_sets + s];
   const DecAdam c = {st.lr, st.beta1, st.beta2, 1.0f - st.beta1, 1.0f - st.beta2, st.bias1, st.bias2};
-  decoder_update_elem(blockIdx.x * 256 + threadIdx.x, n_dec, n_adam_w01, H, O, st.blocks, S.dw_partials, c,
+  decoder_update_elem(blockIdx.x * 64 + threadIdx.x, n_dec, n_adam_w01, H, O, st.blocks, S.dw_partials, c,
                       S.grad, S.m, S.v, S.p, S.sse_partials, S.sse_out);
 }
 
@@ -1618,7 +1691,7 @@ int ntc_decoder_update_sets(int32_t hidden_dim, int32_t feature_dim, int32_t out
   NTC_RETURN_IF(!sets || !steps || !step, NTC_E_INVALID);
   const int H = hidden_dim;
   const int n_dec = H * feature_dim + H * H + 16 * H;
-  hipLaunchKernelGGL(k_decoder_update_sets, dim3(blocks_for(n_dec, 256), n_sets), dim3(256), 0, stream, n_dec,
+  hipLaunchKernelGGL(k_decoder_update_sets, dim3(blocks_for(n_dec, 64), n_sets), dim3(64), 0, stream, n_dec,
                      H * feature_dim + H * H, H, out_dim, sets, steps, n_sets, step);
   return NTC_LAUNCH_STATUS();
 }
@@ -1664,9 +1737,30 @@ int ntc_decoder_update(int32_t hidden_dim, int32_t feature_dim, int32_t out_dim,
   const int H = hidden_dim;
   const int n_dec = H * feature_dim + H * H + 16 * H;
   DecAdam c = {lr, beta1, beta2, 1.0f - beta1, 1.0f - beta2, bias1, bias2};
-  hipLaunchKernelGGL(k_decoder_update, dim3(blocks_for(n_dec, 256)), dim3(256), 0, stream, n_dec,
+  hipLaunchKernelGGL(k_decoder_update, dim3(blocks_for(n_dec, 64)), dim3(64), 0, stream, n_dec,
                      H * feature_dim + H * H, H, out_dim, blocks, dw_partials, c, grad, m, v, p, sse_partials,
-                     sse_out);
+                     sse_out, TableLayout{}, 0, 0, (uint16_t*)nullptr);
+  return NTC_LAUNCH_STATUS();
+}
+
+int ntc_decoder_update_bf16(const ntc_feature_desc* fd, int32_t hidden_dim, int32_t out_dim, int32_t blocks,
+                            const float* dw_partials, float lr, float beta1, float beta2, float bias1, float bias2,
+                            float* grad, float* m, float* v, float* p, uint16_t* table,
+                            const double* sse_partials, double* sse_out, hipStream_t stream)
+{
+  NTC_RETURN_IF(!fd || !valid_feature_desc(fd), NTC_E_INVALID);
+  NTC_RETURN_IF(hidden_dim % 64 != 0 || hidden_dim < 64 || hidden_dim > 256, NTC_E_INVALID);
+  NTC_RETURN_IF(out_dim < 1 || out_dim > 16 || blocks < 1, NTC_E_INVALID);
+  NTC_RETURN_IF(!dw_partials || !grad || !m || !v || !p || !table, NTC_E_INVALID);
+  Shape s;
+  shape_of(*fd, hidden_dim, 0, s);
+  const TableLayout L{s.H, s.FP, s.G};
+  const int F = fd->feature_dim, H = hidden_dim;
+  const int gslots = g1_base(*fd) + fd->g1_channels;
+  const int n_dec = H * F + H * H + 16 * H;
+  DecAdam c = {lr, beta1, beta2, 1.0f - beta1, 1.0f - beta2, bias1, bias2};
+  hipLaunchKernelGGL(k_decoder_update, dim3(blocks_for(n_dec, 64)), dim3(64), 0, stream, n_dec, H * F + H * H, H,
+                     out_dim, blocks, dw_partials, c, grad, m, v, p, sse_partials, sse_out, L, F, gslots, table);
   return NTC_LAUNCH_STATUS();
 }
 } // extern "C"

@@ -368,6 +368,48 @@ _PART_CASES = {
 }
 
 
+@pytest.mark.gpu
+@pytest.mark.parametrize("kind,Od,blocks", [("c1_bilinear_relu", 9, 253), ("c3_bilinear_h128", 9, 256),
+                                            ("c2_bilinear", 4, 37), ("ref_bpp_2_25", 16, 1)])
+def test_decoder_update_refreshes_tables(oracle, kind, Od, blocks):
+    """ntc_decoder_update_bf16 (the update writes the BF16 tables it changes)
+    against ntc_decoder_update followed by a full ntc_pack_decoder_bf16, bit
+    for bit, over two updates in a row (Adam moments non-zero the second time)."""
+    lib = G.lib()
+    Lay, act = _case(oracle, kind)
+    fd = oracle.feature_desc(Lay, 0)
+    fdc = G.feature_desc(fd)
+    H, F = Lay.hidden_dim, Lay.feature_dim
+    rng = np.random.default_rng(zlib.crc32(f"tables-{kind}-{Od}".encode()))
+    dec = np.concatenate([(rng.uniform(-1, 1, H * F) / np.sqrt(F)).astype(np.float32),
+                          (rng.uniform(-1, 1, H * H) / np.sqrt(H)).astype(np.float32),
+                          (rng.uniform(-1, 1, 16 * H) / np.sqrt(H)).astype(np.float32)])
+    n_dec = dec.size
+    n_tab = lib.ntc_fused_table_elems(G.byref(fdc), H)
+    sides = []
+    for _ in range(2):
+        p = G.dev(dec)
+        tab = G.empty((n_tab,), np.uint16)
+        G.ok(lib.ntc_pack_decoder_bf16(G.byref(fdc), H, Od, G.p(p), G.p(tab), None))
+        sides.append(dict(p=p, tab=tab, grad=G.empty((n_dec,), np.float32), m=G.dev(np.zeros(n_dec, np.float32)),
+                          v=G.dev(np.zeros(n_dec, np.float32)), sse=G.empty((1,), np.float64)))
+    for it, (lr, b1c, b2c) in enumerate(((1e-2, 10.0, 1000.0), (3e-3, 5.26, 500.25))):
+        part = G.dev((rng.standard_normal(blocks * n_dec) * 1e-2).astype(np.float32))
+        ssep = G.dev(rng.uniform(0, 1, blocks))
+        a, b = sides
+        G.ok(lib.ntc_decoder_update_bf16(G.byref(fdc), H, Od, blocks, G.p(part), lr, 0.9, 0.999, b1c, b2c,
+                                         G.p(a["grad"]), G.p(a["m"]), G.p(a["v"]), G.p(a["p"]), G.p(a["tab"]),
+                                         G.p(ssep), G.p(a["sse"]), None))
+        G.ok(lib.ntc_decoder_update(H, F, Od, blocks, G.p(part), lr, 0.9, 0.999, b1c, b2c, G.p(b["grad"]),
+                                    G.p(b["m"]), G.p(b["v"]), G.p(b["p"]), G.p(ssep), G.p(b["sse"]), None))
+        G.ok(lib.ntc_pack_decoder_bf16(G.byref(fdc), H, Od, G.p(b["p"]), G.p(b["tab"]), None))
+        for k in ("p", "tab", "grad", "m", "v", "sse"):
+            ga, gb = G.host(a[k]), G.host(b[k])
+            assert np.array_equal(ga.view(np.uint8), gb.view(np.uint8)), (it, k)
+        # the update moved the parameters (the tables are not trivially equal)
+        assert not np.array_equal(G.host(a["p"]), dec)
+
+
 @pytest.mark.parametrize("case", list(_PART_CASES))
 def test_grid_update_parts_match_tiles_update(case):
     """The cold part (cells outside every tile rectangle, gradient untouched)
