@@ -145,7 +145,23 @@ __host__ __device__ __forceinline__ int kperm_col(int k)
   return (k & ~31) + 8 * (q >> 2) + (q & 3) + 4 * (r >> 4);
 }
 
-// Weight table layout (elements, BF16), identical in HBM and LDS.
+// Column of element (row, col) of a table row of wd BF16 elements: the
+// 16-byte chunks of a row are XOR-permuted by row so that the sixteen rows one
+// MFMA operand read touches (same logical chunk) fall on different LDS banks
+// (64 banks of 4 bytes: without it rows of 256 bytes all start on bank 0).
+// An involution within the row; rows whose chunk count is not a power of two
+// are left as they are.
+__host__ __device__ __forceinline__ int swz(int row, int col, int wd)
+{
+  const int nc = wd >> 3;
+  if (nc & (nc - 1)) return col;
+  const int per = nc >= 16 ? 1 : 16 / nc;  // rows per 256 bytes
+  const int m = (nc >= 16 ? 16 : nc) - 1;
+  return ((((col >> 3) ^ ((row / per) & m))) << 3) | (col & 7);
+}
+
+// Weight table layout (elements, BF16), identical in HBM and LDS; the columns
+// of every row are placed by swz.
 struct TableLayout
 {
   int H, FP, G;
@@ -803,7 +819,7 @@ __device__ __forceinline__ void fused_body(const FusedParams& P, const int vblk,
   using CF = Cfg<H, FP, G, C0, C1, G0MODE, PEC, LODS, ACT, NT>;
   constexpr int kNT = NT, kPass = CF::PASS;
   constexpr int MT = CF::MT, KF = CF::KF, KH = CF::KH, GT = CF::GT, MTW = CF::MTW, NTO = CF::NTO;
-  constexpr int kRing = 4;  // weight fragments read ahead of their MFMA
+  constexpr int kRing = 8;  // fragments read ahead of their MFMA
   constexpr TableLayout TLY = {H, FP, G};
   __shared__ __attribute__((aligned(16))) uint16_t smem[CF::LDS];
   uint16_t* wt = smem;
@@ -920,7 +936,10 @@ __device__ __forceinline__ void fused_body(const FusedParams& P, const int vblk,
 #pragma unroll
         for (int nt = 0; nt < kNT; ++nt) xb[j][nt] = frag_lds(&xnf[tloc[nt] * CF::XS + 32 * j + 8 * g]);
       lds_ring<MT * KF, kRing>(
-          [&](int i) { return &wt[TLY.oW0() + (16 * (i % MT) + n16) * FP + 32 * (i / MT) + 8 * g]; },
+          [&](int i) {
+            const int row = 16 * (i % MT) + n16;
+            return &wt[TLY.oW0() + row * FP + swz(row, 32 * (i / MT) + 8 * g, FP)];
+          },
           [&](int i, const bf16x8 a) {
             const int r = i % MT, j = i / MT;
             if (j == 0)
@@ -948,7 +967,10 @@ __device__ __forceinline__ void fused_body(const FusedParams& P, const int vblk,
     // layer 1
     f32x4 z1[MT][kNT];
     lds_ring<MT * KH, kRing>(
-        [&](int i) { return &wt[TLY.oW1() + (16 * (i % MT) + n16) * H + 32 * (i / MT) + 8 * g]; },
+        [&](int i) {
+          const int row = 16 * (i % MT) + n16;
+          return &wt[TLY.oW1() + row * H + swz(row, 32 * (i / MT) + 8 * g, H)];
+        },
         [&](int i, const bf16x8 a) {
           const int r = i % MT, j = i / MT;
           if (j == 0)
@@ -979,7 +1001,7 @@ __device__ __forceinline__ void fused_body(const FusedParams& P, const int vblk,
     for (int nt = 0; nt < kNT; ++nt)
     {
       f32x4 yh = {0, 0, 0, 0};
-      lds_ring<KH, kRing>([&](int j) { return &wt[TLY.oWo() + n16 * H + 32 * j + 8 * g]; },
+      lds_ring<KH, kRing>([&](int j) { return &wt[TLY.oWo() + n16 * H + swz(n16, 32 * j + 8 * g, H)]; },
                           [&](int j, const bf16x8 a) { yh = mfma(a, a1f[j][nt], yh); });
       float v[8];
 #pragma unroll
@@ -1004,7 +1026,7 @@ __device__ __forceinline__ void fused_body(const FusedParams& P, const int vblk,
     bf16x8 dz1f[KH][kNT];
     {
       f32x4 dz1[MT][kNT];
-      lds_ring<MT, kRing>([&](int r) { return &wt[TLY.oWoT() + (16 * r + n16) * 32 + 8 * g]; },
+      lds_ring<MT, kRing>([&](int r) { return &wt[TLY.oWoT() + (16 * r + n16) * 32 + swz(16 * r + n16, 8 * g, 32)]; },
                           [&](int r, const bf16x8 a) {
 #pragma unroll
                             for (int nt = 0; nt < kNT; ++nt)
@@ -1035,7 +1057,10 @@ __device__ __forceinline__ void fused_body(const FusedParams& P, const int vblk,
     {
       f32x4 dz0[MT][kNT];
       lds_ring<MT * KH, kRing>(
-          [&](int i) { return &wt[TLY.oW1T() + (16 * (i % MT) + n16) * H + 32 * (i / MT) + 8 * g]; },
+          [&](int i) {
+            const int row = 16 * (i % MT) + n16;
+            return &wt[TLY.oW1T() + row * H + swz(row, 32 * (i / MT) + 8 * g, H)];
+          },
           [&](int i, const bf16x8 a) {
             const int r = i % MT, j = i / MT;
             if (j == 0)
@@ -1081,7 +1106,10 @@ __device__ __forceinline__ void fused_body(const FusedParams& P, const int vblk,
         float* drow = dxs + tloc[nt] * CF::GS;
         f32x4 dx = {0, 0, 0, 0};
         lds_ring<GT * KH, kRing>(
-            [&](int q) { return &wt[TLY.oW0T() + (16 * (q / KH) + n16) * H + 32 * (q % KH) + 8 * g]; },
+            [&](int q) {
+              const int row = 16 * (q / KH) + n16;
+              return &wt[TLY.oW0T() + row * H + swz(row, 32 * (q % KH) + 8 * g, H)];
+            },
             [&](int q, const bf16x8 a) {
           const int r = q / KH, j = q % KH;
           if (j == 0) dx = f32x4{0, 0, 0, 0};
@@ -1295,33 +1323,33 @@ __host__ __device__ __forceinline__ void pack_table_elem(const TableLayout& L, i
   float v = 0.0f;
   if (e < L.oW1())
   {
-    const int m = e / L.FP, c = e % L.FP;
+    const int m = e / L.FP, c = swz(m, e % L.FP, L.FP);
     v = c < F ? w0[m * F + c] : 0.0f;
   }
   else if (e < L.oWo())
   {
-    const int i = e - L.oW1(), m = i / H, pos = i % H;
+    const int i = e - L.oW1(), m = i / H, pos = swz(m, i % H, H);
     v = w1[m * H + kperm_pos(pos)];
   }
   else if (e < L.oWoT())
   {
-    const int i = e - L.oWo(), o = i / H, pos = i % H;
+    const int i = e - L.oWo(), o = i / H, pos = swz(o, i % H, H);
     v = o < O ? wo[o * H + kperm_pos(pos)] : 0.0f;
   }
   else if (e < L.oW1T())
   {
-    const int i = e - L.oWoT(), h = i / 32, pos = i % 32, gg = pos >> 3, ee = pos & 7;
+    const int i = e - L.oWoT(), h = i / 32, pos = swz(h, i % 32, 32), gg = pos >> 3, ee = pos & 7;
     const int o = 4 * gg + ee;
     v = (ee < 4 && o < O) ? wo[o * H + h] : 0.0f;
   }
   else if (e < L.oW0T())
   {
-    const int i = e - L.oW1T(), hin = i / H, pos = i % H;
+    const int i = e - L.oW1T(), hin = i / H, pos = swz(hin, i % H, H);
     v = w1[kperm_pos(pos) * H + hin];
   }
   else
   {
-    const int i = e - L.oW0T(), s = i / H, pos = i % H;
+    const int i = e - L.oW0T(), s = i / H, pos = swz(s, i % H, H);
     v = s < gslots ? w0[kperm_pos(pos) * F + s] : 0.0f;
   }
   tab[e] = f2bf(v);
@@ -1338,20 +1366,20 @@ __host__ __device__ __forceinline__ void table_scatter(const TableLayout& L, int
   if (e < H * F)
   {
     const int m = e / F, c = e % F;
-    tab[L.oW0() + m * L.FP + c] = b;
-    if (c < gslots) tab[L.oW0T() + c * H + kperm_col(m)] = b;
+    tab[L.oW0() + m * L.FP + swz(m, c, L.FP)] = b;
+    if (c < gslots) tab[L.oW0T() + c * H + swz(c, kperm_col(m), H)] = b;
   }
   else if (e < H * F + H * H)
   {
     const int i = e - H * F, m = i / H, k = i % H;
-    tab[L.oW1() + m * H + kperm_col(k)] = b;
-    tab[L.oW1T() + k * H + kperm_col(m)] = b;
+    tab[L.oW1() + m * H + swz(m, kperm_col(k), H)] = b;
+    tab[L.oW1T() + k * H + swz(k, kperm_col(m), H)] = b;
   }
   else
   {
     const int i = e - H * F - H * H, o = i / H, h = i % H;
-    tab[L.oWo() + o * H + kperm_col(h)] = b;
-    tab[L.oWoT() + h * 32 + 8 * (o >> 2) + (o & 3)] = b;
+    tab[L.oWo() + o * H + swz(o, kperm_col(h), H)] = b;
+    tab[L.oWoT() + h * 32 + swz(h, 8 * (o >> 2) + (o & 3), 32)] = b;
   }
 }
 
