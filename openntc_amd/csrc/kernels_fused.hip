@@ -827,6 +827,7 @@ __device__ __forceinline__ void fused_body(const FusedParams& P, const int vblk,
   uint16_t* uni = xfn + CF::XFN;
   uint16_t* xnf = uni;                       // [PASS][XS] during gather/forward
   float* sbox = reinterpret_cast<float*>(uni + kPass * CF::XS);  // staged boxes during gather
+  // stA, stB, stY: rows of texels, columns placed by swz
   uint16_t* stA = uni;                       // [H][64]  A0 | dZ0
   uint16_t* stB = uni + H * 64;              // [H][64]  dZ1 | A1
   uint16_t* stY = uni + 2 * H * 64;          // [16][64] dY
@@ -1166,24 +1167,33 @@ __device__ __forceinline__ void fused_body(const FusedParams& P, const int vblk,
     for (int rnd = 0; rnd < kNT; ++rnd)
     {
       const int col = wave * 16 + n16;
+      // swz(h, col, 64) for the rows this lane writes: h = kperm(j, g, e) and
+      // 4 g + i give (h >> 1) & 7 = (2 g + ((e >> 1) & 1)) & 7, two columns
+      int scol[2];
+#pragma unroll
+      for (int b = 0; b < 2; ++b) scol[b] = swz(2 * (2 * g + b), col, 64);
 #pragma unroll
       for (int j = 0; j < KH; ++j)
 #pragma unroll
         for (int e = 0; e < 8; ++e)
         {
           const int h = kperm(j, g, e);
-          stA[h * 64 + col] = frag_elem(a0f[j][rnd], e);
-          stB[h * 64 + col] = frag_elem(dz1f[j][rnd], e);
+          stA[h * 64 + scol[(e >> 1) & 1]] = frag_elem(a0f[j][rnd], e);
+          stB[h * 64 + scol[(e >> 1) & 1]] = frag_elem(dz1f[j][rnd], e);
         }
       __syncthreads();
       // dW1[h_out][h_in] += dZ1 A0^T, rows of this wave
       {
         bf16x8 a;
         lds_ring<MTW * 2 * MT, kRing>(
-            [&](int i) { return &stA[(16 * (i % MT) + n16) * 64 + 32 * ((i / MT) % 2) + 8 * g]; },
+            [&](int i) {
+              const int row = 16 * (i % MT) + n16;
+              return &stA[row * 64 + swz(row, 32 * ((i / MT) % 2) + 8 * g, 64)];
+            },
             [&](int i, const bf16x8 b) {
               const int mt = i / (2 * MT), kc = (i / MT) % 2, c = i % MT;
-              if (c == 0) a = frag_lds(&stB[(CF::RW * wave + 16 * mt + n16) * 64 + 32 * kc + 8 * g]);
+              const int row = CF::RW * wave + 16 * mt + n16;
+              if (c == 0) a = frag_lds(&stB[row * 64 + swz(row, 32 * kc + 8 * g, 64)]);
               dw1[mt][c] = mfma(a, b, dw1[mt][c]);
             });
       }
@@ -1194,11 +1204,11 @@ __device__ __forceinline__ void fused_body(const FusedParams& P, const int vblk,
         for (int e = 0; e < 8; ++e)
         {
           const int h = kperm(j, g, e);
-          stA[h * 64 + col] = frag_elem(dz0f[j][rnd], e);
-          stB[h * 64 + col] = frag_elem(a1f[j][rnd], e);
+          stA[h * 64 + scol[(e >> 1) & 1]] = frag_elem(dz0f[j][rnd], e);
+          stB[h * 64 + scol[(e >> 1) & 1]] = frag_elem(a1f[j][rnd], e);
         }
 #pragma unroll
-      for (int i = 0; i < 4; ++i) stY[(4 * g + i) * 64 + col] = bf16_dev(dy[rnd][i]);
+      for (int i = 0; i < 4; ++i) stY[(4 * g + i) * 64 + scol[i >> 1]] = bf16_dev(dy[rnd][i]);
       __syncthreads();
       // dW0[h][f] += dZ0 X^T, X^T read from xfn in the staged texel order
       {
@@ -1209,10 +1219,14 @@ __device__ __forceinline__ void fused_body(const FusedParams& P, const int vblk,
         };
         bf16x8 a;
         lds_ring<MTW * 2 * NC, kRing>(
-            [&](int i) { return &xfn[(16 * (i % NC) + n16) * kPass + xcol((i / NC) % 2)]; },
+            [&](int i) {
+              const int row = 16 * (i % NC) + n16;
+              return &xfn[row * kPass + xcol((i / NC) % 2)];
+            },
             [&](int i, const bf16x8 b) {
               const int mt = i / (2 * NC), kc = (i / NC) % 2, c = i % NC;
-              if (c == 0) a = frag_lds(&stA[(CF::RW * wave + 16 * mt + n16) * 64 + 32 * kc + 8 * g]);
+              const int row = CF::RW * wave + 16 * mt + n16;
+              if (c == 0) a = frag_lds(&stA[row * 64 + swz(row, 32 * kc + 8 * g, 64)]);
               dw0[mt][c] = mfma(a, b, dw0[mt][c]);
             });
       }
@@ -1220,12 +1234,12 @@ __device__ __forceinline__ void fused_body(const FusedParams& P, const int vblk,
 #pragma unroll
       for (int kc = 0; kc < 2; ++kc)
       {
-        const bf16x8 a = frag_lds(&stY[n16 * 64 + 32 * kc + 8 * g]);
+        const bf16x8 a = frag_lds(&stY[n16 * 64 + swz(n16, 32 * kc + 8 * g, 64)]);
 #pragma unroll
         for (int c = 0; c < NTO; ++c)
         {
           const int hc = (wave * NTO + c) * 16 + n16;
-          dwo[c] = mfma(a, frag_lds(&stB[hc * 64 + 32 * kc + 8 * g]), dwo[c]);
+          dwo[c] = mfma(a, frag_lds(&stB[hc * 64 + swz(hc, 32 * kc + 8 * g, 64)]), dwo[c]);
         }
       }
       __syncthreads();
