@@ -442,7 +442,9 @@ __device__ __forceinline__ void box_gradients(const AxisTab& tb, const Box& bx, 
     const int m = 16 * mt + n;
     const int v = small_div(m, bx.wr), u = m - v * bx.w;  // v >= bx.h past the last cell: no corner matches
     const int ch = 16 * ct + n;
-    f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+    // two accumulators (even and odd K steps) so that consecutive MFMAs do
+    // not wait for each other; summed at the end
+    f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f}, acc1 = acc;
 #pragma unroll
     for (int c = 0; c < NC; ++c)
     {
@@ -461,9 +463,14 @@ __device__ __forceinline__ void box_gradients(const AxisTab& tb, const Box& bx, 
       for (int kk = 0; kk < KS; ++kk)
       {
         const float d = ch < C ? dxs[(4 * kk + q) * GS + slot0 + c * C + ch] : 0.0f;
-        acc = mfma_f32(wx[kk & 3] * wy[kk >> 2], d, acc);
+        if (kk & 1)
+          acc1 = mfma_f32(wx[kk & 3] * wy[kk >> 2], d, acc1);
+        else
+          acc = mfma_f32(wx[kk & 3] * wy[kk >> 2], d, acc);
       }
     }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) acc[i] += acc1[i];
 #pragma unroll
     for (int i = 0; i < 4; ++i)
     {
@@ -1254,7 +1261,12 @@ __device__ __forceinline__ void fused_body(const FusedParams& P, const int vblk,
     if (P.scatter && bx0.fits)
       box_gradients<C0, G0MODE == NTC_G0_CORNERS, CF::GS, kPass>(tab[0], bx0, dxs, 0, P.dg0, D0, wave, lane);
     if (P.scatter && bx1.fits)
-      box_gradients<C1, false, CF::GS, kPass>(tab[1], bx1, dxs, CF::G1B, P.dg1, D1, wave, lane);
+    {
+      // g1's tiles start on the wave after g0's last one
+      const int t0 = bx0.fits ? ((bx0.w * bx0.h + 15) / 16) * ((C0 + 15) / 16) : 0;
+      box_gradients<C1, false, CF::GS, kPass>(tab[1], bx1, dxs, CF::G1B, P.dg1, D1, (wave + 4 - (t0 & 3)) & 3,
+                                              lane);
+    }
     cur = nxt;
     tb ^= 1;
   }
