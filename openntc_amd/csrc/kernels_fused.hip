@@ -99,6 +99,20 @@ __device__ __forceinline__ bf16x8 frag_pack(const float* v)
   return __builtin_bit_cast(bf16x8, u);
 }
 
+// Stages dword d of fragment f (rows h and h + 1 of this lane's texel column)
+// into a [row][64] BF16 array as 32-bit writes: the lane holding an even
+// column writes row h for its column and its neighbour's, the odd lane row
+// h + 1 for both (the neighbours swap dwords with a DPP quad permutation).
+// sel: 0x05040100 on even lanes, 0x03020706 on odd ones; at: the dword index
+// of (h + odd, even column).
+__device__ __forceinline__ void stage_pair(uint16_t* st, const bf16x8& f, int d, uint32_t sel, int at)
+{
+  const uint4 u = __builtin_bit_cast(uint4, f);
+  const uint32_t mine = d == 0 ? u.x : (d == 1 ? u.y : (d == 2 ? u.z : u.w));
+  const uint32_t theirs = (uint32_t)__builtin_amdgcn_mov_dpp((int)mine, 0xB1, 0xF, 0xF, false);
+  reinterpret_cast<uint32_t*>(st)[at] = __builtin_amdgcn_perm(theirs, mine, sel);
+}
+
 __device__ __forceinline__ uint16_t frag_elem(const bf16x8& f, int e)
 {
   const uint4 u = __builtin_bit_cast(uint4, f);
@@ -1179,14 +1193,20 @@ __device__ __forceinline__ void fused_body(const FusedParams& P, const int vblk,
       int scol[2];
 #pragma unroll
       for (int b = 0; b < 2; ++b) scol[b] = swz(2 * (2 * g + b), col, 64);
+      // the same for the column pair (col & ~1) of the 32-bit writes
+      const int odd = n16 & 1;
+      const uint32_t sel = odd ? 0x03020706u : 0x05040100u;
+      int pcol[2];
+#pragma unroll
+      for (int b = 0; b < 2; ++b) pcol[b] = (odd * 64 + swz(2 * (2 * g + b), col & ~1, 64)) >> 1;
 #pragma unroll
       for (int j = 0; j < KH; ++j)
 #pragma unroll
-        for (int e = 0; e < 8; ++e)
+        for (int d = 0; d < 4; ++d)
         {
-          const int h = kperm(j, g, e);
-          stA[h * 64 + scol[(e >> 1) & 1]] = frag_elem(a0f[j][rnd], e);
-          stB[h * 64 + scol[(e >> 1) & 1]] = frag_elem(dz1f[j][rnd], e);
+          const int at = kperm(j, g, 2 * d) * 32 + pcol[d & 1];
+          stage_pair(stA, a0f[j][rnd], d, sel, at);
+          stage_pair(stB, dz1f[j][rnd], d, sel, at);
         }
       __syncthreads();
       // dW1[h_out][h_in] += dZ1 A0^T, rows of this wave
@@ -1208,11 +1228,11 @@ __device__ __forceinline__ void fused_body(const FusedParams& P, const int vblk,
 #pragma unroll
       for (int j = 0; j < KH; ++j)
 #pragma unroll
-        for (int e = 0; e < 8; ++e)
+        for (int d = 0; d < 4; ++d)
         {
-          const int h = kperm(j, g, e);
-          stA[h * 64 + scol[(e >> 1) & 1]] = frag_elem(dz0f[j][rnd], e);
-          stB[h * 64 + scol[(e >> 1) & 1]] = frag_elem(a1f[j][rnd], e);
+          const int at = kperm(j, g, 2 * d) * 32 + pcol[d & 1];
+          stage_pair(stA, dz0f[j][rnd], d, sel, at);
+          stage_pair(stB, a1f[j][rnd], d, sel, at);
         }
 #pragma unroll
       for (int i = 0; i < 4; ++i) stY[(4 * g + i) * 64 + scol[i >> 1]] = bf16_dev(dy[rnd][i]);
