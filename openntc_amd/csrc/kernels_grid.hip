@@ -269,6 +269,20 @@ __device__ __forceinline__ int wave_min(int v)
 // wave: the step loop runs from the wave's smallest counter with the step
 // constants as wave-uniform loads (the next step's issued before the current
 // one is applied), and a lane applies the steps from its own counter on.
+// Correctly rounded square root of y for y == 0 or 2^-96 <= y < +inf: the
+// hardware estimate (within 1 ulp) and the choice between it and its two
+// neighbours by the sign of the FMA residuals, the steps the compiler's exact
+// sqrtf takes on that range (below 2^-96 it scales y by 2^32 first, and it
+// passes 0 and +inf through; the residual test gives 0 for 0 by itself).
+__device__ __forceinline__ float sqrt_rn_wide(float y)
+{
+  const float s = __builtin_amdgcn_sqrtf(y);
+  const float sd = __int_as_float(__float_as_int(s) - 1), su = __int_as_float(__float_as_int(s) + 1);
+  float r = fmaf(-sd, s, y) <= 0.0f ? sd : s;
+  r = fmaf(-su, s, y) > 0.0f ? su : r;
+  return r;
+}
+
 __device__ __forceinline__ void lazy_run(const LazyArgs& L, const GridSeg& s, int64_t k, int from, bool live)
 {
   const bool mine = live && from < L.upto;
@@ -303,15 +317,36 @@ __device__ __forceinline__ void lazy_run(const LazyArgs& L, const GridSeg& s, in
       // adam_one with g = 0, its gradient terms (1-b1) 0 and ((1-b2) 0) 0
       // formed once (wave-uniform) instead of per element: the same values
       const float zm = c.omb1 * 0.0f, zv = (c.omb2 * 0.0f) * 0.0f;
+      float me[4], y[4];
+      bool low = false;
 #pragma unroll
       for (int e = 0; e < 4; ++e)
       {
-        const float me = fmaf(c.b1, mi[e], zm);
-        const float ve = fmaf(c.b2, vi[e], zv);
-        const float pe = pi[e] - (c.lr * (me * c.bias1)) / (__builtin_sqrtf(ve * c.bias2) + 1e-8f);
-        mi[e] = me;
-        vi[e] = ve;
-        pi[e] = fmaxf(s.lo, fminf(s.hi, pe));
+        me[e] = fmaf(c.b1, mi[e], zm);
+        vi[e] = fmaf(c.b2, vi[e], zv);
+        y[e] = vi[e] * c.bias2;
+        low |= y[e] > 0.0f && y[e] < 0x1p-96f;
+      }
+      // the exact sqrtf where some lane of the wave has a tiny positive y,
+      // else the same bits without its scaling and special-value steps
+      float r[4];
+      if (__builtin_amdgcn_ballot_w64(low) != 0)
+      {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) r[e] = __builtin_sqrtf(y[e]);
+      }
+      else
+      {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) r[e] = sqrt_rn_wide(y[e]);
+      }
+#pragma unroll
+      for (int e = 0; e < 4; ++e)
+      {
+        const float pe = pi[e] - (c.lr * (me[e] * c.bias1)) / (r[e] + 1e-8f);
+        mi[e] = me[e];
+        // fmaxf(lo, fminf(hi, pe)) for every non-NaN pe (lo <= hi)
+        pi[e] = __builtin_amdgcn_fmed3f(pe, s.lo, s.hi);
       }
     }
     lr = nlr;
