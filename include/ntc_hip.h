@@ -258,6 +258,24 @@ int ntc_grid_lazy_hot_f32(const ntc_feature_desc* fd, const ntc_tiles* tiles,
                           float* g0, float* dg0, float* g1, float* dg1,
                           const ntc_grid_adam* adam, int32_t step, int32_t* last0,
                           int32_t* last1, hipStream_t stream);
+/* ntc_grid_lazy_hot_f32 of one step and ntc_grid_lazy_catchup_f32 of the next
+ * (next_* arguments; skip: the first step's tiles when both are on one level,
+ * so that the two touch different cells) in one launch, on the blocks of each
+ * part: the step's tail on one stream, with no second stream to wait for.
+ * Not across a flush (as the catch-up issued ahead). Context::Train keeps
+ * the two-stream form, which overlaps the catch-up with the decoder update
+ * as well (0.477 against 0.488 ms per step at 4096^2). No counterpart in the
+ * reference. */
+int ntc_grid_lazy_hot_catchup_f32(const ntc_feature_desc* fd, const ntc_tiles* tiles,
+                                  float* g0, float* dg0, float* g1, float* dg1,
+                                  const ntc_grid_adam* adam, int32_t step,
+                                  int32_t* last0, int32_t* last1,
+                                  const ntc_feature_desc* next_fd,
+                                  const ntc_tiles* next_tiles, const ntc_tiles* skip,
+                                  float* next_g0, float* next_g1,
+                                  const ntc_grid_adam* next_adam, const float* steps,
+                                  int32_t upto, int32_t* next_last0,
+                                  int32_t* next_last1, hipStream_t stream);
 /* out[i] = value, i < n (kernel fill) */
 int ntc_fill_i32(int64_t n, int32_t value, int32_t* out, hipStream_t stream);
 /* zero-fill by kernel (graph-safe replacement for memset) */
@@ -570,10 +588,10 @@ int ntc_ctx_flush_grids(ntc_context* ctx);
 /* Device pointer to the context's stream (for timing with events). */
 void* ntc_ctx_stream(ntc_context* ctx);
 /* Average durations over the last Train call with timing on, measured with
- * events: {fused kernel, decoder update + table pack, grid update, steps
- * timed, grid updates timed, grid updates that skipped the zero gradient
- * outside the tile footprints, cold part, hot part (ms; the split update),
- * steps with the split update}; -1 when not recorded. */
+ * events: {fused kernel, decoder update (with its table refresh), grid
+ * update, steps timed, grid updates timed, grid updates that skipped the
+ * zero gradient outside the tile footprints, deferred catch-up, hot part
+ * (ms), steps with the deferred update}; -1 when not recorded. */
 int ntc_ctx_kernel_times(ntc_context* ctx, float* out, int32_t n);
 /* enabled: 0 off; 1 events around every kernel of each fused step; 2 around
  * the fused kernel only (fewer events between the step's kernels; the other

@@ -197,6 +197,8 @@ SIGNATURES = {
     "ntc_grid_update_tiles_part_f32": (_I, [_P, _P, _P, _P, _P, _P, _P, _I, _I, _P]),
     "ntc_grid_lazy_catchup_f32": (_I, [_P, _P, _P, _P, _P, _P, _P, _I, _P, _P, _P]),
     "ntc_grid_lazy_hot_f32": (_I, [_P, _P, _P, _P, _P, _P, _P, _I, _P, _P, _P]),
+    "ntc_grid_lazy_hot_catchup_f32": (_I, [_P, _P, _P, _P, _P, _P, _P, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _I, _P, _P,
+                                           _P]),
     "ntc_fill_i32": (_I, [C.c_int64, _I, _P, _P]),
     "ntc_decode_table_bytes": (C.c_size_t, [_I, _I, _I, _I]),
     "ntc_build_decode_table_f": (_I, [_I, _I, _I, _I, _P, _F, _F, _P]),

@@ -474,6 +474,25 @@ __global__ __launch_bounds__(kThreads) void k_grid_lazy_hot(LazyArgs L)
   lazy_hot_body(L, blockIdx.x, gridDim.x, blockIdx.y);
 }
 
+// One launch for a step's hot part (blockIdx.y < 32: H, its rectangle slot)
+// and the next step's catch-up (blockIdx.y >= 32: C, which leaves out H's
+// rectangles when both are on one level), each on its own blocks: the two
+// touch different cells. nbx_h / nbx_c: blocks per rectangle of each part.
+__global__ __launch_bounds__(kThreads) void k_grid_lazy_hot_catchup(LazyArgs H, LazyArgs C, int nbx_h, int nbx_c)
+{
+  const int y = (int)blockIdx.y, bx = (int)blockIdx.x;
+  if (y < 32)
+  {
+    if (y >= 2 * H.A.tl.n_tiles || bx >= nbx_h) return;
+    lazy_hot_body(H, bx, nbx_h, y);
+  }
+  else
+  {
+    if (y - 32 >= 2 * C.A.tl.n_tiles || bx >= nbx_c) return;
+    lazy_catchup_rects_body(C, bx, nbx_c, y - 32);
+  }
+}
+
 __global__ __launch_bounds__(kThreads) void k_fill_i32(int64_t n, int32_t value, int32_t* __restrict__ out)
 {
   for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads)
@@ -812,6 +831,34 @@ int ntc_grid_lazy_hot_f32(const ntc_feature_desc* fd, const ntc_tiles* tiles, fl
   const int st = lazy_args_of(fd, tiles, g0, dg0, g1, dg1, adam, kUnused, step, last0, last1, L);
   if (st != NTC_OK) return st;
   hipLaunchKernelGGL(k_grid_lazy_hot, dim3(rect_blocks(tiles, L.A, 256), 32), dim3(kThreads), 0, stream, L);
+  return NTC_LAUNCH_STATUS();
+}
+
+int ntc_grid_lazy_hot_catchup_f32(const ntc_feature_desc* fd, const ntc_tiles* tiles, float* g0, float* dg0,
+                                   float* g1, float* dg1, const ntc_grid_adam* adam, int32_t step, int32_t* last0,
+                                   int32_t* last1, const ntc_feature_desc* next_fd, const ntc_tiles* next_tiles,
+                                   const ntc_tiles* skip, float* next_g0, float* next_g1,
+                                   const ntc_grid_adam* next_adam, const float* steps, int32_t upto,
+                                   int32_t* next_last0, int32_t* next_last1, hipStream_t stream)
+{
+  NTC_RETURN_IF(!tiles || !next_tiles, NTC_E_INVALID);
+  LazyArgs H, C;
+  static const float kUnused[3] = {0, 0, 0};
+  int st = lazy_args_of(fd, tiles, g0, dg0, g1, dg1, adam, kUnused, step, last0, last1, H);
+  if (st != NTC_OK) return st;
+  st = lazy_args_of(next_fd, next_tiles, next_g0, next_g0, next_g1, next_g1, next_adam, steps, upto, next_last0,
+                    next_last1, C);
+  if (st != NTC_OK) return st;
+  if (skip)
+  {
+    NTC_RETURN_IF(!valid_tiles(skip) || skip->n_tiles > 16, NTC_E_INVALID);
+    C.skip = *skip;
+    C.n_skip = skip->n_tiles;
+  }
+  const int nbx_h = rect_blocks(tiles, H.A, 256);
+  const int nbx_c = upto == 0 ? 0 : rect_blocks(next_tiles, C.A, 256);
+  hipLaunchKernelGGL(k_grid_lazy_hot_catchup, dim3(nbx_h > nbx_c ? nbx_h : nbx_c, nbx_c > 0 ? 64 : 32),
+                     dim3(kThreads), 0, stream, H, C, nbx_h, nbx_c);
   return NTC_LAUNCH_STATUS();
 }
 

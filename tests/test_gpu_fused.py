@@ -457,7 +457,7 @@ def test_grid_update_parts_match_tiles_update(case):
         assert np.array_equal(runs["cold_only"][k][3], init[k][3])
 
 
-@pytest.mark.parametrize("pipelined", [False, True])
+@pytest.mark.parametrize("pipelined", [False, True, "merged"])
 @pytest.mark.parametrize("case", list(_PART_CASES))
 def test_grid_lazy_matches_eager(case, pipelined):
     """Twelve steps with fresh tile origins, gradients and Adam constants per
@@ -466,8 +466,10 @@ def test_grid_lazy_matches_eager(case, pipelined):
     there, the other cells' zero-gradient steps applied only at a flush).
     Pipelined: the catch-up of step k + 1 leaves out step k's rectangles and
     runs before step k's hot part (as it may beside it); never across a
-    flush. After a flush at step 7 and at the end, p, m, v and the gradient
-    agree bit for bit, and the cell counters hold the step count."""
+    flush. Merged: step k's hot part and step k + 1's catch-up in one launch
+    (ntc_grid_lazy_hot_catchup_f32). After a flush at step 7 and at the end,
+    p, m, v and the gradient agree bit for bit, and the cell counters hold
+    the step count."""
     D0, D1, C0, C1, M, T, org = _PART_CASES[case]
     n_tiles, K, flush_at = len(org) // 2, 12, 7
     lib = G.lib()
@@ -523,13 +525,23 @@ def test_grid_lazy_matches_eager(case, pipelined):
                                            G.p(eager[1][0]), G.p(eager[1][3]), G.byref(ad), None), "eager")
         if issued != k:
             catch_up(k, None)
-        if pipelined and k + 1 < K and k + 1 != flush_at:
+        ahead = pipelined and k + 1 < K and k + 1 != flush_at
+        if ahead and pipelined != "merged":
             catch_up(k + 1, tls[k])
             issued = k + 1
         ad = adam(lazy, k)
-        G.ok(lib.ntc_grid_lazy_hot_f32(G.byref(fdc), G.byref(tls[k]), G.p(lazy[0][0]), G.p(lazy[0][3]),
-                                       G.p(lazy[1][0]), G.p(lazy[1][3]), G.byref(ad), k, G.p(last[0]),
-                                       G.p(last[1]), None), "hot")
+        if ahead and pipelined == "merged":
+            ad0 = adam(lazy, 0)
+            G.ok(lib.ntc_grid_lazy_hot_catchup_f32(
+                G.byref(fdc), G.byref(tls[k]), G.p(lazy[0][0]), G.p(lazy[0][3]), G.p(lazy[1][0]),
+                G.p(lazy[1][3]), G.byref(ad), k, G.p(last[0]), G.p(last[1]), G.byref(fdc), G.byref(tls[k + 1]),
+                G.byref(tls[k]), G.p(lazy[0][0]), G.p(lazy[1][0]), G.byref(ad0), G.p(tab), k + 1, G.p(last[0]),
+                G.p(last[1]), None), "hot + catch-up")
+            issued = k + 1
+        else:
+            G.ok(lib.ntc_grid_lazy_hot_f32(G.byref(fdc), G.byref(tls[k]), G.p(lazy[0][0]), G.p(lazy[0][3]),
+                                           G.p(lazy[1][0]), G.p(lazy[1][3]), G.byref(ad), k, G.p(last[0]),
+                                           G.p(last[1]), None), "hot")
         if k + 1 == flush_at:
             flush(k + 1)
             compare(k + 1)
