@@ -382,19 +382,21 @@ __device__ __forceinline__ void lazy_catchup_rects_body(const LazyArgs& L, int b
   const int grid = slot >= nt, b = grid ? slot - nt : slot;
   const int4 r = rect[grid][b];
   const int D = A.D[grid], Q = A.Q[grid];
-  const int64_t items = (int64_t)r.y * r.w * Q;
+  // below 2^31 items (the grids' quads: checked on the host), 32-bit index
+  // arithmetic
+  const uint32_t items = (uint32_t)r.y * (uint32_t)r.w * (uint32_t)Q;
   // uniform trip count: every lane of a wave takes part in lazy_run
-  for (int64_t i0 = (int64_t)bx * kThreads; i0 < items; i0 += (int64_t)nbx * kThreads)
+  for (uint32_t i0 = (uint32_t)bx * kThreads; i0 < items; i0 += (uint32_t)nbx * kThreads)
   {
-    const int64_t i = i0 + threadIdx.x;
+    const uint32_t i = i0 + threadIdx.x;
     bool live = i < items;
     int64_t k = 0;
     int from = L.upto;
     if (live)
     {
-      const int q = (int)(i % Q);
-      const int c = (int)(i / Q);
-      const int iy = c / r.y, ix = c - iy * r.y;
+      const uint32_t c = i / (uint32_t)Q;
+      const int q = (int)(i - c * (uint32_t)Q);
+      const int iy = (int)(c / (uint32_t)r.y), ix = (int)c - iy * r.y;
       int x = r.x + ix, y = r.z + iy;
       x -= x >= D ? D : 0;
       y -= y >= D ? D : 0;
@@ -445,13 +447,13 @@ __device__ __forceinline__ void lazy_hot_body(const LazyArgs& L, int bx, int nbx
   const int4 r = rect[grid][b];
   const int D = A.D[grid], Q = A.Q[grid];
   const GridSeg& s = A.s[grid];
-  const int64_t items = (int64_t)r.y * r.w * Q;
+  const uint32_t items = (uint32_t)r.y * (uint32_t)r.w * (uint32_t)Q;  // below 2^31, as above
   const f4 zero = {0.f, 0.f, 0.f, 0.f};
-  for (int64_t i = (int64_t)bx * kThreads + threadIdx.x; i < items; i += (int64_t)nbx * kThreads)
+  for (uint32_t i = (uint32_t)bx * kThreads + threadIdx.x; i < items; i += (uint32_t)nbx * kThreads)
   {
-    const int q = (int)(i % Q);
-    const int c = (int)(i / Q);
-    const int iy = c / r.y, ix = c - iy * r.y;
+    const uint32_t c = i / (uint32_t)Q;
+    const int q = (int)(i - c * (uint32_t)Q);
+    const int iy = (int)(c / (uint32_t)r.y), ix = (int)c - iy * r.y;
     int x = r.x + ix, y = r.z + iy;
     x -= x >= D ? D : 0;
     y -= y >= D ? D : 0;
@@ -782,6 +784,8 @@ static int lazy_args_of(const ntc_feature_desc* fd, const ntc_tiles* tiles, floa
   ntc_tiles all = {kNoOrigin, 1, 1, 1, 0.0f};
   const int st = tile_update_of(fd, tiles ? tiles : &all, g0, dg0, g1, dg1, adam, L.A);
   if (st != NTC_OK) return st;
+  // the rectangle walks index a grid's quads in 32 bits
+  NTC_RETURN_IF(L.A.n4 >= ((int64_t)1 << 31), NTC_E_INVALID);
   L.tab = tab;
   L.b1 = adam->beta1;
   L.b2 = adam->beta2;
