@@ -667,6 +667,16 @@ __device__ __forceinline__ void gather_staged(const FusedParams& P, const PassGe
       xnf[t * CF::XS + slot] = b;
       xfn[slot * CF::PASS + t] = b;
     };
+    // four consecutive slots from slot0 (a multiple of 4): one 8-byte write
+    // into the texel's X row (rows of XS elements, XS a multiple of 4)
+    auto put4 = [&](int slot0, float a, float b, float c, float d) {
+      const uint32_t lo = pk_bf16(a, b), hi = pk_bf16(c, d);
+      *reinterpret_cast<uint2*>(&xnf[t * CF::XS + slot0]) = make_uint2(lo, hi);
+      xfn[slot0 * CF::PASS + t] = (uint16_t)lo;
+      xfn[(slot0 + 1) * CF::PASS + t] = (uint16_t)(lo >> 16);
+      xfn[(slot0 + 2) * CF::PASS + t] = (uint16_t)hi;
+      xfn[(slot0 + 3) * CF::PASS + t] = (uint16_t)(hi >> 16);
+    };
     if (qi < Q0 + Q1)
     {
       const bool g1 = qi >= Q0;
@@ -688,10 +698,15 @@ __device__ __forceinline__ void gather_staged(const FusedParams& P, const PassGe
 #pragma unroll
         for (int c = 0; c < 4; ++c)
         {
-          if (4 * q + 0 < C0) put(c * C0 + 4 * q + 0, val[c].x);
-          if (4 * q + 1 < C0) put(c * C0 + 4 * q + 1, val[c].y);
-          if (4 * q + 2 < C0) put(c * C0 + 4 * q + 2, val[c].z);
-          if (4 * q + 3 < C0) put(c * C0 + 4 * q + 3, val[c].w);
+          if (C0 % 4 == 0)
+            put4(c * C0 + 4 * q, val[c].x, val[c].y, val[c].z, val[c].w);
+          else
+          {
+            if (4 * q + 0 < C0) put(c * C0 + 4 * q + 0, val[c].x);
+            if (4 * q + 1 < C0) put(c * C0 + 4 * q + 1, val[c].y);
+            if (4 * q + 2 < C0) put(c * C0 + 4 * q + 2, val[c].z);
+            if (4 * q + 3 < C0) put(c * C0 + 4 * q + 3, val[c].w);
+          }
         }
       }
       else
@@ -706,9 +721,12 @@ __device__ __forceinline__ void gather_staged(const FusedParams& P, const PassGe
           acc[2] = fmaf(w[c], val[c].z, acc[2]);
           acc[3] = fmaf(w[c], val[c].w, acc[3]);
         }
+        if (C0 % 4 == 0 && C1 % 4 == 0 && CF::G1B % 4 == 0)
+          put4(base, acc[0], acc[1], acc[2], acc[3]);
+        else
 #pragma unroll
-        for (int i = 0; i < 4; ++i)
-          if (4 * q + i < C) put(base + i, acc[i]);
+          for (int i = 0; i < 4; ++i)
+            if (4 * q + i < C) put(base + i, acc[i]);
       }
     }
     else
