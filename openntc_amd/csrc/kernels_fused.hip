@@ -668,14 +668,21 @@ __device__ __forceinline__ void gather_staged(const FusedParams& P, const PassGe
       xfn[slot * CF::PASS + t] = b;
     };
     // four consecutive slots from slot0 (a multiple of 4): one 8-byte write
-    // into the texel's X row (rows of XS elements, XS a multiple of 4)
+    // into the texel's X row (rows of XS elements, XS a multiple of 4), and
+    // 32-bit writes of texel pairs into the slot rows of X^T: texels t and
+    // t ^ 1 sit in neighbouring lanes of one wave (all 64 lanes run every
+    // item of a wave), which swap dwords with a DPP quad permutation; the
+    // even lane writes slot rows slot0 and slot0 + 2, the odd lane the others
     auto put4 = [&](int slot0, float a, float b, float c, float d) {
       const uint32_t lo = pk_bf16(a, b), hi = pk_bf16(c, d);
       *reinterpret_cast<uint2*>(&xnf[t * CF::XS + slot0]) = make_uint2(lo, hi);
-      xfn[slot0 * CF::PASS + t] = (uint16_t)lo;
-      xfn[(slot0 + 1) * CF::PASS + t] = (uint16_t)(lo >> 16);
-      xfn[(slot0 + 2) * CF::PASS + t] = (uint16_t)hi;
-      xfn[(slot0 + 3) * CF::PASS + t] = (uint16_t)(hi >> 16);
+      const int odd = t & 1;
+      const uint32_t sel = odd ? 0x03020706u : 0x05040100u;
+      uint32_t* x32 = reinterpret_cast<uint32_t*>(xfn);
+      const uint32_t lo_n = (uint32_t)__builtin_amdgcn_mov_dpp((int)lo, 0xB1, 0xF, 0xF, false);
+      const uint32_t hi_n = (uint32_t)__builtin_amdgcn_mov_dpp((int)hi, 0xB1, 0xF, 0xF, false);
+      x32[((slot0 + odd) * CF::PASS + (t & ~1)) >> 1] = __builtin_amdgcn_perm(lo_n, lo, sel);
+      x32[((slot0 + 2 + odd) * CF::PASS + (t & ~1)) >> 1] = __builtin_amdgcn_perm(hi_n, hi, sel);
     };
     if (qi < Q0 + Q1)
     {
