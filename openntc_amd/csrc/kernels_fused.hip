@@ -739,7 +739,19 @@ __device__ __forceinline__ void gather_staged(const FusedParams& P, const PassGe
     else
     {
       const int xu = pg.xu0 + x, yu = pg.yu0 + y;
-      gather_rest<CF, PEC, LODS>(P, xu >= M ? xu - M : xu, yu >= M ? yu - M : yu, put);
+      if constexpr (C0 % 4 == 0 && C1 % 4 == 0 && CF::G1B % 4 == 0)
+      {
+        // the remaining slots (positional encoding, lod, zero padding) in
+        // groups of four
+        constexpr int NR = CF::KF * 32 - CF::GSLOTS;
+        float v[NR];
+        gather_rest<CF, PEC, LODS>(P, xu >= M ? xu - M : xu, yu >= M ? yu - M : yu,
+                                   [&](int slot, float f) { v[slot - CF::GSLOTS] = f; });
+#pragma unroll
+        for (int r = 0; r < NR; r += 4) put4(CF::GSLOTS + r, v[r], v[r + 1], v[r + 2], v[r + 3]);
+      }
+      else
+        gather_rest<CF, PEC, LODS>(P, xu >= M ? xu - M : xu, yu >= M ? yu - M : yu, put);
     }
   }
 }
