@@ -1174,6 +1174,17 @@ __device__ __forceinline__ void fused_body(const FusedParams& P, const int vblk,
           if (j == 0) dx = f32x4{0, 0, 0, 0};
           dx = mfma(a, dz0f[j][nt], dx);
           if (j != KH - 1 || !tv) return;
+          // slots 16 r + 4 g .. + 3 lie in one grid's range when its slot
+          // counts are multiples of 4: one 16-byte write of the texel's dX row
+          if constexpr (CF::G1B % 4 == 0 && CF::GSLOTS % 4 == 0)
+          {
+            const int s0 = 16 * r + 4 * g;
+            if ((s0 < CF::G1B && bx0.fits) || (s0 >= CF::G1B && s0 < CF::GSLOTS && bx1.fits))
+            {
+              *reinterpret_cast<f32x4*>(drow + s0) = dx;
+              return;
+            }
+          }
 #pragma unroll
           for (int i = 0; i < 4; ++i)
           {
