@@ -574,3 +574,201 @@ def test_decode_int8_matches_hlsl_restatement(oracle, act):
     ref = oracle.decode(0, act, F, Fp, H, O, x, dec=np.concatenate([w0.ravel(), w1.ravel(), np.concatenate(
         [wo, np.zeros((16 - O, H), np.float32)]).ravel()]))
     assert np.max(np.abs(got - ref)) < 0.1 * np.max(np.abs(ref))
+
+
+# ---- parameter init (libopenntc.cpp:60-78, 132-152, 483-501) --------------------
+class _PyMT:
+    """std::mt19937 written out here from the published algorithm
+    (Matsumoto-Nishimura 1998; init_genrand seeding), independent of the
+    oracle's C generator."""
+
+    def __init__(self, seed):
+        self.mt = [0] * 624
+        self.mt[0] = seed & 0xFFFFFFFF
+        for i in range(1, 624):
+            self.mt[i] = (1812433253 * (self.mt[i - 1] ^ (self.mt[i - 1] >> 30)) + i) & 0xFFFFFFFF
+        self.i = 624
+
+    def next(self):
+        if self.i >= 624:
+            for k in range(624):
+                y = (self.mt[k] & 0x80000000) | (self.mt[(k + 1) % 624] & 0x7FFFFFFF)
+                self.mt[k] = self.mt[(k + 397) % 624] ^ (y >> 1) ^ (0x9908B0DF if y & 1 else 0)
+            self.i = 0
+        y = self.mt[self.i]
+        self.i += 1
+        y ^= y >> 11
+        y ^= (y << 7) & 0x9D2C5680
+        y ^= (y << 15) & 0xEFC60000
+        y ^= y >> 18
+        return y
+
+
+def test_python_mt19937_known_answer():
+    m = _PyMT(5489)
+    for _ in range(9999):
+        m.next()
+    assert m.next() == 4123659995
+
+
+def _draws(m, n, lo, hi):
+    # std::uniform_real_distribution<float>(lo, hi) over 24-bit uniforms:
+    # lo + (hi - lo) * u in float32 (the oracle's stated definition of an
+    # implementation-defined distribution)
+    f = np.float32
+    u = np.array([(m.next() >> 8) for _ in range(n)], np.float64) * 2.0 ** -24
+    return (f(lo) + (f(hi) - f(lo)) * u.astype(np.float32)).astype(np.float32)
+
+
+def test_init_params_draw_order_ranges_and_fan_in(oracle):
+    """Replays BeginTraining's draws with an independent generator: per level
+    g0 then g1 (libopenntc.cpp:485-493), then W0, W1, Wout (:496-504), each
+    Kaiming-uniform with bound 1/sqrt(row length) (:60-78). A swapped order,
+    levels drawn grid-major, or a fan-in taken from the wrong axis fails."""
+    L = oracle.layout_custom(4, 2, 4, 8, 2, 64, 1, 1, 64, out_dim=7, mip_min_dim=4)
+    assert L.level_count >= 2
+    H, F = L.hidden_dim, L.feature_dim
+    P = oracle.init_params(oracle.MT(99), L, -0.5, 0.5)
+    m = _PyMT(99)
+    for lvl in range(L.level_count):
+        n0 = L.g0_dim[lvl] ** 2 * L.g0_channels
+        n1 = L.g1_dim[lvl] ** 2 * L.g1_channels
+        assert np.array_equal(P.g0[lvl].ravel(), _draws(m, n0, -0.5, 0.5)), f"g0 level {lvl}"
+        assert np.array_equal(P.g1[lvl].ravel(), _draws(m, n1, -0.5, 0.5)), f"g1 level {lvl}"
+    b0 = np.float32(1.0) / np.sqrt(np.float32(F))
+    b1 = np.float32(1.0) / np.sqrt(np.float32(H))
+    w0 = P.dec[: H * F]
+    w1 = P.dec[H * F: H * F + H * H]
+    wo = P.dec[H * F + H * H:]
+    assert np.array_equal(w0, _draws(m, H * F, -b0, b0))
+    assert np.array_equal(w1, _draws(m, H * H, -b1, b1))
+    assert np.array_equal(wo, _draws(m, 16 * H, -b1, b1))
+    # the bounds are the fan-in ones: each tensor fills its own range
+    for w, b in ((w0, b0), (w1, b1), (wo, b1)):
+        assert np.all(np.abs(w) <= b) and np.max(np.abs(w)) > 0.97 * b
+    assert F != H and abs(b0 - b1) > 0.05  # the two bounds are told apart
+    # Adam moments start at zero
+    assert not np.any(P.dec_m) and not np.any(P.dec_v)
+    assert all(not np.any(a) for a in P.g0m + P.g0v + P.g1m + P.g1v)
+    # and the next draw after the init is the generator's next output
+    m2 = oracle.MT(99)
+    oracle.init_params(m2, L, -0.5, 0.5)
+    assert m2.next() == m.next()
+
+
+# ---- calibration (libopenntc.cpp:898-970) ----------------------------------------
+def _hardgelu64(z):
+    return np.where(z < -1.5, 0.0, np.where(z < 1.5, (z / 3.0) * (z + 1.5), z))
+
+
+def _act64(act, z):
+    return np.maximum(z, 0.0) if act == 1 else _hardgelu64(z)
+
+
+@pytest.mark.parametrize("act", [0, 1])
+def test_calibrate_max_abs_vs_float64_walk(oracle, act):
+    """max |A0| and max |A1| over every texel of every mip against a numpy
+    FP64 forward over each whole mip (gathered as one mip-sized tile). The
+    decoder is built so that each maximum sits where a plausible walk mistake
+    misses it: max |A0| in the last tile of mip 0's batch (a latent bump read
+    only there at full strength), max |A1| in the last mip (a unit driven by
+    the lod slot), and a unit whose pre-activation reaches -2x the maximum
+    (the maximum of the pre-activations would differ)."""
+    L = oracle.layout_custom(4, 4, 4, 8, 2, 64, 0, act, 512, mip_min_dim=4)
+    H, F = L.hidden_dim, L.feature_dim
+    assert L.lod_slot == 1 and L.mip_count == 8
+    lod = 4 * L.g0_channels + L.g1_channels + 12
+    P = oracle.init_params(oracle.MT(5), L, -0.5, 0.5)
+    P.g0[0][200, 200, 0] = 30.0  # corner 0 of texel (401, 401): mip 0, last tile
+    rng = np.random.default_rng(6)
+    w0 = (rng.uniform(-1, 1, (H, F)) * 0.05).astype(np.float32)
+    w0[0, 0], w0[0, lod] = 1.0, -100.0   # +30 at mip 0, at most ~16 at mip 1
+    w0[1, lod] = 8.0                     # largest at the last mip (lod feature 1)
+    w0[2, 0] = -2.0                      # pre-activation -60 at the bump
+    w1 = (rng.uniform(-1, 1, (H, H)) * 0.01).astype(np.float32)
+    w1[0, 1] = 5.0                       # A1 unit 0 follows A0 unit 1
+    P.dec[: H * F] = w0.ravel()
+    P.dec[H * F: H * F + H * H] = w1.ravel()
+    tr = oracle.Trainer(L, P, [np.zeros((L.mip_dim[m], L.mip_dim[m], L.out_dim), np.uint8)
+                               for m in range(L.mip_count)], max_batch=4 * 65536)
+    got = tr.calibrate()
+    w0d, w1d = w0.astype(np.float64), w1.astype(np.float64)
+    mx = np.zeros((L.mip_count, 2))
+    zmax = 0.0
+    for mip in range(L.mip_count):
+        M = L.mip_dim[mip]
+        lvl = oracle.level_for_lod(L, mip)
+        fd = oracle.feature_desc(L, lvl)
+        x = oracle.draw_features(fd, np.array([0, 0], np.int32), M, M, mip / (L.mip_count - 1),
+                                 P.g0[lvl], P.g1[lvl], dtype=np.float64)
+        z0 = w0d @ x
+        a0 = _act64(act, z0)
+        a1 = _act64(act, w1d @ a0)
+        mx[mip] = np.abs(a0).max(), np.abs(a1).max()
+        zmax = max(zmax, np.abs(z0).max())
+    want = mx.max(axis=0)
+    assert np.argmax(mx[:, 0]) == 0 and np.argmax(mx[:, 1]) == L.mip_count - 1
+    assert mx[1:, 0].max() < 0.6 * want[0] and mx[:-1, 1].max() < 0.9 * want[1] and zmax > 1.9 * want[0]
+    assert np.allclose(got, want, rtol=2e-5, atol=0), (got, want)
+
+
+# ---- texture packaging (ntc_kernel.cu:533-567, 665-686; libopenntc.cpp:1195-1199, 1324-1362)
+def test_build_package_channel_map_sort_and_interleave(oracle):
+    """Hand-built sources whose RGBA bytes name their source, channel and
+    position. The package must hold, per texel, the selected channels of each
+    source in channel-string order, the sources stably sorted by semantic
+    (Albedo 1 < Metallic 6 < Normal 7 < Roughness 9; the two Metallic sources
+    keep their manifest order), and each coarser mip the per-channel Lanczos
+    filter of the one above."""
+    L = oracle.layout_custom(4, 4, 4, 8, 2, 64, 1, 1, 16, out_dim=9, mip_min_dim=4)
+    dim = 16
+    yy, xx = np.meshgrid(np.arange(dim), np.arange(dim), indexing="ij")
+
+    def src(sid):
+        img = np.zeros((dim, dim, 4), np.uint8)
+        for c in range(4):
+            img[:, :, c] = (sid * 64 + c * 16 + (xx + 3 * yy) % 16).astype(np.uint8)
+        return img
+
+    # (rgba, semantic, channel string) in manifest order
+    sources = [(src(0), 7, "RGB"),   # Normal
+               (src(1), 6, "B"),     # Metallic #1
+               (src(2), 1, "BGR"),   # Albedo, reversed channel order
+               (src(3), 9, "A"),     # Roughness
+               (src(1), 6, "R")]     # Metallic #2 (same image, other channel)
+    pk = oracle.build_package(sources, L)
+    # expected out channels: (source id, RGBA index)
+    order = [(2, 2), (2, 1), (2, 0), (1, 2), (1, 0), (0, 0), (0, 1), (0, 2), (3, 3)]
+    assert pk[0].shape == (dim, dim, len(order))
+    for o, (sid, c) in enumerate(order):
+        want = (sid * 64 + c * 16 + (xx + 3 * yy) % 16).astype(np.uint8)
+        assert np.array_equal(pk[0][:, :, o], want), o
+    assert len(pk) == L.mip_count
+    for m in range(1, L.mip_count):
+        M = L.mip_dim[m]
+        assert pk[m].shape == (M, M, len(order))
+        for o, (sid, c) in enumerate(order):
+            plane = pk[m - 1][:, :, o:o + 1]
+            assert np.array_equal(pk[m][:, :, o], oracle.filter_lanczos(plane)[:, :, 0]), (m, o)
+
+
+# ---- FP64 decode (the error oracle of every evaluation precision) ---------------------
+@pytest.mark.parametrize("act", [0, 1])
+def test_decode_f64_vs_numpy_chain(oracle, act):
+    """Wout[:O] act(W1 act(W0 x)) in numpy float64 (HardGELU from
+    ntc_kernel.cu:13-20). Rows of Wout from O on must not reach the output;
+    pre-activations straddle the HardGELU knees at +-1.5."""
+    F, H, O, B = 36, 64, 9, 301
+    rng = np.random.default_rng(40 + act)
+    w0 = rng.uniform(-1, 1, (H, F)) * 0.6
+    w1 = rng.uniform(-1, 1, (H, H)) * 0.35
+    wo = rng.uniform(-1, 1, (16, H)) * 0.3
+    wo[O:] = 1e6  # never read
+    x = rng.uniform(-1, 1, (F, B))
+    dec = np.concatenate([w0.ravel(), w1.ravel(), wo.ravel()])
+    got = oracle.decode_f64(act, F, H, O, x, dec)
+    z0 = w0 @ x
+    assert np.mean(np.abs(z0) > 1.5) > 0.05 and np.mean(np.abs(z0) < 1.5) > 0.3
+    want = wo[:O] @ _act64(act, w1 @ _act64(act, z0))
+    assert got.shape == (O, B)
+    assert np.allclose(got, want, rtol=1e-12, atol=1e-12)
