@@ -657,6 +657,40 @@ int ntc_image_decode_rgba8(const char* path, int32_t* width, int32_t* height,
 int ntc_synthetic_pbr(int32_t dim, uint32_t seed, uint8_t* albedo,
                       uint8_t* normal, uint8_t* orm);
 
+/* ---- ONTC container, GPU-free ---------------------------------------------
+ * The reference's free functions Dump, Load, FillNTCConstants and
+ * FileData::Data (openntc_runtime.h:123-150, openntc_runtime.cpp:156-468)
+ * for FFI callers. ntc_compressed carries the fields of CompressedData
+ * (openntc_runtime.h:91-121) as plain C types; its blob pointers are not
+ * owned. Status values are openntc::Result. */
+typedef struct ntc_compressed
+{
+  const void* g0[6];
+  const void* g1[6];
+  const void* decoder;
+  uint64_t g0_size[6], g1_size[6], g0_offset[6], g1_offset[6];
+  uint64_t g0_total_size, g1_total_size, decoder_size;
+  int32_t g0_grid_dim[6], g1_grid_dim[6];
+  int32_t g0_bits, g1_bits, g0_channels, g1_channels;
+  int32_t dim, mip_count, level_count, channel_count;
+  int32_t channel_semantics[16];
+  int32_t profile;
+  float max_abs_a1, max_abs_a2, s_a1, s_a2; /* CalibrationData */
+} ntc_compressed;
+
+typedef struct ntc_file ntc_file; /* openntc::FileData */
+
+/* Dump (openntc_runtime.cpp:179-253). */
+int ntc_container_dump(const char* path, const ntc_compressed* data);
+/* Load (openntc_runtime.cpp:307-456): on success *out owns the file's bytes
+ * and stays valid until ntc_file_free; on failure *out is NULL. */
+int ntc_container_load(const char* path, ntc_file** out);
+/* FileData::Data (openntc_runtime.cpp:465-468): pointers into the file. */
+int ntc_file_data(const ntc_file* f, ntc_compressed* out);
+void ntc_file_free(ntc_file* f);
+/* FillNTCConstants (openntc_runtime.cpp:156-177) into 240 bytes. */
+int ntc_fill_constants(const ntc_compressed* data, void* constants240);
+
 #ifdef __cplusplus
 }
 #endif

@@ -102,6 +102,21 @@ class ManifestSource(C.Structure):
                 ("semantic", C.c_int32), ("is_srgb", C.c_int32), ("vertical_flip", C.c_int32)]
 
 
+class Compressed(C.Structure):
+    """ntc_compressed: the fields of openntc::CompressedData as plain C types."""
+    _fields_ = [("g0", C.c_void_p * 6), ("g1", C.c_void_p * 6), ("decoder", C.c_void_p),
+                ("g0_size", C.c_uint64 * 6), ("g1_size", C.c_uint64 * 6),
+                ("g0_offset", C.c_uint64 * 6), ("g1_offset", C.c_uint64 * 6),
+                ("g0_total_size", C.c_uint64), ("g1_total_size", C.c_uint64),
+                ("decoder_size", C.c_uint64), ("g0_grid_dim", C.c_int32 * 6),
+                ("g1_grid_dim", C.c_int32 * 6), ("g0_bits", C.c_int32), ("g1_bits", C.c_int32),
+                ("g0_channels", C.c_int32), ("g1_channels", C.c_int32), ("dim", C.c_int32),
+                ("mip_count", C.c_int32), ("level_count", C.c_int32), ("channel_count", C.c_int32),
+                ("channel_semantics", C.c_int32 * 16), ("profile", C.c_int32),
+                ("max_abs_a1", C.c_float), ("max_abs_a2", C.c_float), ("s_a1", C.c_float),
+                ("s_a2", C.c_float)]
+
+
 class BatchStats(C.Structure):
     _fields_ = [("n_sets", C.c_int32), ("steps", C.c_int32), ("steps_done", C.c_int32),
                 ("pass_texels", C.c_int32), ("fused_grid", C.c_int32), ("grid_update_grid", C.c_int32),
@@ -224,6 +239,11 @@ SIGNATURES = {
     "ntc_batch_destroy": (None, [_P]),
     "ntc_manifest_parse": (_I, [C.c_char_p, _I, _P, _P, _P]),
     "ntc_image_decode_rgba8": (_I, [C.c_char_p, _P, _P, _P, _SZ]),
+    "ntc_container_dump": (_I, [C.c_char_p, _P]),
+    "ntc_container_load": (_I, [C.c_char_p, _P]),
+    "ntc_file_data": (_I, [_P, _P]),
+    "ntc_file_free": (None, [_P]),
+    "ntc_fill_constants": (_I, [_P, _P]),
 }
 
 _lib = None

@@ -3,7 +3,7 @@
 HIP sources (*.hip) go through hipcc with --offload-arch=gfx950; host sources
 (*.cpp) through g++ against the HIP runtime headers. Both with
 -ffp-contract=off: every fused multiply-add in the library is an explicit fma.
-Usage: python -m openntc_amd.build [-j N]
+Usage: python -m openntc_amd.build [-j N] [--incremental]
 """
 from __future__ import annotations
 
@@ -38,9 +38,20 @@ CXX_FLAGS = [
 ]
 
 
-def _compile(src: str) -> str:
+def _newest_header() -> float:
+    hdrs = [os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith(".h")]
+    for d, _, fs in os.walk(os.path.join(ROOT, "include")):
+        hdrs += [os.path.join(d, f) for f in fs if f.endswith(".h")]
+    return max(os.path.getmtime(h) for h in hdrs)
+
+
+def _compile(src: str, incremental: bool = False) -> str:
     path = os.path.join(CSRC, src)
     obj = os.path.join(OBJ, src + ".o")
+    if incremental and os.path.exists(obj):
+        t = os.path.getmtime(obj)
+        if t > os.path.getmtime(path) and t > _newest_header():
+            return obj
     if src.endswith(".hip"):
         cmd = [os.path.join(ROCM, "bin", "hipcc"), *HIPCC_FLAGS, *INCLUDES, "-c", path, "-o", obj]
     else:
@@ -51,11 +62,13 @@ def _compile(src: str) -> str:
     return obj
 
 
-def build(jobs: int = 8, verbose: bool = True) -> str:
+def build(jobs: int = 8, verbose: bool = True, incremental: bool = False) -> str:
+    """Compiles every source (incremental: only those older than their
+    object or than any header) and links the library."""
     os.makedirs(OBJ, exist_ok=True)
     sources = [s for s in HIP_SOURCES + CXX_SOURCES if os.path.exists(os.path.join(CSRC, s))]
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
-        objs = list(ex.map(_compile, sources))
+        objs = list(ex.map(lambda s: _compile(s, incremental), sources))
     tmp = LIB + ".tmp"
     cmd = [os.path.join(ROCM, "bin", "hipcc"), "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp,
            *objs, "-lpthread", "-lz"]
@@ -71,9 +84,10 @@ def build(jobs: int = 8, verbose: bool = True) -> str:
 if __name__ == "__main__":
     ap = argparse.ArgumentParser()
     ap.add_argument("-j", type=int, default=8)
+    ap.add_argument("--incremental", action="store_true")
     a = ap.parse_args()
     try:
-        build(a.j)
+        build(a.j, incremental=a.incremental)
     except RuntimeError as e:
         print(e, file=sys.stderr)
         sys.exit(1)

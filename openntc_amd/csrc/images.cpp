@@ -505,7 +505,7 @@ bool DecodeJpeg(const std::vector<uint8_t>& f, std::vector<uint8_t>& rgba, int& 
       J.height = Be16(s + 1);
       J.width = Be16(s + 3);
       const int nc = s[5];
-      if ((nc != 1 && nc != 3) || len < 8 + 3 * nc || J.width < 1 || J.height < 1) return false;
+      if ((nc != 1 && nc != 3) || len < 8 + 3 * nc || !SaneImageSize(J.width, J.height)) return false;
       J.comp.resize(nc);
       for (int i = 0; i < nc; ++i)
       {
@@ -691,9 +691,10 @@ bool DecodeBmp(const std::vector<uint8_t>& f, std::vector<uint8_t>& rgba, int& w
     compression = (int)Le32(d + 30);
   }
   if (compression != 0 && compression != 3) return false;  // run-length and other codings
+  if (height == INT32_MIN) return false;
   const bool top_down = height < 0;
   if (top_down) height = -height;
-  if (width < 1 || height < 1 || width > (1 << 16) || height > (1 << 16)) return false;
+  if (!SaneImageSize(width, height)) return false;
   uint32_t mr = 0, mg = 0, mb = 0, ma = 0;
   if (bpp == 16)
   {

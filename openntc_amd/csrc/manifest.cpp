@@ -181,7 +181,7 @@ bool DecodePng(const std::vector<uint8_t>& f, std::vector<uint8_t>& rgba, int& w
       iend = true;
     pos += 12 + len;
   }
-  if (!ihdr || p.w <= 0 || p.h <= 0 || p.w > 1 << 16 || p.h > 1 << 16) return false;
+  if (!ihdr || !img::SaneImageSize(p.w, p.h)) return false;
   const int d = p.depth;
   const bool depth_ok = (p.ctype == 0 && (d == 1 || d == 2 || d == 4 || d == 8 || d == 16)) ||
                         (p.ctype == 3 && (d == 1 || d == 2 || d == 4 || d == 8)) ||
@@ -202,6 +202,9 @@ bool DecodePng(const std::vector<uint8_t>& f, std::vector<uint8_t>& rgba, int& w
       const int pw = (p.w - ax0[k] + adx[k] - 1) / adx[k], ph = (p.h - ay0[k] + ady[k] - 1) / ady[k];
       if (pw > 0 && ph > 0) raw_size += (stride_of(pw) + 1) * (size_t)ph;
     }
+  // deflate expands at most 1032:1 (RFC 1951 stored / fixed-code bounds):
+  // data claiming more cannot be in the file
+  if (raw_size > 1032 * (uint64_t)z.size() + 64) return false;
   std::vector<uint8_t> raw(raw_size);
   z_stream zs = {};
   if (inflateInit(&zs) != Z_OK) return false;
@@ -249,15 +252,17 @@ bool DecodeTga(const std::vector<uint8_t>& f, std::vector<uint8_t>& rgba, int& w
   const bool rle = type == 10 || type == 11, grey = type == 3 || type == 11;
   if (cmtype > 1 || !(type == 2 || type == 3 || type == 10 || type == 11) || w <= 0 || h <= 0) return false;
   if (grey ? bits != 8 : (bits != 24 && bits != 32)) return false;
+  if (!img::SaneImageSize(w, h)) return false;
   const int bypp = bits / 8;
   size_t pos = 18 + (size_t)idlen + (cmtype ? (size_t)cmlen * ((cmbits + 7) / 8) : 0);
+  if (pos > f.size()) return false;
   const size_t n = (size_t)w * h;
+  // the file must hold the pixels (raw) or their packets (RLE: one header
+  // byte per at most 128 pixels) before anything is sized from the header
+  if (!rle ? f.size() - pos < n * bypp : n > 128 * (f.size() - pos)) return false;
   std::vector<uint8_t> px(n * bypp);
   if (!rle)
-  {
-    if (f.size() < pos + px.size()) return false;
     memcpy(px.data(), &f[pos], px.size());
-  }
   else
   {
     size_t k = 0;
@@ -331,7 +336,7 @@ bool DecodePnm(const std::vector<uint8_t>& f, std::vector<uint8_t>& rgba, int& w
   w = (int)vals[0];
   h = (int)vals[1];
   const long maxv = vals[2];
-  if (w <= 0 || h <= 0 || maxv <= 0 || maxv > 65535) return false;
+  if (!img::SaneImageSize(w, h) || maxv <= 0 || maxv > 65535) return false;
   const int bps = maxv > 255 ? 2 : 1;
   const size_t n = (size_t)w * h;
   if (f.size() < pos + n * nc * bps) return false;
@@ -356,14 +361,24 @@ bool DecodePnm(const std::vector<uint8_t>& f, std::vector<uint8_t>& rgba, int& w
 
 bool DecodeImage(const std::string& path, std::vector<uint8_t>& rgba, int& w, int& h)
 {
-  std::vector<uint8_t> f;
-  if (!ReadFile(path, f)) return false;
-  // the order stb_image tries formats in (those read here)
-  if (img::DecodeJpeg(f, rgba, w, h)) return true;
-  if (DecodePng(f, rgba, w, h)) return true;
-  if (img::DecodeBmp(f, rgba, w, h)) return true;
-  if (DecodePnm(f, rgba, w, h)) return true;
-  return DecodeTga(f, rgba, w, h);
+  // an allocation that fails fails the load (FileNotFound, as a null
+  // stbi_load does in the reference), never the process
+  try
+  {
+    std::vector<uint8_t> f;
+    if (!ReadFile(path, f)) return false;
+    // the order stb_image tries formats in (those read here)
+    if (img::DecodeJpeg(f, rgba, w, h)) return true;
+    if (DecodePng(f, rgba, w, h)) return true;
+    if (img::DecodeBmp(f, rgba, w, h)) return true;
+    if (DecodePnm(f, rgba, w, h)) return true;
+    return DecodeTga(f, rgba, w, h);
+  }
+  catch (const std::bad_alloc&)
+  {
+    rgba.clear();
+    return false;
+  }
 }
 
 // libopenntc.cpp:1103-1121

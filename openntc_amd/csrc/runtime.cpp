@@ -6,6 +6,7 @@
 // sorted order, then blobs g0_0.., g1_0.., decoder, each starting at a
 // multiple of 256 bytes from the end of the JSON.
 #include <libopenntc/openntc_runtime.h>
+#include <ntc_hip.h>
 
 #include "json.h"
 
@@ -338,7 +339,10 @@ Result Load(const std::string& path, FileData& o)
     if (!n || n->kind != Json::String || !of || of->kind != Json::Number || !of->is_int || !sz ||
         sz->kind != Json::Number || !sz->is_int || of->inum < 0 || sz->inum < 0)
       return Result::InvalidFile;
-    if (blob_off + (uint64_t)of->inum + (uint64_t)sz->inum > file_size) return Result::InvalidFile;
+    // offset and size checked one at a time against what the file holds
+    // after the JSON, so no sum of two 63-bit values can wrap
+    const uint64_t room = file_size - blob_off;
+    if ((uint64_t)of->inum > room || (uint64_t)sz->inum > room - (uint64_t)of->inum) return Result::InvalidFile;
     for (Slot& s : slots)
       if (n->str == s.name)
       {
@@ -382,3 +386,134 @@ FileData::~FileData()
 
 const CompressedData& FileData::Data() { return data_; }
 } // namespace openntc
+
+// ---- C ABI of the container (include/ntc_hip.h) ----------------------------------------------
+struct ntc_file
+{
+  openntc::FileData f;
+};
+
+namespace
+{
+void ToCompressed(const ntc_compressed& c, openntc::CompressedData& d)
+{
+  d = {};
+  for (int i = 0; i < openntc::kMaxLevels; ++i)
+  {
+    d.g0_[i] = (uint32_t*)c.g0[i];
+    d.g1_[i] = (uint32_t*)c.g1[i];
+    d.g0_size_[i] = c.g0_size[i];
+    d.g1_size_[i] = c.g1_size[i];
+    d.g0_offset_[i] = c.g0_offset[i];
+    d.g1_offset_[i] = c.g1_offset[i];
+    d.g0_grid_dim_[i] = c.g0_grid_dim[i];
+    d.g1_grid_dim_[i] = c.g1_grid_dim[i];
+  }
+  d.decoder_ = (void*)c.decoder;
+  d.g0_total_size_ = c.g0_total_size;
+  d.g1_total_size_ = c.g1_total_size;
+  d.decoder_size_ = c.decoder_size;
+  d.g0_bits_per_channel_ = c.g0_bits;
+  d.g1_bits_per_channel_ = c.g1_bits;
+  d.g0_channels_ = c.g0_channels;
+  d.g1_channels_ = c.g1_channels;
+  d.dim_ = c.dim;
+  d.mip_count_ = c.mip_count;
+  d.level_count_ = c.level_count;
+  d.channel_count_ = c.channel_count;
+  for (int i = 0; i < openntc::kMaxChannels; ++i) d.channel_semantics_[i] = (openntc::Semantic)c.channel_semantics[i];
+  d.profile_ = (openntc::Profile)c.profile;
+  d.caldata_ = {c.max_abs_a1, c.max_abs_a2, c.s_a1, c.s_a2};
+}
+
+void FromCompressed(const openntc::CompressedData& d, ntc_compressed& c)
+{
+  c = {};
+  for (int i = 0; i < openntc::kMaxLevels; ++i)
+  {
+    c.g0[i] = d.g0_[i];
+    c.g1[i] = d.g1_[i];
+    c.g0_size[i] = d.g0_size_[i];
+    c.g1_size[i] = d.g1_size_[i];
+    c.g0_offset[i] = d.g0_offset_[i];
+    c.g1_offset[i] = d.g1_offset_[i];
+    c.g0_grid_dim[i] = d.g0_grid_dim_[i];
+    c.g1_grid_dim[i] = d.g1_grid_dim_[i];
+  }
+  c.decoder = d.decoder_;
+  c.g0_total_size = d.g0_total_size_;
+  c.g1_total_size = d.g1_total_size_;
+  c.decoder_size = d.decoder_size_;
+  c.g0_bits = d.g0_bits_per_channel_;
+  c.g1_bits = d.g1_bits_per_channel_;
+  c.g0_channels = d.g0_channels_;
+  c.g1_channels = d.g1_channels_;
+  c.dim = d.dim_;
+  c.mip_count = d.mip_count_;
+  c.level_count = d.level_count_;
+  c.channel_count = d.channel_count_;
+  for (int i = 0; i < openntc::kMaxChannels; ++i) c.channel_semantics[i] = (int32_t)d.channel_semantics_[i];
+  c.profile = (int32_t)d.profile_;
+  c.max_abs_a1 = d.caldata_.max_abs_a1_;
+  c.max_abs_a2 = d.caldata_.max_abs_a2_;
+  c.s_a1 = d.caldata_.s_a1_;
+  c.s_a2 = d.caldata_.s_a2_;
+}
+} // namespace
+
+extern "C" {
+int ntc_container_dump(const char* path, const ntc_compressed* data)
+{
+  if (!path || !data) return (int)openntc::Result::InvalidState;
+  openntc::CompressedData d;
+  ToCompressed(*data, d);
+  try
+  {
+    return (int)openntc::Dump(path, d);
+  }
+  catch (const std::bad_alloc&)
+  {
+    return (int)openntc::Result::AllocationFailure;
+  }
+}
+
+int ntc_container_load(const char* path, ntc_file** out)
+{
+  if (!out) return (int)openntc::Result::InvalidState;
+  *out = nullptr;
+  if (!path) return (int)openntc::Result::FileNotFound;
+  try
+  {
+    std::unique_ptr<ntc_file> f(new ntc_file());
+    const openntc::Result r = openntc::Load(path, f->f);
+    if (r != openntc::Result::Success) return (int)r;
+    *out = f.release();
+    return 0;
+  }
+  catch (const std::bad_alloc&)
+  {
+    return (int)openntc::Result::AllocationFailure;
+  }
+}
+
+int ntc_file_data(const ntc_file* f, ntc_compressed* out)
+{
+  if (!f || !out) return (int)openntc::Result::InvalidState;
+  FromCompressed(const_cast<ntc_file*>(f)->f.Data(), *out);
+  return 0;
+}
+
+void ntc_file_free(ntc_file* f) { delete f; }
+
+int ntc_fill_constants(const ntc_compressed* data, void* constants240)
+{
+  if (!data || !constants240) return (int)openntc::Result::InvalidState;
+  openntc::CompressedData d;
+  ToCompressed(*data, d);
+  openntc::NTCConstants k;
+  openntc::FillNTCConstants(d, k);
+  std::memcpy(constants240, &k, sizeof(k));
+  return 0;
+}
+} // extern "C"
+

@@ -383,3 +383,41 @@ def test_truncated_and_unsupported_images_fail(tmp_path):
     rle = tmp_path / "rle.bmp"
     rle.write_bytes(bytes(bmp))
     assert decode(rle)[0] == FILE_NOT_FOUND
+
+
+def _peak_rss_kb():
+    import resource
+    return resource.getrusage(resource.RUSAGE_SELF).ru_maxrss
+
+
+@pytest.mark.parametrize("kind", ["tga", "tga_rle", "png", "jpeg", "jpeg_progressive", "bmp", "pnm"])
+def test_headers_claiming_huge_images_fail_without_allocating(tmp_path, kind):
+    """Short files whose headers claim 65535 x 65535 (or up to 2^31) pixels:
+    the decode fails (FileNotFound) before sizing anything from the header,
+    and the process does not grow by gigabytes (a 65535^2 RGBA image is 16 GiB)."""
+    p = tmp_path / f"huge.{kind}"
+    W = H = 65535
+    if kind.startswith("tga"):
+        hdr = bytearray(18)
+        hdr[2] = 10 if kind == "tga_rle" else 2
+        struct.pack_into("<HH", hdr, 12, W, H)
+        hdr[16], hdr[17] = 32, 0x20
+        p.write_bytes(bytes(hdr) + bytes(64))
+    elif kind == "png":
+        ihdr = struct.pack(">IIBBBBB", W, H, 8, 6, 0, 0, 0)
+        p.write_bytes(b"\x89PNG\r\n\x1a\n" + _chunk(b"IHDR", ihdr) + _chunk(b"IDAT", zlib.compress(bytes(1000)))
+                      + _chunk(b"IEND", b""))
+    elif kind.startswith("jpeg"):
+        sof = 0xC2 if kind == "jpeg_progressive" else 0xC0
+        body = struct.pack(">BHHB", 8, H, W, 3) + bytes([1, 0x22, 0, 2, 0x11, 1, 3, 0x11, 1])
+        p.write_bytes(b"\xff\xd8" + bytes([0xFF, sof]) + struct.pack(">H", len(body) + 2) + body + b"\xff\xd9")
+    elif kind == "bmp":
+        hdr = bytearray(54)
+        hdr[0:2] = b"BM"
+        struct.pack_into("<IIiiHHI", hdr, 10, 54, 40, 2 ** 31 - 1, -2 ** 31, 1, 24, 0)
+        p.write_bytes(bytes(hdr) + bytes(64))
+    else:
+        p.write_bytes(b"P6\n65535 65535\n255\n" + bytes(64))
+    before = _peak_rss_kb()
+    assert decode(p)[0] == FILE_NOT_FOUND
+    assert _peak_rss_kb() - before < 256 * 1024  # kB
