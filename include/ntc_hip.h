@@ -285,9 +285,8 @@ int ntc_fill_zero(void* p, size_t bytes, hipStream_t stream);
  * One persistent launch per step does the gather, the decoder forward, the
  * loss, the hand-written backward and the latent-gradient scatter
  * (replacing launch_draw_features .. launch_accumulate_grid_gradients and the
- * nine SGEMMs, libopenntc.cpp:621-709). It recomputes the forward, so it is
- * the backward half of the forward/backward pair; the forward half for
- * evaluation is the plain path. MFMA operands are BF16, accumulation FP32,
+ * nine SGEMMs, libopenntc.cpp:621-709); ntc_step_forward / ntc_step_backward
+ * below are the same step as a forward / backward pair. MFMA operands are BF16, accumulation FP32,
  * master latents and weights stay FP32. */
 typedef struct ntc_fused_args
 {
@@ -323,6 +322,30 @@ int ntc_pack_decoder_bf16(const ntc_feature_desc* fd, int32_t hidden_dim,
 /* tiles.tile_dim must be a power of two (NTC_E_INVALID otherwise), as every
  * mip side of a Context is. */
 int ntc_fused_train_step(const ntc_fused_args* args, hipStream_t stream);
+/* The same step as a forward / backward pair, split where the reference
+ * splits it, at the loss (forward: draw_features + 3 GEMMs + activations,
+ * libopenntc.cpp:621-655; backward: 6 GEMMs + activation backward +
+ * accumulate_grid_gradients, :672-709 and ntc_kernel.cu:261-373). Arguments
+ * as ntc_fused_train_step's.
+ * ntc_step_forward writes yhat [O][B] (FP32, columns in batch order: tile,
+ * then rows of the tile); with targets it also writes sse_partials (the SSE
+ * of the batch is their ordered sum); g0/g1 are only read.
+ * ntc_step_backward takes dyhat = dL/dYhat [O][B] in place of the loss
+ * gradient the training step forms from the targets (loss_scale and targets
+ * are not used), and writes the decoder-gradient partials and (scatter = 1)
+ * adds the latent gradients into dg0/dg1. Nothing is kept between the two
+ * calls: the backward gathers and runs the forward again (recompute policy).
+ * Given dyhat = loss_scale * (Y* - yhat) formed in FP32 from the forward's
+ * yhat, the pair gives the training step's results. */
+int ntc_step_forward(const ntc_fused_args* args, float* yhat, hipStream_t stream);
+int ntc_step_backward(const ntc_fused_args* args, const float* dyhat, hipStream_t stream);
+/* Ordered sum of the workgroup partials into grad [H*F + H*H + 16*H] (the
+ * decoder update's gradient, bit for bit, without the Adam step); sse_out
+ * (may be NULL) receives the ordered sum of sse_partials. */
+int ntc_reduce_decoder_partials(int32_t hidden_dim, int32_t feature_dim,
+                                int32_t blocks, const float* dw_partials,
+                                float* grad, const double* sse_partials,
+                                double* sse_out, hipStream_t stream);
 /* Sums the workgroup partials in order into grad, then Adam on W0 | W1 |
  * Wout rows < O (update_adam x3, libopenntc.cpp:722-754); sse_out (may be
  * NULL) receives the ordered sum of sse_partials. */

@@ -239,6 +239,9 @@ SIGNATURES = {
     "ntc_batch_destroy": (None, [_P]),
     "ntc_manifest_parse": (_I, [C.c_char_p, _I, _P, _P, _P]),
     "ntc_image_decode_rgba8": (_I, [C.c_char_p, _P, _P, _P, _SZ]),
+    "ntc_step_forward": (_I, [_P, _P, _P]),
+    "ntc_step_backward": (_I, [_P, _P, _P]),
+    "ntc_reduce_decoder_partials": (_I, [_I, _I, _I, _P, _P, _P, _P, _P]),
     "ntc_container_dump": (_I, [C.c_char_p, _P]),
     "ntc_container_load": (_I, [C.c_char_p, _P]),
     "ntc_file_data": (_I, [_P, _P]),

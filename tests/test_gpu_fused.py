@@ -142,7 +142,7 @@ def _step_case(oracle, kind, lod, n_tiles, dim=256, mip_min=4):
     a.g0, a.g1, a.weights, a.targets = G.p(d_g0), G.p(d_g1), G.p(tab), G.p(d_pkg)
     a.loss_scale = float(scale)
     a.scatter = 1
-    return dict(a=a, fdc=fdc, H=H, F=F, Od=Od, act=act, B=B, dec=dec, g0=g0, g1=g1, support=support,
+    return dict(a=a, fdc=fdc, H=H, F=F, Od=Od, act=act, B=B, dec=dec, g0=g0, g1=g1, pkg=pkg, support=support,
                 fd=fd, origins=origins, T=T, M=M, bdx=np.ascontiguousarray(bdx),
                 ref=(sse, dw0, dw1, dwout, rdg0, rdg1), bref=(bsse, bw0, bw1, bwo, bdg0, bdg1))
 
