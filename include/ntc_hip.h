@@ -346,6 +346,20 @@ int ntc_reduce_decoder_partials(int32_t hidden_dim, int32_t feature_dim,
                                 int32_t blocks, const float* dw_partials,
                                 float* grad, const double* sse_partials,
                                 double* sse_out, hipStream_t stream);
+/* ntc_fused_train_step followed by ntc_reduce_decoder_partials, with no
+ * workspace argument and no allocation: no counterpart in the reference; for
+ * callers that cannot allocate device memory (code under graph capture,
+ * serving loops, harnesses that forbid hipMalloc). args->dw_partials and
+ * args->sse_partials must be NULL; the partials live in a device array of
+ * the library, one per GPU, held until the process ends, of
+ * ntc_fused_train_step_noalloc_max() floats (256 x 32768): blocks * (decoder
+ * elements) above that, or blocks above 256, return NTC_E_WORKSPACE. Writes
+ * grad (the decoder gradient, as ntc_reduce_decoder_partials) and *sse_out
+ * (may be NULL); latent gradients as ntc_fused_train_step. Calls that
+ * overlap on one GPU share the array and race. */
+size_t ntc_fused_train_step_noalloc_max(void);
+int ntc_fused_train_step_noalloc(const ntc_fused_args* args, float* grad,
+                                 double* sse_out, hipStream_t stream);
 /* Sums the workgroup partials in order into grad, then Adam on W0 | W1 |
  * Wout rows < O (update_adam x3, libopenntc.cpp:722-754); sse_out (may be
  * NULL) receives the ordered sum of sse_partials. */

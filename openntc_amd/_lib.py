@@ -242,6 +242,8 @@ SIGNATURES = {
     "ntc_step_forward": (_I, [_P, _P, _P]),
     "ntc_step_backward": (_I, [_P, _P, _P]),
     "ntc_reduce_decoder_partials": (_I, [_I, _I, _I, _P, _P, _P, _P, _P]),
+    "ntc_fused_train_step_noalloc_max": (C.c_size_t, []),
+    "ntc_fused_train_step_noalloc": (_I, [_P, _P, _P, _P]),
     "ntc_container_dump": (_I, [C.c_char_p, _P]),
     "ntc_container_load": (_I, [C.c_char_p, _P]),
     "ntc_file_data": (_I, [_P, _P]),

@@ -8,6 +8,8 @@
 // dL/dYhat. Recompute policy: nothing is kept between the two calls; the
 // backward gathers and runs the forward again, which costs a third of the
 // step's MFMA work and no HBM for activations.
+#include <mutex>
+
 #include "fused_body.h"
 
 namespace
@@ -149,5 +151,70 @@ int ntc_reduce_decoder_partials(int32_t hidden_dim, int32_t feature_dim, int32_t
   hipLaunchKernelGGL(k_reduce_partials, dim3(blocks_for(n_dec, 64)), dim3(64), 0, stream, n_dec, blocks,
                      dw_partials, grad, sse_partials, sse_out);
   return NTC_LAUNCH_STATUS();
+}
+} // extern "C"
+
+// ---- the training step without a workspace argument ----------------------------------
+// For callers that cannot allocate device memory. The workgroup partials of
+// the decoder gradient and of the SSE live in one file-scope device array,
+// one copy per GPU, held until the process ends: kNoAllocFloats floats of
+// partials (256 workgroups x 32768 decoder elements, 32 MiB), then
+// kNoAllocBlocks doubles. Every call writes each partial slot it reads in
+// full, so nothing is cleared between calls; calls that overlap on one GPU
+// share the array and race.
+namespace
+{
+constexpr int64_t kNoAllocFloats = (int64_t)256 * 32768;
+constexpr int kNoAllocBlocks = 256;
+static __device__ __align__(256) float scratch[kNoAllocFloats + 2 * kNoAllocBlocks];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// The device address of the current GPU's copy of the array, looked up once
+// per device (the array's name is only a host-side stand-in).
+int scratch_for_current_device(void** out)
+{
+  int dev = 0;
+  NTC_HIP_TRY(hipGetDevice(&dev));
+  NTC_RETURN_IF(dev < 0 || dev >= kMaxDevices, NTC_E_INVALID);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev])
+  {
+    void* ptr = nullptr;
+    NTC_HIP_TRY(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    NTC_RETURN_IF(!ptr, NTC_E_INVALID);
+    g_scratch_ptr[dev] = ptr;
+  }
+  *out = g_scratch_ptr[dev];
+  return NTC_OK;
+}
+} // namespace
+
+extern "C" {
+size_t ntc_fused_train_step_noalloc_max(void) { return (size_t)kNoAllocFloats; }
+
+int ntc_fused_train_step_noalloc(const ntc_fused_args* a, float* grad, double* sse_out, hipStream_t stream)
+{
+  NTC_RETURN_IF(!a || !grad, NTC_E_INVALID);
+  NTC_RETURN_IF(a->dw_partials || a->sse_partials, NTC_E_INVALID);  // the partials are internal here
+  NTC_RETURN_IF(a->hidden_dim < 1 || a->feat.feature_dim < 1, NTC_E_INVALID);
+  const int64_t dec = (int64_t)a->hidden_dim * a->feat.feature_dim + (int64_t)a->hidden_dim * a->hidden_dim +
+                      16 * (int64_t)a->hidden_dim;
+  NTC_RETURN_IF(a->blocks < 1 || a->blocks > kNoAllocBlocks, NTC_E_WORKSPACE);
+  NTC_RETURN_IF((int64_t)a->blocks * dec > kNoAllocFloats, NTC_E_WORKSPACE);
+  void* base = nullptr;
+  const int st = scratch_for_current_device(&base);
+  if (st != NTC_OK) return st;
+  float* part = static_cast<float*>(base);
+  double* ssep = reinterpret_cast<double*>(part + kNoAllocFloats);
+  ntc_fused_args b = *a;
+  b.dw_partials = part;
+  b.sse_partials = ssep;
+  const int s1 = ntc_fused_train_step(&b, stream);
+  if (s1 != NTC_OK) return s1;
+  return ntc_reduce_decoder_partials(a->hidden_dim, a->feat.feature_dim, a->blocks, part, grad, ssep, sse_out,
+                                     stream);
 }
 } // extern "C"
