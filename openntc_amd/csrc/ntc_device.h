@@ -93,7 +93,9 @@ struct Footprint
 template <typename T>
 __device__ __forceinline__ void footprint_axis(int x, int M, int D, int& x0, T& f)
 {
-  const T u = ((T)x + (T)0.5) / (T)M;
+  // (x + 0.5) / M: for a power-of-two side (every mip side) the quotient is
+  // exact, so one exponent shift gives the division's bits
+  const T u = (M & (M - 1)) == 0 ? ldexp((T)x + (T)0.5, -__builtin_ctz((unsigned)M)) : ((T)x + (T)0.5) / (T)M;
   const T g = u * (T)D - (T)0.5;
   const T fl = floor(g);
   x0 = (int)fl;
@@ -205,7 +207,8 @@ template <typename T>
 __device__ __forceinline__ T tri_wave(int coord, int P, int h)
 {
   const T phase = h == 0 ? (T)0 : (T)0.25 * (T)P;
-  const T t = ((T)coord + (T)0.5 + phase) / (T)P;
+  const T t = (P & (P - 1)) == 0 ? ldexp((T)coord + (T)0.5 + phase, -__builtin_ctz((unsigned)P))
+                                  : ((T)coord + (T)0.5 + phase) / (T)P;  // exact for power-of-two periods
   const T s = t - floor(t);
   return (T)1 - (T)4 * fabs(s - (T)0.5);
 }
