@@ -284,7 +284,8 @@ def time_to_psnr_report(res, w, total):
 
 def eval_precisions(ntc, reps):
     """Full-chain evaluation of the trained evaluation workload at FP32, BF16,
-    FP8 E4M3 and INT8 (the exported decoder): per-precision wall time of one
+    FP8 E4M3, INT8 (the exported decoder) and BF16 on the matrix cores:
+    per-precision wall time of one
     Context::EvalEx walk (gather + targets + decode + FP64 SSE over all mips),
     decode throughput and PSNR overall and per channel group."""
     lib = ntc.load()
@@ -305,7 +306,8 @@ def eval_precisions(ntc, reps):
     texels = sum(lay.mip_dim[m] ** 2 for m in range(lay.mip_count))
     flop_per_texel = 2.0 * (F * H + H * H + H * O)
     res = {}
-    for prec in (ntc.EvalPrecision.FP32, ntc.EvalPrecision.BF16, ntc.EvalPrecision.FP8, ntc.EvalPrecision.INT8):
+    for prec in (ntc.EvalPrecision.FP32, ntc.EvalPrecision.BF16, ntc.EvalPrecision.FP8, ntc.EvalPrecision.INT8,
+                 ntc.EvalPrecision.BF16_MFMA):
         ctx.EvalEx(prec)  # untimed: builds the precision's tables (FP8 runs its calibration walk)
         lib.ntc_dev_synchronize()
         t0 = time.perf_counter()
@@ -316,7 +318,7 @@ def eval_precisions(ntc, reps):
         res[prec.name] = {"eval_ms": dt * 1e3, "texels_per_s": texels / dt, "psnr": ev.psnr,
                           "psnr_groups": group_psnr(ev.channel_sse, texels)}
         log(f"  {prec.name}: {dt * 1e3:.1f} ms per walk, psnr {ev.psnr:.4f}")
-    out = {"metric": "evaluation decode over all mips, 4096^2 9-ch (FP32 / BF16 / FP8 E4M3 / INT8)",
+    out = {"metric": "evaluation decode over all mips, 4096^2 9-ch (FP32 / BF16 / FP8 E4M3 / INT8 / BF16 MFMA)",
            "unit": "texels/s", "texels_per_walk": texels, "flop_per_texel": flop_per_texel, "reps": reps,
            "train_s": t_train,
            "config": {"workload": "4096^2 9-ch synthetic set; g0 12ch@2048^2 4-bit, g1 16ch@1024^2 2-bit; "

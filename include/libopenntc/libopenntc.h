@@ -212,6 +212,7 @@ enum class EvalPrecision : int32_t
   BF16 = 1,
   FP8 = 2,
   INT8 = 3,
+  BF16_MFMA = 4,  // BF16 rounding points, MFMA accumulation order (the fast BF16 decode)
 };
 
 struct LayoutInfo

@@ -84,6 +84,7 @@ class EvalPrecision(enum.IntEnum):
     BF16 = 1
     FP8 = 2
     INT8 = 3
+    BF16_MFMA = 4
 
 
 @dataclass

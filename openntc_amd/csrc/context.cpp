@@ -984,7 +984,7 @@ EvalResults Context::Impl::Evaluate(const Context& ctx, EvalPrecision p, EvalDet
 {
   err = Result::Success;
   const int prec = (int)p;
-  if (prec < NTC_PREC_FP32 || prec > NTC_PREC_INT8)
+  if (prec < NTC_PREC_FP32 || prec > NTC_PREC_BF16_MFMA)
   {
     err = Result::InvalidState;
     return EvalResults{};

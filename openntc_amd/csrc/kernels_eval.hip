@@ -28,6 +28,13 @@
 
 using namespace ntc;
 
+// The matrix-core decode (kernels_eval_mfma.hip).
+size_t ntc_mfma_table_bytes(int F, int H);
+int ntc_mfma_table_build(int F, int H, int O, const float* dec, void* table);
+int ntc_mfma_decode(bool gather, int H, int F, int O, int act, int64_t B, const void* table, const float* x,
+                    const float* yt, float* y, double* sse, void* workspace, const ntc_feature_desc* fd,
+                    const ntc_tiles* tl, const float* g0, const float* g1, const uint8_t* pkg, hipStream_t stream);
+
 namespace
 {
 constexpr int kThreads = 256;
@@ -279,6 +286,7 @@ extern "C" {
 size_t ntc_decode_table_bytes(int32_t precision, int32_t feature_dim, int32_t feature_dim_padded, int32_t hidden_dim)
 {
   const int64_t H = hidden_dim;
+  if (precision == NTC_PREC_BF16_MFMA) return ntc_mfma_table_bytes(feature_dim, hidden_dim);
   if (precision == NTC_PREC_INT8)
     return sizeof(float) * kHeader + 4 * ((feature_dim_padded / 4) * H + H * (H / 4) + H * 16 + 2 * H + 16);
   return sizeof(float) * (kHeader + (int64_t)feature_dim * H + H * H + H * 16);
@@ -287,9 +295,15 @@ size_t ntc_decode_table_bytes(int32_t precision, int32_t feature_dim, int32_t fe
 int ntc_build_decode_table_f(int32_t precision, int32_t F, int32_t H, int32_t O, const float* dec, float max_a0,
                              float max_a1, void* table)
 {
-  NTC_RETURN_IF(precision != NTC_PREC_FP32 && precision != NTC_PREC_BF16 && precision != NTC_PREC_FP8,
+  NTC_RETURN_IF(precision != NTC_PREC_FP32 && precision != NTC_PREC_BF16 && precision != NTC_PREC_FP8 &&
+                    precision != NTC_PREC_BF16_MFMA,
                 NTC_E_INVALID);
   NTC_RETURN_IF(!dec || !table || F < 1 || (H != 64 && H != 128) || O < 1 || O > 16, NTC_E_INVALID);
+  if (precision == NTC_PREC_BF16_MFMA)
+  {
+    NTC_RETURN_IF(F > 96, NTC_E_INVALID);
+    return ntc_mfma_table_build(F, H, O, dec, table);
+  }
   const float* w0 = dec;
   const float* w1 = w0 + (int64_t)H * F;
   const float* wo = w1 + (int64_t)H * H;
@@ -376,6 +390,12 @@ int ntc_decode_sse_f32(int32_t precision, int32_t activation, int32_t F, int32_t
   NTC_RETURN_IF(!table || !x || !y_target || !sse || !workspace || batch < 1, NTC_E_INVALID);
   NTC_RETURN_IF(F < 1 || F > 256 || O < 1 || O > 16 || (H != 64 && H != 128), NTC_E_INVALID);
   NTC_RETURN_IF(precision == NTC_PREC_INT8 && (Fp < F || Fp % 16), NTC_E_INVALID);
+  if (precision == NTC_PREC_BF16_MFMA)
+  {
+    NTC_RETURN_IF(F > 96, NTC_E_INVALID);
+    return ntc_mfma_decode(false, H, F, O, activation, batch, table, x, y_target, y_out, sse, workspace, nullptr,
+                           nullptr, nullptr, nullptr, nullptr, stream);
+  }
   const DecodeKernel k = H == 64 ? decode_kernel<64>(precision) : decode_kernel<128>(precision);
   NTC_RETURN_IF(!k, NTC_E_INVALID);
   DecodeArgs a;
@@ -405,9 +425,15 @@ int ntc_decode_tiles_sse_f32(int32_t precision, int32_t activation, const ntc_fe
   const int F = fd->feature_dim;
   NTC_RETURN_IF(O < 1 || O > 16 || (H != 64 && H != 128), NTC_E_INVALID);
   NTC_RETURN_IF(precision == NTC_PREC_INT8 && (Fp < F || Fp % 16), NTC_E_INVALID);
+  const int64_t batch = batch_of(*tiles);
+  if (precision == NTC_PREC_BF16_MFMA)
+  {
+    NTC_RETURN_IF(F > 96, NTC_E_INVALID);
+    return ntc_mfma_decode(true, H, F, O, activation, batch, table, nullptr, nullptr, nullptr, sse, workspace, fd,
+                           tiles, g0, g1, package, stream);
+  }
   const DecodeKernel k = H == 64 ? decode_kernel<64, true>(precision) : decode_kernel<128, true>(precision);
   NTC_RETURN_IF(!k, NTC_E_INVALID);
-  const int64_t batch = batch_of(*tiles);
   DecodeArgs a = {};
   a.act = activation;
   a.F = F;
