@@ -1097,12 +1097,30 @@ EvalResults Context::Impl::Evaluate(const Context& ctx, EvalPrecision p, EvalDet
       err = Result::CudaFailure;
       return EvalResults{};
     }
-    for (const auto& b : batches)
+    if (prec == NTC_PREC_BF16_MFMA)
     {
-      if ((err = DecodeTiles(b[0], eval_origins.as<int32_t>() + b[1], b[2], b[3], prec, sse)) != Result::Success)
-        return EvalResults{};
-      count += (double)b[2] * b[3] * b[3];
+      // the matrix-core decode sums in FP64 per workgroup whatever the batch
+      // size: one launch per up to 64 tiles of a mip (contiguous origins)
+      for (size_t i = 0; i < batches.size();)
+      {
+        size_t j = i;
+        int tiles = 0;
+        for (; j < batches.size() && batches[j][0] == batches[i][0] && tiles + batches[j][2] <= 64; ++j)
+          tiles += batches[j][2];
+        const auto& b = batches[i];
+        if ((err = DecodeTiles(b[0], eval_origins.as<int32_t>() + b[1], tiles, b[3], prec, sse)) != Result::Success)
+          return EvalResults{};
+        count += (double)tiles * b[3] * b[3];
+        i = j;
+      }
     }
+    else
+      for (const auto& b : batches)
+      {
+        if ((err = DecodeTiles(b[0], eval_origins.as<int32_t>() + b[1], b[2], b[3], prec, sse)) != Result::Success)
+          return EvalResults{};
+        count += (double)b[2] * b[3] * b[3];
+      }
   }
   else
     err = WalkEvalTiles(*this, [&](int mip, const int32_t* origins, int nb, int T) -> Result {

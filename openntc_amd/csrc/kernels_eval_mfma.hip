@@ -109,8 +109,8 @@ __device__ __forceinline__ void gather_fixed(const ntc_feature_desc& fd, const n
 
 template <int H, int FP, int NTW, bool GATHER, int ACT, int C0 = 0, int C1 = 0, int G0MODE = 0, int PEC = 0,
           int LODS = 0>
-__global__ __launch_bounds__(kThreads) void k_decode_bf16_mfma(const uint16_t* __restrict__ wtab, int F,
-                                                             int O, int64_t B, const float* __restrict__ x,
+__global__ __launch_bounds__(kThreads) void k_decode_bf16_mfma(const uint16_t* __restrict__ wtab, int F, int O,
+                                                             int64_t B, const float* __restrict__ x,
                                                              const float* __restrict__ yt, float* __restrict__ y,
                                                              double* __restrict__ part, ntc_feature_desc fd,
                                                              ntc_tiles tl, const float* __restrict__ g0,
@@ -119,15 +119,13 @@ __global__ __launch_bounds__(kThreads) void k_decode_bf16_mfma(const uint16_t* _
 {
   constexpr int KF = FP / 32, KH = H / 32, MT = H / 16;
   constexpr int XS = FP + 8;  // BF16 elements per texel row (rows 16 bytes apart in bank groups)
-  constexpr int TPW = 16 * NTW, CHUNK = 4 * TPW;
   constexpr int WE = H * FP + H * H + 16 * H;
+  constexpr int SUB = 64 / (16 * NTW);  // decoder passes over a wave's 64 texels
   constexpr TableLayout TLY = {H, FP, 16};
   __shared__ __attribute__((aligned(16))) uint16_t wt[WE];
-  __shared__ __attribute__((aligned(16))) uint16_t xs[CHUNK * XS];
-  __shared__ uint8_t ts[CHUNK * 16];  // target bytes [texel][o] (GATHER) ...
-  __shared__ float unorm[256];        // byte / 255, the division's bits
-  __shared__ float tf[GATHER ? 1 : CHUNK * 16];  // ... or target values (X from HBM)
-  static_assert(CHUNK * XS * 2 >= kThreads * 4 * 8, "the channel reduction reuses the feature rows");
+  __shared__ __attribute__((aligned(16))) uint16_t xs_all[4 * 64 * XS];  // each wave's texel rows
+  __shared__ __attribute__((aligned(16))) float ts_all[4 * 64 * 16];    // each wave's target values [texel][o]
+  __shared__ float unorm[256];                                         // byte / 255, the division's bits
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, n16 = lane & 15, g = lane >> 4;
   {
     const uint4* src = reinterpret_cast<const uint4*>(wtab);
@@ -135,17 +133,20 @@ __global__ __launch_bounds__(kThreads) void k_decode_bf16_mfma(const uint16_t* _
     for (int i = tid; i < WE / 8; i += kThreads) dst[i] = src[i];
     unorm[tid] = (float)tid / 255.0f;
   }
+  __syncthreads();
+  uint16_t* const xs = xs_all + wave * 64 * XS;
+  float* const ts = ts_all + wave * 64 * 16;
   double sse[4] = {0.0, 0.0, 0.0, 0.0};
-  for (int64_t c0 = (int64_t)blockIdx.x * CHUNK; c0 < B; c0 += (int64_t)gridDim.x * CHUNK)
+  // Each wave walks chunks of 64 texels on its own (its rows of xs and ts
+  // are its own, so no workgroup barrier): every lane gathers one texel, then
+  // the wave decodes the 64.
+  for (int64_t c0 = ((int64_t)blockIdx.x * 4 + wave) * 64; c0 < B; c0 += (int64_t)gridDim.x * 4 * 64)
   {
-    __syncthreads();  // the previous chunk's rows are read (and the tables copied)
-    if (tid < CHUNK)
     {
-      const int64_t t = c0 + tid;
-      const bool live = t < B;
-      const int64_t tc = live ? t : 0;
+      const int64_t t = c0 + lane;
+      const int64_t tc = t < B ? t : 0;
       // the texel's features as BF16 pairs (slots arrive in order), zeros to FP
-      uint32_t* row = reinterpret_cast<uint32_t*>(xs + tid * XS);
+      uint32_t* row = reinterpret_cast<uint32_t*>(xs + lane * XS);
       uint32_t lo = 0;
       auto put = [&](int slot, float v) {
         if (slot & 1)
@@ -172,100 +173,113 @@ __global__ __launch_bounds__(kThreads) void k_decode_bf16_mfma(const uint16_t* _
           const int py = (int)(((uint32_t)(xy >> lt) + (uint32_t)tl.origins[2 * b + 1]) & mm);
           tb = ((int64_t)py * tl.mip_dim + px) * O;
         }
-        for (int o = 0; o < O; ++o) ts[tid * 16 + o] = pkg[tb + o];
+        for (int o = 0; o < O; ++o) ts[lane * 16 + o] = unorm[pkg[tb + o]];
       }
       else
       {
         for (int k = 0; k < F; ++k) put(k, x[(int64_t)k * B + tc]);
-        for (int o = 0; o < O; ++o) tf[tid * 16 + o] = yt[(int64_t)o * B + tc];
+        for (int o = 0; o < O; ++o) ts[lane * 16 + o] = yt[(int64_t)o * B + tc];
       }
       if (F & 1) row[F >> 1] = lo;
       for (int k = (F + 1) >> 1; k < FP / 2; ++k) row[k] = 0;
     }
-    __syncthreads();
-    // layer 0: Z0 = W0 X, each weight fragment feeding the wave's NTW column tiles
-    f32x4 z[MT][NTW];
+    // the wave's own LDS writes complete before its lanes read each other's rows
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 #pragma unroll
-    for (int r = 0; r < MT; ++r)
-#pragma unroll
-      for (int nt = 0; nt < NTW; ++nt) z[r][nt] = f32x4{0, 0, 0, 0};
-#pragma unroll
-    for (int j = 0; j < KF; ++j)
+    for (int sb = 0; sb < SUB; ++sb)
     {
-      bf16x8 xb[NTW];
-#pragma unroll
-      for (int nt = 0; nt < NTW; ++nt) xb[nt] = frag_lds(&xs[(wave * TPW + nt * 16 + n16) * XS + 32 * j + 8 * g]);
+      const int tb0 = sb * 16 * NTW;  // first texel of this decoder pass within the wave's 64
+      // layer 0: Z0 = W0 X, each weight fragment feeding NTW column tiles
+      f32x4 z[MT][NTW];
 #pragma unroll
       for (int r = 0; r < MT; ++r)
+#pragma unroll
+        for (int nt = 0; nt < NTW; ++nt) z[r][nt] = f32x4{0, 0, 0, 0};
+#pragma unroll
+      for (int j = 0; j < KF; ++j)
       {
-        const int row = 16 * r + n16;
-        const bf16x8 a = frag_lds(&wt[TLY.oW0() + row * FP + swz(row, 32 * j + 8 * g, FP)]);
+        bf16x8 xb[NTW];
 #pragma unroll
-        for (int nt = 0; nt < NTW; ++nt) z[r][nt] = mfma(a, xb[nt], z[r][nt]);
-      }
-    }
-    // activation, BF16, and two accumulator tiles per B fragment of the next layer
-    bf16x8 af[KH][NTW];
-    auto to_frags = [&]() {
+        for (int nt = 0; nt < NTW; ++nt) xb[nt] = frag_lds(&xs[(tb0 + nt * 16 + n16) * XS + 32 * j + 8 * g]);
 #pragma unroll
-      for (int j = 0; j < KH; ++j)
-#pragma unroll
-        for (int nt = 0; nt < NTW; ++nt)
+        for (int r = 0; r < MT; ++r)
         {
-          float v[8];
+          const int row = 16 * r + n16;
+          const bf16x8 a = frag_lds(&wt[TLY.oW0() + row * FP + swz(row, 32 * j + 8 * g, FP)]);
 #pragma unroll
-          for (int i = 0; i < 4; ++i)
-          {
-            v[i] = act_fwd<float>(ACT, z[2 * j][nt][i]);
-            v[4 + i] = act_fwd<float>(ACT, z[2 * j + 1][nt][i]);
-          }
-          af[j][nt] = frag_pack(v);
+          for (int nt = 0; nt < NTW; ++nt) z[r][nt] = mfma(a, xb[nt], z[r][nt]);
         }
-    };
-    to_frags();
-    // layer 1
+      }
+      // activation, BF16, and two accumulator tiles per B fragment of the next layer
+      bf16x8 af[KH][NTW];
+      auto to_frags = [&]() {
 #pragma unroll
-    for (int r = 0; r < MT; ++r)
+        for (int j = 0; j < KH; ++j)
 #pragma unroll
-      for (int nt = 0; nt < NTW; ++nt) z[r][nt] = f32x4{0, 0, 0, 0};
+          for (int nt = 0; nt < NTW; ++nt)
+          {
+            float v[8];
 #pragma unroll
-    for (int j = 0; j < KH; ++j)
+            for (int i = 0; i < 4; ++i)
+            {
+              v[i] = act_fwd<float>(ACT, z[2 * j][nt][i]);
+              v[4 + i] = act_fwd<float>(ACT, z[2 * j + 1][nt][i]);
+            }
+            af[j][nt] = frag_pack(v);
+          }
+      };
+      to_frags();
+      // layer 1
 #pragma unroll
       for (int r = 0; r < MT; ++r)
-      {
-        const int row = 16 * r + n16;
-        const bf16x8 a = frag_lds(&wt[TLY.oW1() + row * H + swz(row, 32 * j + 8 * g, H)]);
 #pragma unroll
-        for (int nt = 0; nt < NTW; ++nt) z[r][nt] = mfma(a, af[j][nt], z[r][nt]);
-      }
-    to_frags();
-    // output layer and the squared errors of outputs 4g + i
-#pragma unroll
-    for (int nt = 0; nt < NTW; ++nt)
-    {
-      f32x4 yh = {0, 0, 0, 0};
+        for (int nt = 0; nt < NTW; ++nt) z[r][nt] = f32x4{0, 0, 0, 0};
 #pragma unroll
       for (int j = 0; j < KH; ++j)
-        yh = mfma(frag_lds(&wt[TLY.oWo() + n16 * H + swz(n16, 32 * j + 8 * g, H)]), af[j][nt], yh);
-      const int loc = wave * TPW + nt * 16 + n16;
-      const int64_t t = c0 + loc;
-      const bool live = t < B;
 #pragma unroll
-      for (int i = 0; i < 4; ++i)
+        for (int r = 0; r < MT; ++r)
+        {
+          const int row = 16 * r + n16;
+          const bf16x8 a = frag_lds(&wt[TLY.oW1() + row * H + swz(row, 32 * j + 8 * g, H)]);
+#pragma unroll
+          for (int nt = 0; nt < NTW; ++nt) z[r][nt] = mfma(a, af[j][nt], z[r][nt]);
+        }
+      to_frags();
+      // output layer and the squared errors of outputs 4g + i
+#pragma unroll
+      for (int nt = 0; nt < NTW; ++nt)
       {
-        const int o = 4 * g + i;
-        const bool use = live && o < O;
-        const int oc = o < O ? o : 0;
-        const float target = GATHER ? unorm[ts[loc * 16 + oc]] : tf[loc * 16 + oc];
-        const float r = target - yh[i];
-        sse[i] += use ? (double)r * (double)r : 0.0;
-        if (!GATHER && y && use) y[(int64_t)o * B + t] = yh[i];
+        f32x4 yh = {0, 0, 0, 0};
+#pragma unroll
+        for (int j = 0; j < KH; ++j)
+          yh = mfma(frag_lds(&wt[TLY.oWo() + n16 * H + swz(n16, 32 * j + 8 * g, H)]), af[j][nt], yh);
+        const int loc = tb0 + nt * 16 + n16;
+        const int64_t t = c0 + loc;
+        const bool live = t < B;
+        const float4 tv = *reinterpret_cast<const float4*>(&ts[loc * 16 + 4 * g]);
+        const float tgt[4] = {tv.x, tv.y, tv.z, tv.w};
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+        {
+          const int o = 4 * g + i;
+          const bool use = live && o < O;
+          const float r = tgt[i] - yh[i];
+          sse[i] += use ? (double)r * (double)r : 0.0;
+          if (!GATHER && y && use) y[(int64_t)o * B + t] = yh[i];
+        }
       }
     }
+    // the decoder passes' reads of xs and ts are done before the next gather writes them
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   }
   // channel o = 4g + i: the 16 lanes of group g in every wave, in a fixed order
-  __syncthreads();  // the last chunk's rows are read
-  double* red = reinterpret_cast<double*>(xs);
+  __syncthreads();
+  double* red = reinterpret_cast<double*>(xs_all);
+  static_assert(4 * 64 * XS * 2 >= kThreads * 4 * 8, "the channel reduction reuses the feature rows");
 #pragma unroll
   for (int i = 0; i < 4; ++i) red[tid * 4 + i] = sse[i];
   __syncthreads();
@@ -326,7 +340,7 @@ MfmaDecode mfma_kernel(int H, int FP, int act)
                              : mfma_kernel_act<GATHER, NTC_ACT_HARDGELU>(H, FP);
 }
 
-int chunk_of(int H) { return H == 64 ? 256 : 128; }
+int chunk_of(int) { return 256; }
 
 // Workgroups for a batch: persistent, two per CU (each copies the weight
 // tables once and walks its share of the chunks), at most one per chunk and
