@@ -307,7 +307,7 @@ def eval_precisions(ntc, reps):
     flop_per_texel = 2.0 * (F * H + H * H + H * O)
     res = {}
     for prec in (ntc.EvalPrecision.FP32, ntc.EvalPrecision.BF16, ntc.EvalPrecision.FP8, ntc.EvalPrecision.INT8,
-                 ntc.EvalPrecision.BF16_MFMA):
+                 ntc.EvalPrecision.BF16_MFMA, ntc.EvalPrecision.FP8_MFMA):
         ctx.EvalEx(prec)  # untimed: builds the precision's tables (FP8 runs its calibration walk)
         lib.ntc_dev_synchronize()
         t0 = time.perf_counter()

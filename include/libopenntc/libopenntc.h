@@ -213,6 +213,7 @@ enum class EvalPrecision : int32_t
   FP8 = 2,
   INT8 = 3,
   BF16_MFMA = 4,  // BF16 rounding points, MFMA accumulation order (the fast BF16 decode)
+  FP8_MFMA = 5,   // FP8 E4M3 rounding points and scales, MFMA accumulation order
 };
 
 struct LayoutInfo

@@ -466,16 +466,17 @@ int ntc_step_advance(int32_t* step, hipStream_t stream);
  * max_a0/max_a1), exact products, FP32 accumulation. INT8: int8 features
  * clamp(round(128 x)), the exported blob's int8 weights and row scales,
  * int32 dot products, per-layer requantisation with rcp_s_a1/rcp_s_a2.
- * BF16_MFMA: the BF16 rounding points on the matrix cores
- * (v_mfma_f32_16x16x32_bf16, FP32 accumulation in the MFMA's order, so the
- * outputs differ from BF16's fixed-order chains by accumulation rounding);
- * its table holds BF16 weights; feature_dim_padded unused. The FP32, BF16,
+ * BF16_MFMA / FP8_MFMA: BF16's / FP8's rounding points on the matrix cores
+ * (v_mfma_f32_16x16x32_bf16 / _fp8_fp8, FP32 accumulation in the MFMA's
+ * order, so the outputs differ from the fixed-order chains by accumulation
+ * rounding); their tables hold BF16 / E4M3 weights; feature_dim_padded unused. The FP32, BF16,
  * FP8 and INT8 forms are reproducible bit for bit on the CPU. */
 #define NTC_PREC_FP32 0
 #define NTC_PREC_BF16 1
 #define NTC_PREC_FP8 2
 #define NTC_PREC_INT8 3
 #define NTC_PREC_BF16_MFMA 4
+#define NTC_PREC_FP8_MFMA 5
 size_t ntc_decode_table_bytes(int32_t precision, int32_t feature_dim, int32_t feature_dim_padded,
                               int32_t hidden_dim);
 /* FP32 / BF16 / FP8 table (host memory) from the FP32 decoder

@@ -85,6 +85,7 @@ class EvalPrecision(enum.IntEnum):
     FP8 = 2
     INT8 = 3
     BF16_MFMA = 4
+    FP8_MFMA = 5
 
 
 @dataclass

@@ -984,7 +984,7 @@ EvalResults Context::Impl::Evaluate(const Context& ctx, EvalPrecision p, EvalDet
 {
   err = Result::Success;
   const int prec = (int)p;
-  if (prec < NTC_PREC_FP32 || prec > NTC_PREC_BF16_MFMA)
+  if (prec < NTC_PREC_FP32 || prec > NTC_PREC_FP8_MFMA)
   {
     err = Result::InvalidState;
     return EvalResults{};
@@ -1027,7 +1027,7 @@ EvalResults Context::Impl::Evaluate(const Context& ctx, EvalPrecision p, EvalDet
         return EvalResults{};
       }
       float ma0 = 0.0f, ma1 = 0.0f;
-      if (p == EvalPrecision::FP8)
+      if (p == EvalPrecision::FP8 || p == EvalPrecision::FP8_MFMA)
       {
         const CalibrationData c = RunCalibration(1.0f, err);
         if (err != Result::Success) return EvalResults{};
@@ -1097,7 +1097,7 @@ EvalResults Context::Impl::Evaluate(const Context& ctx, EvalPrecision p, EvalDet
       err = Result::CudaFailure;
       return EvalResults{};
     }
-    if (prec == NTC_PREC_BF16_MFMA)
+    if (prec == NTC_PREC_BF16_MFMA || prec == NTC_PREC_FP8_MFMA)
     {
       // the matrix-core decode sums in FP64 per workgroup whatever the batch
       // size: one launch per up to 64 tiles of a mip (contiguous origins)

@@ -23,15 +23,16 @@
 #include <cstring>
 #include <vector>
 
+#include "eval_formats.h"
 #include "ntc_device.h"
 #include "ntc_host.h"
 
 using namespace ntc;
 
 // The matrix-core decode (kernels_eval_mfma.hip).
-size_t ntc_mfma_table_bytes(int F, int H);
-int ntc_mfma_table_build(int F, int H, int O, const float* dec, void* table);
-int ntc_mfma_decode(bool gather, int H, int F, int O, int act, int64_t B, const void* table, const float* x,
+size_t ntc_mfma_table_bytes(bool fp8, int F, int H);
+int ntc_mfma_table_build(bool fp8, int F, int H, int O, const float* dec, const float* s, void* table);
+int ntc_mfma_decode(bool fp8, bool gather, int H, int F, int O, int act, int64_t B, const void* table, const float* x,
                     const float* yt, float* y, double* sse, void* workspace, const ntc_feature_desc* fd,
                     const ntc_tiles* tl, const float* g0, const float* g1, const uint8_t* pkg, hipStream_t stream);
 
@@ -39,45 +40,6 @@ namespace
 {
 constexpr int kThreads = 256;
 constexpr int kHeader = 8;  // floats ahead of the weights in a table
-
-// ---- numeric formats (host and device) ----------------------------------------------------
-__host__ __device__ inline uint32_t f_bits(float f)
-{
-  uint32_t u;
-  memcpy(&u, &f, 4);
-  return u;
-}
-
-__host__ __device__ inline float f_from(uint32_t u)
-{
-  float f;
-  memcpy(&f, &u, 4);
-  return f;
-}
-
-__host__ __device__ inline float bf16_round(float x)
-{
-  uint32_t u = f_bits(x);
-  u += 0x7FFFu + ((u >> 16) & 1u);
-  return f_from(u & 0xFFFF0000u);
-}
-
-// OCP FP8 E4M3 (bias 7, largest finite 448, 0x7F NaN, no infinities) quantise-dequantise of
-// v * r: round to nearest even, saturating (NaN saturates to 448 with its sign). Normal range:
-// the float's mantissa rounded to 3 bits on its bits (a carry moves into the exponent; the clamp
-// to 448 keeps the result finite); below 2^-6: multiples of 2^-9. Formed on the float directly,
-// without an E4M3 code in between: equal, on every float bit pattern, to encoding the value to
-// its E4M3 byte and decoding it again.
-__host__ __device__ inline float fp8_q(float v, float r)
-{
-  const uint32_t u = f_bits(v * r);
-  float a = f_from(u & 0x7FFFFFFFu);
-  a = a < 448.0f ? a : 448.0f;
-  uint32_t n = f_bits(a);
-  n = (n + 0x7FFFFu + ((n >> 20) & 1u)) & 0xFFF00000u;
-  const float q = a < 0x1p-6f ? rintf(a * 512.0f) * 0x1p-9f : f_from(n);
-  return f_from(f_bits(q) | (u & 0x80000000u));
-}
 
 __host__ __device__ inline int clamp_s8(float f)
 {
@@ -286,7 +248,8 @@ extern "C" {
 size_t ntc_decode_table_bytes(int32_t precision, int32_t feature_dim, int32_t feature_dim_padded, int32_t hidden_dim)
 {
   const int64_t H = hidden_dim;
-  if (precision == NTC_PREC_BF16_MFMA) return ntc_mfma_table_bytes(feature_dim, hidden_dim);
+  if (precision == NTC_PREC_BF16_MFMA || precision == NTC_PREC_FP8_MFMA)
+    return ntc_mfma_table_bytes(precision == NTC_PREC_FP8_MFMA, feature_dim, hidden_dim);
   if (precision == NTC_PREC_INT8)
     return sizeof(float) * kHeader + 4 * ((feature_dim_padded / 4) * H + H * (H / 4) + H * 16 + 2 * H + 16);
   return sizeof(float) * (kHeader + (int64_t)feature_dim * H + H * H + H * 16);
@@ -296,14 +259,10 @@ int ntc_build_decode_table_f(int32_t precision, int32_t F, int32_t H, int32_t O,
                              float max_a1, void* table)
 {
   NTC_RETURN_IF(precision != NTC_PREC_FP32 && precision != NTC_PREC_BF16 && precision != NTC_PREC_FP8 &&
-                    precision != NTC_PREC_BF16_MFMA,
+                    precision != NTC_PREC_BF16_MFMA && precision != NTC_PREC_FP8_MFMA,
                 NTC_E_INVALID);
   NTC_RETURN_IF(!dec || !table || F < 1 || (H != 64 && H != 128) || O < 1 || O > 16, NTC_E_INVALID);
-  if (precision == NTC_PREC_BF16_MFMA)
-  {
-    NTC_RETURN_IF(F > 96, NTC_E_INVALID);
-    return ntc_mfma_table_build(F, H, O, dec, table);
-  }
+  const bool fp8 = precision == NTC_PREC_FP8 || precision == NTC_PREC_FP8_MFMA;
   const float* w0 = dec;
   const float* w1 = w0 + (int64_t)H * F;
   const float* wo = w1 + (int64_t)H * H;
@@ -313,7 +272,7 @@ int ntc_build_decode_table_f(int32_t precision, int32_t F, int32_t H, int32_t O,
   float* wot = w1t + (int64_t)H * H;
   // FP8 per-layer scales: s_x = 1/256, s_w = max|W| / 448, s_a = max|a| / 448
   float s[6] = {1.0f / 256.0f, 1, 1, 1, 1, 1};
-  if (precision == NTC_PREC_FP8)
+  if (fp8)
   {
     const float m0 = max_abs(w0, (int64_t)H * F), m1 = max_abs(w1, (int64_t)H * H), mo = max_abs(wo, (int64_t)O * H);
     s[1] = m0 > 0.0f ? m0 / 448.0f : 1.0f;
@@ -321,6 +280,11 @@ int ntc_build_decode_table_f(int32_t precision, int32_t F, int32_t H, int32_t O,
     s[3] = mo > 0.0f ? mo / 448.0f : 1.0f;
     s[4] = max_a0 > 0.0f ? max_a0 / 448.0f : 1.0f;
     s[5] = max_a1 > 0.0f ? max_a1 / 448.0f : 1.0f;
+  }
+  if (precision == NTC_PREC_BF16_MFMA || precision == NTC_PREC_FP8_MFMA)
+  {
+    NTC_RETURN_IF(F > 96, NTC_E_INVALID);
+    return ntc_mfma_table_build(fp8, F, H, O, dec, s, table);
   }
   float r[6];
   for (int i = 0; i < 6; ++i) r[i] = 1.0f / s[i];
@@ -390,11 +354,11 @@ int ntc_decode_sse_f32(int32_t precision, int32_t activation, int32_t F, int32_t
   NTC_RETURN_IF(!table || !x || !y_target || !sse || !workspace || batch < 1, NTC_E_INVALID);
   NTC_RETURN_IF(F < 1 || F > 256 || O < 1 || O > 16 || (H != 64 && H != 128), NTC_E_INVALID);
   NTC_RETURN_IF(precision == NTC_PREC_INT8 && (Fp < F || Fp % 16), NTC_E_INVALID);
-  if (precision == NTC_PREC_BF16_MFMA)
+  if (precision == NTC_PREC_BF16_MFMA || precision == NTC_PREC_FP8_MFMA)
   {
     NTC_RETURN_IF(F > 96, NTC_E_INVALID);
-    return ntc_mfma_decode(false, H, F, O, activation, batch, table, x, y_target, y_out, sse, workspace, nullptr,
-                           nullptr, nullptr, nullptr, nullptr, stream);
+    return ntc_mfma_decode(precision == NTC_PREC_FP8_MFMA, false, H, F, O, activation, batch, table, x, y_target,
+                           y_out, sse, workspace, nullptr, nullptr, nullptr, nullptr, nullptr, stream);
   }
   const DecodeKernel k = H == 64 ? decode_kernel<64>(precision) : decode_kernel<128>(precision);
   NTC_RETURN_IF(!k, NTC_E_INVALID);
@@ -426,11 +390,11 @@ int ntc_decode_tiles_sse_f32(int32_t precision, int32_t activation, const ntc_fe
   NTC_RETURN_IF(O < 1 || O > 16 || (H != 64 && H != 128), NTC_E_INVALID);
   NTC_RETURN_IF(precision == NTC_PREC_INT8 && (Fp < F || Fp % 16), NTC_E_INVALID);
   const int64_t batch = batch_of(*tiles);
-  if (precision == NTC_PREC_BF16_MFMA)
+  if (precision == NTC_PREC_BF16_MFMA || precision == NTC_PREC_FP8_MFMA)
   {
     NTC_RETURN_IF(F > 96, NTC_E_INVALID);
-    return ntc_mfma_decode(true, H, F, O, activation, batch, table, nullptr, nullptr, nullptr, sse, workspace, fd,
-                           tiles, g0, g1, package, stream);
+    return ntc_mfma_decode(precision == NTC_PREC_FP8_MFMA, true, H, F, O, activation, batch, table, nullptr,
+                           nullptr, nullptr, sse, workspace, fd, tiles, g0, g1, package, stream);
   }
   const DecodeKernel k = H == 64 ? decode_kernel<64, true>(precision) : decode_kernel<128, true>(precision);
   NTC_RETURN_IF(!k, NTC_E_INVALID);

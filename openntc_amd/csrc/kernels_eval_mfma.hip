@@ -1,18 +1,26 @@
-// Evaluation decode on the matrix cores: the BF16 decode as
-// v_mfma_f32_16x16x32_bf16 chains with FP32 accumulation (features, weights
-// and hidden activations rounded to BF16, nearest even, as in the VALU
-// decode of kernels_eval.hip, whose fixed-order FMA chains stay the
-// bit-exact reference). Replaces, per evaluation batch, the reference's
-// draw_features + three SGEMMs + activations + Sdot (libopenntc.cpp:972-1053).
+// Evaluation decode on the matrix cores, at the rounding points of the VALU
+// decode of kernels_eval.hip (whose fixed-order FMA chains stay the
+// bit-exact reference) with the MFMA's FP32 accumulation order:
+//   BF16: features, weights and hidden activations rounded to BF16 (nearest
+//         even), v_mfma_f32_16x16x32_bf16;
+//   FP8:  OCP E4M3 operands with per-layer scales (features x / s_x, weights
+//         W / s_w, hidden activations a / s_a, saturating at 448),
+//         v_mfma_f32_16x16x32_fp8_fp8 (exact products), each accumulator
+//         scaled by s_w * s_in afterwards.
+// Replaces, per evaluation batch, the reference's draw_features + three
+// SGEMMs + activations + Sdot (libopenntc.cpp:972-1053).
 //
-// A workgroup (4 waves) walks the batch in chunks of 4 x 16 x NTW texels:
-// every thread gathers one texel's features (noise off) into an LDS row as
-// BF16 and its target values; then each wave runs the decoder for its 16 x NTW
-// texels with the weights as the MFMA A operand (the fused step's tables:
-// W0 rows, W1 and Wout with permuted columns, so that a layer's two
-// accumulator tiles are the next layer's B fragment) and adds the squared
-// errors per output channel in FP64. Each workgroup writes its 16 channel sums
-// in a fixed order.
+// Each wave walks chunks of 64 texels on its own: every lane gathers one
+// texel's features (noise off) into the wave's LDS rows in the operand format
+// and its target values; then the wave decodes them, 16 x NTW texels per pass,
+// with the weights as the MFMA A operand (the fused step's table layout: W0
+// rows, W1 and Wout with permuted columns, so that a layer's two accumulator
+// tiles are the next layer's B fragment), and adds the squared errors per
+// output channel in FP64. Each workgroup writes its 16 channel sums in a
+// fixed order.
+#include <type_traits>
+
+#include "eval_formats.h"
 #include "fused_body.h"
 
 namespace
@@ -107,34 +115,95 @@ __device__ __forceinline__ void gather_fixed(const ntc_feature_desc& fd, const n
   }
 }
 
-template <int H, int FP, int NTW, bool GATHER, int ACT, int C0 = 0, int C1 = 0, int G0MODE = 0, int PEC = 0,
-          int LODS = 0>
-__global__ __launch_bounds__(kThreads) void k_decode_bf16_mfma(const uint16_t* __restrict__ wtab, int F, int O,
-                                                             int64_t B, const float* __restrict__ x,
-                                                             const float* __restrict__ yt, float* __restrict__ y,
-                                                             double* __restrict__ part, ntc_feature_desc fd,
-                                                             ntc_tiles tl, const float* __restrict__ g0,
-                                                             const float* __restrict__ g1,
-                                                             const uint8_t* __restrict__ pkg)
+// Operand format of the matrix-core decode: BF16 (16-byte fragments of 8
+// elements) or E4M3 (8-byte fragments).
+template <bool FP8>
+struct Op
 {
+  typedef typename std::conditional<FP8, uint8_t, uint16_t>::type El;
+  typedef typename std::conditional<FP8, long, bf16x8>::type Frag;
+  static constexpr int PAD = FP8 ? 16 : 8;  // elements after a texel row (16 bytes)
+  __device__ static Frag load(const El* p)
+  {
+    if constexpr (FP8)
+      return *reinterpret_cast<const long*>(p);
+    else
+      return frag_lds(p);
+  }
+  __device__ static f32x4 mma(const Frag& a, const Frag& b, const f32x4& c)
+  {
+    if constexpr (FP8)
+      return __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8(a, b, c, 0, 0, 0);
+    else
+      return mfma(a, b, c);
+  }
+  // v * r clamped to the E4M3 range: v_cvt_pk_fp8_f32 (round to nearest
+  // even) of it is the byte of fp8_q(v, r), which rounds to nearest even after
+  // the same clamp
+  __device__ static float sat(float v, float r) { return __builtin_amdgcn_fmed3f(v * r, -448.0f, 448.0f); }
+  // operand element of v (FP8: the E4M3 byte of v * r)
+  __device__ static uint32_t elem(float v, float r)
+  {
+    if constexpr (FP8)
+      return (uint32_t)__builtin_amdgcn_cvt_pk_fp8_f32(sat(v, r), 0.0f, 0, false) & 0xFFu;
+    else
+      return bf16_dev(v);
+  }
+  // 8 values as one B fragment (FP8: of v * r)
+  __device__ static Frag pack(const float (&v)[8], float r)
+  {
+    if constexpr (FP8)
+    {
+      uint32_t w[2];
+#pragma unroll
+      for (int h = 0; h < 2; ++h)
+      {
+        int x = __builtin_amdgcn_cvt_pk_fp8_f32(sat(v[4 * h], r), sat(v[4 * h + 1], r), 0, false);
+        x = __builtin_amdgcn_cvt_pk_fp8_f32(sat(v[4 * h + 2], r), sat(v[4 * h + 3], r), x, true);
+        w[h] = (uint32_t)x;
+      }
+      return (long)(((uint64_t)w[1] << 32) | w[0]);
+    }
+    else
+      return frag_pack(v);
+  }
+};
+
+template <bool FP8, int H, int FP, int NTW, bool GATHER, int ACT, int C0 = 0, int C1 = 0, int G0MODE = 0,
+          int PEC = 0, int LODS = 0>
+__global__ __launch_bounds__(kThreads) void k_decode_mfma(const uint8_t* __restrict__ table, int F, int O, int64_t B,
+                                                        const float* __restrict__ x, const float* __restrict__ yt,
+                                                        float* __restrict__ y, double* __restrict__ part,
+                                                        ntc_feature_desc fd, ntc_tiles tl,
+                                                        const float* __restrict__ g0, const float* __restrict__ g1,
+                                                        const uint8_t* __restrict__ pkg)
+{
+  typedef Op<FP8> O8;
+  typedef typename O8::El El;
+  typedef typename O8::Frag Frag;
   constexpr int KF = FP / 32, KH = H / 32, MT = H / 16;
-  constexpr int XS = FP + 8;  // BF16 elements per texel row (rows 16 bytes apart in bank groups)
+  constexpr int XS = FP + O8::PAD;  // elements per texel row
   constexpr int WE = H * FP + H * H + 16 * H;
   constexpr int SUB = 64 / (16 * NTW);  // decoder passes over a wave's 64 texels
   constexpr TableLayout TLY = {H, FP, 16};
-  __shared__ __attribute__((aligned(16))) uint16_t wt[WE];
-  __shared__ __attribute__((aligned(16))) uint16_t xs_all[4 * 64 * XS];  // each wave's texel rows
-  __shared__ __attribute__((aligned(16))) float ts_all[4 * 64 * 16];    // each wave's target values [texel][o]
-  __shared__ float unorm[256];                                         // byte / 255, the division's bits
+  static_assert((WE * sizeof(El)) % 16 == 0 && (XS * sizeof(El)) % 16 == 0, "16-byte rows");
+  __shared__ __attribute__((aligned(16))) El wt[WE];
+  __shared__ __attribute__((aligned(16))) El xs_all[4 * 64 * XS];    // each wave's texel rows
+  __shared__ __attribute__((aligned(16))) float ts_all[4 * 64 * 16];  // each wave's target values [texel][o]
+  __shared__ float unorm[256];                                        // byte / 255, the division's bits
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, n16 = lane & 15, g = lane >> 4;
+  // header: FP8 chain scales s_w0 s_x, s_w1 s_a0, s_wo s_a1 and the reciprocals r_x, r_a0, r_a1
+  const float* hdr = reinterpret_cast<const float*>(table);
+  const float sc0 = FP8 ? hdr[0] : 1.0f, sc1 = FP8 ? hdr[1] : 1.0f, sco = FP8 ? hdr[2] : 1.0f;
+  const float rx = FP8 ? hdr[3] : 1.0f, ra0 = FP8 ? hdr[4] : 1.0f, ra1 = FP8 ? hdr[5] : 1.0f;
   {
-    const uint4* src = reinterpret_cast<const uint4*>(wtab);
+    const uint4* src = reinterpret_cast<const uint4*>(table + kHeaderBytes);
     uint4* dst = reinterpret_cast<uint4*>(wt);
-    for (int i = tid; i < WE / 8; i += kThreads) dst[i] = src[i];
+    for (int i = tid; i < (int)(WE * sizeof(El) / 16); i += kThreads) dst[i] = src[i];
     unorm[tid] = (float)tid / 255.0f;
   }
   __syncthreads();
-  uint16_t* const xs = xs_all + wave * 64 * XS;
+  El* const xs = xs_all + wave * 64 * XS;
   float* const ts = ts_all + wave * 64 * 16;
   double sse[4] = {0.0, 0.0, 0.0, 0.0};
   // Each wave walks chunks of 64 texels on its own (its rows of xs and ts
@@ -145,14 +214,19 @@ __global__ __launch_bounds__(kThreads) void k_decode_bf16_mfma(const uint16_t* _
     {
       const int64_t t = c0 + lane;
       const int64_t tc = t < B ? t : 0;
-      // the texel's features as BF16 pairs (slots arrive in order), zeros to FP
+      // the texel's features in the operand format (slots arrive in order),
+      // written as 32-bit words, zeros to FP
       uint32_t* row = reinterpret_cast<uint32_t*>(xs + lane * XS);
-      uint32_t lo = 0;
+      constexpr int PER = 4 / sizeof(El);  // elements per word
+      uint32_t acc = 0;
       auto put = [&](int slot, float v) {
-        if (slot & 1)
-          row[slot >> 1] = lo | ((uint32_t)bf16_dev(v) << 16);
-        else
-          lo = bf16_dev(v);
+        const int k = slot % PER;
+        acc |= O8::elem(v, rx) << (8 * sizeof(El) * k);
+        if (k == PER - 1)
+        {
+          row[slot / PER] = acc;
+          acc = 0;
+        }
       };
       if constexpr (GATHER)
       {
@@ -180,8 +254,8 @@ __global__ __launch_bounds__(kThreads) void k_decode_bf16_mfma(const uint16_t* _
         for (int k = 0; k < F; ++k) put(k, x[(int64_t)k * B + tc]);
         for (int o = 0; o < O; ++o) ts[lane * 16 + o] = yt[(int64_t)o * B + tc];
       }
-      if (F & 1) row[F >> 1] = lo;
-      for (int k = (F + 1) >> 1; k < FP / 2; ++k) row[k] = 0;
+      if (F % PER) row[F / PER] = acc;
+      for (int k = (F + PER - 1) / PER; k < FP / PER; ++k) row[k] = 0;
     }
     // the wave's own LDS writes complete before its lanes read each other's rows
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -200,21 +274,22 @@ __global__ __launch_bounds__(kThreads) void k_decode_bf16_mfma(const uint16_t* _
 #pragma unroll
       for (int j = 0; j < KF; ++j)
       {
-        bf16x8 xb[NTW];
+        Frag xb[NTW];
 #pragma unroll
-        for (int nt = 0; nt < NTW; ++nt) xb[nt] = frag_lds(&xs[(tb0 + nt * 16 + n16) * XS + 32 * j + 8 * g]);
+        for (int nt = 0; nt < NTW; ++nt) xb[nt] = O8::load(&xs[(tb0 + nt * 16 + n16) * XS + 32 * j + 8 * g]);
 #pragma unroll
         for (int r = 0; r < MT; ++r)
         {
           const int row = 16 * r + n16;
-          const bf16x8 a = frag_lds(&wt[TLY.oW0() + row * FP + swz(row, 32 * j + 8 * g, FP)]);
+          const Frag a = O8::load(&wt[TLY.oW0() + row * FP + swz(row, 32 * j + 8 * g, FP)]);
 #pragma unroll
-          for (int nt = 0; nt < NTW; ++nt) z[r][nt] = mfma(a, xb[nt], z[r][nt]);
+          for (int nt = 0; nt < NTW; ++nt) z[r][nt] = O8::mma(a, xb[nt], z[r][nt]);
         }
       }
-      // activation, BF16, and two accumulator tiles per B fragment of the next layer
-      bf16x8 af[KH][NTW];
-      auto to_frags = [&]() {
+      // activation (of the scaled chain), the operand format, and two
+      // accumulator tiles per B fragment of the next layer
+      Frag af[KH][NTW];
+      auto to_frags = [&](float sc, float r) {
 #pragma unroll
         for (int j = 0; j < KH; ++j)
 #pragma unroll
@@ -224,13 +299,13 @@ __global__ __launch_bounds__(kThreads) void k_decode_bf16_mfma(const uint16_t* _
 #pragma unroll
             for (int i = 0; i < 4; ++i)
             {
-              v[i] = act_fwd<float>(ACT, z[2 * j][nt][i]);
-              v[4 + i] = act_fwd<float>(ACT, z[2 * j + 1][nt][i]);
+              v[i] = act_fwd<float>(ACT, FP8 ? z[2 * j][nt][i] * sc : z[2 * j][nt][i]);
+              v[4 + i] = act_fwd<float>(ACT, FP8 ? z[2 * j + 1][nt][i] * sc : z[2 * j + 1][nt][i]);
             }
-            af[j][nt] = frag_pack(v);
+            af[j][nt] = O8::pack(v, r);
           }
       };
-      to_frags();
+      to_frags(sc0, ra0);
       // layer 1
 #pragma unroll
       for (int r = 0; r < MT; ++r)
@@ -242,11 +317,11 @@ __global__ __launch_bounds__(kThreads) void k_decode_bf16_mfma(const uint16_t* _
         for (int r = 0; r < MT; ++r)
         {
           const int row = 16 * r + n16;
-          const bf16x8 a = frag_lds(&wt[TLY.oW1() + row * H + swz(row, 32 * j + 8 * g, H)]);
+          const Frag a = O8::load(&wt[TLY.oW1() + row * H + swz(row, 32 * j + 8 * g, H)]);
 #pragma unroll
-          for (int nt = 0; nt < NTW; ++nt) z[r][nt] = mfma(a, af[j][nt], z[r][nt]);
+          for (int nt = 0; nt < NTW; ++nt) z[r][nt] = O8::mma(a, af[j][nt], z[r][nt]);
         }
-      to_frags();
+      to_frags(sc1, ra1);
       // output layer and the squared errors of outputs 4g + i
 #pragma unroll
       for (int nt = 0; nt < NTW; ++nt)
@@ -254,7 +329,7 @@ __global__ __launch_bounds__(kThreads) void k_decode_bf16_mfma(const uint16_t* _
         f32x4 yh = {0, 0, 0, 0};
 #pragma unroll
         for (int j = 0; j < KH; ++j)
-          yh = mfma(frag_lds(&wt[TLY.oWo() + n16 * H + swz(n16, 32 * j + 8 * g, H)]), af[j][nt], yh);
+          yh = O8::mma(O8::load(&wt[TLY.oWo() + n16 * H + swz(n16, 32 * j + 8 * g, H)]), af[j][nt], yh);
         const int loc = tb0 + nt * 16 + n16;
         const int64_t t = c0 + loc;
         const bool live = t < B;
@@ -265,9 +340,10 @@ __global__ __launch_bounds__(kThreads) void k_decode_bf16_mfma(const uint16_t* _
         {
           const int o = 4 * g + i;
           const bool use = live && o < O;
-          const float r = tgt[i] - yh[i];
+          const float yo = FP8 ? yh[i] * sco : yh[i];
+          const float r = tgt[i] - yo;
           sse[i] += use ? (double)r * (double)r : 0.0;
-          if (!GATHER && y && use) y[(int64_t)o * B + t] = yh[i];
+          if (!GATHER && y && use) y[(int64_t)o * B + t] = yo;
         }
       }
     }
@@ -279,7 +355,7 @@ __global__ __launch_bounds__(kThreads) void k_decode_bf16_mfma(const uint16_t* _
   // channel o = 4g + i: the 16 lanes of group g in every wave, in a fixed order
   __syncthreads();
   double* red = reinterpret_cast<double*>(xs_all);
-  static_assert(4 * 64 * XS * 2 >= kThreads * 4 * 8, "the channel reduction reuses the feature rows");
+  static_assert(4 * 64 * XS * sizeof(El) >= kThreads * 4 * 8, "the channel reduction reuses the feature rows");
 #pragma unroll
   for (int i = 0; i < 4; ++i) red[tid * 4 + i] = sse[i];
   __syncthreads();
@@ -299,7 +375,7 @@ __global__ __launch_bounds__(256) void k_decode_mfma_sum(int blocks, const doubl
   sum_channel_partials(blocks, part, sse);
 }
 
-typedef void (*MfmaDecode)(const uint16_t*, int, int, int64_t, const float*, const float*, float*, double*,
+typedef void (*MfmaDecode)(const uint8_t*, int, int, int64_t, const float*, const float*, float*, double*,
                            ntc_feature_desc, ntc_tiles, const float*, const float*, const uint8_t*);
 
 template <int H, int FP, int G, int C0, int C1, int G0MODE, int PEC, int LODS, int ACT>
@@ -311,62 +387,84 @@ constexpr bool layout_is(const ntc_feature_desc& fd, int h)
 
 #define NTC_EVAL_LAYOUT(H, FP, G, C0, C1, MODE, PEC, LODS, ACT)                                        \
   if (layout_is<H, FP, G, C0, C1, MODE, PEC, LODS, ACT>(fd, h) && act == ACT)                           \
-    return k_decode_bf16_mfma<H, FP, H == 64 ? 4 : 2, true, ACT, C0, C1, MODE, PEC, LODS>;
+    return k_decode_mfma<FP8, H, FP, H == 64 ? 4 : 2, true, ACT, C0, C1, MODE, PEC, LODS>;
 
 // The gathering kernel specialised for a compiled layout and activation,
 // else null.
+template <bool FP8>
 MfmaDecode mfma_kernel_fixed(const ntc_feature_desc& fd, int h, int act)
 {
   NTC_FUSED_LAYOUTS(NTC_EVAL_LAYOUT, NTC_EVAL_LAYOUT)
   return nullptr;
 }
 
-template <bool GATHER, int ACT>
+template <bool FP8, bool GATHER, int ACT>
 MfmaDecode mfma_kernel_act(int H, int FP)
 {
-  if (H == 64 && FP == 32) return k_decode_bf16_mfma<64, 32, 4, GATHER, ACT>;
-  if (H == 64 && FP == 64) return k_decode_bf16_mfma<64, 64, 4, GATHER, ACT>;
-  if (H == 64 && FP == 96) return k_decode_bf16_mfma<64, 96, 4, GATHER, ACT>;
-  if (H == 128 && FP == 32) return k_decode_bf16_mfma<128, 32, 2, GATHER, ACT>;
-  if (H == 128 && FP == 64) return k_decode_bf16_mfma<128, 64, 2, GATHER, ACT>;
-  if (H == 128 && FP == 96) return k_decode_bf16_mfma<128, 96, 2, GATHER, ACT>;
+  if (H == 64 && FP == 32) return k_decode_mfma<FP8, 64, 32, 4, GATHER, ACT>;
+  if (H == 64 && FP == 64) return k_decode_mfma<FP8, 64, 64, 4, GATHER, ACT>;
+  if (H == 64 && FP == 96) return k_decode_mfma<FP8, 64, 96, 4, GATHER, ACT>;
+  if (H == 128 && FP == 32) return k_decode_mfma<FP8, 128, 32, 2, GATHER, ACT>;
+  if (H == 128 && FP == 64) return k_decode_mfma<FP8, 128, 64, 2, GATHER, ACT>;
+  if (H == 128 && FP == 96) return k_decode_mfma<FP8, 128, 96, 2, GATHER, ACT>;
   return nullptr;
 }
 
-template <bool GATHER>
+template <bool FP8, bool GATHER>
 MfmaDecode mfma_kernel(int H, int FP, int act)
 {
-  return act == NTC_ACT_RELU ? mfma_kernel_act<GATHER, NTC_ACT_RELU>(H, FP)
-                             : mfma_kernel_act<GATHER, NTC_ACT_HARDGELU>(H, FP);
+  return act == NTC_ACT_RELU ? mfma_kernel_act<FP8, GATHER, NTC_ACT_RELU>(H, FP)
+                             : mfma_kernel_act<FP8, GATHER, NTC_ACT_HARDGELU>(H, FP);
 }
 
-int chunk_of(int) { return 256; }
-
 // Workgroups for a batch: persistent, two per CU (each copies the weight
-// tables once and walks its share of the chunks), at most one per chunk and
-// per 256 texels (the partial slots ntc_decode_workspace_bytes sizes).
-int mfma_blocks(int64_t B, int H)
+// tables once and walks its share of the chunks), at most one per 256
+// texels (the partial slots ntc_decode_workspace_bytes sizes).
+int mfma_blocks(int64_t B)
 {
   int dev = 0, cus = 256;
   if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
     cus = 256;
-  int64_t n = (B + chunk_of(H) - 1) / chunk_of(H);
-  const int64_t cap = (B + 255) / 256;
-  n = n < cap ? n : cap;
+  const int64_t n = (B + 255) / 256;
   return (int)(n < 2 * (int64_t)cus ? n : 2 * (int64_t)cus);
 }
 } // namespace
 
-// Host side of the table: header, then the BF16 tables W0 [H][FP] | W1 [H][H]
-// (columns permuted) | Wout [16][H] (columns permuted) in the fused step's
-// layout, every row's 16-byte chunks placed by the bank swizzle.
-size_t ntc_mfma_table_bytes(int F, int H) { return kHeaderBytes + 2 * (size_t)(H * round_up(F, 32) + H * H + 16 * H); }
+// Host side of the table: a header of 8 floats (FP8: the chain scales
+// s_w0 s_x, s_w1 s_a0, s_wo s_a1 and the reciprocals r_x, r_a0, r_a1), then
+// the weights in the operand format, W0 [H][FP] | W1 [H][H] (columns
+// permuted) | Wout [16][H] (columns permuted), in the fused step's layout,
+// every row's 8-element chunks placed by the bank swizzle.
+size_t ntc_mfma_table_bytes(bool fp8, int F, int H)
+{
+  return kHeaderBytes + (fp8 ? 1 : 2) * (size_t)(H * round_up(F, 32) + H * H + 16 * H);
+}
 
-int ntc_mfma_table_build(int F, int H, int O, const float* dec, void* table)
+// s: the FP8 scales {s_x, s_w0, s_w1, s_wo, s_a0, s_a1} (unused for BF16)
+int ntc_mfma_table_build(bool fp8, int F, int H, int O, const float* dec, const float* s, void* table)
 {
   const int FP = round_up(F, 32);
   std::memset(table, 0, kHeaderBytes);
-  uint16_t* tab = reinterpret_cast<uint16_t*>(static_cast<uint8_t*>(table) + kHeaderBytes);
+  float* hdr = static_cast<float*>(table);
+  float r[6] = {1, 1, 1, 1, 1, 1};
+  if (fp8)
+  {
+    for (int i = 0; i < 6; ++i) r[i] = 1.0f / s[i];
+    hdr[0] = s[1] * s[0];
+    hdr[1] = s[2] * s[4];
+    hdr[2] = s[3] * s[5];
+    hdr[3] = r[0];
+    hdr[4] = r[4];
+    hdr[5] = r[5];
+  }
+  uint8_t* t8 = static_cast<uint8_t*>(table) + kHeaderBytes;
+  uint16_t* t16 = reinterpret_cast<uint16_t*>(t8);
+  auto put = [&](int idx, float v, int layer) {
+    if (fp8)
+      t8[idx] = e4m3_byte(fp8_q(v, r[layer]));
+    else
+      t16[idx] = f2bf(v);
+  };
   const TableLayout L{H, FP, 16};
   const float* w0 = dec;
   const float* w1 = w0 + H * F;
@@ -375,31 +473,34 @@ int ntc_mfma_table_build(int F, int H, int O, const float* dec, void* table)
     for (int c = 0; c < FP; ++c)
     {
       const int src = swz(m, c, FP);
-      tab[L.oW0() + m * FP + c] = f2bf(src < F ? w0[m * F + src] : 0.0f);
+      put(L.oW0() + m * FP + c, src < F ? w0[m * F + src] : 0.0f, 1);
     }
   for (int m = 0; m < H; ++m)
-    for (int c = 0; c < H; ++c) tab[L.oW1() + m * H + c] = f2bf(w1[m * H + kperm_pos(swz(m, c, H))]);
+    for (int c = 0; c < H; ++c) put(L.oW1() + m * H + c, w1[m * H + kperm_pos(swz(m, c, H))], 2);
   for (int o = 0; o < 16; ++o)
-    for (int c = 0; c < H; ++c) tab[L.oWo() + o * H + c] = o < O ? f2bf(wo[o * H + kperm_pos(swz(o, c, H))]) : 0;
+    for (int c = 0; c < H; ++c) put(L.oWo() + o * H + c, o < O ? wo[o * H + kperm_pos(swz(o, c, H))] : 0.0f, 3);
   return NTC_OK;
 }
 
-int ntc_mfma_decode(bool gather, int H, int F, int O, int act, int64_t B, const void* table, const float* x,
+int ntc_mfma_decode(bool fp8, bool gather, int H, int F, int O, int act, int64_t B, const void* table, const float* x,
                     const float* yt, float* y, double* sse, void* workspace, const ntc_feature_desc* fd,
                     const ntc_tiles* tl, const float* g0, const float* g1, const uint8_t* pkg, hipStream_t stream)
 {
   const int FP = round_up(F, 32);
   NTC_RETURN_IF(act != NTC_ACT_RELU && act != NTC_ACT_HARDGELU, NTC_E_INVALID);
-  MfmaDecode k = gather ? mfma_kernel_fixed(*fd, H, act) : nullptr;
-  if (!k) k = gather ? mfma_kernel<true>(H, FP, act) : mfma_kernel<false>(H, FP, act);
+  MfmaDecode k = nullptr;
+  if (gather) k = fp8 ? mfma_kernel_fixed<true>(*fd, H, act) : mfma_kernel_fixed<false>(*fd, H, act);
+  if (!k)
+    k = fp8 ? (gather ? mfma_kernel<true, true>(H, FP, act) : mfma_kernel<true, false>(H, FP, act))
+            : (gather ? mfma_kernel<false, true>(H, FP, act) : mfma_kernel<false, false>(H, FP, act));
   NTC_RETURN_IF(!k, NTC_E_INVALID);
   NTC_RETURN_IF(gather && !pow2_tiles(*tl), NTC_E_INVALID);
-  const int blocks = mfma_blocks(B, H);
-  const uint16_t* w = reinterpret_cast<const uint16_t*>(static_cast<const uint8_t*>(table) + kHeaderBytes);
+  const int blocks = mfma_blocks(B);
   double* part = static_cast<double*>(workspace);
   const ntc_feature_desc fdv = fd ? *fd : ntc_feature_desc{};
   const ntc_tiles tlv = tl ? *tl : ntc_tiles{};
-  hipLaunchKernelGGL(k, dim3(blocks), dim3(kThreads), 0, stream, w, F, O, B, x, yt, y, part, fdv, tlv, g0, g1, pkg);
+  hipLaunchKernelGGL(k, dim3(blocks), dim3(kThreads), 0, stream, static_cast<const uint8_t*>(table), F, O, B, x, yt, y,
+                     part, fdv, tlv, g0, g1, pkg);
   hipLaunchKernelGGL(k_decode_mfma_sum, dim3(O), dim3(256), 0, stream, blocks, part, sse);
   return NTC_LAUNCH_STATUS();
 }
