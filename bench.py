@@ -163,9 +163,19 @@ def footprint_cells(si, M, D):
     return int(mask.sum())
 
 
-def cpu_baseline(ctx, first_step, n_steps):
+def cpu_model():
+    try:
+        for ln in open("/proc/cpuinfo"):
+            if ln.startswith("model name"):
+                return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(ctx, first_step, n_steps, warm=5):
     """The CPU oracle (C, OpenMP, FP32) running the same schedule steps from
-    the same initial parameters."""
+    the same initial parameters: `warm` untimed steps, then `n_steps` timed."""
     from oracle import oracle as O
 
     lay = ctx.layout()
@@ -176,7 +186,8 @@ def cpu_baseline(ctx, first_step, n_steps):
     steps = O.schedule(mt, L, TOTAL_STEPS, TILES, 1e-2, 1e-3)
     pk = [ctx.read_package(m) for m in range(lay.mip_count)]
     tr = O.Trainer(L, P, pk, seed=123, max_batch=TILES * 65536)
-    tr.step(steps[0], 0)  # warm caches / page in
+    for s in range(max(0, first_step - warm), first_step):  # warm caches / page in
+        tr.step(steps[s], s)
     texels = 0
     t0 = time.perf_counter()
     for s in range(first_step, first_step + n_steps):
@@ -185,8 +196,70 @@ def cpu_baseline(ctx, first_step, n_steps):
     dt = time.perf_counter() - t0
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     return {"value": texels / dt, "unit": "texel-samples/s", "cores": cores, "kind": "port",
-            "sample": f"{n_steps} schedule steps ({texels} texels) of the same workload after 1 warm-up step, "
-                      f"CPU oracle (C, OpenMP, FP32), {dt:.1f} s"}
+            "cpu_model": cpu_model(),
+            "sample": f"{n_steps} schedule steps ({texels} texels) of the same workload after {warm} warm-up "
+                      f"steps, CPU oracle (C, OpenMP, FP32, {cores} threads), {dt:.1f} s"}
+
+
+def lod0_micro(ntc, ctx, reps=50, seed=7):
+    """The fused training kernel alone at a fixed lod 0 (8 random 256^2 tiles
+    of the 4096^2 mip, the context's current latents and decoder, noise on):
+    wall time per launch over `reps` back-to-back launches."""
+    from openntc_amd import _lib as L
+    lib = ntc.load()
+    lay = ctx.layout()
+    H, F, O = lay.hidden_dim, lay.feature_dim, lay.out_dim
+    fd = lay.feat  # level 0, the level lod 0 samples
+    g0, g1 = ctx.read_grid(0, 0), ctx.read_grid(1, 0)
+    dec = ctx.read_decoder()
+    pkg = ctx.read_package(0)
+    rng = np.random.default_rng(seed)
+    origins = rng.integers(0, DIM, 2 * TILES).astype(np.int32)
+    bufs = []
+
+    def dev(a=None, nbytes=None):
+        p = ctypes.c_void_p()
+        n = a.nbytes if a is not None else nbytes
+        ntc.check(lib.ntc_dev_alloc(n, ctypes.byref(p)), "ntc_dev_alloc")
+        bufs.append(p)
+        if a is not None:
+            a = np.ascontiguousarray(a)
+            ntc.check(lib.ntc_dev_upload(p, a.ctypes.data, a.nbytes), "upload")
+        return p.value
+    try:
+        d_dec = dev(dec)
+        tab = dev(nbytes=2 * lib.ntc_fused_table_elems(ctypes.byref(fd), H))
+        ntc.check(lib.ntc_pack_decoder_bf16(ctypes.byref(fd), H, O, d_dec, tab, None), "pack")
+        maxb = ctypes.c_int32()
+        lib.ntc_fused_max_blocks(ctypes.byref(maxb))
+        B = TILES * 256 * 256
+        blocks = lib.ntc_fused_blocks_for(ctypes.byref(fd), H, lay.activation, B, maxb.value)
+        a = L.FusedArgs()
+        a.feat = fd
+        a.tiles = L.Tiles(dev(origins), TILES, 256, DIM, 0.0)
+        a.noise = L.Noise(123, 1, 0, 1, lay.g0_delta, lay.g1_delta)
+        a.hidden_dim, a.out_dim, a.activation = H, O, lay.activation
+        a.g0, a.g1, a.weights, a.targets = dev(g0), dev(g1), tab, dev(pkg)
+        a.loss_scale = -2.0 / (O * B)
+        a.scatter = 1
+        a.dg0, a.dg1 = dev(nbytes=g0.nbytes), dev(nbytes=g1.nbytes)
+        a.dw_partials, a.sse_partials = dev(nbytes=4 * blocks * dec.size), dev(nbytes=8 * blocks)
+        a.blocks = blocks
+        for _ in range(3):
+            ntc.check(lib.ntc_fused_train_step(ctypes.byref(a), None), "fused step")
+        lib.ntc_dev_synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            lib.ntc_fused_train_step(ctypes.byref(a), None)
+        ntc.check(lib.ntc_dev_synchronize(), "synchronize")
+        ms = (time.perf_counter() - t0) * 1e3 / reps
+    finally:
+        for p in bufs:
+            lib.ntc_dev_free(p)
+    flop = 6.0 * (F * H + H * H + H * O) * B
+    return {"kernel": "k_fused_train", "lod": 0, "texels": B, "ms": ms, "texels_per_s": B / (ms * 1e-3),
+            "tflops": flop / (ms * 1e-3) / 1e12, "mfma_frac": flop / (ms * 1e-3) / 1e12 / BF16_PEAK_TFLOPS,
+            "sample": f"{reps} back-to-back launches, wall clock"}
 
 
 def pmc_traffic(kernel):
@@ -208,7 +281,7 @@ def pmc_traffic(kernel):
     return None
 
 
-def time_to_psnr(ntc, chunk, workload=None, only=None):
+def time_to_psnr(ntc, chunk, workload=None, only=None, report=True):
     """Seconds of training for the fused BF16 path to reach the plain FP32
     path's final PSNR - 0.05 dB on the same workload and seed (default: the
     4096^2 bench workload). Both paths run the whole schedule; the fused one
@@ -233,7 +306,7 @@ def time_to_psnr(ntc, chunk, workload=None, only=None):
             if r.returncode != 0:
                 raise SystemExit(f"time-to-psnr {path} run failed ({r.returncode})")
             res[path] = json.loads(r.stdout.strip().splitlines()[-1])
-        return time_to_psnr_report(res, w, total)
+        return time_to_psnr_report(res, w, total, report)
     lib = ntc.load()
     res = {}
     for path in (only,):
@@ -262,7 +335,7 @@ def time_to_psnr(ntc, chunk, workload=None, only=None):
     print(json.dumps(res[only]), flush=True)
 
 
-def time_to_psnr_report(res, w, total):
+def time_to_psnr_report(res, w, total, report=True):
     target = res["plain"]["final_psnr"] - 0.05
     hit = next((t for t in res["fused"]["trace"] if t["psnr"] >= target), None)
     lw = w["layout"]
@@ -279,7 +352,9 @@ def time_to_psnr_report(res, w, total):
            "seconds_to_target": hit["train_s"] if hit else None,
            "steps_to_target": hit["steps"] if hit else None,
            "speedup_full_schedule": res["plain"]["train_s"] / res["fused"]["train_s"]}
-    print(json.dumps(out), flush=True)
+    if report:
+        print(json.dumps(out), flush=True)
+    return out
 
 
 def eval_precisions(ntc, reps):
@@ -412,10 +487,12 @@ def main():
     ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--path", choices=["fused", "plain"], default="fused")
-    ap.add_argument("--cpu-steps", type=int, default=40)
+    ap.add_argument("--cpu-steps", type=int, default=50)
     ap.add_argument("--detail-steps", type=int, default=100,
                     help="steps after the timed region over which the step's other kernels are timed")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-time-to-psnr", action="store_true",
+                    help="leave out the seconds-to-FP32-PSNR pair (two full schedules, about a minute)")
     ap.add_argument("--time-to-psnr", type=int, default=0, metavar="CHUNK",
                     help="only measure the seconds for the fused path to reach the plain path's final PSNR, "
                          "evaluating every CHUNK steps (runs both paths over the whole schedule)")
@@ -448,6 +525,12 @@ def main():
     if a.sets:
         sets_bench(ntc, a.warmup, a.steps)
         return
+    ttp = None
+    if not a.no_time_to_psnr and a.path == "fused":
+        # the second half of the metric: each path in a child process started
+        # before this process does any GPU work
+        log("time to FP32 PSNR: the plain and the fused path over the whole schedule, one child each")
+        ttp = time_to_psnr(ntc, 500, report=False)
     path = ntc.TrainPath.Fused if a.path == "fused" else ntc.TrainPath.Plain
     if a.warmup + a.steps + (a.detail_steps if a.path == "fused" else 0) > TOTAL_STEPS:
         raise SystemExit(f"warmup + steps + detail steps exceed the {TOTAL_STEPS}-step schedule")
@@ -621,6 +704,14 @@ def main():
                                    "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": f["mfma_frac"],
                                    "traffic": pmc_traffic("k_fused_train")}
 
+    if a.path == "fused":
+        out["kernels"]["fused_train_step_lod0"] = lod0_micro(ntc, ctx)
+        log(f"lod-0 fused kernel: {out['kernels']['fused_train_step_lod0']['ms']:.4f} ms")
+    if ttp is not None:
+        out["time_to_psnr"] = {k: ttp[k] for k in ("metric", "plain_fp32", "target_psnr", "final_within_0.05dB",
+                                                    "seconds_to_target", "steps_to_target",
+                                                    "speedup_full_schedule")}
+        out["time_to_psnr"]["fused_bf16"] = {k: ttp["fused_bf16"][k] for k in ("train_s", "final_psnr", "trace")}
     if not a.no_cpu_baseline:
         log(f"cpu baseline: {a.cpu_steps} oracle steps")
         out["cpu_baseline"] = cpu_baseline(ctx, a.warmup, a.cpu_steps)
