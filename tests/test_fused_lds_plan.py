@@ -1,0 +1,298 @@
+"""Host emulation of the fused training kernel's LDS plan and index math
+(openntc_amd/csrc/fused_body.h), for every layout the library compiles.
+
+- The plan: the BF16 array `smem` (weight tables | X^T | a union region that
+  holds X and the staged footprint boxes during the gather, the
+  decoder-gradient staging rows during the rounds) and the separate static
+  arrays dX (`dxs`), target words, axis tables and tile origins. Its byte
+  total, restated from the kernel's constants, must equal the group segment
+  size the compiler recorded for each kernel in the built code object, and
+  fit the 160 KiB of LDS a workgroup can have.
+- The indices: every LDS element each lane of each wave reads or writes
+  (weight-table fragments, X rows and X^T rows, staging rows, dX rows,
+  staged boxes, target words), over every loop variable, evaluated with the
+  kernel's own index expressions and checked against the bounds of the
+  region it belongs to; global indices of the partial writes and of the
+  decoder update's reduction against their buffers.
+
+No GPU is used. `_plan` and the index expressions restate the kernel's; a
+change to a stride there that is not made here fails the size check.
+"""
+from __future__ import annotations
+
+import os
+import re
+import shutil
+import subprocess
+import tempfile
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+BODY = os.path.join(ROOT, "openntc_amd", "csrc", "fused_body.h")
+OBJ = os.path.join(ROOT, "openntc_amd", "_build", "kernels_fused.hip.o")
+LLVM = "/opt/rocm/llvm/bin"
+LDS_LIMIT = 160 * 1024
+THREADS = 256
+CORNERS, BILINEAR = 0, 1
+
+
+def _layouts():
+    """(H, FP, G, C0, C1, G0MODE, PEC, LODS, ACT) of every compiled layout,
+    read from the NTC_FUSED_LAYOUTS list."""
+    src = open(BODY).read()
+    block = src[src.index("#define NTC_FUSED_LAYOUTS"):]
+    block = block[:block.index("\n\n")]
+    out = []
+    for m in re.finditer(r"X2?\((\d+), (\d+), (\d+), (\d+), (\d+), NTC_G0_(\w+), (\d), (\d), NTC_ACT_(\w+)\)", block):
+        H, FP, G, C0, C1 = (int(m.group(i)) for i in range(1, 6))
+        mode = CORNERS if m.group(6) == "CORNERS" else BILINEAR
+        act = 1 if m.group(9) == "RELU" else 0
+        out.append((H, FP, G, C0, C1, mode, int(m.group(7)), int(m.group(8)), act))
+    assert len(out) >= 9
+    return out
+
+
+def _nt(H):
+    return 1 if H >= 128 else 2
+
+
+def _plan(H, FP, G, C0, C1, mode, NT):
+    """The kernel's Cfg constants and LDS byte total."""
+    PASS = 64 * NT
+    BH = PASS // 16
+    WT = H * FP + 2 * H * H + 48 * H + G * H
+    XFN = FP * PASS
+    XS = FP + 8
+    STAGE_B = 2 * H * 64 + 16 * 64
+    WMAX = 18
+    Q0, Q1 = (C0 + 3) // 4, (C1 + 3) // 4
+    CP0, CP1 = 4 * Q0, 4 * Q1
+    G1B = 4 * C0 if mode == CORNERS else C0
+    GSLOTS = G1B + C1
+    GS_BANK = G + (16 - G) % 64
+
+    def lds_bytes(sw, sv, gs):
+        u = PASS * XS + 2 * sw * sv * (CP0 + CP1)
+        return 2 * (WT + XFN + max(u, STAGE_B)) + 4 * PASS * gs + 4 * 192 + 4 * 128 + 16 * PASS
+
+    cands = [(WMAX, BH + 3, GS_BANK), (WMAX, BH + 3, G + 4), (10, BH // 2 + 3, GS_BANK), (10, BH // 2 + 3, G + 4)]
+    sel = next((i for i, c in enumerate(cands) if lds_bytes(*c) <= LDS_LIMIT), 4)
+    assert sel < 4, "layout does not fit"
+    SW, SV, GS = cands[sel]
+    UNION = max(PASS * XS + 2 * SW * SV * (CP0 + CP1), STAGE_B)
+    return dict(PASS=PASS, BH=BH, WT=WT, XFN=XFN, XS=XS, UNION=UNION, SW=SW, SV=SV, GS=GS, CP0=CP0, CP1=CP1,
+                Q0=Q0, Q1=Q1, G1B=G1B, GSLOTS=GSLOTS, MT=H // 16, KF=FP // 32, KH=H // 32, GT=G // 16,
+                RW=H // 4, MTW=H // 64, NTO=H // 64, SBOX0=SW * SV * CP0,
+                DXS=PASS * GS, total=lds_bytes(SW, SV, GS))
+
+
+def swz(row, col, wd):
+    nc = wd >> 3
+    if nc & (nc - 1):
+        return col
+    per = 1 if nc >= 16 else 16 // nc
+    m = (16 if nc >= 16 else nc) - 1
+    return (((col >> 3) ^ ((row // per) & m)) << 3) | (col & 7)
+
+
+def kperm(j, g, e):
+    return 32 * j + (4 * g + e if e < 4 else 16 + 4 * g + (e - 4))
+
+
+class Region:
+    """Element bounds of one LDS array (or sub-range); records every index."""
+
+    def __init__(self, name, size):
+        self.name, self.size, self.lo, self.hi = name, size, 1 << 60, -1
+
+    def touch(self, idx, width=1):
+        idx = np.asarray(idx, np.int64)
+        self.lo = min(self.lo, int(idx.min()))
+        self.hi = max(self.hi, int(idx.max()) + width - 1)
+
+    def check(self):
+        assert self.hi >= 0, f"{self.name}: never touched"
+        assert 0 <= self.lo and self.hi < self.size, f"{self.name}: [{self.lo}, {self.hi}] outside [0, {self.size})"
+
+
+def _emulate(H, FP, G, C0, C1, mode, PEC, LODS, NT):
+    P = _plan(H, FP, G, C0, C1, mode, NT)
+    PASS, XS = P["PASS"], P["XS"]
+    lanes = np.arange(64)
+    n16, g = lanes & 15, lanes >> 4
+    wt = Region("weight tables", P["WT"])
+    xnf = Region("X rows", PASS * XS)
+    xfn = Region("X^T rows", P["XFN"])
+    sbox = Region("staged boxes (floats)", P["UNION"] // 2 - PASS * XS // 2)
+    stA, stB, stY = Region("stA", H * 64), Region("stB", H * 64), Region("stY", 16 * 64)
+    dxs = Region("dX", P["DXS"])
+    tgt = Region("target words", PASS * 4)
+    # the union holds X + boxes, then stA | stB | stY
+    assert PASS * XS + 2 * P["SW"] * P["SV"] * (P["CP0"] + P["CP1"]) <= P["UNION"]
+    assert 2 * H * 64 + 16 * 64 <= P["UNION"]
+    oW0, oW1 = 0, H * FP
+    oWo = oW1 + H * H
+    oWoT = oWo + 16 * H
+    oW1T = oWoT + 32 * H
+    oW0T = oW1T + H * H
+    assert oW0T + G * H == P["WT"]
+    vswz = np.vectorize(swz)
+    for wave in range(4):
+        for nt in range(NT):
+            tloc = wave * 16 * NT + nt * 16 + n16
+            for j in range(P["KF"]):  # layer 0: X rows and W0
+                xnf.touch(tloc * XS + 32 * j + 8 * g, 8)
+                for r in range(P["MT"]):
+                    row = 16 * r + n16
+                    wt.touch(oW0 + row * FP + vswz(row, 32 * j + 8 * g, FP), 8)
+            for j in range(P["KH"]):
+                for r in range(P["MT"]):
+                    row = 16 * r + n16
+                    wt.touch(oW1 + row * H + vswz(row, 32 * j + 8 * g, H), 8)
+                    wt.touch(oW1T + row * H + vswz(row, 32 * j + 8 * g, H), 8)
+                wt.touch(oWo + n16 * H + vswz(n16, 32 * j + 8 * g, H), 8)
+                for q in range(P["GT"]):
+                    row = 16 * q + n16
+                    wt.touch(oW0T + row * H + vswz(row, 32 * j + 8 * g, H), 8)
+            for r in range(P["MT"]):
+                row = 16 * r + n16
+                wt.touch(oWoT + row * 32 + vswz(row, 8 * g, 32), 8)
+            for r in range(P["GT"]):  # dX rows: 16-byte or single writes of slots < GSLOTS
+                s0 = 16 * r + 4 * g
+                ok = s0 < P["GSLOTS"]
+                if ok.any():
+                    dxs.touch((tloc * P["GS"] + s0)[ok], 4)
+            tgt.touch(tloc * 4 + g)
+        # decoder-gradient rounds
+        col = wave * 16 + n16
+        odd = n16 & 1
+        pcol = [(odd * 64 + vswz(2 * (2 * g + b), col & ~1, 64)) >> 1 for b in range(2)]
+        scol = [vswz(2 * (2 * g + b), col, 64) for b in range(2)]
+        for j in range(P["KH"]):
+            for d in range(4):
+                at = kperm(j, g, 2 * d) * 32 + pcol[d & 1]  # dword index
+                for st in (stA, stB):
+                    st.touch(2 * at, 2)
+        for i in range(4):
+            stY.touch((4 * g + i) * 64 + scol[i >> 1])
+        for mt in range(P["MTW"]):
+            for kc in range(2):
+                row = P["RW"] * wave + 16 * mt + n16
+                stB.touch(row * 64 + vswz(row, 32 * kc + 8 * g, 64), 8)  # dW1 A operand
+                stA.touch(row * 64 + vswz(row, 32 * kc + 8 * g, 64), 8)  # dW0 A operand
+                for c in range(P["MT"]):
+                    rr = 16 * c + n16
+                    stA.touch(rr * 64 + vswz(rr, 32 * kc + 8 * g, 64), 8)
+        for rnd in range(NT):
+            for kc in range(2):
+                kk = 32 * kc + 8 * g
+                xc = (kk >> 4) * 16 * NT + 16 * rnd + (kk & 15)
+                for c in range(FP // 16):
+                    xfn.touch((16 * c + n16) * PASS + xc, 8)
+        for kc in range(2):
+            stY.touch(n16 * 64 + vswz(n16, 32 * kc + 8 * g, 64), 8)
+            for c in range(P["NTO"]):
+                hc = (wave * P["NTO"] + c) * 16 + n16
+                stB.touch(hc * 64 + vswz(hc, 32 * kc + 8 * g, 64), 8)
+    # staged gather: put4 (8-byte X-row writes, 32-bit X^T pair writes) and
+    # single puts for every slot < FP of every texel; box reads for boxes up
+    # to SW x SV cells
+    t = np.arange(PASS)
+    for slot0 in range(0, FP, 4):
+        xnf.touch(t * XS + slot0, 4)
+        for oddv in (0, 1):
+            tt = t[(t & 1) == oddv]
+            for s in (slot0 + oddv, slot0 + 2 + oddv):
+                xfn.touch(2 * (((s * PASS + (tt & ~1)) >> 1)), 2)
+    for slot in range(FP):
+        xnf.touch(t * XS + slot)
+        xfn.touch(slot * PASS + t)
+    for g1, (Q, CP) in enumerate(((P["Q0"], P["CP0"]), (P["Q1"], P["CP1"]))):
+        base = P["SBOX0"] if g1 else 0
+        bw, bh = P["SW"], P["SV"]
+        ay, ax = np.meshgrid(np.arange(bh), np.arange(bw), indexing="ij")
+        for q in range(Q):
+            sbox.touch(base + 4 * q + (ay * bw + ax) * CP, 4)  # float4 reads
+            cb = np.arange(bw * bh)
+            sbox.touch(base + cb * CP + 4 * q, 4)  # prefetch_commit writes
+    # box gradients: dX reads of slot0 + c*C + ch, ch < C, texel 4kk + q
+    for slot0, C, nc in ((0, C0, 4 if mode == CORNERS else 1), (P["G1B"], C1, 1)):
+        tex = np.arange(PASS)
+        for c in range(nc):
+            for ch in range(C):
+                dxs.touch(tex * P["GS"] + slot0 + c * C + ch)
+    for r in (wt, xnf, xfn, sbox, stA, stB, stY, dxs, tgt):
+        r.check()
+    # every dX slot stays inside its row (no row reaches into the next texel's)
+    assert P["GSLOTS"] <= G <= P["GS"]
+    return P
+
+
+@pytest.mark.parametrize("lay", _layouts(), ids=lambda l: "H%d-F%d-G%d-c%d-%d-m%d-a%d" % (l[0], l[1], l[2], l[3], l[4], l[5], l[8]))
+def test_fused_lds_indices_in_bounds(lay):
+    H, FP, G, C0, C1, mode, PEC, LODS, act = lay
+    P = _emulate(H, FP, G, C0, C1, mode, PEC, LODS, _nt(H))
+    assert P["total"] <= LDS_LIMIT
+
+
+def _code_object_lds():
+    """{mangled kernel name: group segment bytes} of the built fused kernels."""
+    if not os.path.exists(OBJ) or not os.path.exists(os.path.join(LLVM, "llvm-readelf")):
+        pytest.skip("library not built here (openntc_amd/_build) or no LLVM tools")
+    with tempfile.TemporaryDirectory() as d:
+        fat, co = os.path.join(d, "fat.bin"), os.path.join(d, "k.co")
+        subprocess.run([os.path.join(LLVM, "llvm-objcopy"), "--dump-section", f".hip_fatbin={fat}", OBJ],
+                       check=True, capture_output=True)
+        subprocess.run([os.path.join(LLVM, "clang-offload-bundler"), "--unbundle", "--type=o", f"--input={fat}",
+                        "--targets=hipv4-amdgcn-amd-amdhsa--gfx950", f"--output={co}"], check=True,
+                       capture_output=True)
+        notes = subprocess.run([os.path.join(LLVM, "llvm-readelf"), "--notes", co], check=True,
+                               capture_output=True, text=True).stdout
+    out, size = {}, None
+    for ln in notes.splitlines():
+        m = re.match(r"\s*\.group_segment_fixed_size:\s+(\d+)", ln)
+        if m:
+            size = int(m.group(1))
+        m = re.match(r"\s*\.name:\s+(_Z\S+)", ln)
+        if m and size is not None:
+            out[m.group(1)] = size
+            size = None
+    return out
+
+
+def test_lds_plan_matches_code_object():
+    sizes = _code_object_lds()
+    seen = 0
+    for name, size in sizes.items():
+        m = re.search(r"k_fused_train(?:_sets)?I((?:Li-?\d+E)+)E", name)
+        if not m:
+            continue
+        H, FP, G, C0, C1, mode, PEC, LODS, act, NT = (int(v) for v in re.findall(r"Li(-?\d+)E", m.group(1)))
+        P = _plan(H, FP, G, C0, C1, mode, NT)
+        assert size == P["total"], (name, size, P["total"])
+        assert size <= LDS_LIMIT
+        seen += 1
+    assert seen >= 2 * len(_layouts()) - 4  # k_fused_train and k_fused_train_sets instances
+
+
+def _reduction_reads(blocks, n_dec):
+    """Highest partial index k_decoder_update reads (blocks of 32 loads, then
+    the rest one by one)."""
+    hi = -1
+    b = 0
+    while b + 32 <= blocks:
+        hi = max(hi, (b + 31) * n_dec + n_dec - 1)
+        b += 32
+    while b < blocks:
+        hi = max(hi, b * n_dec + n_dec - 1)
+        b += 1
+    return hi
+
+
+@pytest.mark.parametrize("blocks", [1, 31, 32, 33, 64, 255, 256, 257, 512])
+def test_decoder_reduction_reads_in_bounds(blocks):
+    H, F = 128, 40
+    n_dec = H * F + H * H + 16 * H
+    assert _reduction_reads(blocks, n_dec) == blocks * n_dec - 1
