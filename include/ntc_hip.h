@@ -641,6 +641,20 @@ int ntc_ctx_kernel_times(ntc_context* ctx, float* out, int32_t n);
  * the fused kernel only (fewer events between the step's kernels; the other
  * figures of ntc_ctx_kernel_times are then -1). Other values: InvalidState. */
 int ntc_ctx_set_timing(ntc_context* ctx, int32_t enabled);
+/* Fault injection for tests of the error paths. No counterpart in the
+ * reference; it makes the paths where the reference returns CudaFailure or
+ * AllocationFailure after a failed CUDA call (libopenntc.cpp:17-30, 506-527)
+ * reachable on demand. Arms `site` so that its nth check from now on (counted
+ * over every context of the process, n >= 1) fails; n = 0 disarms. Sites:
+ * NTC_FAULT_ALLOC, a device allocation of a Context or Tensor (hipMalloc is
+ * not called; out of memory); NTC_FAULT_LAUNCH, a kernel launch of a Context's
+ * step, update, evaluation or export (the launch is skipped); NTC_FAULT_COPY,
+ * a Context's blocking copy between host and device (skipped). Returns 0, or
+ * NTC_E_INVALID for an unknown site or n < 0. */
+#define NTC_FAULT_ALLOC 1
+#define NTC_FAULT_LAUNCH 2
+#define NTC_FAULT_COPY 3
+int ntc_debug_inject_fault(int32_t site, int32_t nth);
 
 /* ---- several texture sets trained together --------------------------------
  * No counterpart in the reference (one Context trains one set). The contexts

@@ -193,6 +193,7 @@ SIGNATURES = {
     "ntc_ctx_kernel_times": (_I, [_P, _P, _I]),
     "ntc_ctx_flush_grids": (_I, [_P]),
     "ntc_ctx_set_timing": (_I, [_P, _I]),
+    "ntc_debug_inject_fault": (_I, [_I, _I]),
     "ntc_synthetic_pbr": (_I, [_I, _U, _P, _P, _P]),
     "ntc_dev_alloc": (_I, [_SZ, _P]),
     "ntc_dev_free": (_I, [_P]),

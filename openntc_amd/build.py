@@ -78,7 +78,31 @@ def build(jobs: int = 8, verbose: bool = True, incremental: bool = False) -> str
     os.replace(tmp, LIB)
     if verbose:
         print(f"built {LIB}")
+    build_api_flow(verbose)
     return LIB
+
+
+API_FLOW_SRC = os.path.join(ROOT, "tests", "cpp", "api_flow.cpp")
+
+
+def build_api_flow(verbose: bool = True) -> str | None:
+    """The C++ caller of the public API (tests/cpp/api_flow.cpp), linked
+    against the library the way an application links it; found at run time
+    relative to its own directory."""
+    if not os.path.exists(API_FLOW_SRC):
+        return None
+    out = API_FLOW_SRC[:-len(".cpp")]
+    cmd = ["g++", "-std=c++20", "-O1", "-Wall", "-D__HIP_PLATFORM_AMD__", *INCLUDES, "-I",
+           os.path.join(ROCM, "include"), API_FLOW_SRC, "-o", out + ".tmp", "-L", HERE, "-lopenntc_amd",
+           "-L", os.path.join(ROCM, "lib"), "-lamdhip64", "-Wl,-rpath,$ORIGIN/../../openntc_amd",
+           "-Wl,-rpath," + os.path.join(ROCM, "lib")]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"api_flow build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    os.replace(out + ".tmp", out)
+    if verbose:
+        print(f"built {out}")
+    return out
 
 
 if __name__ == "__main__":

@@ -419,4 +419,11 @@ int ntc_ctx_set_timing(ntc_context* c, int32_t enabled)
   c->ctx.GetImpl()->timing = enabled;
   return 0;
 }
+
+int ntc_debug_inject_fault(int32_t site, int32_t nth)
+{
+  if (site < NTC_FAULT_ALLOC || site > NTC_FAULT_COPY || nth < 0) return NTC_E_INVALID;
+  openntc::ArmFault(site, nth);
+  return 0;
+}
 } // extern "C"

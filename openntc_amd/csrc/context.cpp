@@ -27,10 +27,10 @@
     if ((expr) != hipSuccess) return ret; \
   } while (0)
 
-#define CTX_NTC(expr)                            \
-  do                                             \
-  {                                              \
-    if ((expr) != NTC_OK) return Result::CudaFailure; \
+#define CTX_NTC(expr)                                                                \
+  do                                                                                 \
+  {                                                                                  \
+    if (FaultHit(kFaultLaunch) || (expr) != NTC_OK) return Result::CudaFailure; \
   } while (0)
 
 namespace openntc
@@ -60,12 +60,35 @@ int32_t DrawUniformInt(std::mt19937& g, int32_t lo, int32_t hi)
 
 bool DrawBernoulli(std::mt19937& g, double p) { return ((double)(uint32_t)g() * 0x1p-32) < p; }
 
+// ---- fault injection ---------------------------------------------------------------
+namespace
+{
+std::atomic<int> g_fault[4];
+}
+
+void ArmFault(int site, int nth) { g_fault[site].store(nth < 0 ? 0 : nth); }
+
+bool FaultHit(int site)
+{
+  int left = g_fault[site].load(std::memory_order_relaxed);
+  while (left > 0)
+    if (g_fault[site].compare_exchange_weak(left, left - 1)) return left == 1;
+  return false;
+}
+
+// hipMalloc behind the allocation fault site.
+static hipError_t DeviceAlloc(void** p, size_t n)
+{
+  if (FaultHit(kFaultAlloc)) return hipErrorOutOfMemory;
+  return hipMalloc(p, n);
+}
+
 // ---- DeviceBuffer ----------------------------------------------------------------
 hipError_t DeviceBuffer::Alloc(size_t n)
 {
   Free();
   if (n == 0) return hipSuccess;
-  hipError_t e = hipMalloc(&ptr, n);
+  hipError_t e = DeviceAlloc(&ptr, n);
   if (e != hipSuccess)
   {
     ptr = nullptr;
@@ -90,7 +113,7 @@ Result Tensor2d::Init(int x, int y)
   Destroy();
   shape_[0] = x;
   shape_[1] = y;
-  if (hipMalloc((void**)&dev_, sizeof(float) * (size_t)x * y) != hipSuccess) return Result::AllocationFailure;
+  if (DeviceAlloc((void**)&dev_, sizeof(float) * (size_t)x * y) != hipSuccess) return Result::AllocationFailure;
   initialized_ = true;
   return Result::Success;
 }
@@ -124,7 +147,7 @@ Result Tensor3d::Init(int x, int y, int z)
   shape_[0] = x;
   shape_[1] = y;
   shape_[2] = z;
-  if (hipMalloc((void**)&dev_, sizeof(float) * (size_t)x * y * z) != hipSuccess) return Result::AllocationFailure;
+  if (DeviceAlloc((void**)&dev_, sizeof(float) * (size_t)x * y * z) != hipSuccess) return Result::AllocationFailure;
   initialized_ = true;
   return Result::Success;
 }
@@ -158,7 +181,7 @@ Result U8Tensor3d::Init(int x, int y, int z)
   shape_[0] = x;
   shape_[1] = y;
   shape_[2] = z;
-  if (hipMalloc((void**)&dev_, (size_t)x * y * z) != hipSuccess) return Result::AllocationFailure;
+  if (DeviceAlloc((void**)&dev_, (size_t)x * y * z) != hipSuccess) return Result::AllocationFailure;
   initialized_ = true;
   return Result::Success;
 }
@@ -184,7 +207,7 @@ Result IntTensor1d::Init(int x)
 {
   Destroy();
   shape_[0] = x;
-  if (hipMalloc((void**)&dev_, sizeof(int) * (size_t)x) != hipSuccess) return Result::AllocationFailure;
+  if (DeviceAlloc((void**)&dev_, sizeof(int) * (size_t)x) != hipSuccess) return Result::AllocationFailure;
   initialized_ = true;
   return Result::Success;
 }
@@ -1467,11 +1490,12 @@ TrainProgress Context::Train(int32_t num_batches)
   TrainProgress tp = {};
   tp.batches_complete_ = I.batch_i;
   tp.total_batches_ = I.batch_count;
+  // a refused call is reported in the progress only; the last error stays
+  // (libopenntc.cpp:536-548)
   if (!I.manifest_loaded || (I.phase != TrainPhase::TrainInProgress && I.phase != TrainPhase::TrainComplete))
   {
     tp.phase_ = TrainPhase::TrainError;
     tp.result_ = Result::InvalidState;
-    I.last_error = Result::InvalidState;
     return tp;
   }
   const int remaining = std::max(0, I.batch_count - I.batch_i);

@@ -33,6 +33,19 @@ struct StepPlan
   int32_t grid_update, lock;
 };
 
+// Fault injection for tests of the error paths: the armed site's n-th check
+// from now on fails as HIP would have (allocation: out of memory; launch: the
+// launch is skipped and reported failed; copy: the copy is skipped and
+// reported failed). Process-wide, one-shot per arming.
+enum FaultSite : int
+{
+  kFaultAlloc = 1,
+  kFaultLaunch = 2,
+  kFaultCopy = 3,
+};
+bool FaultHit(int site);
+void ArmFault(int site, int nth);
+
 struct DeviceBuffer
 {
   void* ptr = nullptr;
@@ -174,6 +187,7 @@ struct Context::Impl
   // not wait for the kernels that produce or read the buffer.
   hipError_t CopySync(void* dst, const void* src, size_t bytes, hipMemcpyKind kind)
   {
+    if (FaultHit(kFaultCopy)) return hipErrorUnknown;
     const hipError_t e = hipMemcpyAsync(dst, src, bytes, kind, stream);
     return e == hipSuccess ? hipStreamSynchronize(stream) : e;
   }

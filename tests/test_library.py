@@ -85,3 +85,27 @@ def test_context_without_gpu_reports_failure():
     prog = ctx.Train(1)
     assert prog.phase == ntc.TrainPhase.TrainError and prog.result == ntc.Result.InvalidState
     assert ctx.LoadManifest("/nonexistent.json") in (ntc.Result.FileNotFound, ntc.Result.InvalidState)
+
+
+def test_fault_injection_switch_arguments():
+    """The fault-injection switch takes the three sites and n >= 0 (n = 0
+    disarms) and refuses anything else; arming touches no device."""
+    lib = ntc.load()
+    for site in (1, 2, 3):
+        assert lib.ntc_debug_inject_fault(site, 3) == 0
+        assert lib.ntc_debug_inject_fault(site, 0) == 0
+    assert lib.ntc_debug_inject_fault(0, 1) == L.NTC_E_INVALID
+    assert lib.ntc_debug_inject_fault(4, 1) == L.NTC_E_INVALID
+    assert lib.ntc_debug_inject_fault(2, -1) == L.NTC_E_INVALID
+
+
+def test_cpp_api_caller_links():
+    """The C++ caller of the public API (tests/cpp/api_flow.cpp) is built
+    against the library and its dynamic references resolve; without its
+    arguments it prints its usage and touches no device."""
+    import subprocess
+    exe = os.path.join(ROOT, "tests", "cpp", "api_flow")
+    if not os.path.exists(exe):
+        pytest.skip("tests/cpp/api_flow not built (python -m openntc_amd.build)")
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 2 and "usage: api_flow" in r.stderr
