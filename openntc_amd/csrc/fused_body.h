@@ -58,26 +58,62 @@ __device__ __forceinline__ bf16x8 frag_lds(const uint16_t* p)
   return __builtin_bit_cast(bf16x8, v);
 }
 
-// body(i, fragment at addr(i)) for i = 0 .. N-1, each fragment read from LDS
-// RING uses ahead: with one wave per SIMD nothing else hides a read's latency,
-// and the compiler left alone reads each fragment just before its MFMA.
+typedef short v4i16 __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(3))) v4i16 lds_v4i16;
+
+// ds_read_b64_tr_b16: within each 16-lane group, lane i receives element
+// (i & 3) of the 8 bytes that lanes 4k + (i >> 2) address, for k = 0..3. When
+// lane 4k + q addresses four consecutive BF16 of row R(k) at column block q of
+// a row-major table, lane i therefore gets column i of rows R(0..3): four
+// elements of a transposed fragment.
+__device__ __forceinline__ uint2 tr_read(const uint16_t* p)
+{
+  const v4i16 v = __builtin_amdgcn_ds_read_tr16_b64_v4i16((lds_v4i16*)p);
+  return __builtin_bit_cast(uint2, v);
+}
+
+// A fragment of a transposed table from two transposing reads: elements 0..3
+// from rows R(0..3) (address p0), 4..7 from rows R(4..7) (address p1).
+__device__ __forceinline__ bf16x8 frag_tr(const uint16_t* p0, const uint16_t* p1)
+{
+  const uint2 a = tr_read(p0), b = tr_read(p1);
+  return __builtin_bit_cast(bf16x8, make_uint4(a.x, a.y, b.x, b.y));
+}
+
+// Elements 0..3 from one transposing read, 4..7 zero.
+__device__ __forceinline__ bf16x8 frag_tr_half(const uint16_t* p0)
+{
+  const uint2 a = tr_read(p0);
+  return __builtin_bit_cast(bf16x8, make_uint4(a.x, a.y, 0u, 0u));
+}
+
+// body(i, load(i)) for i = 0 .. N-1, each fragment loaded from LDS RING uses
+// ahead: with one wave per SIMD nothing else hides a read's latency, and the
+// compiler left alone reads each fragment just before its MFMA.
 // The layers walk their K chunks outermost (i = j * MT + r): consecutive MFMAs
 // write different accumulators, and each accumulator still sums its K chunks
 // in order.
-template <int N, int RING, class Addr, class Body>
-__device__ __forceinline__ void lds_ring(Addr addr, Body body)
+template <int N, int RING, class Load, class Body>
+__device__ __forceinline__ void lds_ring_ld(Load load, Body body)
 {
   constexpr int R = RING < N ? RING : N;
   bf16x8 ring[R];
 #pragma unroll
-  for (int q = 0; q < R; ++q) ring[q] = frag_lds(addr(q));
+  for (int q = 0; q < R; ++q) ring[q] = load(q);
 #pragma unroll
   for (int i = 0; i < N; ++i)
   {
     const bf16x8 a = ring[i % R];
-    if (i + R < N) ring[i % R] = frag_lds(addr(i + R));
+    if (i + R < N) ring[i % R] = load(i + R);
     body(i, a);
   }
+}
+
+// lds_ring_ld of plain 16-byte fragment reads at addr(i).
+template <int N, int RING, class Addr, class Body>
+__device__ __forceinline__ void lds_ring(Addr addr, Body body)
+{
+  lds_ring_ld<N, RING>([&](int i) { return frag_lds(addr(i)); }, body);
 }
 
 typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
@@ -179,18 +215,23 @@ __host__ __device__ __forceinline__ int swz(int row, int col, int wd)
 }
 
 // Weight table layout (elements, BF16), identical in HBM and LDS; the columns
-// of every row are placed by swz.
+// of every row are placed by swz. The backward reads the transposed operands
+// (W1^T, W0^T, Wout^T) from the same tables with transposing LDS reads.
 struct TableLayout
 {
   int H, FP, G;
   __host__ __device__ int oW0() const { return 0; }                   // [H][FP]
   __host__ __device__ int oW1() const { return H * FP; }              // [H][H]  cols kperm
   __host__ __device__ int oWo() const { return oW1() + H * H; }       // [16][H] cols kperm
-  __host__ __device__ int oWoT() const { return oWo() + 16 * H; }     // [H][32] cols (4g+e, e<4)
-  __host__ __device__ int oW1T() const { return oWoT() + 32 * H; }    // [H][H]  W1^T, cols kperm
-  __host__ __device__ int oW0T() const { return oW1T() + H * H; }     // [G][H]  W0^T, cols kperm
-  __host__ __device__ int total() const { return oW0T() + G * H; }
+  __host__ __device__ int total() const { return oWo() + 16 * H; }
 };
+
+// Addresses of the transposing reads that form the A fragment of a
+// transposed table (the transposed M index is a table column, its K index a
+// table row), for lane (g = lane / 16, i = lane % 16), half h of K chunk j:
+// lane 4k + q reads row 32 j + 16 h + 4 g + k (the hidden unit at K position
+// 32 j + 8 g + 4 h + k of the kperm order), logical columns 16 r + 4 q .. + 3.
+__host__ __device__ __forceinline__ int tr_row(int j, int h, int g, int i) { return 32 * j + 16 * h + 4 * g + (i >> 2); }
 
 struct FusedParams
 {
@@ -242,7 +283,7 @@ struct Cfg
   static constexpr int G1B = G0MODE == NTC_G0_CORNERS ? 4 * C0 : C0;
   static constexpr int GSLOTS = G1B + C1;
   static constexpr int PE_FIRST = PEC == 3 ? 8 : 4;
-  static constexpr int WT = H * FP + 2 * H * H + 48 * H + G * H;
+  static constexpr int WT = H * FP + H * H + 16 * H;
   static constexpr int XFN = FP * PASS;
   static constexpr int XS = FP + 8;  // xnf row stride (elements): spreads texel rows over LDS banks
   static constexpr int STAGE_B = 2 * H * 64 + 16 * 64;
@@ -1123,8 +1164,12 @@ __device__ __forceinline__ void fused_body(const FusedParams& P, const int vblk,
     if constexpr (MODE != kForward)
     {
       f32x4 dz1[MT][kNT];
-      lds_ring<MT, kRing>([&](int r) { return &wt[TLY.oWoT() + (16 * r + n16) * 32 + swz(16 * r + n16, 8 * g, 32)]; },
-                          [&](int r, const bf16x8 a) {
+      lds_ring_ld<MT, kRing>(
+          [&](int r) {
+            const int o = 4 * g + (n16 >> 2), c = kperm_col(16 * r + 4 * (n16 & 3));
+            return frag_tr_half(&wt[TLY.oWo() + o * H + swz(o, c, H)]);
+          },
+          [&](int r, const bf16x8 a) {
 #pragma unroll
                             for (int nt = 0; nt < kNT; ++nt)
                             {
@@ -1154,10 +1199,11 @@ __device__ __forceinline__ void fused_body(const FusedParams& P, const int vblk,
     if constexpr (MODE != kForward)
     {
       f32x4 dz0[MT][kNT];
-      lds_ring<MT * KH, kRing>(
+      lds_ring_ld<MT * KH, kRing>(
           [&](int i) {
-            const int row = 16 * (i % MT) + n16;
-            return &wt[TLY.oW1T() + row * H + swz(row, 32 * (i / MT) + 8 * g, H)];
+            const int j = i / MT, c = kperm_col(16 * (i % MT) + 4 * (n16 & 3));
+            const int r0 = tr_row(j, 0, g, n16), r1 = tr_row(j, 1, g, n16);
+            return frag_tr(&wt[TLY.oW1() + r0 * H + swz(r0, c, H)], &wt[TLY.oW1() + r1 * H + swz(r1, c, H)]);
           },
           [&](int i, const bf16x8 a) {
             const int r = i % MT, j = i / MT;
@@ -1203,10 +1249,11 @@ __device__ __forceinline__ void fused_body(const FusedParams& P, const int vblk,
         const Footprint<float> f1 = footprint<float>(tx[nt], ty[nt], M, D1);
         float* drow = dxs + tloc[nt] * CF::GS;
         f32x4 dx = {0, 0, 0, 0};
-        lds_ring<GT * KH, kRing>(
+        lds_ring_ld<GT * KH, kRing>(
             [&](int q) {
-              const int row = 16 * (q / KH) + n16;
-              return &wt[TLY.oW0T() + row * H + swz(row, 32 * (q % KH) + 8 * g, H)];
+              const int j = q % KH, c = 16 * (q / KH) + 4 * (n16 & 3);
+              const int r0 = tr_row(j, 0, g, n16), r1 = tr_row(j, 1, g, n16);
+              return frag_tr(&wt[TLY.oW0() + r0 * FP + swz(r0, c, FP)], &wt[TLY.oW0() + r1 * FP + swz(r1, c, FP)]);
             },
             [&](int q, const bf16x8 a) {
           const int r = q / KH, j = q % KH;

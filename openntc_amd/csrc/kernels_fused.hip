@@ -50,26 +50,10 @@ __host__ __device__ __forceinline__ void pack_table_elem(const TableLayout& L, i
     const int i = e - L.oW1(), m = i / H, pos = swz(m, i % H, H);
     v = w1[m * H + kperm_pos(pos)];
   }
-  else if (e < L.oWoT())
+  else
   {
     const int i = e - L.oWo(), o = i / H, pos = swz(o, i % H, H);
     v = o < O ? wo[o * H + kperm_pos(pos)] : 0.0f;
-  }
-  else if (e < L.oW1T())
-  {
-    const int i = e - L.oWoT(), h = i / 32, pos = swz(h, i % 32, 32), gg = pos >> 3, ee = pos & 7;
-    const int o = 4 * gg + ee;
-    v = (ee < 4 && o < O) ? wo[o * H + h] : 0.0f;
-  }
-  else if (e < L.oW0T())
-  {
-    const int i = e - L.oW1T(), hin = i / H, pos = swz(hin, i % H, H);
-    v = w1[kperm_pos(pos) * H + hin];
-  }
-  else
-  {
-    const int i = e - L.oW0T(), s = i / H, pos = swz(s, i % H, H);
-    v = s < gslots ? w0[kperm_pos(pos) * F + s] : 0.0f;
   }
   tab[e] = f2bf(v);
 }
@@ -86,19 +70,16 @@ __host__ __device__ __forceinline__ void table_scatter(const TableLayout& L, int
   {
     const int m = e / F, c = e % F;
     tab[L.oW0() + m * L.FP + swz(m, c, L.FP)] = b;
-    if (c < gslots) tab[L.oW0T() + c * H + swz(c, kperm_col(m), H)] = b;
   }
   else if (e < H * F + H * H)
   {
     const int i = e - H * F, m = i / H, k = i % H;
     tab[L.oW1() + m * H + swz(m, kperm_col(k), H)] = b;
-    tab[L.oW1T() + k * H + swz(k, kperm_col(m), H)] = b;
   }
   else
   {
     const int i = e - H * F - H * H, o = i / H, h = i % H;
     tab[L.oWo() + o * H + swz(o, kperm_col(h), H)] = b;
-    tab[L.oWoT() + h * 32 + swz(h, 8 * (o >> 2) + (o & 3), 32)] = b;
   }
 }
 

@@ -62,7 +62,7 @@ def _plan(H, FP, G, C0, C1, mode, NT):
     """The kernel's Cfg constants and LDS byte total."""
     PASS = 64 * NT
     BH = PASS // 16
-    WT = H * FP + 2 * H * H + 48 * H + G * H
+    WT = H * FP + H * H + 16 * H
     XFN = FP * PASS
     XS = FP + 8
     STAGE_B = 2 * H * 64 + 16 * 64
@@ -101,6 +101,35 @@ def kperm(j, g, e):
     return 32 * j + (4 * g + e if e < 4 else 16 + 4 * g + (e - 4))
 
 
+def kperm_pos(pos):
+    j, r = pos >> 5, pos & 31
+    return kperm(j, r >> 3, r & 7)
+
+
+def kperm_col(k):
+    r = k & 31
+    q = r & 15
+    return (k & ~31) + 8 * (q >> 2) + (q & 3) + 4 * (r >> 4)
+
+
+def tr_row(j, h, g, i):
+    return 32 * j + 16 * h + 4 * g + (i >> 2)
+
+
+def tr_read(table, addrs):
+    """ds_read_b64_tr_b16 over 64 lanes: within each 16-lane group, lane i
+    receives element (i & 3) of the 8 bytes lane 4k + (i >> 2) addresses, for
+    k = 0..3 (the instruction's lane mapping, established on the GPU with a
+    probe that read a table of distinct values)."""
+    out = np.zeros((64, 4), table.dtype)
+    for lane in range(64):
+        base, i = lane & ~15, lane & 15
+        for k in range(4):
+            src = base + 4 * k + (i >> 2)
+            out[lane, k] = table[addrs[src] + (i & 3)]
+    return out
+
+
 class Region:
     """Element bounds of one LDS array (or sub-range); records every index."""
 
@@ -134,11 +163,9 @@ def _emulate(H, FP, G, C0, C1, mode, PEC, LODS, NT):
     assert 2 * H * 64 + 16 * 64 <= P["UNION"]
     oW0, oW1 = 0, H * FP
     oWo = oW1 + H * H
-    oWoT = oWo + 16 * H
-    oW1T = oWoT + 32 * H
-    oW0T = oW1T + H * H
-    assert oW0T + G * H == P["WT"]
+    assert oWo + 16 * H == P["WT"]
     vswz = np.vectorize(swz)
+    vkcol = np.vectorize(kperm_col)
     for wave in range(4):
         for nt in range(NT):
             tloc = wave * 16 * NT + nt * 16 + n16
@@ -151,14 +178,20 @@ def _emulate(H, FP, G, C0, C1, mode, PEC, LODS, NT):
                 for r in range(P["MT"]):
                     row = 16 * r + n16
                     wt.touch(oW1 + row * H + vswz(row, 32 * j + 8 * g, H), 8)
-                    wt.touch(oW1T + row * H + vswz(row, 32 * j + 8 * g, H), 8)
+                    # W1^T by transposing reads of W1 (8 bytes a lane)
+                    c = vkcol(16 * r + 4 * (n16 & 3))
+                    for h in range(2):
+                        tr = tr_row(j, h, g, n16)
+                        wt.touch(oW1 + tr * H + vswz(tr, c, H), 4)
                 wt.touch(oWo + n16 * H + vswz(n16, 32 * j + 8 * g, H), 8)
-                for q in range(P["GT"]):
-                    row = 16 * q + n16
-                    wt.touch(oW0T + row * H + vswz(row, 32 * j + 8 * g, H), 8)
-            for r in range(P["MT"]):
-                row = 16 * r + n16
-                wt.touch(oWoT + row * 32 + vswz(row, 8 * g, 32), 8)
+                for q in range(P["GT"]):  # W0^T by transposing reads of W0
+                    c = 16 * q + 4 * (n16 & 3)
+                    for h in range(2):
+                        tr = tr_row(j, h, g, n16)
+                        wt.touch(oW0 + tr * FP + vswz(tr, c, FP), 4)
+            for r in range(P["MT"]):  # Wout^T by transposing reads of Wout
+                o = 4 * g + (n16 >> 2)
+                wt.touch(oWo + o * H + vswz(o, vkcol(16 * r + 4 * (n16 & 3)), H), 4)
             for r in range(P["GT"]):  # dX rows: 16-byte or single writes of slots < GSLOTS
                 s0 = 16 * r + 4 * g
                 ok = s0 < P["GSLOTS"]
@@ -296,3 +329,59 @@ def test_decoder_reduction_reads_in_bounds(blocks):
     H, F = 128, 40
     n_dec = H * F + H * H + 16 * H
     assert _reduction_reads(blocks, n_dec) == blocks * n_dec - 1
+
+
+@pytest.mark.parametrize("H,FP", [(128, 64), (64, 96), (64, 32)])
+def test_transposing_reads_form_transposed_fragments(H, FP):
+    """The backward's A fragments of W1^T, W0^T and Wout^T, assembled from the
+    forward tables (swizzled columns, kperm column order of W1 and Wout) with
+    the transposing reads' lane mapping, equal the fragments of the plain
+    transposed matrices in the K order of the B operands (kperm: the hidden
+    unit at K position p is kperm_pos(p); outputs 4g..4g+3 then zeros)."""
+    rng = np.random.default_rng(H + FP)
+    G = 32
+    w0 = rng.integers(1, 60000, (H, FP)).astype(np.int64)
+    w1 = rng.integers(1, 60000, (H, H)).astype(np.int64)
+    wo = rng.integers(1, 60000, (16, H)).astype(np.int64)
+    # the forward tables as the pack writes them
+    t0 = np.zeros(H * FP, np.int64)
+    t1 = np.zeros(H * H, np.int64)
+    to = np.zeros(16 * H, np.int64)
+    for m in range(H):
+        for c in range(FP):
+            t0[m * FP + swz(m, c, FP)] = w0[m, c]
+        for k in range(H):
+            t1[m * H + swz(m, kperm_col(k), H)] = w1[m, k]
+    for o in range(16):
+        for h in range(H):
+            to[o * H + swz(o, kperm_col(h), H)] = wo[o, h]
+    lanes = np.arange(64)
+    n16, g = lanes & 15, lanes >> 4
+    for j in range(H // 32):
+        for r in range(H // 16):
+            c = np.array([kperm_col(16 * r + 4 * (i & 3)) for i in n16])
+            parts = []
+            for h in range(2):
+                tr = tr_row(j, h, g, n16)
+                parts.append(tr_read(t1, tr * H + np.array([swz(a, b, H) for a, b in zip(tr, c)])))
+            frag = np.concatenate(parts, axis=1)
+            for lane in range(64):
+                want = [w1[kperm_pos(32 * j + 8 * g[lane] + e), 16 * r + n16[lane]] for e in range(8)]
+                assert list(frag[lane]) == want, ("W1^T", j, r, lane)
+        for q in range(G // 16):
+            c = 16 * q + 4 * (n16 & 3)
+            parts = []
+            for h in range(2):
+                tr = tr_row(j, h, g, n16)
+                parts.append(tr_read(t0, tr * FP + np.array([swz(a, b, FP) for a, b in zip(tr, c)])))
+            frag = np.concatenate(parts, axis=1)
+            for lane in range(64):
+                want = [w0[kperm_pos(32 * j + 8 * g[lane] + e), 16 * q + n16[lane]] for e in range(8)]
+                assert list(frag[lane]) == want, ("W0^T", j, q, lane)
+    for r in range(H // 16):
+        o = 4 * g + (n16 >> 2)
+        c = np.array([kperm_col(16 * r + 4 * (i & 3)) for i in n16])
+        frag = tr_read(to, o * H + np.array([swz(a, b, H) for a, b in zip(o, c)]))
+        for lane in range(64):
+            want = [wo[4 * g[lane] + e, 16 * r + n16[lane]] for e in range(4)]
+            assert list(frag[lane]) == want, ("Wout^T", r, lane)
