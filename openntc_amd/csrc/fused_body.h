@@ -204,14 +204,22 @@ __host__ __device__ __forceinline__ int kperm_col(int k)
 // MFMA operand read touches (same logical chunk) fall on different LDS banks
 // (64 banks of 4 bytes: without it rows of 256 bytes all start on bank 0).
 // An involution within the row; rows whose chunk count is not a power of two
-// are left as they are.
+// are left as they are. For rows of 256 bytes or more the XOR key is row mod
+// 16 with its two 2-bit halves swapped: the transposing reads of a W^T
+// fragment take rows 4 g + k (k = 0..3) at chunks 4 a + q (q = 0..3) in one
+// 16-lane group, and the key 4 k + g puts those sixteen (k, q) on sixteen
+// different chunks (the plain key 4 g + k left four lanes on each).
 __host__ __device__ __forceinline__ int swz(int row, int col, int wd)
 {
   const int nc = wd >> 3;
   if (nc & (nc - 1)) return col;
-  const int per = nc >= 16 ? 1 : 16 / nc;  // rows per 256 bytes
-  const int m = (nc >= 16 ? 16 : nc) - 1;
-  return ((((col >> 3) ^ ((row / per) & m))) << 3) | (col & 7);
+  if (nc >= 16)
+  {
+    const int r = row & 15, key = ((r & 3) << 2) | (r >> 2);
+    return (((col >> 3) ^ key) << 3) | (col & 7);
+  }
+  const int per = 16 / nc;  // rows per 256 bytes
+  return ((((col >> 3) ^ ((row / per) & (nc - 1)))) << 3) | (col & 7);
 }
 
 // Weight table layout (elements, BF16), identical in HBM and LDS; the columns
@@ -286,7 +294,11 @@ struct Cfg
   static constexpr int WT = H * FP + H * H + 16 * H;
   static constexpr int XFN = FP * PASS;
   static constexpr int XS = FP + 8;  // xnf row stride (elements): spreads texel rows over LDS banks
-  static constexpr int STAGE_B = 2 * H * 64 + 16 * 64;
+  // Decoder-gradient staging of a 64-texel round: [H][64] BF16 rows of
+  // A0 and dZ1, then (second phase) dZ0 and A1 in the same rows, plus dY
+  // [16][64]; or, where the LDS holds them, all four arrays at once (one
+  // staging phase and one barrier less per round).
+  static constexpr int STAGE2 = 2 * H * 64 + 16 * 64, STAGE4 = 4 * H * 64 + 16 * 64;
   static constexpr int BH = PASS / 16;
   // Latent gradients of a blocked pass (16 x BH texels) come from the
   // per-texel dX when the pass footprint spans at most WMAX x VMAX cells of
@@ -302,20 +314,22 @@ struct Cfg
   // dX row stride = 16 mod 64 floats puts the 4 texel rows of one FP32 MFMA
   // B-operand read in distinct LDS banks
   static constexpr int GS_BANK = G + (((16 - G) % 64) + 64) % 64;
-  static constexpr int lds_bytes(int sw, int sv, int gs)
+  static constexpr int lds_bytes(int sw, int sv, int gs, int stage)
   {
     const int u = PASS * XS + 2 * sw * sv * (CP0 + CP1);
-    return 2 * (WT + XFN + (u > STAGE_B ? u : STAGE_B)) + 4 * PASS * gs + 4 * 192 + 4 * 128 + 16 * PASS;
+    return 2 * (WT + XFN + (u > stage ? u : stage)) + 4 * PASS * gs + 4 * 192 + 4 * 128 + 16 * PASS;
   }
   static constexpr int LIM = 160 * 1024;
-  static constexpr int SEL = lds_bytes(WMAX, BH + 3, GS_BANK) <= LIM       ? 0
-                             : lds_bytes(WMAX, BH + 3, G + 4) <= LIM       ? 1
-                             : lds_bytes(10, BH / 2 + 3, GS_BANK) <= LIM   ? 2
-                             : lds_bytes(10, BH / 2 + 3, G + 4) <= LIM     ? 3
+  static constexpr int SEL = lds_bytes(WMAX, BH + 3, GS_BANK, STAGE2) <= LIM       ? 0
+                             : lds_bytes(WMAX, BH + 3, G + 4, STAGE2) <= LIM       ? 1
+                             : lds_bytes(10, BH / 2 + 3, GS_BANK, STAGE2) <= LIM   ? 2
+                             : lds_bytes(10, BH / 2 + 3, G + 4, STAGE2) <= LIM     ? 3
                                                                            : 4;
   static constexpr int SW = SEL <= 1 ? WMAX : (SEL <= 3 ? 10 : 0);
   static constexpr int SV = SEL <= 1 ? BH + 3 : (SEL <= 3 ? BH / 2 + 3 : 0);
   static constexpr int GS = (SEL == 0 || SEL == 2) ? GS_BANK : G + 4;  // dX row stride (floats)
+  static constexpr bool ONE = lds_bytes(SW, SV, GS, STAGE4) <= LIM;  // single staging phase
+  static constexpr int STAGE_B = ONE ? STAGE4 : STAGE2;
   static constexpr int DXS = PASS * GS;
   static constexpr int SBOX0 = SW * SV * CP0;  // g1 box offset in the staged boxes (floats)
   static constexpr int UNION_U = PASS * XS + 2 * SW * SV * (CP0 + CP1);
@@ -326,7 +340,7 @@ struct Cfg
   static_assert(UNION * 2 >= 8 * 256, "final reduction reuses the union region");
   static_assert(16 + BH <= 128, "axis table threads");
   static_assert((PASS * XS * 2) % 16 == 0, "staged boxes 16-byte aligned");
-  static_assert(lds_bytes(SW, SV, GS) <= LIM, "LDS budget");
+  static_assert(lds_bytes(SW, SV, GS, STAGE_B) <= LIM, "LDS budget");
   // box cell indices (below WMAX * VMAX + 16) divided by box widths (at most WMAX)
   static_assert((WMAX * VMAX + 16) * WMAX < 65536, "small_div range");
 };
@@ -953,10 +967,14 @@ __device__ __forceinline__ void fused_body(const FusedParams& P, const int vblk,
   uint16_t* uni = xfn + CF::XFN;
   uint16_t* xnf = uni;                       // [PASS][XS] during gather/forward
   float* sbox = reinterpret_cast<float*>(uni + kPass * CF::XS);  // staged boxes during gather
-  // stA, stB, stY: rows of texels, columns placed by swz
-  uint16_t* stA = uni;                       // [H][64]  A0 | dZ0
-  uint16_t* stB = uni + H * 64;              // [H][64]  dZ1 | A1
-  uint16_t* stY = uni + 2 * H * 64;          // [16][64] dY
+  // staging rows of texels, columns placed by swz: A0, dZ1, dZ0, A1 (the
+  // last two in A0's and dZ1's rows after the first phase unless all four
+  // fit), dY
+  uint16_t* stA = uni;                                  // [H][64] A0
+  uint16_t* stB = uni + H * 64;                         // [H][64] dZ1
+  uint16_t* stC = CF::ONE ? uni + 2 * H * 64 : stA;     // [H][64] dZ0
+  uint16_t* stD = CF::ONE ? uni + 3 * H * 64 : stB;     // [H][64] A1
+  uint16_t* stY = uni + (CF::ONE ? 4 : 2) * H * 64;     // [16][64] dY
   __shared__ __attribute__((aligned(16))) float dxs[CF::DXS];  // dX of the grid slots [texel][GS]
   __shared__ AxisTab atab[4];                                  // corner tables of g0, g1; two passes
   __shared__ uint32_t orig_s[128];                             // tile origins mod M
@@ -1341,7 +1359,15 @@ __device__ __forceinline__ void fused_body(const FusedParams& P, const int vblk,
           const int at = kperm(j, g, 2 * d) * 32 + pcol[d & 1];
           stage_pair(stA, a0f[j][rnd], d, sel, at);
           stage_pair(stB, dz1f[j][rnd], d, sel, at);
+          if constexpr (CF::ONE)
+          {
+            stage_pair(stC, dz0f[j][rnd], d, sel, at);
+            stage_pair(stD, a1f[j][rnd], d, sel, at);
+          }
         }
+      if constexpr (CF::ONE)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) stY[(4 * g + i) * 64 + scol[i >> 1]] = bf16_dev(dy[rnd][i]);
       __syncthreads();
       // dW1[h_out][h_in] += dZ1 A0^T, rows of this wave
       {
@@ -1358,19 +1384,22 @@ __device__ __forceinline__ void fused_body(const FusedParams& P, const int vblk,
               dw1[mt][c] = mfma(a, b, dw1[mt][c]);
             });
       }
-      __syncthreads();
+      if constexpr (!CF::ONE)
+      {
+        __syncthreads();
 #pragma unroll
-      for (int j = 0; j < KH; ++j)
+        for (int j = 0; j < KH; ++j)
 #pragma unroll
-        for (int d = 0; d < 4; ++d)
-        {
-          const int at = kperm(j, g, 2 * d) * 32 + pcol[d & 1];
-          stage_pair(stA, dz0f[j][rnd], d, sel, at);
-          stage_pair(stB, a1f[j][rnd], d, sel, at);
-        }
+          for (int d = 0; d < 4; ++d)
+          {
+            const int at = kperm(j, g, 2 * d) * 32 + pcol[d & 1];
+            stage_pair(stC, dz0f[j][rnd], d, sel, at);
+            stage_pair(stD, a1f[j][rnd], d, sel, at);
+          }
 #pragma unroll
-      for (int i = 0; i < 4; ++i) stY[(4 * g + i) * 64 + scol[i >> 1]] = bf16_dev(dy[rnd][i]);
-      __syncthreads();
+        for (int i = 0; i < 4; ++i) stY[(4 * g + i) * 64 + scol[i >> 1]] = bf16_dev(dy[rnd][i]);
+        __syncthreads();
+      }
       // dW0[h][f] += dZ0 X^T, X^T read from xfn in the staged texel order
       {
         constexpr int NC = FP / 16;
@@ -1387,7 +1416,7 @@ __device__ __forceinline__ void fused_body(const FusedParams& P, const int vblk,
             [&](int i, const bf16x8 b) {
               const int mt = i / (2 * NC), kc = (i / NC) % 2, c = i % NC;
               const int row = CF::RW * wave + 16 * mt + n16;
-              if (c == 0) a = frag_lds(&stA[row * 64 + swz(row, 32 * kc + 8 * g, 64)]);
+              if (c == 0) a = frag_lds(&stC[row * 64 + swz(row, 32 * kc + 8 * g, 64)]);
               dw0[mt][c] = mfma(a, b, dw0[mt][c]);
             });
       }
@@ -1400,7 +1429,7 @@ __device__ __forceinline__ void fused_body(const FusedParams& P, const int vblk,
         for (int c = 0; c < NTO; ++c)
         {
           const int hc = (wave * NTO + c) * 16 + n16;
-          dwo[c] = mfma(a, frag_lds(&stB[hc * 64 + swz(hc, 32 * kc + 8 * g, 64)]), dwo[c]);
+          dwo[c] = mfma(a, frag_lds(&stD[hc * 64 + swz(hc, 32 * kc + 8 * g, 64)]), dwo[c]);
         }
       }
       __syncthreads();

@@ -65,7 +65,7 @@ def _plan(H, FP, G, C0, C1, mode, NT):
     WT = H * FP + H * H + 16 * H
     XFN = FP * PASS
     XS = FP + 8
-    STAGE_B = 2 * H * 64 + 16 * 64
+    STAGE2, STAGE4 = 2 * H * 64 + 16 * 64, 4 * H * 64 + 16 * 64
     WMAX = 18
     Q0, Q1 = (C0 + 3) // 4, (C1 + 3) // 4
     CP0, CP1 = 4 * Q0, 4 * Q1
@@ -73,28 +73,32 @@ def _plan(H, FP, G, C0, C1, mode, NT):
     GSLOTS = G1B + C1
     GS_BANK = G + (16 - G) % 64
 
-    def lds_bytes(sw, sv, gs):
+    def lds_bytes(sw, sv, gs, stage=STAGE2):
         u = PASS * XS + 2 * sw * sv * (CP0 + CP1)
-        return 2 * (WT + XFN + max(u, STAGE_B)) + 4 * PASS * gs + 4 * 192 + 4 * 128 + 16 * PASS
+        return 2 * (WT + XFN + max(u, stage)) + 4 * PASS * gs + 4 * 192 + 4 * 128 + 16 * PASS
 
     cands = [(WMAX, BH + 3, GS_BANK), (WMAX, BH + 3, G + 4), (10, BH // 2 + 3, GS_BANK), (10, BH // 2 + 3, G + 4)]
     sel = next((i for i, c in enumerate(cands) if lds_bytes(*c) <= LDS_LIMIT), 4)
     assert sel < 4, "layout does not fit"
     SW, SV, GS = cands[sel]
+    ONE = lds_bytes(SW, SV, GS, STAGE4) <= LDS_LIMIT  # all four staging arrays at once
+    STAGE_B = STAGE4 if ONE else STAGE2
     UNION = max(PASS * XS + 2 * SW * SV * (CP0 + CP1), STAGE_B)
     return dict(PASS=PASS, BH=BH, WT=WT, XFN=XFN, XS=XS, UNION=UNION, SW=SW, SV=SV, GS=GS, CP0=CP0, CP1=CP1,
                 Q0=Q0, Q1=Q1, G1B=G1B, GSLOTS=GSLOTS, MT=H // 16, KF=FP // 32, KH=H // 32, GT=G // 16,
                 RW=H // 4, MTW=H // 64, NTO=H // 64, SBOX0=SW * SV * CP0,
-                DXS=PASS * GS, total=lds_bytes(SW, SV, GS))
+                DXS=PASS * GS, ONE=ONE, STAGE_B=STAGE_B, total=lds_bytes(SW, SV, GS, STAGE_B))
 
 
 def swz(row, col, wd):
     nc = wd >> 3
     if nc & (nc - 1):
         return col
-    per = 1 if nc >= 16 else 16 // nc
-    m = (16 if nc >= 16 else nc) - 1
-    return (((col >> 3) ^ ((row // per) & m)) << 3) | (col & 7)
+    if nc >= 16:
+        r = row & 15
+        return (((col >> 3) ^ (((r & 3) << 2) | (r >> 2))) << 3) | (col & 7)
+    per = 16 // nc
+    return (((col >> 3) ^ ((row // per) & (nc - 1))) << 3) | (col & 7)
 
 
 def kperm(j, g, e):
@@ -156,11 +160,13 @@ def _emulate(H, FP, G, C0, C1, mode, PEC, LODS, NT):
     xfn = Region("X^T rows", P["XFN"])
     sbox = Region("staged boxes (floats)", P["UNION"] // 2 - PASS * XS // 2)
     stA, stB, stY = Region("stA", H * 64), Region("stB", H * 64), Region("stY", 16 * 64)
+    # dZ0 and A1 rows: their own arrays when all four fit, else A0's and dZ1's
+    stC, stD = (Region("stC", H * 64), Region("stD", H * 64)) if P["ONE"] else (stA, stB)
     dxs = Region("dX", P["DXS"])
     tgt = Region("target words", PASS * 4)
     # the union holds X + boxes, then stA | stB | stY
     assert PASS * XS + 2 * P["SW"] * P["SV"] * (P["CP0"] + P["CP1"]) <= P["UNION"]
-    assert 2 * H * 64 + 16 * 64 <= P["UNION"]
+    assert P["STAGE_B"] <= P["UNION"] and P["STAGE_B"] == (4 if P["ONE"] else 2) * H * 64 + 16 * 64
     oW0, oW1 = 0, H * FP
     oWo = oW1 + H * H
     assert oWo + 16 * H == P["WT"]
@@ -206,7 +212,7 @@ def _emulate(H, FP, G, C0, C1, mode, PEC, LODS, NT):
         for j in range(P["KH"]):
             for d in range(4):
                 at = kperm(j, g, 2 * d) * 32 + pcol[d & 1]  # dword index
-                for st in (stA, stB):
+                for st in (stA, stB, stC, stD):
                     st.touch(2 * at, 2)
         for i in range(4):
             stY.touch((4 * g + i) * 64 + scol[i >> 1])
@@ -214,7 +220,7 @@ def _emulate(H, FP, G, C0, C1, mode, PEC, LODS, NT):
             for kc in range(2):
                 row = P["RW"] * wave + 16 * mt + n16
                 stB.touch(row * 64 + vswz(row, 32 * kc + 8 * g, 64), 8)  # dW1 A operand
-                stA.touch(row * 64 + vswz(row, 32 * kc + 8 * g, 64), 8)  # dW0 A operand
+                stC.touch(row * 64 + vswz(row, 32 * kc + 8 * g, 64), 8)  # dW0 A operand
                 for c in range(P["MT"]):
                     rr = 16 * c + n16
                     stA.touch(rr * 64 + vswz(rr, 32 * kc + 8 * g, 64), 8)
@@ -228,7 +234,7 @@ def _emulate(H, FP, G, C0, C1, mode, PEC, LODS, NT):
             stY.touch(n16 * 64 + vswz(n16, 32 * kc + 8 * g, 64), 8)
             for c in range(P["NTO"]):
                 hc = (wave * P["NTO"] + c) * 16 + n16
-                stB.touch(hc * 64 + vswz(hc, 32 * kc + 8 * g, 64), 8)
+                stD.touch(hc * 64 + vswz(hc, 32 * kc + 8 * g, 64), 8)
     # staged gather: put4 (8-byte X-row writes, 32-bit X^T pair writes) and
     # single puts for every slot < FP of every texel; box reads for boxes up
     # to SW x SV cells
@@ -256,7 +262,7 @@ def _emulate(H, FP, G, C0, C1, mode, PEC, LODS, NT):
         for c in range(nc):
             for ch in range(C):
                 dxs.touch(tex * P["GS"] + slot0 + c * C + ch)
-    for r in (wt, xnf, xfn, sbox, stA, stB, stY, dxs, tgt):
+    for r in (wt, xnf, xfn, sbox, stA, stB, stC, stD, stY, dxs, tgt):
         r.check()
     # every dX slot stays inside its row (no row reaches into the next texel's)
     assert P["GSLOTS"] <= G <= P["GS"]
@@ -275,8 +281,11 @@ def _code_object_lds():
     if not os.path.exists(OBJ) or not os.path.exists(os.path.join(LLVM, "llvm-readelf")):
         pytest.skip("library not built here (openntc_amd/_build) or no LLVM tools")
     with tempfile.TemporaryDirectory() as d:
-        fat, co = os.path.join(d, "fat.bin"), os.path.join(d, "k.co")
-        subprocess.run([os.path.join(LLVM, "llvm-objcopy"), "--dump-section", f".hip_fatbin={fat}", OBJ],
+        fat, co, obj = os.path.join(d, "fat.bin"), os.path.join(d, "k.co"), os.path.join(d, "k.o")
+        # objcopy rewrites its input: work on a copy, so the build's object
+        # keeps its time stamp (an incremental build compares it with sources)
+        shutil.copyfile(OBJ, obj)
+        subprocess.run([os.path.join(LLVM, "llvm-objcopy"), "--dump-section", f".hip_fatbin={fat}", obj],
                        check=True, capture_output=True)
         subprocess.run([os.path.join(LLVM, "clang-offload-bundler"), "--unbundle", "--type=o", f"--input={fat}",
                         "--targets=hipv4-amdgcn-amd-amdhsa--gfx950", f"--output={co}"], check=True,
@@ -385,3 +394,30 @@ def test_transposing_reads_form_transposed_fragments(H, FP):
         for lane in range(64):
             want = [wo[4 * g[lane] + e, 16 * r + n16[lane]] for e in range(4)]
             assert list(frag[lane]) == want, ("Wout^T", r, lane)
+
+
+def _banks(byte_addrs, width):
+    """Distinct 4-byte LDS banks (of 64) touched by one access of `width`
+    bytes per lane at each address."""
+    out = []
+    for a in byte_addrs:
+        out += [((a + 4 * w) // 4) % 64 for w in range(width // 4)]
+    return out
+
+
+@pytest.mark.parametrize("j,r", [(j, r) for j in range(4) for r in range(8)])
+def test_transposing_reads_of_128_wide_tables_are_conflict_free(j, r):
+    """Within each 16-lane group, the 16 transposing reads (8 bytes each) of
+    a W1^T fragment at H = 128 cover 32 distinct banks, and the 16 plain
+    16-byte fragment reads of the forward W1 cover all 64."""
+    H = 128
+    for g in range(4):
+        for h in range(2):
+            addrs = []
+            for i in range(16):
+                row = tr_row(j, h, g, i)
+                col = kperm_col(16 * r + 4 * (i & 3))
+                addrs.append(2 * (row * H + swz(row, col, H)))
+            assert len(set(_banks(addrs, 8))) == 32, (g, h)
+        fwd = [2 * ((16 * r + i) * H + swz(16 * r + i, 32 * j + 8 * g, H)) for i in range(16)]
+        assert len(set(_banks(fwd, 16))) == 64
