@@ -292,8 +292,8 @@ struct Cfg
   static constexpr int GSLOTS = G1B + C1;
   static constexpr int PE_FIRST = PEC == 3 ? 8 : 4;
   static constexpr int WT = H * FP + H * H + 16 * H;
-  static constexpr int XFN = FP * PASS;
-  static constexpr int XS = FP + 8;  // xnf row stride (elements): spreads texel rows over LDS banks
+  static constexpr int XS = FP + 8;  // X row stride (elements): spreads texel rows over LDS banks
+  static constexpr int XR = PASS * XS;  // X rows [PASS][XS], kept from the gather to the rounds
   // Decoder-gradient staging of a 64-texel round: [H][64] BF16 rows of
   // A0 and dZ1, then (second phase) dZ0 and A1 in the same rows, plus dY
   // [16][64]; or, where the LDS holds them, all four arrays at once (one
@@ -316,8 +316,8 @@ struct Cfg
   static constexpr int GS_BANK = G + (((16 - G) % 64) + 64) % 64;
   static constexpr int lds_bytes(int sw, int sv, int gs, int stage)
   {
-    const int u = PASS * XS + 2 * sw * sv * (CP0 + CP1);
-    return 2 * (WT + XFN + (u > stage ? u : stage)) + 4 * PASS * gs + 4 * 192 + 4 * 128 + 16 * PASS;
+    const int u = 2 * sw * sv * (CP0 + CP1);
+    return 2 * (WT + XR + (u > stage ? u : stage)) + 4 * PASS * gs + 4 * 192 + 4 * 128 + 16 * PASS;
   }
   static constexpr int LIM = 160 * 1024;
   static constexpr int SEL = lds_bytes(WMAX, BH + 3, GS_BANK, STAGE2) <= LIM       ? 0
@@ -332,14 +332,14 @@ struct Cfg
   static constexpr int STAGE_B = ONE ? STAGE4 : STAGE2;
   static constexpr int DXS = PASS * GS;
   static constexpr int SBOX0 = SW * SV * CP0;  // g1 box offset in the staged boxes (floats)
-  static constexpr int UNION_U = PASS * XS + 2 * SW * SV * (CP0 + CP1);
+  static constexpr int UNION_U = 2 * SW * SV * (CP0 + CP1);
   static constexpr int UNION = UNION_U > STAGE_B ? UNION_U : STAGE_B;
-  static constexpr int LDS = WT + XFN + UNION;
+  static constexpr int LDS = WT + XR + UNION;
   static_assert(FP % 32 == 0 && H % 64 == 0 && G % 16 == 0, "tile shapes");
   static_assert(GSLOTS <= G && G <= FP, "grid slots");
   static_assert(UNION * 2 >= 8 * 256, "final reduction reuses the union region");
   static_assert(16 + BH <= 128, "axis table threads");
-  static_assert((PASS * XS * 2) % 16 == 0, "staged boxes 16-byte aligned");
+  static_assert(((WT + XR) * 2) % 16 == 0, "staged boxes 16-byte aligned");
   static_assert(lds_bytes(SW, SV, GS, STAGE_B) <= LIM, "LDS budget");
   // box cell indices (below WMAX * VMAX + 16) divided by box widths (at most WMAX)
   static_assert((WMAX * VMAX + 16) * WMAX < 65536, "small_div range");
@@ -736,7 +736,7 @@ __device__ __forceinline__ void gather_rest(const FusedParams& P, int px, int py
 // weights and the fma order are those of the per-texel gather.
 template <class CF, int C0, int C1, int G0MODE, int PEC, int LODS>
 __device__ __forceinline__ void gather_staged(const FusedParams& P, const PassGeo& pg, const AxisTab* tab,
-                                              const float* sbox, int tid, uint16_t* xnf, uint16_t* xfn)
+                                              const float* sbox, int tid, uint16_t* xnf)
 {
   constexpr int Q0 = CF::Q0, Q1 = CF::Q1, QI = Q0 + Q1 + 1;
   const int M = P.tl.mip_dim;
@@ -744,27 +744,11 @@ __device__ __forceinline__ void gather_staged(const FusedParams& P, const PassGe
   {
     const int qi = item / CF::PASS, t = item % CF::PASS;
     const int x = t & 15, y = t >> 4;
-    auto put = [&](int slot, float v) {
-      const uint16_t b = bf16_dev(v);
-      xnf[t * CF::XS + slot] = b;
-      xfn[slot * CF::PASS + t] = b;
-    };
+    auto put = [&](int slot, float v) { xnf[t * CF::XS + slot] = bf16_dev(v); };
     // four consecutive slots from slot0 (a multiple of 4): one 8-byte write
-    // into the texel's X row (rows of XS elements, XS a multiple of 4), and
-    // 32-bit writes of texel pairs into the slot rows of X^T: texels t and
-    // t ^ 1 sit in neighbouring lanes of one wave (all 64 lanes run every
-    // item of a wave), which swap dwords with a DPP quad permutation; the
-    // even lane writes slot rows slot0 and slot0 + 2, the odd lane the others
+    // into the texel's X row (rows of XS elements, XS a multiple of 4)
     auto put4 = [&](int slot0, float a, float b, float c, float d) {
-      const uint32_t lo = pk_bf16(a, b), hi = pk_bf16(c, d);
-      *reinterpret_cast<uint2*>(&xnf[t * CF::XS + slot0]) = make_uint2(lo, hi);
-      const int odd = t & 1;
-      const uint32_t sel = odd ? 0x03020706u : 0x05040100u;
-      uint32_t* x32 = reinterpret_cast<uint32_t*>(xfn);
-      const uint32_t lo_n = (uint32_t)__builtin_amdgcn_mov_dpp((int)lo, 0xB1, 0xF, 0xF, false);
-      const uint32_t hi_n = (uint32_t)__builtin_amdgcn_mov_dpp((int)hi, 0xB1, 0xF, 0xF, false);
-      x32[((slot0 + odd) * CF::PASS + (t & ~1)) >> 1] = __builtin_amdgcn_perm(lo_n, lo, sel);
-      x32[((slot0 + 2 + odd) * CF::PASS + (t & ~1)) >> 1] = __builtin_amdgcn_perm(hi_n, hi, sel);
+      *reinterpret_cast<uint2*>(&xnf[t * CF::XS + slot0]) = make_uint2(pk_bf16(a, b), pk_bf16(c, d));
     };
     if (qi < Q0 + Q1)
     {
@@ -843,14 +827,10 @@ __device__ __forceinline__ void gather_staged(const FusedParams& P, const PassGe
 // g1 and the remaining slots.
 template <class CF, int C0, int C1, int G0MODE, int PEC, int LODS>
 __device__ __forceinline__ void gather_part(const FusedParams& P, int part, int tl, bool valid, int px, int py,
-                                            uint16_t* xnf, uint16_t* xfn)
+                                            uint16_t* xnf)
 {
   constexpr int FP = CF::KF * 32;
-  auto put = [&](int slot, float v) {
-    const uint16_t b = bf16_dev(v);
-    xnf[tl * CF::XS + slot] = b;
-    xfn[slot * CF::PASS + tl] = b;
-  };
+  auto put = [&](int slot, float v) { xnf[tl * CF::XS + slot] = bf16_dev(v); };
   const int M = P.tl.mip_dim;
   if (part == 0)
   {
@@ -963,10 +943,9 @@ __device__ __forceinline__ void fused_body(const FusedParams& P, const int vblk,
   constexpr TableLayout TLY = {H, FP, G};
   __shared__ __attribute__((aligned(16))) uint16_t smem[CF::LDS];
   uint16_t* wt = smem;
-  uint16_t* xfn = smem + CF::WT;
-  uint16_t* uni = xfn + CF::XFN;
-  uint16_t* xnf = uni;                       // [PASS][XS] during gather/forward
-  float* sbox = reinterpret_cast<float*>(uni + kPass * CF::XS);  // staged boxes during gather
+  uint16_t* xnf = smem + CF::WT;                       // X rows [PASS][XS]
+  uint16_t* uni = xnf + CF::XR;
+  float* sbox = reinterpret_cast<float*>(uni);         // staged boxes during the gather
   // staging rows of texels, columns placed by swz: A0, dZ1, dZ0, A1 (the
   // last two in A0's and dZ1's rows after the first phase unless all four
   // fit), dY
@@ -1046,7 +1025,7 @@ __device__ __forceinline__ void fused_body(const FusedParams& P, const int vblk,
     // ---- P0: gather ----
     if (cur.staged)
     {
-      gather_staged<CF, C0, C1, G0MODE, PEC, LODS>(P, cur, tab, sbox, tid, xnf, xfn);
+      gather_staged<CF, C0, C1, G0MODE, PEC, LODS>(P, cur, tab, sbox, tid, xnf);
     }
     else
     {
@@ -1055,7 +1034,7 @@ __device__ __forceinline__ void fused_body(const FusedParams& P, const int vblk,
       for (int tl = tid & 127; tl < kPass; tl += 128)
       {
         const PassTexel pt = pass_texel(pm, pass, tl, kPass, orig_s, T, M, P.B);
-        gather_part<CF, C0, C1, G0MODE, PEC, LODS>(P, part, tl, pt.valid, pt.x, pt.y, xnf, xfn);
+        gather_part<CF, C0, C1, G0MODE, PEC, LODS>(P, part, tl, pt.valid, pt.x, pt.y, xnf);
       }
     }
     __syncthreads();
@@ -1400,18 +1379,20 @@ __device__ __forceinline__ void fused_body(const FusedParams& P, const int vblk,
         for (int i = 0; i < 4; ++i) stY[(4 * g + i) * 64 + scol[i >> 1]] = bf16_dev(dy[rnd][i]);
         __syncthreads();
       }
-      // dW0[h][f] += dZ0 X^T, X^T read from xfn in the staged texel order
+      // dW0[h][f] += dZ0 X^T: the X^T fragments by transposing reads of the
+      // X rows, K position p of the round at texel (p / 16) 16 NT + 16 rnd
+      // + p % 16 (the staged texel order)
       {
         constexpr int NC = FP / 16;
-        auto xcol = [&](int kc) {
-          const int kk = 32 * kc + 8 * g;
-          return (kk >> 4) * 16 * kNT + 16 * rnd + (kk & 15);
+        auto xrow = [&](int kc, int h) {
+          const int p = 32 * kc + 8 * g + 4 * h + (n16 >> 2);
+          return ((p >> 4) * 16 * kNT + 16 * rnd + (p & 15)) * CF::XS;
         };
         bf16x8 a;
-        lds_ring<MTW * 2 * NC, kRing>(
+        lds_ring_ld<MTW * 2 * NC, kRing>(
             [&](int i) {
-              const int row = 16 * (i % NC) + n16;
-              return &xfn[row * kPass + xcol((i / NC) % 2)];
+              const int c = 16 * (i % NC) + 4 * (n16 & 3), kc = (i / NC) % 2;
+              return frag_tr(&xnf[xrow(kc, 0) + c], &xnf[xrow(kc, 1) + c]);
             },
             [&](int i, const bf16x8 b) {
               const int mt = i / (2 * NC), kc = (i / NC) % 2, c = i % NC;

@@ -63,7 +63,6 @@ def _plan(H, FP, G, C0, C1, mode, NT):
     PASS = 64 * NT
     BH = PASS // 16
     WT = H * FP + H * H + 16 * H
-    XFN = FP * PASS
     XS = FP + 8
     STAGE2, STAGE4 = 2 * H * 64 + 16 * 64, 4 * H * 64 + 16 * 64
     WMAX = 18
@@ -74,8 +73,8 @@ def _plan(H, FP, G, C0, C1, mode, NT):
     GS_BANK = G + (16 - G) % 64
 
     def lds_bytes(sw, sv, gs, stage=STAGE2):
-        u = PASS * XS + 2 * sw * sv * (CP0 + CP1)
-        return 2 * (WT + XFN + max(u, stage)) + 4 * PASS * gs + 4 * 192 + 4 * 128 + 16 * PASS
+        u = 2 * sw * sv * (CP0 + CP1)
+        return 2 * (WT + PASS * XS + max(u, stage)) + 4 * PASS * gs + 4 * 192 + 4 * 128 + 16 * PASS
 
     cands = [(WMAX, BH + 3, GS_BANK), (WMAX, BH + 3, G + 4), (10, BH // 2 + 3, GS_BANK), (10, BH // 2 + 3, G + 4)]
     sel = next((i for i, c in enumerate(cands) if lds_bytes(*c) <= LDS_LIMIT), 4)
@@ -83,8 +82,8 @@ def _plan(H, FP, G, C0, C1, mode, NT):
     SW, SV, GS = cands[sel]
     ONE = lds_bytes(SW, SV, GS, STAGE4) <= LDS_LIMIT  # all four staging arrays at once
     STAGE_B = STAGE4 if ONE else STAGE2
-    UNION = max(PASS * XS + 2 * SW * SV * (CP0 + CP1), STAGE_B)
-    return dict(PASS=PASS, BH=BH, WT=WT, XFN=XFN, XS=XS, UNION=UNION, SW=SW, SV=SV, GS=GS, CP0=CP0, CP1=CP1,
+    UNION = max(2 * SW * SV * (CP0 + CP1), STAGE_B)
+    return dict(PASS=PASS, BH=BH, WT=WT, XS=XS, UNION=UNION, SW=SW, SV=SV, GS=GS, CP0=CP0, CP1=CP1,
                 Q0=Q0, Q1=Q1, G1B=G1B, GSLOTS=GSLOTS, MT=H // 16, KF=FP // 32, KH=H // 32, GT=G // 16,
                 RW=H // 4, MTW=H // 64, NTO=H // 64, SBOX0=SW * SV * CP0,
                 DXS=PASS * GS, ONE=ONE, STAGE_B=STAGE_B, total=lds_bytes(SW, SV, GS, STAGE_B))
@@ -157,15 +156,14 @@ def _emulate(H, FP, G, C0, C1, mode, PEC, LODS, NT):
     n16, g = lanes & 15, lanes >> 4
     wt = Region("weight tables", P["WT"])
     xnf = Region("X rows", PASS * XS)
-    xfn = Region("X^T rows", P["XFN"])
-    sbox = Region("staged boxes (floats)", P["UNION"] // 2 - PASS * XS // 2)
+    sbox = Region("staged boxes (floats)", P["UNION"] // 2)
     stA, stB, stY = Region("stA", H * 64), Region("stB", H * 64), Region("stY", 16 * 64)
     # dZ0 and A1 rows: their own arrays when all four fit, else A0's and dZ1's
     stC, stD = (Region("stC", H * 64), Region("stD", H * 64)) if P["ONE"] else (stA, stB)
     dxs = Region("dX", P["DXS"])
     tgt = Region("target words", PASS * 4)
     # the union holds X + boxes, then stA | stB | stY
-    assert PASS * XS + 2 * P["SW"] * P["SV"] * (P["CP0"] + P["CP1"]) <= P["UNION"]
+    assert 2 * P["SW"] * P["SV"] * (P["CP0"] + P["CP1"]) <= P["UNION"]
     assert P["STAGE_B"] <= P["UNION"] and P["STAGE_B"] == (4 if P["ONE"] else 2) * H * 64 + 16 * 64
     oW0, oW1 = 0, H * FP
     oWo = oW1 + H * H
@@ -224,30 +222,25 @@ def _emulate(H, FP, G, C0, C1, mode, PEC, LODS, NT):
                 for c in range(P["MT"]):
                     rr = 16 * c + n16
                     stA.touch(rr * 64 + vswz(rr, 32 * kc + 8 * g, 64), 8)
-        for rnd in range(NT):
+        for rnd in range(NT):  # X^T by transposing reads of the X rows
             for kc in range(2):
-                kk = 32 * kc + 8 * g
-                xc = (kk >> 4) * 16 * NT + 16 * rnd + (kk & 15)
-                for c in range(FP // 16):
-                    xfn.touch((16 * c + n16) * PASS + xc, 8)
+                for h in range(2):
+                    p = 32 * kc + 8 * g + 4 * h + (n16 >> 2)
+                    trow = (p >> 4) * 16 * NT + 16 * rnd + (p & 15)
+                    for c in range(FP // 16):
+                        xnf.touch(trow * XS + 16 * c + 4 * (n16 & 3), 4)
         for kc in range(2):
             stY.touch(n16 * 64 + vswz(n16, 32 * kc + 8 * g, 64), 8)
             for c in range(P["NTO"]):
                 hc = (wave * P["NTO"] + c) * 16 + n16
                 stD.touch(hc * 64 + vswz(hc, 32 * kc + 8 * g, 64), 8)
-    # staged gather: put4 (8-byte X-row writes, 32-bit X^T pair writes) and
-    # single puts for every slot < FP of every texel; box reads for boxes up
-    # to SW x SV cells
+    # staged gather: put4 (8-byte X-row writes) and single puts for every
+    # slot < FP of every texel; box reads for boxes up to SW x SV cells
     t = np.arange(PASS)
     for slot0 in range(0, FP, 4):
         xnf.touch(t * XS + slot0, 4)
-        for oddv in (0, 1):
-            tt = t[(t & 1) == oddv]
-            for s in (slot0 + oddv, slot0 + 2 + oddv):
-                xfn.touch(2 * (((s * PASS + (tt & ~1)) >> 1)), 2)
     for slot in range(FP):
         xnf.touch(t * XS + slot)
-        xfn.touch(slot * PASS + t)
     for g1, (Q, CP) in enumerate(((P["Q0"], P["CP0"]), (P["Q1"], P["CP1"]))):
         base = P["SBOX0"] if g1 else 0
         bw, bh = P["SW"], P["SV"]
@@ -262,7 +255,7 @@ def _emulate(H, FP, G, C0, C1, mode, PEC, LODS, NT):
         for c in range(nc):
             for ch in range(C):
                 dxs.touch(tex * P["GS"] + slot0 + c * C + ch)
-    for r in (wt, xnf, xfn, sbox, stA, stB, stC, stD, stY, dxs, tgt):
+    for r in (wt, xnf, sbox, stA, stB, stC, stD, stY, dxs, tgt):
         r.check()
     # every dX slot stays inside its row (no row reaches into the next texel's)
     assert P["GSLOTS"] <= G <= P["GS"]
@@ -421,3 +414,36 @@ def test_transposing_reads_of_128_wide_tables_are_conflict_free(j, r):
             assert len(set(_banks(addrs, 8))) == 32, (g, h)
         fwd = [2 * ((16 * r + i) * H + swz(16 * r + i, 32 * j + 8 * g, H)) for i in range(16)]
         assert len(set(_banks(fwd, 16))) == 64
+
+
+@pytest.mark.parametrize("FP,NT", [(64, 1), (32, 2), (64, 2), (96, 2)])
+def test_x_transposing_reads_form_xt_fragments(FP, NT):
+    """The dW0 B fragments (K = the round's 32 texels of a K chunk, N = 16
+    feature slots) read from the X rows with transposing reads equal X^T in
+    the staged texel order, for every round, K chunk and slot tile; within a
+    16-lane group the reads are bank-conflict free for the 64-slot rows."""
+    PASS, XS = 64 * NT, FP + 8
+    X = np.arange(PASS * XS, dtype=np.int64)  # X[t][f] = t * XS + f
+    lanes = np.arange(64)
+    n16, g = lanes & 15, lanes >> 4
+    for rnd in range(NT):
+        for kc in range(2):
+            for c in range(FP // 16):
+                parts = []
+                for h in range(2):
+                    p = 32 * kc + 8 * g + 4 * h + (n16 >> 2)
+                    trow = (p >> 4) * 16 * NT + 16 * rnd + (p & 15)
+                    addrs = trow * XS + 16 * c + 4 * (n16 & 3)
+                    parts.append(tr_read(X, addrs))
+                    if FP == 64:
+                        for g0 in range(4):
+                            b = _banks([2 * a for a in addrs[16 * g0:16 * g0 + 16]], 8)
+                            assert len(set(b)) == 32
+                frag = np.concatenate(parts, axis=1)
+                for lane in range(64):
+                    want = []
+                    for e in range(8):
+                        p = 32 * kc + 8 * g[lane] + e
+                        t = (p >> 4) * 16 * NT + 16 * rnd + (p & 15)
+                        want.append(t * XS + 16 * c + n16[lane])
+                    assert list(frag[lane]) == want, (rnd, kc, c, lane)
