@@ -139,20 +139,6 @@ __device__ __forceinline__ bf16x8 frag_pack(const float* v)
   return __builtin_bit_cast(bf16x8, u);
 }
 
-// Stages dword d of fragment f (rows h and h + 1 of this lane's texel column)
-// into a [row][64] BF16 array as 32-bit writes: the lane holding an even
-// column writes row h for its column and its neighbour's, the odd lane row
-// h + 1 for both (the neighbours swap dwords with a DPP quad permutation).
-// sel: 0x05040100 on even lanes, 0x03020706 on odd ones; at: the dword index
-// of (h + odd, even column).
-__device__ __forceinline__ void stage_pair(uint16_t* st, const bf16x8& f, int d, uint32_t sel, int at)
-{
-  const uint4 u = __builtin_bit_cast(uint4, f);
-  const uint32_t mine = d == 0 ? u.x : (d == 1 ? u.y : (d == 2 ? u.z : u.w));
-  const uint32_t theirs = (uint32_t)__builtin_amdgcn_mov_dpp((int)mine, 0xB1, 0xF, 0xF, false);
-  reinterpret_cast<uint32_t*>(st)[at] = __builtin_amdgcn_perm(theirs, mine, sel);
-}
-
 __device__ __forceinline__ uint16_t frag_elem(const bf16x8& f, int e)
 {
   const uint4 u = __builtin_bit_cast(uint4, f);
@@ -294,11 +280,12 @@ struct Cfg
   static constexpr int WT = H * FP + H * H + 16 * H;
   static constexpr int XS = FP + 8;  // X row stride (elements): spreads texel rows over LDS banks
   static constexpr int XR = PASS * XS;  // X rows [PASS][XS], kept from the gather to the rounds
-  // Decoder-gradient staging of a 64-texel round: [H][64] BF16 rows of
-  // A0 and dZ1, then (second phase) dZ0 and A1 in the same rows, plus dY
-  // [16][64]; or, where the LDS holds them, all four arrays at once (one
-  // staging phase and one barrier less per round).
-  static constexpr int STAGE2 = 2 * H * 64 + 16 * 64, STAGE4 = 4 * H * 64 + 16 * 64;
+  // Decoder-gradient staging of a 64-texel round, texel-major: A0, dZ1, dZ0
+  // and A1 as [64][SH] BF16 rows (a texel's H values in the kperm position
+  // order, one 16-byte store per fragment; SH = H + 16 puts the four rows
+  // of a transposing read on distinct banks), dY as [64][16].
+  static constexpr int SH = H + 16;
+  static constexpr int STAGE = 4 * 64 * SH + 64 * 16;
   static constexpr int BH = PASS / 16;
   // Latent gradients of a blocked pass (16 x BH texels) come from the
   // per-texel dX when the pass footprint spans at most WMAX x VMAX cells of
@@ -320,27 +307,25 @@ struct Cfg
     return 2 * (WT + XR + (u > stage ? u : stage)) + 4 * PASS * gs + 4 * 192 + 4 * 128 + 16 * PASS;
   }
   static constexpr int LIM = 160 * 1024;
-  static constexpr int SEL = lds_bytes(WMAX, BH + 3, GS_BANK, STAGE2) <= LIM       ? 0
-                             : lds_bytes(WMAX, BH + 3, G + 4, STAGE2) <= LIM       ? 1
-                             : lds_bytes(10, BH / 2 + 3, GS_BANK, STAGE2) <= LIM   ? 2
-                             : lds_bytes(10, BH / 2 + 3, G + 4, STAGE2) <= LIM     ? 3
+  static constexpr int SEL = lds_bytes(WMAX, BH + 3, GS_BANK, STAGE) <= LIM       ? 0
+                             : lds_bytes(WMAX, BH + 3, G + 4, STAGE) <= LIM       ? 1
+                             : lds_bytes(10, BH / 2 + 3, GS_BANK, STAGE) <= LIM   ? 2
+                             : lds_bytes(10, BH / 2 + 3, G + 4, STAGE) <= LIM     ? 3
                                                                            : 4;
   static constexpr int SW = SEL <= 1 ? WMAX : (SEL <= 3 ? 10 : 0);
   static constexpr int SV = SEL <= 1 ? BH + 3 : (SEL <= 3 ? BH / 2 + 3 : 0);
   static constexpr int GS = (SEL == 0 || SEL == 2) ? GS_BANK : G + 4;  // dX row stride (floats)
-  static constexpr bool ONE = lds_bytes(SW, SV, GS, STAGE4) <= LIM;  // single staging phase
-  static constexpr int STAGE_B = ONE ? STAGE4 : STAGE2;
   static constexpr int DXS = PASS * GS;
   static constexpr int SBOX0 = SW * SV * CP0;  // g1 box offset in the staged boxes (floats)
   static constexpr int UNION_U = 2 * SW * SV * (CP0 + CP1);
-  static constexpr int UNION = UNION_U > STAGE_B ? UNION_U : STAGE_B;
+  static constexpr int UNION = UNION_U > STAGE ? UNION_U : STAGE;
   static constexpr int LDS = WT + XR + UNION;
   static_assert(FP % 32 == 0 && H % 64 == 0 && G % 16 == 0, "tile shapes");
   static_assert(GSLOTS <= G && G <= FP, "grid slots");
   static_assert(UNION * 2 >= 8 * 256, "final reduction reuses the union region");
   static_assert(16 + BH <= 128, "axis table threads");
   static_assert(((WT + XR) * 2) % 16 == 0, "staged boxes 16-byte aligned");
-  static_assert(lds_bytes(SW, SV, GS, STAGE_B) <= LIM, "LDS budget");
+  static_assert(lds_bytes(SW, SV, GS, STAGE) <= LIM, "LDS budget");
   // box cell indices (below WMAX * VMAX + 16) divided by box widths (at most WMAX)
   static_assert((WMAX * VMAX + 16) * WMAX < 65536, "small_div range");
 };
@@ -946,14 +931,12 @@ __device__ __forceinline__ void fused_body(const FusedParams& P, const int vblk,
   uint16_t* xnf = smem + CF::WT;                       // X rows [PASS][XS]
   uint16_t* uni = xnf + CF::XR;
   float* sbox = reinterpret_cast<float*>(uni);         // staged boxes during the gather
-  // staging rows of texels, columns placed by swz: A0, dZ1, dZ0, A1 (the
-  // last two in A0's and dZ1's rows after the first phase unless all four
-  // fit), dY
-  uint16_t* stA = uni;                                  // [H][64] A0
-  uint16_t* stB = uni + H * 64;                         // [H][64] dZ1
-  uint16_t* stC = CF::ONE ? uni + 2 * H * 64 : stA;     // [H][64] dZ0
-  uint16_t* stD = CF::ONE ? uni + 3 * H * 64 : stB;     // [H][64] A1
-  uint16_t* stY = uni + (CF::ONE ? 4 : 2) * H * 64;     // [16][64] dY
+  // decoder-gradient staging of a round: texel rows
+  uint16_t* stA = uni;                       // [64][SH] A0
+  uint16_t* stB = uni + 64 * CF::SH;         // [64][SH] dZ1
+  uint16_t* stC = uni + 2 * 64 * CF::SH;     // [64][SH] dZ0
+  uint16_t* stD = uni + 3 * 64 * CF::SH;     // [64][SH] A1
+  uint16_t* stY = uni + 4 * 64 * CF::SH;     // [64][16] dY
   __shared__ __attribute__((aligned(16))) float dxs[CF::DXS];  // dX of the grid slots [texel][GS]
   __shared__ AxisTab atab[4];                                  // corner tables of g0, g1; two passes
   __shared__ uint32_t orig_s[128];                             // tile origins mod M
@@ -1318,68 +1301,44 @@ __device__ __forceinline__ void fused_body(const FusedParams& P, const int vblk,
 #pragma unroll
     for (int rnd = 0; rnd < (MODE == kForward ? 0 : kNT); ++rnd)
     {
-      const int col = wave * 16 + n16;
-      // swz(h, col, 64) for the rows this lane writes: h = kperm(j, g, e) and
-      // 4 g + i give (h >> 1) & 7 = (2 g + ((e >> 1) & 1)) & 7, two columns
-      int scol[2];
+      // this wave's tile rnd: texel rows wave * 16 + n16 of the round, one
+      // 16-byte store per fragment (positions 32 j + 8 g .. + 7), dY as 4
+      // outputs 4 g .. 4 g + 3
+      {
+        const int row = wave * 16 + n16;
 #pragma unroll
-      for (int b = 0; b < 2; ++b) scol[b] = swz(2 * (2 * g + b), col, 64);
-      // the same for the column pair (col & ~1) of the 32-bit writes
-      const int odd = n16 & 1;
-      const uint32_t sel = odd ? 0x03020706u : 0x05040100u;
-      int pcol[2];
-#pragma unroll
-      for (int b = 0; b < 2; ++b) pcol[b] = (odd * 64 + swz(2 * (2 * g + b), col & ~1, 64)) >> 1;
-#pragma unroll
-      for (int j = 0; j < KH; ++j)
-#pragma unroll
-        for (int d = 0; d < 4; ++d)
+        for (int j = 0; j < KH; ++j)
         {
-          const int at = kperm(j, g, 2 * d) * 32 + pcol[d & 1];
-          stage_pair(stA, a0f[j][rnd], d, sel, at);
-          stage_pair(stB, dz1f[j][rnd], d, sel, at);
-          if constexpr (CF::ONE)
-          {
-            stage_pair(stC, dz0f[j][rnd], d, sel, at);
-            stage_pair(stD, a1f[j][rnd], d, sel, at);
-          }
+          const int at = row * CF::SH + 32 * j + 8 * g;
+          *reinterpret_cast<bf16x8*>(&stA[at]) = a0f[j][rnd];
+          *reinterpret_cast<bf16x8*>(&stB[at]) = dz1f[j][rnd];
+          *reinterpret_cast<bf16x8*>(&stC[at]) = dz0f[j][rnd];
+          *reinterpret_cast<bf16x8*>(&stD[at]) = a1f[j][rnd];
         }
-      if constexpr (CF::ONE)
-#pragma unroll
-        for (int i = 0; i < 4; ++i) stY[(4 * g + i) * 64 + scol[i >> 1]] = bf16_dev(dy[rnd][i]);
+        *reinterpret_cast<uint2*>(&stY[row * 16 + 4 * g]) =
+            make_uint2(pk_bf16(dy[rnd][0], dy[rnd][1]), pk_bf16(dy[rnd][2], dy[rnd][3]));
+      }
       __syncthreads();
-      // dW1[h_out][h_in] += dZ1 A0^T, rows of this wave
+      // Every operand of the products below has the round's texels as its K
+      // index: a fragment is two transposing reads of texel rows 32 kc + 8 g
+      // + 4 h + k (k = 0..3 over the lanes 4 k + q), at columns c0 + 4 q.
+      const int trr0 = 8 * g + (n16 >> 2), tq = 4 * (n16 & 3);
+      auto frag_st = [&](const uint16_t* st, int stride, int kc, int c0) {
+        const uint16_t* p = st + (32 * kc + trr0) * stride + c0 + tq;
+        return frag_tr(p, p + 4 * stride);
+      };
+      // dW1[p_out][p_in] += dZ1 A0^T over positions; rows RW wave .. of this wave
       {
         bf16x8 a;
-        lds_ring<MTW * 2 * MT, kRing>(
-            [&](int i) {
-              const int row = 16 * (i % MT) + n16;
-              return &stA[row * 64 + swz(row, 32 * ((i / MT) % 2) + 8 * g, 64)];
-            },
+        lds_ring_ld<MTW * 2 * MT, kRing>(
+            [&](int i) { return frag_st(stA, CF::SH, (i / MT) % 2, 16 * (i % MT)); },
             [&](int i, const bf16x8 b) {
               const int mt = i / (2 * MT), kc = (i / MT) % 2, c = i % MT;
-              const int row = CF::RW * wave + 16 * mt + n16;
-              if (c == 0) a = frag_lds(&stB[row * 64 + swz(row, 32 * kc + 8 * g, 64)]);
+              if (c == 0) a = frag_st(stB, CF::SH, kc, CF::RW * wave + 16 * mt);
               dw1[mt][c] = mfma(a, b, dw1[mt][c]);
             });
       }
-      if constexpr (!CF::ONE)
-      {
-        __syncthreads();
-#pragma unroll
-        for (int j = 0; j < KH; ++j)
-#pragma unroll
-          for (int d = 0; d < 4; ++d)
-          {
-            const int at = kperm(j, g, 2 * d) * 32 + pcol[d & 1];
-            stage_pair(stC, dz0f[j][rnd], d, sel, at);
-            stage_pair(stD, a1f[j][rnd], d, sel, at);
-          }
-#pragma unroll
-        for (int i = 0; i < 4; ++i) stY[(4 * g + i) * 64 + scol[i >> 1]] = bf16_dev(dy[rnd][i]);
-        __syncthreads();
-      }
-      // dW0[h][f] += dZ0 X^T: the X^T fragments by transposing reads of the
+      // dW0[p][f] += dZ0 X^T: the X^T fragments by transposing reads of the
       // X rows, K position p of the round at texel (p / 16) 16 NT + 16 rnd
       // + p % 16 (the staged texel order)
       {
@@ -1391,27 +1350,22 @@ __device__ __forceinline__ void fused_body(const FusedParams& P, const int vblk,
         bf16x8 a;
         lds_ring_ld<MTW * 2 * NC, kRing>(
             [&](int i) {
-              const int c = 16 * (i % NC) + 4 * (n16 & 3), kc = (i / NC) % 2;
+              const int c = 16 * (i % NC) + tq, kc = (i / NC) % 2;
               return frag_tr(&xnf[xrow(kc, 0) + c], &xnf[xrow(kc, 1) + c]);
             },
             [&](int i, const bf16x8 b) {
               const int mt = i / (2 * NC), kc = (i / NC) % 2, c = i % NC;
-              const int row = CF::RW * wave + 16 * mt + n16;
-              if (c == 0) a = frag_lds(&stC[row * 64 + swz(row, 32 * kc + 8 * g, 64)]);
+              if (c == 0) a = frag_st(stC, CF::SH, kc, CF::RW * wave + 16 * mt);
               dw0[mt][c] = mfma(a, b, dw0[mt][c]);
             });
       }
-      // dWout[o][h] += dY A1^T, column tiles of this wave
+      // dWout[o][p] += dY A1^T, column tiles of this wave
 #pragma unroll
       for (int kc = 0; kc < 2; ++kc)
       {
-        const bf16x8 a = frag_lds(&stY[n16 * 64 + swz(n16, 32 * kc + 8 * g, 64)]);
+        const bf16x8 a = frag_st(stY, 16, kc, 0);
 #pragma unroll
-        for (int c = 0; c < NTO; ++c)
-        {
-          const int hc = (wave * NTO + c) * 16 + n16;
-          dwo[c] = mfma(a, frag_lds(&stD[hc * 64 + swz(hc, 32 * kc + 8 * g, 64)]), dwo[c]);
-        }
+        for (int c = 0; c < NTO; ++c) dwo[c] = mfma(a, frag_st(stD, CF::SH, kc, (wave * NTO + c) * 16), dwo[c]);
       }
       __syncthreads();
     }
@@ -1449,9 +1403,10 @@ __device__ __forceinline__ void fused_body(const FusedParams& P, const int vblk,
 #pragma unroll
     for (int i = 0; i < 4; ++i)
     {
-      const int row = CF::RW * wave + 16 * mt + 4 * g + i;
+      // accumulator rows and dW1 columns are kperm positions
+      const int row = kperm_pos(CF::RW * wave + 16 * mt + 4 * g + i);
 #pragma unroll
-      for (int c = 0; c < MT; ++c) pw1[row * H + 16 * c + n16] = dw1[mt][c][i];
+      for (int c = 0; c < MT; ++c) pw1[row * H + kperm_pos(16 * c + n16)] = dw1[mt][c][i];
 #pragma unroll
       for (int c = 0; c < FP / 16; ++c)
       {
@@ -1463,7 +1418,7 @@ __device__ __forceinline__ void fused_body(const FusedParams& P, const int vblk,
 #pragma unroll
   for (int c = 0; c < NTO; ++c)
 #pragma unroll
-    for (int i = 0; i < 4; ++i) pwo[(4 * g + i) * H + (wave * NTO + c) * 16 + n16] = dwo[c][i];
+    for (int i = 0; i < 4; ++i) pwo[(4 * g + i) * H + kperm_pos((wave * NTO + c) * 16 + n16)] = dwo[c][i];
   }
 
   // SSE of this workgroup (the union region is free after the last pass)

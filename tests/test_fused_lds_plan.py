@@ -64,7 +64,8 @@ def _plan(H, FP, G, C0, C1, mode, NT):
     BH = PASS // 16
     WT = H * FP + H * H + 16 * H
     XS = FP + 8
-    STAGE2, STAGE4 = 2 * H * 64 + 16 * 64, 4 * H * 64 + 16 * 64
+    SH = H + 16
+    STAGE = 4 * 64 * SH + 64 * 16
     WMAX = 18
     Q0, Q1 = (C0 + 3) // 4, (C1 + 3) // 4
     CP0, CP1 = 4 * Q0, 4 * Q1
@@ -72,7 +73,7 @@ def _plan(H, FP, G, C0, C1, mode, NT):
     GSLOTS = G1B + C1
     GS_BANK = G + (16 - G) % 64
 
-    def lds_bytes(sw, sv, gs, stage=STAGE2):
+    def lds_bytes(sw, sv, gs, stage=STAGE):
         u = 2 * sw * sv * (CP0 + CP1)
         return 2 * (WT + PASS * XS + max(u, stage)) + 4 * PASS * gs + 4 * 192 + 4 * 128 + 16 * PASS
 
@@ -80,13 +81,11 @@ def _plan(H, FP, G, C0, C1, mode, NT):
     sel = next((i for i, c in enumerate(cands) if lds_bytes(*c) <= LDS_LIMIT), 4)
     assert sel < 4, "layout does not fit"
     SW, SV, GS = cands[sel]
-    ONE = lds_bytes(SW, SV, GS, STAGE4) <= LDS_LIMIT  # all four staging arrays at once
-    STAGE_B = STAGE4 if ONE else STAGE2
-    UNION = max(2 * SW * SV * (CP0 + CP1), STAGE_B)
+    UNION = max(2 * SW * SV * (CP0 + CP1), STAGE)
     return dict(PASS=PASS, BH=BH, WT=WT, XS=XS, UNION=UNION, SW=SW, SV=SV, GS=GS, CP0=CP0, CP1=CP1,
                 Q0=Q0, Q1=Q1, G1B=G1B, GSLOTS=GSLOTS, MT=H // 16, KF=FP // 32, KH=H // 32, GT=G // 16,
                 RW=H // 4, MTW=H // 64, NTO=H // 64, SBOX0=SW * SV * CP0,
-                DXS=PASS * GS, ONE=ONE, STAGE_B=STAGE_B, total=lds_bytes(SW, SV, GS, STAGE_B))
+                DXS=PASS * GS, SH=SH, STAGE=STAGE, total=lds_bytes(SW, SV, GS))
 
 
 def swz(row, col, wd):
@@ -157,14 +156,14 @@ def _emulate(H, FP, G, C0, C1, mode, PEC, LODS, NT):
     wt = Region("weight tables", P["WT"])
     xnf = Region("X rows", PASS * XS)
     sbox = Region("staged boxes (floats)", P["UNION"] // 2)
-    stA, stB, stY = Region("stA", H * 64), Region("stB", H * 64), Region("stY", 16 * 64)
-    # dZ0 and A1 rows: their own arrays when all four fit, else A0's and dZ1's
-    stC, stD = (Region("stC", H * 64), Region("stD", H * 64)) if P["ONE"] else (stA, stB)
+    SH = P["SH"]
+    stA, stB, stC, stD = (Region(n, 64 * SH) for n in ("stA", "stB", "stC", "stD"))
+    stY = Region("stY", 64 * 16)
     dxs = Region("dX", P["DXS"])
     tgt = Region("target words", PASS * 4)
     # the union holds X + boxes, then stA | stB | stY
     assert 2 * P["SW"] * P["SV"] * (P["CP0"] + P["CP1"]) <= P["UNION"]
-    assert P["STAGE_B"] <= P["UNION"] and P["STAGE_B"] == (4 if P["ONE"] else 2) * H * 64 + 16 * 64
+    assert P["STAGE"] <= P["UNION"]
     oW0, oW1 = 0, H * FP
     oWo = oW1 + H * H
     assert oWo + 16 * H == P["WT"]
@@ -202,26 +201,26 @@ def _emulate(H, FP, G, C0, C1, mode, PEC, LODS, NT):
                 if ok.any():
                     dxs.touch((tloc * P["GS"] + s0)[ok], 4)
             tgt.touch(tloc * 4 + g)
-        # decoder-gradient rounds
-        col = wave * 16 + n16
-        odd = n16 & 1
-        pcol = [(odd * 64 + vswz(2 * (2 * g + b), col & ~1, 64)) >> 1 for b in range(2)]
-        scol = [vswz(2 * (2 * g + b), col, 64) for b in range(2)]
+        # decoder-gradient rounds: texel-row stores, transposing reads
+        row = wave * 16 + n16
         for j in range(P["KH"]):
-            for d in range(4):
-                at = kperm(j, g, 2 * d) * 32 + pcol[d & 1]  # dword index
-                for st in (stA, stB, stC, stD):
-                    st.touch(2 * at, 2)
-        for i in range(4):
-            stY.touch((4 * g + i) * 64 + scol[i >> 1])
-        for mt in range(P["MTW"]):
-            for kc in range(2):
-                row = P["RW"] * wave + 16 * mt + n16
-                stB.touch(row * 64 + vswz(row, 32 * kc + 8 * g, 64), 8)  # dW1 A operand
-                stC.touch(row * 64 + vswz(row, 32 * kc + 8 * g, 64), 8)  # dW0 A operand
-                for c in range(P["MT"]):
-                    rr = 16 * c + n16
-                    stA.touch(rr * 64 + vswz(rr, 32 * kc + 8 * g, 64), 8)
+            for st in (stA, stB, stC, stD):
+                st.touch(row * SH + 32 * j + 8 * g, 8)
+        stY.touch(row * 16 + 4 * g, 4)
+        trr0, tq = 8 * g + (n16 >> 2), 4 * (n16 & 3)
+
+        def frag_st(reg, stride, kc, c0):
+            for h in range(2):
+                reg.touch((32 * kc + trr0 + 4 * h) * stride + c0 + tq, 4)
+        for kc in range(2):
+            for mt in range(P["MTW"]):
+                frag_st(stB, SH, kc, P["RW"] * wave + 16 * mt)  # dW1 A operand
+                frag_st(stC, SH, kc, P["RW"] * wave + 16 * mt)  # dW0 A operand
+            for c in range(P["MT"]):
+                frag_st(stA, SH, kc, 16 * c)
+            frag_st(stY, 16, kc, 0)
+            for c in range(P["NTO"]):
+                frag_st(stD, SH, kc, (wave * P["NTO"] + c) * 16)
         for rnd in range(NT):  # X^T by transposing reads of the X rows
             for kc in range(2):
                 for h in range(2):
@@ -229,11 +228,6 @@ def _emulate(H, FP, G, C0, C1, mode, PEC, LODS, NT):
                     trow = (p >> 4) * 16 * NT + 16 * rnd + (p & 15)
                     for c in range(FP // 16):
                         xnf.touch(trow * XS + 16 * c + 4 * (n16 & 3), 4)
-        for kc in range(2):
-            stY.touch(n16 * 64 + vswz(n16, 32 * kc + 8 * g, 64), 8)
-            for c in range(P["NTO"]):
-                hc = (wave * P["NTO"] + c) * 16 + n16
-                stD.touch(hc * 64 + vswz(hc, 32 * kc + 8 * g, 64), 8)
     # staged gather: put4 (8-byte X-row writes) and single puts for every
     # slot < FP of every texel; box reads for boxes up to SW x SV cells
     t = np.arange(PASS)
@@ -447,3 +441,29 @@ def test_x_transposing_reads_form_xt_fragments(FP, NT):
                         t = (p >> 4) * 16 * NT + 16 * rnd + (p & 15)
                         want.append(t * XS + 16 * c + n16[lane])
                     assert list(frag[lane]) == want, (rnd, kc, c, lane)
+
+
+@pytest.mark.parametrize("H", [64, 128])
+def test_texel_row_staging_fragments_and_banks(H):
+    """The decoder-gradient operands read back from the texel-row staging
+    with transposing reads are the fragments of the [unit][texel] matrices
+    (K = the round's texels), and each 16-lane group's reads hit 32 distinct
+    banks (rows of H + 16 elements; dY rows of 16)."""
+    SH = H + 16
+    S = np.arange(64 * SH, dtype=np.int64)  # S[t][p] = t * SH + p
+    lanes = np.arange(64)
+    n16, g = lanes & 15, lanes >> 4
+    trr0, tq = 8 * g + (n16 >> 2), 4 * (n16 & 3)
+    for stride in (SH, 16):
+        for kc in range(2):
+            for c0 in range(0, (SH - 16 if stride == SH else 16), 16):
+                parts = []
+                for h in range(2):
+                    addrs = (32 * kc + trr0 + 4 * h) * stride + c0 + tq
+                    parts.append(tr_read(S if stride == SH else np.arange(64 * 16), addrs))
+                    for g0 in range(4):
+                        assert len(set(_banks([2 * a for a in addrs[16 * g0:16 * g0 + 16]], 8))) == 32
+                frag = np.concatenate(parts, axis=1)
+                for lane in range(64):
+                    want = [(32 * kc + 8 * g[lane] + e) * stride + c0 + n16[lane] for e in range(8)]
+                    assert list(frag[lane]) == want
