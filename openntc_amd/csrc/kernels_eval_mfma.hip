@@ -33,8 +33,8 @@ constexpr int kHeaderBytes = 32;
 // as one 16-byte load when the channel count is a multiple of 4. C0 == 0:
 // gather_texel itself (any layout).
 template <int C0, int C1, int G0MODE, int PEC, int LODS, class Put>
-__device__ __forceinline__ void gather_fixed(const ntc_feature_desc& fd, const ntc_tiles& tl, const float* g0,
-                                             const float* g1, int64_t t, Put&& put)
+__device__ __forceinline__ void gather_fixed(const ntc_feature_desc& fd, const ntc_tiles& tl, const uint32_t* orig,
+                                             const float* g0, const float* g1, int64_t t, Put&& put)
 {
   if constexpr (C0 == 0)
   {
@@ -48,8 +48,8 @@ __device__ __forceinline__ void gather_fixed(const ntc_feature_desc& fd, const n
     const int b = (int)(t >> (2 * lt));
     const int xy = (int)(t & (((int64_t)1 << (2 * lt)) - 1));
     const uint32_t mm = (uint32_t)M - 1u;
-    const int px = (int)(((uint32_t)(xy & (T - 1)) + (uint32_t)tl.origins[2 * b]) & mm);
-    const int py = (int)(((uint32_t)(xy >> lt) + (uint32_t)tl.origins[2 * b + 1]) & mm);
+    const int px = (int)(((uint32_t)(xy & (T - 1)) + (uint32_t)orig[2 * b]) & mm);
+    const int py = (int)(((uint32_t)(xy >> lt) + (uint32_t)orig[2 * b + 1]) & mm);
     const Footprint<float> f0 = footprint<float>(px, py, M, fd.g0_dim);
     const Footprint<float> f1 = footprint<float>(px, py, M, fd.g1_dim);
     auto quads = [&](const float* grid, const Footprint<float>& f, int C, int q, float4 (&v)[4]) {
@@ -191,6 +191,7 @@ __global__ __launch_bounds__(kThreads) void k_decode_mfma(const uint8_t* __restr
   __shared__ __attribute__((aligned(16))) El xs_all[4 * 64 * XS];    // each wave's texel rows
   __shared__ __attribute__((aligned(16))) float ts_all[4 * 64 * 16];  // each wave's target values [texel][o]
   __shared__ float unorm[256];                                        // byte / 255, the division's bits
+  __shared__ uint32_t orig_s[128];                                    // tile origins (gather: at most 64 tiles)
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, n16 = lane & 15, g = lane >> 4;
   // header: FP8 chain scales s_w0 s_x, s_w1 s_a0, s_wo s_a1 and the reciprocals r_x, r_a0, r_a1
   const float* hdr = reinterpret_cast<const float*>(table);
@@ -201,6 +202,7 @@ __global__ __launch_bounds__(kThreads) void k_decode_mfma(const uint8_t* __restr
     uint4* dst = reinterpret_cast<uint4*>(wt);
     for (int i = tid; i < (int)(WE * sizeof(El) / 16); i += kThreads) dst[i] = src[i];
     unorm[tid] = (float)tid / 255.0f;
+    if (GATHER && tid < 2 * tl.n_tiles) orig_s[tid] = (uint32_t)tl.origins[tid];
   }
   __syncthreads();
   El* const xs = xs_all + wave * 64 * XS;
@@ -230,7 +232,6 @@ __global__ __launch_bounds__(kThreads) void k_decode_mfma(const uint8_t* __restr
       };
       if constexpr (GATHER)
       {
-        gather_fixed<C0, C1, G0MODE, PEC, LODS>(fd, tl, g0, g1, tc, put);
         int64_t tb;
         if constexpr (C0 == 0)
         {
@@ -243,11 +244,19 @@ __global__ __launch_bounds__(kThreads) void k_decode_mfma(const uint8_t* __restr
           const int T = tl.tile_dim, lt = 31 - __builtin_clz((unsigned)T);
           const int b = (int)(tc >> (2 * lt)), xy = (int)(tc & (((int64_t)1 << (2 * lt)) - 1));
           const uint32_t mm = (uint32_t)tl.mip_dim - 1u;
-          const int px = (int)(((uint32_t)(xy & (T - 1)) + (uint32_t)tl.origins[2 * b]) & mm);
-          const int py = (int)(((uint32_t)(xy >> lt) + (uint32_t)tl.origins[2 * b + 1]) & mm);
+          const int px = (int)(((uint32_t)(xy & (T - 1)) + orig_s[2 * b]) & mm);
+          const int py = (int)(((uint32_t)(xy >> lt) + orig_s[2 * b + 1]) & mm);
           tb = ((int64_t)py * tl.mip_dim + px) * O;
         }
-        for (int o = 0; o < O; ++o) ts[lane * 16 + o] = unorm[pkg[tb + o]];
+        // the target bytes are loaded before the gather's loads so that all
+        // of the texel's loads are in flight together
+        uint32_t tg[16];
+#pragma unroll
+        for (int o = 0; o < 16; ++o) tg[o] = o < O ? pkg[tb + o] : 0u;
+        gather_fixed<C0, C1, G0MODE, PEC, LODS>(fd, tl, orig_s, g0, g1, tc, put);
+#pragma unroll
+        for (int o = 0; o < 16; ++o)
+          if (o < O) ts[lane * 16 + o] = unorm[tg[o]];
       }
       else
       {
@@ -494,7 +503,7 @@ int ntc_mfma_decode(bool fp8, bool gather, int H, int F, int O, int act, int64_t
     k = fp8 ? (gather ? mfma_kernel<true, true>(H, FP, act) : mfma_kernel<true, false>(H, FP, act))
             : (gather ? mfma_kernel<false, true>(H, FP, act) : mfma_kernel<false, false>(H, FP, act));
   NTC_RETURN_IF(!k, NTC_E_INVALID);
-  NTC_RETURN_IF(gather && !pow2_tiles(*tl), NTC_E_INVALID);
+  NTC_RETURN_IF(gather && (!tl || !valid_tiles(tl) || !pow2_tiles(*tl)), NTC_E_INVALID);
   const int blocks = mfma_blocks(B);
   double* part = static_cast<double*>(workspace);
   const ntc_feature_desc fdv = fd ? *fd : ntc_feature_desc{};
