@@ -3,7 +3,10 @@
 // the reference trains one set per Context, libopenntc.cpp:536-879) and all
 // share one decoder shape. A training step is one launch per stage for all
 // sets: the fused step, the ordered decoder-gradient reduction with Adam, the
-// BF16 table refresh and the tile-aware dense grid update. Every per-step
+// BF16 table refresh and the grid update (per set as Context::Train does it:
+// deferred zero-gradient steps, a catch-up of the step's footprint cells
+// before the fused step and the gradient step in them after it, or the
+// tile-aware dense update where the tiles cover much of the set). Every per-step
 // parameter sits in device tables indexed by a device step counter, so the
 // step is captured once as a hipGraph and the same graph is replayed for
 // every step. Per set, the arithmetic is that of Context::Train's fused path.
@@ -66,6 +69,9 @@ struct ntc_batch
   ntc_feature_desc fd = {};  // level-0 description of set 0: the kernels' shape
   int pass = 0, fused_grid = 0, grid_grid = 0;
   DeviceBuffer fparams, wg_first, dsets, dsteps, psets, trecs, blk_first, step_ctr;
+  DeviceBuffer crecs, cfirst, hrecs, hfirst;  // deferred grid steps: catch-up and gradient step
+  int lazy_cgrid = 0, lazy_hgrid = 0, lazy_slots = 0;
+  bool any_eager = false;
   std::vector<int64_t> step_texels;  // all sets' texels of each step
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -80,7 +86,9 @@ struct ntc_batch
   {
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
-    for (DeviceBuffer* b : {&fparams, &wg_first, &dsets, &dsteps, &psets, &trecs, &blk_first, &step_ctr}) b->Free();
+    for (DeviceBuffer* b : {&fparams, &wg_first, &dsets, &dsteps, &psets, &trecs, &blk_first, &step_ctr, &crecs,
+                            &cfirst, &hrecs, &hfirst})
+      b->Free();
     if (stream) (void)hipStreamDestroy(stream);
   }
 
@@ -148,14 +156,22 @@ struct ntc_batch
   // The stage launches of one step (the captured sequence).
   int LaunchStep()
   {
-    int st = ntc_fused_train_sets(&fd, H, act, pass, fparams.ptr, wg_first.as<int32_t>(), n,
-                                  step_ctr.as<int32_t>(), fused_grid, stream);
+    int st = NTC_OK;
+    if (lazy_cgrid > 0)
+      st = ntc_grid_lazy_catchup_sets(crecs.ptr, cfirst.as<int32_t>(), n, step_ctr.as<int32_t>(), lazy_cgrid,
+                                      lazy_slots, stream);
+    if (st == NTC_OK)
+      st = ntc_fused_train_sets(&fd, H, act, pass, fparams.ptr, wg_first.as<int32_t>(), n, step_ctr.as<int32_t>(),
+                                fused_grid, stream);
     if (st == NTC_OK)
       st = ntc_decoder_update_sets(H, F, O, static_cast<const ntc_dec_set*>(dsets.ptr),
                                    static_cast<const ntc_dec_step*>(dsteps.ptr), n, step_ctr.as<int32_t>(), stream);
     if (st == NTC_OK)
       st = ntc_pack_decoder_bf16_sets(&fd, H, O, static_cast<const ntc_pack_set*>(psets.ptr), n, stream);
-    if (st == NTC_OK)
+    if (st == NTC_OK && lazy_hgrid > 0)
+      st = ntc_grid_lazy_hot_sets(hrecs.ptr, hfirst.as<int32_t>(), n, step_ctr.as<int32_t>(), lazy_hgrid, lazy_slots,
+                                  stream);
+    if (st == NTC_OK && any_eager)
       st = ntc_grid_update_tiles_sets(trecs.ptr, blk_first.as<int32_t>(), n, step_ctr.as<int32_t>(), grid_grid,
                                       stream);
     if (st == NTC_OK) st = ntc_step_advance(step_ctr.as<int32_t>(), stream);
@@ -323,6 +339,16 @@ struct ntc_batch
     const size_t tb = ntc_grid_update_tiles_bytes();
     std::vector<uint8_t> tr((size_t)steps * n * tb, 0);
     std::vector<int32_t> bf((size_t)steps * (n + 1), 0);
+    // deferred sets: per (step, set) a catch-up and a gradient-step record;
+    // done[s][l]: the level's steps before this one (each set's counters
+    // start at zero: no set has trained yet)
+    const size_t lb = ntc_grid_lazy_bytes();
+    std::vector<uint8_t> cr((size_t)steps * n * lb, 0), hr((size_t)steps * n * lb, 0);
+    std::vector<int32_t> cf((size_t)steps * (n + 1), 0), hf((size_t)steps * (n + 1), 0);
+    std::vector<std::vector<int32_t>> done(n, std::vector<int32_t>(kMaxLevels, 0));
+    lazy_cgrid = lazy_hgrid = 0;
+    lazy_slots = 0;
+    any_eager = false;
     step_texels.assign(steps, 0);
     const int gbudget = 8 * max_blocks;  // grid-update blocks per step, split by level size
     grid_grid = 1;
@@ -346,12 +372,42 @@ struct ntc_batch
         const ntc_grid_adam adam = {I.g0m[lvl].as<float>(), I.g0v[lvl].as<float>(), I.g1m[lvl].as<float>(),
                                     I.g1v[lvl].as<float>(), st.lr_grid, 0.9f, 0.999f, st.grid_bias1,
                                     st.grid_bias2, -1.0f, 1.0f - I.g0_delta, -1.0f, 1.0f - I.g1_delta};
+        if (I.lazy)
+        {
+          const int32_t upto = done[s][lvl]++;
+          int32_t nb = 0;
+          const size_t at = ((size_t)t * n + s) * lb;
+          if (ntc_grid_lazy_pack(&f, &tl, I.g0[lvl].as<float>(), I.dg0[lvl].as<float>(), I.g1[lvl].as<float>(),
+                                 I.dg1[lvl].as<float>(), &adam, I.lazy_steps[lvl].as<float>(), upto,
+                                 I.lazy_last[lvl][0].as<int32_t>(), I.lazy_last[lvl][1].as<int32_t>(), &cr[at],
+                                 &nb) != NTC_OK ||
+              ntc_grid_lazy_pack(&f, &tl, I.g0[lvl].as<float>(), I.dg0[lvl].as<float>(), I.g1[lvl].as<float>(),
+                                 I.dg1[lvl].as<float>(), &adam, I.lazy_steps[lvl].as<float>(), upto,
+                                 I.lazy_last[lvl][0].as<int32_t>(), I.lazy_last[lvl][1].as<int32_t>(), &hr[at],
+                                 nullptr) != NTC_OK)
+            return Result::InvalidState;
+          // a first visit of the level owes nothing: no catch-up blocks
+          cf[(size_t)t * (n + 1) + s + 1] = upto > 0 ? nb : 0;
+          hf[(size_t)t * (n + 1) + s + 1] = nb;
+          lazy_slots = std::max(lazy_slots, 2 * st.n_tiles);
+          continue;
+        }
+        any_eager = true;
         if (ntc_grid_update_tiles_pack(&f, &tl, I.g0[lvl].as<float>(), I.dg0[lvl].as<float>(), I.g1[lvl].as<float>(),
                                        I.dg1[lvl].as<float>(), &adam, &tr[((size_t)t * n + s) * tb],
                                        &items[s]) != NTC_OK)
           return Result::InvalidState;
         total += items[s];
       }
+      // block offsets of the deferred sets' records (counts written above)
+      for (std::vector<int32_t>* v : {&cf, &hf})
+      {
+        int32_t* o = &(*v)[(size_t)t * (n + 1)];
+        o[0] = 0;
+        for (int s = 0; s < n; ++s) o[s + 1] += o[s];
+      }
+      lazy_cgrid = std::max(lazy_cgrid, (int)cf[(size_t)t * (n + 1) + n]);
+      lazy_hgrid = std::max(lazy_hgrid, (int)hf[(size_t)t * (n + 1) + n]);
       int32_t* f = &bf[(size_t)t * (n + 1)];
       f[0] = 0;
       for (int s = 0; s < n; ++s)
@@ -372,6 +428,13 @@ struct ntc_batch
     if (r == Result::Success) r = Upload(dsteps, dst.data(), dst.size() * sizeof(ntc_dec_step));
     if (r == Result::Success) r = Upload(trecs, tr.data(), tr.size());
     if (r == Result::Success) r = Upload(blk_first, bf.data(), bf.size() * sizeof(int32_t));
+    if (r == Result::Success && lazy_hgrid > 0)
+    {
+      r = Upload(crecs, cr.data(), cr.size());
+      if (r == Result::Success) r = Upload(cfirst, cf.data(), cf.size() * sizeof(int32_t));
+      if (r == Result::Success) r = Upload(hrecs, hr.data(), hr.size());
+      if (r == Result::Success) r = Upload(hfirst, hf.data(), hf.size() * sizeof(int32_t));
+    }
     const int32_t zero = 0;
     if (r == Result::Success) r = Upload(step_ctr, &zero, sizeof(zero));
     return r;
@@ -414,7 +477,19 @@ struct ntc_batch
       else if (LaunchStep() != NTC_OK)
         return Result::CudaFailure;
       texels_done += step_texels[step_i];
-      for (int s = 0; s < n; ++s) impl(s).grid_batch_i[impl(s).plan[step_i].level] += 1;
+      for (int s = 0; s < n; ++s)
+      {
+        Context::Impl& I = impl(s);
+        const StepPlan& st = I.plan[step_i];
+        I.grid_batch_i[st.level] += 1;
+        // the deferred sets' step counters, as Context::Train keeps them (the
+        // lock and the evaluation flush from them)
+        if (I.lazy && st.grid_update)
+        {
+          I.lazy_done[st.level] += 1;
+          I.lazy_behind[st.level] = true;
+        }
+      }
     }
     if (hipStreamSynchronize(stream) != hipSuccess) return Result::CudaFailure;
     for (int s = 0; s < n; ++s)

@@ -547,6 +547,30 @@ __global__ __launch_bounds__(kThreads) void k_grid_update_tiles_sets(const TileU
   grid_update_tiles_body(recs[(int64_t)t * n_sets + s], b - first[s], first[s + 1] - first[s]);
 }
 
+// Deferred grid steps of several sets for the step held in *step: records
+// [steps][n_sets] of LazyArgs, set s on x-blocks [blk_first[s],
+// blk_first[s + 1]) of every rectangle slot blockIdx.y (blk_first as for
+// k_grid_update_tiles_sets). HOT: the step's gradient step in each set's
+// footprint cells; else the catch-up of those cells to the record's step.
+template <bool HOT>
+__global__ __launch_bounds__(kThreads) void k_grid_lazy_sets(const LazyArgs* __restrict__ recs,
+                                                             const int32_t* __restrict__ blk_first, int n_sets,
+                                                             const int32_t* __restrict__ step)
+{
+  const int t = read_step(step);
+  const int32_t* first = blk_first + (int64_t)t * (n_sets + 1);
+  const int b = blockIdx.x;
+  if (b >= first[n_sets]) return;
+  int s = 0;
+  while (s + 1 < n_sets && b >= first[s + 1]) ++s;
+  const LazyArgs& L = recs[(int64_t)t * n_sets + s];
+  if ((int)blockIdx.y >= 2 * L.A.tl.n_tiles) return;
+  if constexpr (HOT)
+    lazy_hot_body(L, b - first[s], first[s + 1] - first[s], blockIdx.y);
+  else
+    lazy_catchup_rects_body(L, b - first[s], first[s + 1] - first[s], blockIdx.y);
+}
+
 __global__ __launch_bounds__(kThreads) void k_grid_update_tail(int64_t beg, int64_t n, AdamConst c,
                                                                float lo, float hi, float* __restrict__ g,
                                                                float* __restrict__ m, float* __restrict__ v,
@@ -864,6 +888,48 @@ int ntc_grid_lazy_hot_catchup_f32(const ntc_feature_desc* fd, const ntc_tiles* t
   hipLaunchKernelGGL(k_grid_lazy_hot_catchup, dim3(nbx_h > nbx_c ? nbx_h : nbx_c, nbx_c > 0 ? 64 : 32),
                      dim3(kThreads), 0, stream, H, C, nbx_h, nbx_c);
   return NTC_LAUNCH_STATUS();
+}
+
+size_t ntc_grid_lazy_bytes(void) { return sizeof(LazyArgs); }
+
+int ntc_grid_lazy_pack(const ntc_feature_desc* fd, const ntc_tiles* tiles, float* g0, float* dg0, float* g1,
+                       float* dg1, const ntc_grid_adam* adam, const float* steps, int32_t upto, int32_t* last0,
+                       int32_t* last1, void* record, int32_t* rect_blocks_out)
+{
+  NTC_RETURN_IF(!record || !tiles, NTC_E_INVALID);
+  LazyArgs L;
+  const int st = lazy_args_of(fd, tiles, g0, dg0, g1, dg1, adam, steps, upto, last0, last1, L);
+  if (st != NTC_OK) return st;
+  std::memcpy(record, &L, sizeof(L));
+  if (rect_blocks_out) *rect_blocks_out = rect_blocks(tiles, L.A, 256);
+  return NTC_OK;
+}
+
+static int lazy_sets_launch(bool hot, const void* records, const int32_t* blk_first, int32_t n_sets,
+                            const int32_t* step, int32_t grid, int32_t slots, hipStream_t stream)
+{
+  NTC_RETURN_IF(!records || !blk_first || !step || n_sets < 1 || grid < 1 || slots < 1 || slots > 32,
+                NTC_E_INVALID);
+  const LazyArgs* r = static_cast<const LazyArgs*>(records);
+  if (hot)
+    hipLaunchKernelGGL(k_grid_lazy_sets<true>, dim3(grid, slots), dim3(kThreads), 0, stream, r, blk_first, n_sets,
+                       step);
+  else
+    hipLaunchKernelGGL(k_grid_lazy_sets<false>, dim3(grid, slots), dim3(kThreads), 0, stream, r, blk_first, n_sets,
+                       step);
+  return NTC_LAUNCH_STATUS();
+}
+
+int ntc_grid_lazy_catchup_sets(const void* records, const int32_t* blk_first, int32_t n_sets, const int32_t* step,
+                               int32_t grid, int32_t slots, hipStream_t stream)
+{
+  return lazy_sets_launch(false, records, blk_first, n_sets, step, grid, slots, stream);
+}
+
+int ntc_grid_lazy_hot_sets(const void* records, const int32_t* blk_first, int32_t n_sets, const int32_t* step,
+                           int32_t grid, int32_t slots, hipStream_t stream)
+{
+  return lazy_sets_launch(true, records, blk_first, n_sets, step, grid, slots, stream);
 }
 
 int ntc_fill_i32(int64_t n, int32_t value, int32_t* out, hipStream_t stream)
