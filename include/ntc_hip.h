@@ -457,17 +457,21 @@ int ntc_step_advance(int32_t* step, hipStream_t stream);
  * ntc_grid_lazy_hot_f32): records of ntc_grid_lazy_bytes() from
  * ntc_grid_lazy_pack (the step's tiles, its Adam constants; upto: for the
  * catch-up the number of the level's steps before this one, for the gradient
- * step its own index; rect_blocks: the blocks per footprint rectangle the
- * single-set launch would use), [steps][n_sets], and blk_first as for
- * ntc_grid_update_tiles_sets, split over each set's rectangle blocks; slots:
- * twice the most tiles of a record (rectangle slots, at most 32). The
- * catch-up runs before the step's ntc_fused_train_sets, the gradient step
- * after it. */
+ * step its own index; skip: tiles whose rectangles the catch-up leaves out,
+ * as for ntc_grid_lazy_catchup_f32, or NULL; rect_blocks: the blocks per
+ * footprint rectangle the single-set launch would use), [steps][n_sets], and
+ * blk_first as for ntc_grid_update_tiles_sets, split over each set's
+ * rectangle blocks; slots: twice the most tiles of a record (rectangle
+ * slots, at most 32). A step's cells are caught up before its
+ * ntc_fused_train_sets reads them (for example beside the previous step's
+ * gradient step, leaving out its footprint), the gradient step runs after
+ * the fused launch. */
 size_t ntc_grid_lazy_bytes(void);
-int ntc_grid_lazy_pack(const ntc_feature_desc* fd, const ntc_tiles* tiles, float* g0,
-                       float* dg0, float* g1, float* dg1, const ntc_grid_adam* adam,
-                       const float* steps, int32_t upto, int32_t* last0, int32_t* last1,
-                       void* record, int32_t* rect_blocks);
+int ntc_grid_lazy_pack(const ntc_feature_desc* fd, const ntc_tiles* tiles,
+                       const ntc_tiles* skip, float* g0, float* dg0, float* g1,
+                       float* dg1, const ntc_grid_adam* adam, const float* steps,
+                       int32_t upto, int32_t* last0, int32_t* last1, void* record,
+                       int32_t* rect_blocks);
 int ntc_grid_lazy_catchup_sets(const void* records, const int32_t* blk_first,
                                int32_t n_sets, const int32_t* step, int32_t grid,
                                int32_t slots, hipStream_t stream);

@@ -234,7 +234,7 @@ SIGNATURES = {
     "ntc_grid_update_tiles_pack": (_I, [_P, _P, _P, _P, _P, _P, _P, _P, _P]),
     "ntc_grid_update_tiles_sets": (_I, [_P, _P, _I, _P, _I, _P]),
     "ntc_grid_lazy_bytes": (C.c_size_t, []),
-    "ntc_grid_lazy_pack": (_I, [_P, _P, _P, _P, _P, _P, _P, _P, _I, _P, _P, _P, _P]),
+    "ntc_grid_lazy_pack": (_I, [_P, _P, _P, _P, _P, _P, _P, _P, _P, _I, _P, _P, _P, _P]),
     "ntc_grid_lazy_catchup_sets": (_I, [_P, _P, _I, _P, _I, _I, _P]),
     "ntc_grid_lazy_hot_sets": (_I, [_P, _P, _I, _P, _I, _I, _P]),
     "ntc_step_advance": (_I, [_P, _P]),

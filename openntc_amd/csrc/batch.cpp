@@ -5,8 +5,9 @@
 // sets: the fused step, the ordered decoder-gradient reduction with Adam, the
 // BF16 table refresh and the grid update (per set as Context::Train does it:
 // deferred zero-gradient steps, a catch-up of the step's footprint cells
-// before the fused step and the gradient step in them after it, or the
-// tile-aware dense update where the tiles cover much of the set). Every per-step
+// beside the step before it and the gradient step in them after the fused
+// step, or the tile-aware dense update where the tiles cover much of the
+// set). Every per-step
 // parameter sits in device tables indexed by a device step counter, so the
 // step is captured once as a hipGraph and the same graph is replayed for
 // every step. Per set, the arithmetic is that of Context::Train's fused path.
@@ -65,6 +66,9 @@ struct ntc_batch
   std::vector<ntc_context*> sets;
   int n = 0, steps = 0, step_i = 0;
   hipStream_t stream = nullptr;
+  hipStream_t side = nullptr;  // the deferred sets' catch-up, beside each step
+  hipEvent_t fork_ev = nullptr, join_ev = nullptr;
+  static constexpr int kSideCtr = 16;  // the side stream's step counter: step_ctr[16], its own 64 bytes
   int H = 0, F = 0, O = 0, act = 0, max_blocks = 0;
   ntc_feature_desc fd = {};  // level-0 description of set 0: the kernels' shape
   int pass = 0, fused_grid = 0, grid_grid = 0;
@@ -89,6 +93,9 @@ struct ntc_batch
     for (DeviceBuffer* b : {&fparams, &wg_first, &dsets, &dsteps, &psets, &trecs, &blk_first, &step_ctr, &crecs,
                             &cfirst, &hrecs, &hfirst})
       b->Free();
+    if (fork_ev) (void)hipEventDestroy(fork_ev);
+    if (join_ev) (void)hipEventDestroy(join_ev);
+    if (side) (void)hipStreamDestroy(side);
     if (stream) (void)hipStreamDestroy(stream);
   }
 
@@ -153,16 +160,12 @@ struct ntc_batch
     return Result::Success;
   }
 
-  // The stage launches of one step (the captured sequence).
+  // The stage launches of one step on the main stream (the captured
+  // sequence); they read the step counter step_ctr[0] and advance it.
   int LaunchStep()
   {
-    int st = NTC_OK;
-    if (lazy_cgrid > 0)
-      st = ntc_grid_lazy_catchup_sets(crecs.ptr, cfirst.as<int32_t>(), n, step_ctr.as<int32_t>(), lazy_cgrid,
-                                      lazy_slots, stream);
-    if (st == NTC_OK)
-      st = ntc_fused_train_sets(&fd, H, act, pass, fparams.ptr, wg_first.as<int32_t>(), n, step_ctr.as<int32_t>(),
-                                fused_grid, stream);
+    int st = ntc_fused_train_sets(&fd, H, act, pass, fparams.ptr, wg_first.as<int32_t>(), n, step_ctr.as<int32_t>(),
+                                  fused_grid, stream);
     if (st == NTC_OK)
       st = ntc_decoder_update_sets(H, F, O, static_cast<const ntc_dec_set*>(dsets.ptr),
                                    static_cast<const ntc_dec_step*>(dsteps.ptr), n, step_ctr.as<int32_t>(), stream);
@@ -175,6 +178,23 @@ struct ntc_batch
       st = ntc_grid_update_tiles_sets(trecs.ptr, blk_first.as<int32_t>(), n, step_ctr.as<int32_t>(), grid_grid,
                                       stream);
     if (st == NTC_OK) st = ntc_step_advance(step_ctr.as<int32_t>(), stream);
+    return st;
+  }
+
+  // The deferred sets' catch-up for the step after step_i, on the side
+  // stream beside step_i's launches: it waits for what the main stream held
+  // before them (the previous step's gradient step), reads its own counter
+  // step_ctr[kSideCtr] (it runs while the main stream advances step_ctr[0])
+  // and the next step's launches wait for it. Its records leave out
+  // everything step_i reads or writes.
+  int LaunchCatchup()
+  {
+    int32_t* ctr = step_ctr.as<int32_t>() + kSideCtr;
+    hipError_t e = hipStreamWaitEvent(side, fork_ev, 0);
+    if (e != hipSuccess) return (int)e;
+    int st = ntc_grid_lazy_catchup_sets(crecs.ptr, cfirst.as<int32_t>(), n, ctr, lazy_cgrid, lazy_slots, side);
+    if (st == NTC_OK) st = ntc_step_advance(ctr, side);
+    if (st == NTC_OK && (e = hipEventRecord(join_ev, side)) != hipSuccess) st = (int)e;
     return st;
   }
 
@@ -280,7 +300,11 @@ struct ntc_batch
         return Result::InvalidState;
     }
     if (!ntc_fused_supported(&fd, H, act)) return Result::InvalidState;
-    if (hipStreamCreateWithFlags(&stream, hipStreamNonBlocking) != hipSuccess) return Result::CudaFailure;
+    if (hipStreamCreateWithFlags(&stream, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&side, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&fork_ev, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&join_ev, hipEventDisableTiming) != hipSuccess)
+      return Result::CudaFailure;
 
     // launch shape: given, cached for these sets, or tuned now
     std::string key = std::to_string(H) + "/" + std::to_string(F) + "/" + std::to_string(O) + "/" +
@@ -377,19 +401,43 @@ struct ntc_batch
           const int32_t upto = done[s][lvl]++;
           int32_t nb = 0;
           const size_t at = ((size_t)t * n + s) * lb;
-          if (ntc_grid_lazy_pack(&f, &tl, I.g0[lvl].as<float>(), I.dg0[lvl].as<float>(), I.g1[lvl].as<float>(),
-                                 I.dg1[lvl].as<float>(), &adam, I.lazy_steps[lvl].as<float>(), upto,
-                                 I.lazy_last[lvl][0].as<int32_t>(), I.lazy_last[lvl][1].as<int32_t>(), &cr[at],
-                                 &nb) != NTC_OK ||
-              ntc_grid_lazy_pack(&f, &tl, I.g0[lvl].as<float>(), I.dg0[lvl].as<float>(), I.g1[lvl].as<float>(),
-                                 I.dg1[lvl].as<float>(), &adam, I.lazy_steps[lvl].as<float>(), upto,
-                                 I.lazy_last[lvl][0].as<int32_t>(), I.lazy_last[lvl][1].as<int32_t>(), &hr[at],
-                                 nullptr) != NTC_OK)
+          if (ntc_grid_lazy_pack(&f, &tl, nullptr, I.g0[lvl].as<float>(), I.dg0[lvl].as<float>(),
+                                 I.g1[lvl].as<float>(), I.dg1[lvl].as<float>(), &adam, I.lazy_steps[lvl].as<float>(),
+                                 upto, I.lazy_last[lvl][0].as<int32_t>(), I.lazy_last[lvl][1].as<int32_t>(), &hr[at],
+                                 &nb) != NTC_OK)
             return Result::InvalidState;
-          // a first visit of the level owes nothing: no catch-up blocks
-          cf[(size_t)t * (n + 1) + s + 1] = upto > 0 ? nb : 0;
           hf[(size_t)t * (n + 1) + s + 1] = nb;
           lazy_slots = std::max(lazy_slots, 2 * st.n_tiles);
+          // the next step's catch-up, at this step's index: it runs beside
+          // this step's launches. It leaves out this step's footprint cells
+          // when it is on the same level (this step's fused launch reads
+          // them and its gradient step brings them current) and touches
+          // nothing else this step reads or writes. Step 0 is every level's
+          // first visit and owes nothing; the lock step flushes every level
+          // itself.
+          if (t + 1 < steps && I.plan[t + 1].grid_update && !I.plan[t + 1].lock)
+          {
+            const StepPlan& sn = I.plan[t + 1];
+            const int ln = sn.level;
+            const int32_t un = done[s][ln];  // the level's steps before t + 1
+            if (un > 0)
+            {
+              const ntc_feature_desc fn = I.FeatureDesc(ln);
+              const ntc_tiles tn = {I.origins_table.as<int32_t>() + (int64_t)(t + 1) * 16, sn.n_tiles, sn.tile_dim,
+                                    I.mip_dim[sn.lod], (float)sn.lod / (float)(I.mip_count - 1)};
+              const ntc_grid_adam an = {I.g0m[ln].as<float>(), I.g0v[ln].as<float>(), I.g1m[ln].as<float>(),
+                                        I.g1v[ln].as<float>(), sn.lr_grid, 0.9f, 0.999f, sn.grid_bias1,
+                                        sn.grid_bias2, -1.0f, 1.0f - I.g0_delta, -1.0f, 1.0f - I.g1_delta};
+              int32_t nc = 0;
+              if (ntc_grid_lazy_pack(&fn, &tn, ln == lvl ? &tl : nullptr, I.g0[ln].as<float>(),
+                                     I.dg0[ln].as<float>(), I.g1[ln].as<float>(), I.dg1[ln].as<float>(), &an,
+                                     I.lazy_steps[ln].as<float>(), un, I.lazy_last[ln][0].as<int32_t>(),
+                                     I.lazy_last[ln][1].as<int32_t>(), &cr[at], &nc) != NTC_OK)
+                return Result::InvalidState;
+              cf[(size_t)t * (n + 1) + s + 1] = nc;
+              lazy_slots = std::max(lazy_slots, 2 * sn.n_tiles);
+            }
+          }
           continue;
         }
         any_eager = true;
@@ -435,8 +483,8 @@ struct ntc_batch
       if (r == Result::Success) r = Upload(hrecs, hr.data(), hr.size());
       if (r == Result::Success) r = Upload(hfirst, hf.data(), hf.size() * sizeof(int32_t));
     }
-    const int32_t zero = 0;
-    if (r == Result::Success) r = Upload(step_ctr, &zero, sizeof(zero));
+    const int32_t zero[kSideCtr + 1] = {};
+    if (r == Result::Success) r = Upload(step_ctr, zero, sizeof(zero));
     return r;
   }
 
@@ -452,7 +500,8 @@ struct ntc_batch
       {
         // the lock step quantises every level of every set (all sets share
         // the schedule length, so they lock together)
-        if (hipStreamSynchronize(stream) != hipSuccess) return Result::CudaFailure;
+        if (hipStreamSynchronize(side) != hipSuccess || hipStreamSynchronize(stream) != hipSuccess)
+          return Result::CudaFailure;
         for (int s = 0; s < n; ++s)
         {
           Context::Impl& I = impl(s);
@@ -461,6 +510,11 @@ struct ntc_batch
           if (hipStreamSynchronize(I.stream) != hipSuccess) return Result::CudaFailure;
         }
       }
+      // the step's catch-up (issued beside the last step) before its
+      // launches; this step's position for the next step's catch-up
+      if (lazy_cgrid > 0 && (hipStreamWaitEvent(stream, join_ev, 0) != hipSuccess ||
+                             hipEventRecord(fork_ev, stream) != hipSuccess))
+        return Result::CudaFailure;
       if (use_graph)
       {
         if (!exec)
@@ -476,6 +530,7 @@ struct ntc_batch
       }
       else if (LaunchStep() != NTC_OK)
         return Result::CudaFailure;
+      if (lazy_cgrid > 0 && LaunchCatchup() != NTC_OK) return Result::CudaFailure;
       texels_done += step_texels[step_i];
       for (int s = 0; s < n; ++s)
       {
@@ -491,7 +546,8 @@ struct ntc_batch
         }
       }
     }
-    if (hipStreamSynchronize(stream) != hipSuccess) return Result::CudaFailure;
+    if (hipStreamSynchronize(side) != hipSuccess || hipStreamSynchronize(stream) != hipSuccess)
+      return Result::CudaFailure;
     for (int s = 0; s < n; ++s)
     {
       Context::Impl& I = impl(s);

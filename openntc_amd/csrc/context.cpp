@@ -856,16 +856,20 @@ Result Context::Impl::RunFusedStep(const StepPlan& st, int32_t step_index, hipEv
     if (evd) CTX_HIP(hipEventRecord(evd[4], stream), Result::CudaFailure);
   }
   caught_step = -1;
+  const int nx = step_index + 1;
+  const bool ahead = deferred && nx < train_end && plan[nx].grid_update && !plan[nx].lock;
+  if (ahead) CTX_HIP(hipEventRecord(ready_ev, stream), Result::CudaFailure);  // after the last step's launches
   if (ev) CTX_HIP(hipEventRecord(ev[0], stream), Result::CudaFailure);
   CTX_NTC(ntc_fused_train_step(&a, stream));
   if (ev) CTX_HIP(hipEventRecord(ev[1], stream), Result::CudaFailure);
-  // the next step's catch-up, beside the rest of this step (issued after the
-  // decoder update and table pack instead, it measured 0.709 against 0.674 ms
-  // per step): it leaves out this step's footprint cells when it is on the
-  // same level (the hot part below brings those current) and reads nothing
-  // else this step writes
-  const int nx = step_index + 1;
-  if (deferred && nx < train_end && plan[nx].grid_update && !plan[nx].lock)
+  // the next step's catch-up, on the second stream beside this whole step
+  // (its waves fill the issue slots the fused launch leaves idle): it waits
+  // for the last step's gradient step, leaves out this step's footprint cells
+  // when it is on the same level (the fused launch reads them and the hot
+  // part below brings them current) and touches nothing else this step reads
+  // or writes. Issued after the fused launch, so the fused workgroups, which
+  // need most of a CU's LDS, are placed first.
+  if (ahead)
   {
     const StepPlan& sn = plan[nx];
     const int ln = sn.level;
@@ -877,7 +881,6 @@ Result Context::Impl::RunFusedStep(const StepPlan& st, int32_t step_index, hipEv
                                -1.0f, 1.0f - g0_delta, -1.0f, 1.0f - g1_delta};
     // its step count: this step's hot part counts one more on the same level
     const int32_t upto = lazy_done[ln] + (ln == lvl ? 1 : 0);
-    CTX_HIP(hipEventRecord(ready_ev, stream), Result::CudaFailure);
     CTX_HIP(hipStreamWaitEvent(side, ready_ev, 0), Result::CudaFailure);
     hipEvent_t* evn = evd ? evd + kStepEvents : nullptr;
     if (evn) CTX_HIP(hipEventRecord(evn[3], side), Result::CudaFailure);

@@ -264,11 +264,6 @@ __device__ __forceinline__ int wave_min(int v)
   return __builtin_amdgcn_readfirstlane(v);
 }
 
-// Catch-up of one float4 item per lane to step L.upto (from: the item's
-// cell counter; live: the lane holds an item). Called by every lane of the
-// wave: the step loop runs from the wave's smallest counter with the step
-// constants as wave-uniform loads (the next step's issued before the current
-// one is applied), and a lane applies the steps from its own counter on.
 // Correctly rounded square root of y for y == 0 or 2^-96 <= y < +inf: the
 // hardware estimate (within 1 ulp) and the choice between it and its two
 // neighbours by the sign of the FMA residuals, the steps the compiler's exact
@@ -283,18 +278,11 @@ __device__ __forceinline__ float sqrt_rn_wide(float y)
   return r;
 }
 
-__device__ __forceinline__ void lazy_run(const LazyArgs& L, const GridSeg& s, int64_t k, int from, bool live)
+// Zero-gradient steps lo .. upto - 1 of the lane's item (lo < upto, the
+// wave's smallest counter), applied from the lane's own counter on where run.
+__device__ __forceinline__ void lazy_steps(const LazyArgs& L, const GridSeg& s, int from, bool run, int lo, f4& mi,
+                                           f4& vi, f4& pi)
 {
-  const bool mine = live && from < L.upto;
-  const int lo = wave_min(mine ? from : L.upto);
-  if (lo >= L.upto) return;
-  f4 mi = {0.f, 0.f, 0.f, 0.f}, vi = mi, pi = mi;
-  if (mine)
-  {
-    mi = __builtin_nontemporal_load(&s.m[k]);
-    vi = __builtin_nontemporal_load(&s.v[k]);
-    pi = __builtin_nontemporal_load(&s.p[k]);
-  }
   AdamConst c;
   c.b1 = L.b1;
   c.b2 = L.b2;
@@ -312,7 +300,7 @@ __device__ __forceinline__ void lazy_run(const LazyArgs& L, const GridSeg& s, in
     c.lr = lr;
     c.bias1 = bias1;
     c.bias2 = bias2;
-    if (mine && o >= from)
+    if (run && o >= from)
     {
       // adam_one with g = 0, its gradient terms (1-b1) 0 and ((1-b2) 0) 0
       // formed once (wave-uniform) instead of per element: the same values
@@ -353,6 +341,46 @@ __device__ __forceinline__ void lazy_run(const LazyArgs& L, const GridSeg& s, in
     bias1 = nb1;
     bias2 = nb2;
   }
+}
+
+// Catch-up of one float4 item per lane to step L.upto (from: the item's
+// cell counter; live: the lane holds an item). Called by every lane of the
+// wave: the step loop runs from the wave's smallest counter with the step
+// constants as wave-uniform loads (the next step's issued before the current
+// one is applied), and a lane applies the steps from its own counter on.
+// A lane whose m and v are all zero (a parameter no nonzero gradient has
+// reached yet, so every cell before its first visit) owes only a clamp: each
+// zero-gradient step keeps m = v = +0 and subtracts (lr (0 bias1)) / (0 +
+// eps) = +0 from p, and the clamp is idempotent; it leaves the step loop to
+// the other lanes.
+__device__ __forceinline__ void lazy_run(const LazyArgs& L, const GridSeg& s, int64_t k, int from, bool live)
+{
+  const bool mine = live && from < L.upto;
+  if (wave_min(mine ? from : L.upto) >= L.upto) return;
+  f4 mi = {0.f, 0.f, 0.f, 0.f}, vi = mi, pi = mi;
+  bool flat = false;
+  if (mine)
+  {
+    mi = __builtin_nontemporal_load(&s.m[k]);
+    vi = __builtin_nontemporal_load(&s.v[k]);
+    pi = __builtin_nontemporal_load(&s.p[k]);
+    flat = mi[0] == 0.0f && mi[1] == 0.0f && mi[2] == 0.0f && mi[3] == 0.0f && vi[0] == 0.0f && vi[1] == 0.0f &&
+           vi[2] == 0.0f && vi[3] == 0.0f;
+    if (flat)
+    {
+#pragma unroll
+      for (int e = 0; e < 4; ++e)
+      {
+        mi[e] = 0.0f;
+        vi[e] = 0.0f;
+        pi[e] = __builtin_amdgcn_fmed3f(pi[e], s.lo, s.hi);
+      }
+    }
+  }
+  const bool run = mine && !flat;
+  const int lo = wave_min(run ? from : L.upto);
+  if (lo < L.upto)
+    lazy_steps(L, s, from, run, lo, mi, vi, pi);
   if (mine)
   {
     __builtin_nontemporal_store(mi, &s.m[k]);
@@ -383,29 +411,35 @@ __device__ __forceinline__ void lazy_catchup_rects_body(const LazyArgs& L, int b
   const int4 r = rect[grid][b];
   const int D = A.D[grid], Q = A.Q[grid];
   // below 2^31 items (the grids' quads: checked on the host), 32-bit index
-  // arithmetic
-  const uint32_t items = (uint32_t)r.y * (uint32_t)r.w * (uint32_t)Q;
+  // arithmetic. A wave takes cpw whole cells, Q items each (Q = 3 leaves a
+  // lane idle), so each cell's counter is read and then written by one wave
+  // and no other wave takes the cell.
+  const uint32_t cells = (uint32_t)r.y * (uint32_t)r.w;
+  const int cpw = 64 / Q;
+  const int lane = (int)threadIdx.x & 63, lc = lane / Q, q = lane - lc * Q;
+  const uint32_t per_block = (uint32_t)(kThreads / 64 * cpw);
+  const uint32_t wave_first = (threadIdx.x >> 6) * (uint32_t)cpw;
   // uniform trip count: every lane of a wave takes part in lazy_run
-  for (uint32_t i0 = (uint32_t)bx * kThreads; i0 < items; i0 += (uint32_t)nbx * kThreads)
+  for (uint32_t c0 = (uint32_t)bx * per_block; c0 < cells; c0 += (uint32_t)nbx * per_block)
   {
-    const uint32_t i = i0 + threadIdx.x;
-    bool live = i < items;
-    int64_t k = 0;
+    const uint32_t c = c0 + wave_first + (uint32_t)lc;
+    bool live = lc < cpw && c < cells;
+    int64_t k = 0, cell = 0;
     int from = L.upto;
     if (live)
     {
-      const uint32_t c = i / (uint32_t)Q;
-      const int q = (int)(i - c * (uint32_t)Q);
       const int iy = (int)(c / (uint32_t)r.y), ix = (int)c - iy * r.y;
       int x = r.x + ix, y = r.z + iy;
       x -= x >= D ? D : 0;
       y -= y >= D ? D : 0;
       live = !in_rects(rect[grid], b, x, y, D) && !in_rects(srect[grid], L.n_skip, x, y, D);
-      const int64_t cell = (int64_t)y * D + x;
+      cell = (int64_t)y * D + x;
       k = cell * Q + q;
       if (live) from = L.last[grid][cell];
     }
     lazy_run(L, A.s[grid], k, from, live);
+    // every lane of the cell has read the counter (above, in this wave)
+    if (live && q == 0 && from < L.upto) L.last[grid][cell] = L.upto;
   }
 }
 
@@ -892,14 +926,20 @@ int ntc_grid_lazy_hot_catchup_f32(const ntc_feature_desc* fd, const ntc_tiles* t
 
 size_t ntc_grid_lazy_bytes(void) { return sizeof(LazyArgs); }
 
-int ntc_grid_lazy_pack(const ntc_feature_desc* fd, const ntc_tiles* tiles, float* g0, float* dg0, float* g1,
-                       float* dg1, const ntc_grid_adam* adam, const float* steps, int32_t upto, int32_t* last0,
-                       int32_t* last1, void* record, int32_t* rect_blocks_out)
+int ntc_grid_lazy_pack(const ntc_feature_desc* fd, const ntc_tiles* tiles, const ntc_tiles* skip, float* g0,
+                       float* dg0, float* g1, float* dg1, const ntc_grid_adam* adam, const float* steps,
+                       int32_t upto, int32_t* last0, int32_t* last1, void* record, int32_t* rect_blocks_out)
 {
   NTC_RETURN_IF(!record || !tiles, NTC_E_INVALID);
   LazyArgs L;
   const int st = lazy_args_of(fd, tiles, g0, dg0, g1, dg1, adam, steps, upto, last0, last1, L);
   if (st != NTC_OK) return st;
+  if (skip)
+  {
+    NTC_RETURN_IF(!valid_tiles(skip) || skip->n_tiles > 16, NTC_E_INVALID);
+    L.skip = *skip;
+    L.n_skip = skip->n_tiles;
+  }
   std::memcpy(record, &L, sizeof(L));
   if (rect_blocks_out) *rect_blocks_out = rect_blocks(tiles, L.A, 256);
   return NTC_OK;

@@ -457,7 +457,7 @@ def test_grid_update_parts_match_tiles_update(case):
         assert np.array_equal(runs["cold_only"][k][3], init[k][3])
 
 
-@pytest.mark.parametrize("pipelined", [False, True, "merged"])
+@pytest.mark.parametrize("pipelined", [False, True, "merged", "across"])
 @pytest.mark.parametrize("case", list(_PART_CASES))
 def test_grid_lazy_matches_eager(case, pipelined):
     """Twelve steps with fresh tile origins, gradients and Adam constants per
@@ -465,11 +465,14 @@ def test_grid_lazy_matches_eager(case, pipelined):
     the deferred form (catch-up of the step's footprint cells, the hot step
     there, the other cells' zero-gradient steps applied only at a flush).
     Pipelined: the catch-up of step k + 1 leaves out step k's rectangles and
-    runs before step k's hot part (as it may beside it); never across a
-    flush. Merged: step k's hot part and step k + 1's catch-up in one launch
-    (ntc_grid_lazy_hot_catchup_f32). After a flush at step 7 and at the end,
-    p, m, v and the gradient agree bit for bit, and the cell counters hold
-    the step count."""
+    runs before step k's hot part (as it may beside it). Merged: step k's hot
+    part and step k + 1's catch-up in one launch
+    (ntc_grid_lazy_hot_catchup_f32). Across: pipelined, also when the flush
+    at step 7 comes between the catch-up of step 7 and step 7 itself (the
+    catch-up records the caught-up cells' counters, so the flush leaves them
+    alone). A third of each grid starts with zero Adam state. After the flush
+    at step 7 and at the end, p, m, v and the gradient agree bit for bit, and
+    the cell counters hold the step count."""
     D0, D1, C0, C1, M, T, org = _PART_CASES[case]
     n_tiles, K, flush_at = len(org) // 2, 12, 7
     lib = G.lib()
@@ -478,6 +481,16 @@ def test_grid_lazy_matches_eager(case, pipelined):
     shapes = [(D0, D0, C0), (D1, D1, C1)]
     init = [[rng.uniform(-1, 1, shp).astype(np.float32), (rng.standard_normal(shp) * 1e-3).astype(np.float32),
              rng.uniform(0, 1e-6, shp).astype(np.float32), np.zeros(shp, np.float32)] for shp in shapes]
+    for p0, m0, v0, _ in init:
+        # Adam state still zero (cells no gradient has reached: the catch-up
+        # owes them only the clamp, p above the upper bound included), with
+        # -0 in places, and rows where only m or only v is zero
+        D = p0.shape[0]
+        m0[: D // 3] = 0.0
+        v0[: D // 3] = 0.0
+        m0[: D // 6, ::3] = -0.0
+        m0[D // 3: D // 2] = 0.0
+        v0[D // 2: 2 * D // 3] = 0.0
     consts = np.stack([rng.uniform(0.002, 0.01, K), 1.0 / (1 - 0.9 ** np.arange(1, K + 1)),
                        1.0 / (1 - 0.999 ** np.arange(1, K + 1))], 1).astype(np.float32)
     bounds = (-1.0, 1.0 - 2.0 / 16, -1.0, 1.0 - 2.0 / 16)
@@ -525,7 +538,7 @@ def test_grid_lazy_matches_eager(case, pipelined):
                                            G.p(eager[1][0]), G.p(eager[1][3]), G.byref(ad), None), "eager")
         if issued != k:
             catch_up(k, None)
-        ahead = pipelined and k + 1 < K and k + 1 != flush_at
+        ahead = pipelined and k + 1 < K and (k + 1 != flush_at or pipelined == "across")
         if ahead and pipelined != "merged":
             catch_up(k + 1, tls[k])
             issued = k + 1
