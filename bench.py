@@ -202,9 +202,11 @@ def cpu_baseline(ctx, first_step, n_steps, warm=5):
 
 
 def lod0_micro(ntc, ctx, reps=50, seed=7):
-    """The fused training kernel alone at a fixed lod 0 (8 random 256^2 tiles
-    of the 4096^2 mip, the context's current latents and decoder, noise on):
-    wall time per launch over `reps` back-to-back launches."""
+    """The training step's kernels alone at a fixed lod 0 (8 random 256^2
+    tiles of the 4096^2 mip, the context's current latents and decoder, noise
+    on): wall time per step over `reps` back-to-back steps, of the one-launch
+    kernel and of the three-launch form, whose MLP launch is also timed alone
+    with an event pair right around it in every step."""
     from openntc_amd import _lib as L
     lib = ntc.load()
     lay = ctx.layout()
@@ -226,6 +228,10 @@ def lod0_micro(ntc, ctx, reps=50, seed=7):
             a = np.ascontiguousarray(a)
             ntc.check(lib.ntc_dev_upload(p, a.ctypes.data, a.nbytes), "upload")
         return p.value
+    hip = ctypes.CDLL("libamdhip64.so")
+    events = []
+    flop = 6.0 * (F * H + H * H + H * O) * TILES * 256 * 256
+    out = {"lod": 0, "texels": TILES * 256 * 256, "sample": f"{reps} back-to-back steps, wall clock"}
     try:
         d_dec = dev(dec)
         tab = dev(nbytes=2 * lib.ntc_fused_table_elems(ctypes.byref(fd), H))
@@ -233,7 +239,6 @@ def lod0_micro(ntc, ctx, reps=50, seed=7):
         maxb = ctypes.c_int32()
         lib.ntc_fused_max_blocks(ctypes.byref(maxb))
         B = TILES * 256 * 256
-        blocks = lib.ntc_fused_blocks_for(ctypes.byref(fd), H, lay.activation, B, maxb.value)
         a = L.FusedArgs()
         a.feat = fd
         a.tiles = L.Tiles(dev(origins), TILES, 256, DIM, 0.0)
@@ -243,23 +248,59 @@ def lod0_micro(ntc, ctx, reps=50, seed=7):
         a.loss_scale = -2.0 / (O * B)
         a.scatter = 1
         a.dg0, a.dg1 = dev(nbytes=g0.nbytes), dev(nbytes=g1.nbytes)
-        a.dw_partials, a.sse_partials = dev(nbytes=4 * blocks * dec.size), dev(nbytes=8 * blocks)
-        a.blocks = blocks
-        for _ in range(3):
-            ntc.check(lib.ntc_fused_train_step(ctypes.byref(a), None), "fused step")
-        lib.ntc_dev_synchronize()
-        t0 = time.perf_counter()
-        for _ in range(reps):
-            lib.ntc_fused_train_step(ctypes.byref(a), None)
-        ntc.check(lib.ntc_dev_synchronize(), "synchronize")
-        ms = (time.perf_counter() - t0) * 1e3 / reps
+        a.dw_partials, a.sse_partials = dev(nbytes=4 * maxb.value * dec.size), dev(nbytes=8 * maxb.value)
+
+        def timed(call):
+            for _ in range(3):
+                ntc.check(call(None), "warm-up step")
+            lib.ntc_dev_synchronize()
+            t0 = time.perf_counter()
+            for r in range(reps):
+                call(r)
+            ntc.check(lib.ntc_dev_synchronize(), "synchronize")
+            return (time.perf_counter() - t0) * 1e3 / reps
+
+        a.blocks = lib.ntc_fused_blocks_for(ctypes.byref(fd), H, lay.activation, B, maxb.value)
+        ms = timed(lambda r: lib.ntc_fused_train_step(ctypes.byref(a), None))
+        out["one_launch"] = {"kernel": "k_fused_train", "ms": ms, "texels_per_s": B / (ms * 1e-3),
+                             "tflops": flop / (ms * 1e-3) / 1e12,
+                             "mfma_frac": flop / (ms * 1e-3) / 1e12 / BF16_PEAK_TFLOPS}
+        if lib.ntc_split_supported(ctypes.byref(fd), H, lay.activation):
+            a.blocks = lib.ntc_split_blocks_for(ctypes.byref(fd), H, lay.activation, B, maxb.value)
+            nbytes = lib.ntc_split_workspace_bytes(ctypes.byref(fd), H, lay.activation, B)
+            ws = dev(nbytes=nbytes)
+            for _ in range(2 * reps):
+                e = ctypes.c_void_p()
+                if hip.hipEventCreate(ctypes.byref(e)) != 0:
+                    raise SystemExit("hipEventCreate failed")
+                events.append(e)
+
+            def step(r):
+                st = lib.ntc_split_gather(ctypes.byref(a), ws, nbytes, None)
+                if r is not None:
+                    hip.hipEventRecord(events[2 * r], None)
+                st = st or lib.ntc_split_mlp(ctypes.byref(a), ws, nbytes, None)
+                if r is not None:
+                    hip.hipEventRecord(events[2 * r + 1], None)
+                return st or lib.ntc_split_scatter(ctypes.byref(a), ws, nbytes, None)
+            ms = timed(step)
+            mlp = []
+            for r in range(reps):
+                t = ctypes.c_float()
+                if hip.hipEventElapsedTime(ctypes.byref(t), events[2 * r], events[2 * r + 1]) == 0:
+                    mlp.append(t.value)
+            mlp_ms = float(np.mean(mlp))
+            out["three_launches"] = {"kernels": "k_split_gather + k_mlp_train + k_split_scatter", "ms": ms,
+                                     "texels_per_s": B / (ms * 1e-3), "tflops": flop / (ms * 1e-3) / 1e12,
+                                     "mfma_frac": flop / (ms * 1e-3) / 1e12 / BF16_PEAK_TFLOPS,
+                                     "mlp_ms": mlp_ms, "mlp_tflops": flop / (mlp_ms * 1e-3) / 1e12,
+                                     "mlp_mfma_frac": flop / (mlp_ms * 1e-3) / 1e12 / BF16_PEAK_TFLOPS}
     finally:
+        for e in events:
+            hip.hipEventDestroy(e)
         for p in bufs:
             lib.ntc_dev_free(p)
-    flop = 6.0 * (F * H + H * H + H * O) * B
-    return {"kernel": "k_fused_train", "lod": 0, "texels": B, "ms": ms, "texels_per_s": B / (ms * 1e-3),
-            "tflops": flop / (ms * 1e-3) / 1e12, "mfma_frac": flop / (ms * 1e-3) / 1e12 / BF16_PEAK_TFLOPS,
-            "sample": f"{reps} back-to-back launches, wall clock"}
+    return out
 
 
 def pmc_traffic(kernel):
@@ -583,9 +624,13 @@ def main():
     }
 
     if a.path == "fused":
-        times = (ctypes.c_float * 9)()
-        lib.ntc_ctx_kernel_times(ctx._h, times, 9)
+        times = (ctypes.c_float * 10)()
+        lib.ntc_ctx_kernel_times(ctx._h, times, 10)
         fused_ms = float(times[0])
+        # the three-launch step (gather, MLP, scatter): its MLP launch alone,
+        # events right around it in the same timed steps
+        mlp_ms = float(times[9])
+        split = mlp_ms > 0
         # the step's other kernels: an event around each, over the next
         # `detail_steps` schedule steps (after the timed region)
         lib.ntc_ctx_set_timing(ctx._h, 1)
@@ -594,8 +639,8 @@ def main():
         lib.ntc_dev_synchronize()
         if prog.result != ntc.Result.Success:
             raise SystemExit(f"kernel-time sample failed: {prog}")
-        lib.ntc_ctx_kernel_times(ctx._h, times, 9)
-        _, dec_ms, grid_ms, n_t, n_grid, n_tiles_upd, catch_ms, hot_ms, n_lazy = list(times)
+        lib.ntc_ctx_kernel_times(ctx._h, times, 10)
+        _, dec_ms, grid_ms, n_t, n_grid, n_tiles_upd, catch_ms, hot_ms, n_lazy, _ = list(times)
         _, dinfos = step_texels(ctx, a.warmup + a.steps, a.detail_steps)
         tiles_upd = n_grid > 0 and n_tiles_upd == n_grid
         lazy = n_grid > 0 and n_lazy == n_grid
@@ -631,14 +676,25 @@ def main():
                 return 24.0 * (pl - pf) + 32.0 * pf, 24.0 * (pl - pf) + 32.0 * pf
             return 32.0 * pl, 32.0 * pl
 
-        flops, fbytes, gbytes, cbytes, hbytes, blks, dense = [], [], [], [], [], [], []
+        # three-launch step: the MLP launch reads the feature rows (2 B per
+        # feature slot, F rounded up to 8) and the targets, writes the dX rows
+        # (4 B per grid slot, G of them) and the partials; the gather reads
+        # the footprint cells and writes the feature rows, the scatter reads
+        # the dX rows and adds into the footprint cells' gradients (8 B/param)
+        fxs = (F + 7) // 8 * 8
+        gsl = -(-(lay.g0_channels + lay.g1_channels) // 16) * 16
+        flops, fbytes, gbytes, cbytes, hbytes, blks, dense, mbytes = [], [], [], [], [], [], [], []
         for si in infos:
             B = si.n_tiles * si.tile_dim ** 2
             pl, pf = cells(si)
-            blocks = min(256, (B + 63) // 64)
+            blocks = min(256, (B + (127 if split else 63)) // (128 if split else 64))
             flops.append(6.0 * (F * H + H * H + H * O) * B)
             blks.append(blocks)
-            fbytes.append(4.0 * pf + O * B + 4.0 * blocks * n_dec)
+            if split:
+                mbytes.append(2.0 * fxs * B + O * B + 4.0 * gsl * B + 4.0 * blocks * n_dec)
+                fbytes.append(4.0 * pf + 2.0 * fxs * B + mbytes[-1] + 4.0 * gsl * B + 8.0 * pf)
+            else:
+                fbytes.append(4.0 * pf + O * B + 4.0 * blocks * n_dec)
             if si.grid_update:
                 dense.append(grid_bytes(si, pl, pf)[1])
         for si in dinfos:
@@ -651,7 +707,9 @@ def main():
                 hbytes.append(32.0 * pf)
         fused_tflops = np.mean(flops) / (fused_ms * 1e-3) / 1e12
         fused_gbs = np.mean(fbytes) / (fused_ms * 1e-3) / 1e9
-        kern = {"fused_train_step": {"ms": fused_ms, "tflops": fused_tflops, "mfma_frac": fused_tflops / BF16_PEAK_TFLOPS,
+        kern = {"fused_train_step": {"kernels": ("k_split_gather + k_mlp_train + k_split_scatter" if split
+                                                 else "k_fused_train"),
+                                     "ms": fused_ms, "tflops": fused_tflops, "mfma_frac": fused_tflops / BF16_PEAK_TFLOPS,
                                      "GBps": fused_gbs, "hbm_frac": fused_gbs / HBM_PEAK_GBS,
                                      "flop_per_launch": float(np.mean(flops)),
                                      "bytes_per_launch": float(np.mean(fbytes)),
@@ -683,9 +741,22 @@ def main():
         kern["step"] = {"ms": dt * 1e3 / a.steps, "dense_equivalent_bytes": step_bytes,
                         "achieved": step_bytes / (dt / a.steps) / 1e9, "unit": "GB/s",
                         "frac": step_bytes / (dt / a.steps) / 1e9 / HBM_PEAK_GBS}
+        if split:
+            mlp_tflops = np.mean(flops) / (mlp_ms * 1e-3) / 1e12
+            mlp_gbs = np.mean(mbytes) / (mlp_ms * 1e-3) / 1e9
+            kern["mlp_train"] = {"kernel": "k_mlp_train", "ms": mlp_ms, "tflops": mlp_tflops,
+                                 "mfma_frac": mlp_tflops / BF16_PEAK_TFLOPS, "GBps": mlp_gbs,
+                                 "hbm_frac": mlp_gbs / HBM_PEAK_GBS, "flop_per_launch": float(np.mean(flops)),
+                                 "bytes_per_launch": float(np.mean(mbytes)), "workgroups": float(np.mean(blks)),
+                                 "sample": f"the {a.steps} timed steps, events right around the launch"}
         out["kernels"] = kern
         # the roofline names the longest-running kernel of the step
-        if gbytes and not lazy and grid_ms >= fused_ms:
+        if split:
+            m = kern["mlp_train"]
+            out["roofline"] = {"bound": "mfma", "kernel": "k_mlp_train", "achieved": m["tflops"],
+                               "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": m["mfma_frac"],
+                               "traffic": pmc_traffic("k_mlp_train")}
+        elif gbytes and not lazy and grid_ms >= fused_ms:
             g = kern["grid_update"]
             out["roofline"] = {"bound": "hbm", "kernel": g["kernel"], "achieved": g["achieved"],
                                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": g["frac"],
@@ -703,7 +774,10 @@ def main():
 
     if a.path == "fused":
         out["kernels"]["fused_train_step_lod0"] = lod0_micro(ntc, ctx)
-        log(f"lod-0 fused kernel: {out['kernels']['fused_train_step_lod0']['ms']:.4f} ms")
+        l0 = out["kernels"]["fused_train_step_lod0"]
+        log(f"lod-0 step kernels: one launch {l0['one_launch']['ms']:.4f} ms"
+            + (f", three launches {l0['three_launches']['ms']:.4f} ms (MLP {l0['three_launches']['mlp_ms']:.4f} ms)"
+               if "three_launches" in l0 else ""))
     if ttp is not None:
         out["time_to_psnr"] = {k: ttp[k] for k in ("metric", "plain_fp32", "target_psnr", "final_within_0.05dB",
                                                     "seconds_to_target", "steps_to_target",

@@ -27,6 +27,7 @@ extern "C" {
 #endif
 
 typedef struct ihipStream_t* hipStream_t;
+typedef struct ihipEvent_t* hipEvent_t;
 
 #define NTC_OK 0
 #define NTC_E_INVALID (-1)    /* argument out of range / inconsistent shapes */
@@ -360,6 +361,39 @@ int ntc_reduce_decoder_partials(int32_t hidden_dim, int32_t feature_dim,
 size_t ntc_fused_train_step_noalloc_max(void);
 int ntc_fused_train_step_noalloc(const ntc_fused_args* args, float* grad,
                                  double* sse_out, hipStream_t stream);
+/* The training step of ntc_fused_train_step in three launches (replaces the
+ * same reference stages: draw_features, the forward and backward GEMMs,
+ * activations, loss and accumulate_grid_gradients, libopenntc.cpp:621-709,
+ * ntc_kernel.cu:75-373): ntc_split_gather writes the features of every texel
+ * (with their quantisation noise) into BF16 rows of the workspace,
+ * ntc_split_mlp runs the decoder forward, loss and backward on BF16 MFMA
+ * (decoder-gradient and SSE partials as ntc_fused_train_step's) and writes
+ * the dX of the grid slots into FP32 rows of the workspace, ntc_split_scatter
+ * adds the latent gradients from those rows (scatter = 1; nothing
+ * otherwise). ntc_split_train_step issues the three in order on the stream;
+ * a caller may issue them one by one (with its own work or events between
+ * them) as long as they run in that order on one step's arguments.
+ * Compiled for the H = 128 ReLU layouts (ntc_split_supported). Arguments as
+ * ntc_fused_train_step's, except that blocks must equal
+ * ntc_split_blocks_for(...) (the MLP launch's workgroups, one partial slot
+ * each) and the workspace (256-byte aligned, at least
+ * ntc_split_workspace_bytes(...) bytes) holds the feature rows first, then
+ * the dX rows, both in pass order (128-texel passes). */
+int ntc_split_supported(const ntc_feature_desc* fd, int32_t hidden_dim,
+                        int32_t activation);
+int ntc_split_blocks_for(const ntc_feature_desc* fd, int32_t hidden_dim,
+                         int32_t activation, int64_t batch,
+                         int32_t max_blocks);
+size_t ntc_split_workspace_bytes(const ntc_feature_desc* fd, int32_t hidden_dim,
+                                 int32_t activation, int64_t batch);
+int ntc_split_train_step(const ntc_fused_args* args, void* workspace,
+                         size_t workspace_bytes, hipStream_t stream);
+int ntc_split_gather(const ntc_fused_args* args, void* workspace,
+                     size_t workspace_bytes, hipStream_t stream);
+int ntc_split_mlp(const ntc_fused_args* args, void* workspace,
+                  size_t workspace_bytes, hipStream_t stream);
+int ntc_split_scatter(const ntc_fused_args* args, void* workspace,
+                      size_t workspace_bytes, hipStream_t stream);
 /* Sums the workgroup partials in order into grad, then Adam on W0 | W1 |
  * Wout rows < O (update_adam x3, libopenntc.cpp:722-754); sse_out (may be
  * NULL) receives the ordered sum of sse_partials. */
@@ -658,15 +692,24 @@ int ntc_ctx_flush_grids(ntc_context* ctx);
 /* Device pointer to the context's stream (for timing with events). */
 void* ntc_ctx_stream(ntc_context* ctx);
 /* Average durations over the last Train call with timing on, measured with
- * events: {fused kernel, decoder update (with its table refresh), grid
- * update, steps timed, grid updates timed, grid updates that skipped the
- * zero gradient outside the tile footprints, deferred catch-up, hot part
- * (ms), steps with the deferred update}; -1 when not recorded. */
+ * events: {fused kernel (the three launches of ntc_split_train_step where it
+ * is used), decoder update (with its table refresh), grid update, steps
+ * timed, grid updates timed, grid updates that skipped the zero gradient
+ * outside the tile footprints, deferred catch-up, hot part (ms), steps with
+ * the deferred update, MLP launch of ntc_split_train_step}; -1 when not
+ * recorded. */
 int ntc_ctx_kernel_times(ntc_context* ctx, float* out, int32_t n);
 /* enabled: 0 off; 1 events around every kernel of each fused step; 2 around
  * the fused kernel only (fewer events between the step's kernels; the other
  * figures of ntc_ctx_kernel_times are then -1). Other values: InvalidState. */
 int ntc_ctx_set_timing(ntc_context* ctx, int32_t enabled);
+/* Form of the fused path's step, taken at the next BeginTraining: 0 (the
+ * default) the three launches of ntc_split_train_step where the layout has
+ * them, else the one-launch ntc_fused_train_step; 1 always the one-launch
+ * kernel. Other values: InvalidState. ntc_ctx_fused_form returns the form the
+ * current training uses (0 three launches, 1 one launch). */
+int ntc_ctx_set_fused_form(ntc_context* ctx, int32_t form);
+int ntc_ctx_fused_form(ntc_context* ctx);
 /* Fault injection for tests of the error paths. No counterpart in the
  * reference; it makes the paths where the reference returns CudaFailure or
  * AllocationFailure after a failed CUDA call (libopenntc.cpp:17-30, 506-527)

@@ -193,6 +193,8 @@ SIGNATURES = {
     "ntc_ctx_kernel_times": (_I, [_P, _P, _I]),
     "ntc_ctx_flush_grids": (_I, [_P]),
     "ntc_ctx_set_timing": (_I, [_P, _I]),
+    "ntc_ctx_set_fused_form": (_I, [_P, _I]),
+    "ntc_ctx_fused_form": (_I, [_P]),
     "ntc_debug_inject_fault": (_I, [_I, _I]),
     "ntc_synthetic_pbr": (_I, [_I, _U, _P, _P, _P]),
     "ntc_dev_alloc": (_I, [_SZ, _P]),
@@ -209,6 +211,13 @@ SIGNATURES = {
     "ntc_fused_blocks_for": (_I, [_P, _I, _I, _I64, _I]),
     "ntc_pack_decoder_bf16": (_I, [_P, _I, _I, _P, _P, _P]),
     "ntc_fused_train_step": (_I, [_P, _P]),
+    "ntc_split_supported": (_I, [_P, _I, _I]),
+    "ntc_split_blocks_for": (_I, [_P, _I, _I, _I64, _I]),
+    "ntc_split_workspace_bytes": (_SZ, [_P, _I, _I, _I64]),
+    "ntc_split_train_step": (_I, [_P, _P, _SZ, _P]),
+    "ntc_split_gather": (_I, [_P, _P, _SZ, _P]),
+    "ntc_split_mlp": (_I, [_P, _P, _SZ, _P]),
+    "ntc_split_scatter": (_I, [_P, _P, _SZ, _P]),
     "ntc_grid_update_tiles_f32": (_I, [_P, _P, _P, _P, _P, _P, _P, _P]),
     "ntc_grid_update_tiles_part_f32": (_I, [_P, _P, _P, _P, _P, _P, _P, _I, _I, _P]),
     "ntc_grid_lazy_catchup_f32": (_I, [_P, _P, _P, _P, _P, _P, _P, _I, _P, _P, _P]),

@@ -20,7 +20,7 @@ OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libopenntc_amd.so")
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 
-HIP_SOURCES = ["kernels_plain.hip", "kernels_grid.hip", "kernels_tex.hip", "kernels_fused.hip", "kernels_fused_pair.hip", "kernels_eval.hip", "kernels_eval_mfma.hip"]
+HIP_SOURCES = ["kernels_plain.hip", "kernels_grid.hip", "kernels_tex.hip", "kernels_fused.hip", "kernels_fused_pair.hip", "kernels_split.hip", "kernels_eval.hip", "kernels_eval_mfma.hip"]
 CXX_SOURCES = ["context.cpp", "runtime.cpp", "capi.cpp", "synthetic.cpp", "batch.cpp", "manifest.cpp", "images.cpp"]
 
 INCLUDES = ["-I", os.path.join(ROOT, "include"), "-I", CSRC]

@@ -420,6 +420,21 @@ int ntc_ctx_set_timing(ntc_context* c, int32_t enabled)
   return 0;
 }
 
+int ntc_ctx_set_fused_form(ntc_context* c, int32_t form)
+{
+  if (!c) return (int)Result::InvalidState;
+  if (form < 0 || form > 1) return (int)Result::InvalidState;
+  c->ctx.GetImpl()->fused_form = form;
+  return 0;
+}
+
+int ntc_ctx_fused_form(ntc_context* c)
+{
+  if (!c) return -1;
+  const Context::Impl& I = *c->ctx.GetImpl();
+  return I.split ? 0 : 1;
+}
+
 int ntc_debug_inject_fault(int32_t site, int32_t nth)
 {
   if (site < NTC_FAULT_ALLOC || site > NTC_FAULT_COPY || nth < 0) return NTC_E_INVALID;

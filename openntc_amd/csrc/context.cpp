@@ -838,7 +838,7 @@ Result Context::Impl::RunFusedStep(const StepPlan& st, int32_t step_index, hipEv
   a.sse_partials = sse_part.as<double>();
   const bool tiles_update = st.grid_update && g0_channels % 4 == 0 && g1_channels % 4 == 0 && st.n_tiles <= 16;
   const bool deferred = lazy && st.grid_update;
-  a.blocks = ntc_fused_blocks_for(&fd, H, act, B, max_blocks);
+  a.blocks = split ? ntc_split_blocks_for(&fd, H, act, B, max_blocks) : ntc_fused_blocks_for(&fd, H, act, B, max_blocks);
   // dense Adam + clamp of the level: the gradient is zero outside the tile
   // footprints, so the update reads and clears it only inside them
   const ntc_grid_adam adam = {g0m[lvl].as<float>(), g0v[lvl].as<float>(), g1m[lvl].as<float>(),
@@ -858,9 +858,25 @@ Result Context::Impl::RunFusedStep(const StepPlan& st, int32_t step_index, hipEv
   caught_step = -1;
   const int nx = step_index + 1;
   const bool ahead = deferred && nx < train_end && plan[nx].grid_update && !plan[nx].lock;
-  if (ahead) CTX_HIP(hipEventRecord(ready_ev, stream), Result::CudaFailure);  // after the last step's launches
   if (ev) CTX_HIP(hipEventRecord(ev[0], stream), Result::CudaFailure);
-  CTX_NTC(ntc_fused_train_step(&a, stream));
+  if (split)
+  {
+    // the next step's catch-up starts once the MLP launch is done: beside the
+    // scatter, the decoder update and the hot part (memory-bound, few
+    // registers), not beside the MLP launch, whose waves fill every SIMD's
+    // registers
+    CTX_NTC(ntc_split_gather(&a, split_ws.ptr, split_ws.bytes, stream));
+    if (ev) CTX_HIP(hipEventRecord(ev[6], stream), Result::CudaFailure);
+    CTX_NTC(ntc_split_mlp(&a, split_ws.ptr, split_ws.bytes, stream));
+    if (ev) CTX_HIP(hipEventRecord(ev[7], stream), Result::CudaFailure);
+    if (ahead) CTX_HIP(hipEventRecord(ready_ev, stream), Result::CudaFailure);
+    CTX_NTC(ntc_split_scatter(&a, split_ws.ptr, split_ws.bytes, stream));
+  }
+  else
+  {
+    if (ahead) CTX_HIP(hipEventRecord(ready_ev, stream), Result::CudaFailure);  // after the last step's launches
+    CTX_NTC(ntc_fused_train_step(&a, stream));
+  }
   if (ev) CTX_HIP(hipEventRecord(ev[1], stream), Result::CudaFailure);
   // the next step's catch-up, on the second stream beside this whole step
   // (its waves fill the issue slots the fused launch leaves idle): it waits
@@ -1473,6 +1489,21 @@ void Context::BeginTrainingEx(const TrainInfoEx& info)
   }
 
   I.BuildPlan(I.batch_count);
+  I.split = false;
+  if (I.path == TrainPath::Fused && I.fused_form == 0)
+  {
+    const ntc_feature_desc fd = I.FeatureDesc(0);
+    if (ntc_split_supported(&fd, I.hidden_dim, I.ActCode()))
+    {
+      int64_t max_b = 0;
+      for (int t = 0; t < I.batch_count; ++t)
+        max_b = std::max<int64_t>(max_b, (int64_t)I.plan[t].n_tiles * I.plan[t].tile_dim * I.plan[t].tile_dim);
+      const size_t ws = ntc_split_workspace_bytes(&fd, I.hidden_dim, I.ActCode(), max_b);
+      if (ws == 0) return fail(Result::InvalidState);
+      if (I.split_ws.bytes < ws && I.split_ws.Alloc(ws) != hipSuccess) return fail(Result::AllocationFailure);
+      I.split = true;
+    }
+  }
   std::vector<int32_t> table((size_t)I.batch_count * 16, 0);
   for (int t = 0; t < I.batch_count; ++t) std::memcpy(&table[(size_t)t * 16], I.plan[t].origins, sizeof(int32_t) * 16);
   if (I.origins_table.Alloc(sizeof(int32_t) * table.size()) != hipSuccess) return fail(Result::AllocationFailure);
@@ -1549,13 +1580,14 @@ TrainProgress Context::Train(int32_t num_batches)
   if (hipStreamSynchronize(I.stream) != hipSuccess) return error(Result::CudaFailure);
   if (timed)
   {
-    double fused = 0, decoder = 0, grid = 0, catchup = 0, hot = 0;
+    double fused = 0, decoder = 0, grid = 0, catchup = 0, hot = 0, mlp = 0;
     int grids = 0, deferred = 0;
     for (int s = 0; s < n; ++s)
     {
       hipEvent_t* ev = &I.events[(size_t)s * Impl::kStepEvents];
       float ms = 0, ms2 = 0;
       if (hipEventElapsedTime(&ms, ev[0], ev[1]) == hipSuccess) fused += ms;
+      if (I.split && hipEventElapsedTime(&ms, ev[6], ev[7]) == hipSuccess) mlp += ms;
       if (I.timing != 1) continue;
       if (hipEventElapsedTime(&ms, ev[1], ev[2]) == hipSuccess) decoder += ms;
       if (!I.plan[first + s].grid_update) continue;
@@ -1581,7 +1613,8 @@ TrainProgress Context::Train(int32_t num_batches)
     I.kernel_ms = {(float)(fused / n), I.timing == 1 ? (float)(decoder / n) : -1.0f,
                    grids ? (float)(grid / grids) : -1.0f, (float)n,
                    (float)grids, (float)I.tiles_update_steps, deferred ? (float)(catchup / deferred) : -1.0f,
-                   deferred ? (float)(hot / deferred) : -1.0f, (float)deferred};
+                   deferred ? (float)(hot / deferred) : -1.0f, (float)deferred,
+                   I.split ? (float)(mlp / n) : -1.0f};
   }
   tp.phase_ = TrainPhase::TrainInProgress;
   tp.result_ = Result::Success;

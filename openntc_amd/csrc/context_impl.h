@@ -74,8 +74,9 @@ struct Context::Impl
   int32_t feature_dim = 0, feature_dim_padded = 0, out_dim = 0;
   static constexpr int32_t kOutPadded = 16;
   // timing events per fused step: fused kernel start/end, after the decoder
-  // update + table pack, grid catch-up start/end, end of the grid update
-  static constexpr int32_t kStepEvents = 6;
+  // update + table pack, grid catch-up start/end, end of the grid update,
+  // MLP launch start/end of the three-launch step
+  static constexpr int32_t kStepEvents = 8;
   int32_t min_dimension = Context::kMinDimension;
   int32_t mip_min_dim = 4;
   uint32_t seed = 123;
@@ -123,6 +124,13 @@ struct Context::Impl
   DeviceBuffer dw_part;       // [blocks][decoder elems]
   DeviceBuffer sse_part;      // [blocks]
   int32_t max_blocks = 0;
+  // The step as three launches (gather, MLP, latent scatter; feature and dX
+  // rows in split_ws) where compiled for the layout (ntc_split_supported),
+  // unless fused_form = 1 asks for the one-launch kernel; decided at
+  // BeginTraining.
+  int32_t fused_form = 0;
+  bool split = false;
+  DeviceBuffer split_ws;
   // Deferred zero-gradient grid steps (fused path): per level, the count of
   // the level's Adam steps applied to every g0 / g1 cell, {lr, bias1, bias2}
   // of each of the level's steps, the steps issued so far and whether some
@@ -148,9 +156,11 @@ struct Context::Impl
   // grid-update launches timed, grid updates that skipped the zero gradient
   // outside the tile footprints, catch-up, hot part, steps with the deferred
   // update} over the last Train call (durations averaged; with the deferred
-  // update the grid figure is catch-up + hot). Level 2 records only the fused
-  // kernel's pair (each event recorded between the step's kernels delays the
-  // next one by a few microseconds); the other figures are then -1.
+  // update the grid figure is catch-up + hot), then the MLP launch of the
+  // three-launch step (-1 for the one-launch kernel). Level 2 records only the
+  // fused kernel's pair (each event recorded between the step's kernels delays
+  // the next one by a few microseconds) and the MLP launch's; the other
+  // figures are then -1.
   int32_t timing = 0;
   int32_t tiles_update_steps = 0;
   std::vector<float> kernel_ms;

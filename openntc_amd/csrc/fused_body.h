@@ -669,9 +669,9 @@ __device__ __forceinline__ void prefetch_commit(const Prefetch<CF::PF>& pf, cons
 
 // Issues the loads of the target bytes of this lane's output rows
 // (4(l/16)+i) for its texels of a pass; unconditional, like prefetch_issue.
-template <class CF, int NT>
+template <class CF, int NT, class PP>
 __device__ __forceinline__ void targets_issue(uint32_t (&tg)[NT][4], const PassMap& pm, int64_t pass,
-                                              const FusedParams& P, const uint32_t* orig, int wave, int n16, int g)
+                                              const PP& P, const uint32_t* orig, int wave, int n16, int g)
 {
   const int M = P.tl.mip_dim;
 #pragma unroll

@@ -71,7 +71,7 @@ def _case(oracle, kind, dim=256, mip_min=4):
     if kind == "c1_bilinear_relu":
         return oracle.layout_custom(4, 4, 8, 16, 2, 64, 1, 1, dim, 9, mip_min), 1
     if kind == "c3_bilinear_h128":
-        return oracle.layout_custom(4, 4, 16, 16, 2, 128, 1, 1, 256), 1
+        return oracle.layout_custom(4, 4, 16, 16, 2, 128, 1, 1, dim, 9, mip_min), 1
     if kind == "c2_bilinear":
         return oracle.layout_custom(4, 2, 12, 16, 2, 64, 1, 1, 256), 1
     if kind == "ref_bpp_2_25":
@@ -143,7 +143,7 @@ def _step_case(oracle, kind, lod, n_tiles, dim=256, mip_min=4):
     a.loss_scale = float(scale)
     a.scatter = 1
     return dict(a=a, fdc=fdc, H=H, F=F, Od=Od, act=act, B=B, dec=dec, g0=g0, g1=g1, pkg=pkg, support=support,
-                fd=fd, origins=origins, T=T, M=M, bdx=np.ascontiguousarray(bdx),
+                fd=fd, origins=origins, T=T, M=M, bdx=np.ascontiguousarray(bdx), x32=x32,
                 ref=(sse, dw0, dw1, dwout, rdg0, rdg1), bref=(bsse, bw0, bw1, bwo, bdg0, bdg1))
 
 
