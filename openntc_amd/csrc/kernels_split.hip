@@ -332,10 +332,10 @@ __global__ __launch_bounds__(kThreads) void k_split_scatter(FusedParams P, const
 // wave of the SIMD hides the rest of the latency).
 constexpr int kMlpWaves = 8, kMlpRing = 2;
 
-template <int H, int FP, int G, int ACT>
+template <int H, int FP, int G, int ACT, int FEAT>
 __global__ __launch_bounds__(64 * kMlpWaves, 1) void k_mlp_train(SplitParams P)
 {
-  mlp_body<H, FP, G, ACT, kMlpWaves, kMlpRing>(P, blockIdx.x, gridDim.x);
+  mlp_body<H, FP, G, ACT, kMlpWaves, kMlpRing, FEAT>(P, blockIdx.x, gridDim.x);
 }
 
 typedef void (*GatherKernel)(FusedParams, uint16_t*, int);
@@ -352,7 +352,8 @@ struct SplitEntry
 
 #define NTC_SPLIT(H, FP, G, C0, C1, MODE, PEC, LODS, ACT)                                                  \
   SplitEntry{shape_v<H, FP, G, C0, C1, MODE, PEC, LODS, ACT>(), k_split_gather<FP, G, C0, C1, MODE, PEC, LODS>, \
-             k_mlp_train<H, FP, G, ACT>, k_split_scatter<FP, G, C0, C1, MODE, PEC>},
+             k_mlp_train<H, FP, G, ACT, (MODE == NTC_G0_CORNERS ? 4 * C0 : C0) + C1 + 4 * PEC + LODS>,     \
+             k_split_scatter<FP, G, C0, C1, MODE, PEC>},
 
 // Layouts with the three-launch form: the H = 128 decoders with ReLU.
 const SplitEntry kSplitEntries[] = {NTC_SPLIT(128, 64, 32, 16, 16, NTC_G0_BILINEAR, 2, 0, NTC_ACT_RELU)};

@@ -46,7 +46,7 @@ struct SplitParams
 // kSplitPass. The decoder-gradient rounds take 64 texels (four tiles) each;
 // every wave accumulates H / W rows of dW1 and dW0 and H / (16 W) column
 // tiles of dWout.
-template <int H, int FP, int G, int ACT, int W>
+template <int H, int FP, int G, int ACT, int W, int FEAT>
 struct MlpCfg
 {
   static constexpr int NW = W;
@@ -56,7 +56,17 @@ struct MlpCfg
   static constexpr int ROUNDS = PASS / 64;
   static constexpr int MT = H / 16, KF = FP / 32, KH = H / 32, GT = G / 16;
   static constexpr int RW = H / W, MTW = RW / 16, NTO = H / (16 * W);
+  // dW1 (MT x MT tiles): each wave accumulates a BM1 x BN1 block of tiles
+  // (fewer operand reads per round than a row of tiles); dW0: MTW rows of
+  // tiles of its NCW column tiles that hold features (the padding columns
+  // FEAT .. FP - 1 of the last tiles are never written out)
+  static constexpr int T1 = MT * MT / W;
+  static constexpr int BN1 = T1 >= 16 ? 4 : (T1 >= 4 ? 2 : 1);
+  static constexpr int BM1 = T1 / BN1;
+  static constexpr int NCW = (FEAT + 15) / 16;
   static_assert(NT >= 1 && MTW >= 1 && NTO >= 1 && (W == 4 || NT == 1), "wave shapes");
+  static_assert(MT % BM1 == 0 && MT % BN1 == 0 && (MT / BM1) * (MT / BN1) == W, "dW1 blocks");
+  static_assert(FEAT <= FP && NCW <= FP / 16, "feature columns");
   static constexpr int WT = H * FP + H * H + 16 * H;
   static constexpr int XS = FP + 8;      // LDS X row stride (elements)
   static constexpr int XR = PASS * XS;
@@ -70,10 +80,144 @@ struct MlpCfg
 // Feature row stride in HBM: F rounded up to 8 elements (16-byte chunks).
 __host__ __device__ constexpr int split_fxs(int F) { return (F + 7) & ~7; }
 
-template <int H, int FP, int G, int ACT, int W, int RING>
+typedef short v2i16 __attribute__((ext_vector_type(2)));
+typedef unsigned short v2u16 __attribute__((ext_vector_type(2)));
+
+// ReLU of two packed BF16 values (one v_pk_max_i16): read as signed 16-bit
+// integers the negative values (sign bit set, -0 included) lie below zero,
+// the others keep their order, so max(v, 0) gives the BF16 ReLU. Rounding
+// to BF16 and the ReLU commute, so relu_pk(pk_bf16(a, b)) has the bits of
+// the ReLU applied in FP32 and then rounded.
+__device__ __forceinline__ uint32_t relu_pk(uint32_t w)
+{
+  return __builtin_bit_cast(uint32_t, __builtin_elementwise_max(__builtin_bit_cast(v2i16, w), v2i16{0, 0}));
+}
+
+// ReLU backward of two packed BF16 values: d where the paired activation a
+// (a ReLU output, so +0 or positive: nonzero exactly when the pre-activation
+// was positive) is nonzero, else +0. Masking after the rounding gives the
+// bits of masking in FP32 first.
+__device__ __forceinline__ uint32_t relu_bwd_pk(uint32_t d, uint32_t a)
+{
+  // per half: 0 - a has its sign bit set exactly when 0 < a <= 0x7FFF; the
+  // arithmetic shift spreads it into an all-ones or all-zeros mask
+  const v2i16 m = (v2i16{0, 0} - __builtin_bit_cast(v2i16, a)) >> 15;
+  return d & __builtin_bit_cast(uint32_t, m);
+}
+
+// b / 255.0f for a byte b, correctly rounded, without a division: q = b r
+// with r = fl(1/255), then one FMA correction; equal to the IEEE quotient for
+// every b in 0..255 (checked exhaustively in exact arithmetic).
+__device__ __forceinline__ float byte_over_255(uint32_t b)
+{
+  constexpr float r = 1.0f / 255.0f;
+  const float x = (float)b;
+  const float q = x * r;
+  const float e = fmaf(-q, 255.0f, x);
+  return fmaf(e, r, q);
+}
+
+// Two accumulator tiles (hidden units 4 (l/16) + i of tiles 2 j and 2 j + 1)
+// as one B fragment of the next product (kperm order), after the activation.
+template <int ACT>
+__device__ __forceinline__ bf16x8 act_frag(const f32x4& lo, const f32x4& hi)
+{
+  if constexpr (ACT == NTC_ACT_RELU)
+  {
+    uint4 u;
+    u.x = relu_pk(pk_bf16(lo[0], lo[1]));
+    u.y = relu_pk(pk_bf16(lo[2], lo[3]));
+    u.z = relu_pk(pk_bf16(hi[0], hi[1]));
+    u.w = relu_pk(pk_bf16(hi[2], hi[3]));
+    return __builtin_bit_cast(bf16x8, u);
+  }
+  else
+  {
+    float v[8];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+    {
+      v[i] = act_fwd<float>(ACT, lo[i]);
+      v[4 + i] = act_fwd<float>(ACT, hi[i]);
+    }
+    return frag_pack(v);
+  }
+}
+
+// The backward of the activation for two accumulator tiles of upstream
+// gradients (dA) as one B fragment: ReLU masks the packed BF16 gradient with
+// the activation fragment af of the same positions; other activations use
+// the FP32 pre-activations zlo / zhi.
+template <int ACT>
+__device__ __forceinline__ bf16x8 act_bwd_frag(const f32x4& dlo, const f32x4& dhi, const bf16x8& af,
+                                               const f32x4& zlo, const f32x4& zhi)
+{
+  if constexpr (ACT == NTC_ACT_RELU)
+  {
+    const uint4 a = __builtin_bit_cast(uint4, af);
+    uint4 u;
+    u.x = relu_bwd_pk(pk_bf16(dlo[0], dlo[1]), a.x);
+    u.y = relu_bwd_pk(pk_bf16(dlo[2], dlo[3]), a.y);
+    u.z = relu_bwd_pk(pk_bf16(dhi[0], dhi[1]), a.z);
+    u.w = relu_bwd_pk(pk_bf16(dhi[2], dhi[3]), a.w);
+    return __builtin_bit_cast(bf16x8, u);
+  }
+  else
+  {
+    float v[8];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+    {
+      v[i] = act_bwd<float>(ACT, zlo[i], dlo[i]);
+      v[4 + i] = act_bwd<float>(ACT, zhi[i], dhi[i]);
+    }
+    return frag_pack(v);
+  }
+}
+
+// A pass's feature rows (n 16-byte chunks) held in registers between their
+// global loads and their LDS stores: up to three chunks per thread, as named
+// members (an array here was kept in scratch memory by the compiler, which
+// made every pass wait for its loads where they were issued).
+struct XRows
+{
+  uint4 r0, r1, r2;
+};
+
+template <int kXL, int kThreads>
+__device__ __forceinline__ XRows x_issue(const uint16_t* xg, int64_t pass, int pass_texels, int fxs, int n, int tid)
+{
+  static_assert(kXL >= 1 && kXL <= 3, "prefetch chunks per thread");
+  const uint4* src = reinterpret_cast<const uint4*>(xg + pass * pass_texels * fxs);
+  XRows x;
+  const int c0 = tid, c1 = tid + kThreads, c2 = tid + 2 * kThreads;
+  // unconditional loads from valid addresses
+  x.r0 = src[c0 < n ? c0 : 0];
+  x.r1 = kXL > 1 ? src[c1 < n ? c1 : 0] : make_uint4(0, 0, 0, 0);
+  x.r2 = kXL > 2 ? src[c2 < n ? c2 : 0] : make_uint4(0, 0, 0, 0);
+  return x;
+}
+
+template <int kXL, int kThreads>
+__device__ __forceinline__ void x_commit(const XRows& x, uint16_t* xnf, int xs, int xchunks, int n, int tid)
+{
+  auto put = [&](int c, const uint4& v) {
+    if (c < n)
+    {
+      const int t = c / xchunks, q = c - t * xchunks;
+      *reinterpret_cast<uint4*>(&xnf[t * xs + 8 * q]) = v;
+    }
+  };
+  put(tid, x.r0);
+  if (kXL > 1) put(tid + kThreads, x.r1);
+  if (kXL > 2) put(tid + 2 * kThreads, x.r2);
+}
+
+template <int H, int FP, int G, int ACT, int W, int RING, int FEAT>
 __device__ __forceinline__ void mlp_body(const SplitParams& P, const int vblk, const int vgrid)
 {
-  using CF = MlpCfg<H, FP, G, ACT, W>;
+  using CF = MlpCfg<H, FP, G, ACT, W, FEAT>;
+  constexpr int BM1 = CF::BM1, BN1 = CF::BN1, NCW = CF::NCW;
   constexpr int kNT = CF::NT, kPass = CF::PASS, kThreads = CF::THREADS;
   constexpr int MT = CF::MT, KF = CF::KF, KH = CF::KH, GT = CF::GT, MTW = CF::MTW, NTO = CF::NTO;
   constexpr int kRing = RING;
@@ -104,15 +248,16 @@ __device__ __forceinline__ void mlp_body(const SplitParams& P, const int vblk, c
       if (i % CF::XS >= FXS) xnf[i] = 0;
   }
 
-  f32x4 dw1[MTW][MT], dw0[MTW][FP / 16], dwo[NTO];
+  f32x4 dw1[BM1][BN1], dw0[MTW][NCW], dwo[NTO];
+  const int mb1 = wave / (MT / BN1), nb1 = wave % (MT / BN1);  // this wave's dW1 block
+#pragma unroll
+  for (int a = 0; a < BM1; ++a)
+#pragma unroll
+    for (int c = 0; c < BN1; ++c) dw1[a][c] = f32x4{0, 0, 0, 0};
 #pragma unroll
   for (int a = 0; a < MTW; ++a)
-  {
 #pragma unroll
-    for (int c = 0; c < MT; ++c) dw1[a][c] = f32x4{0, 0, 0, 0};
-#pragma unroll
-    for (int c = 0; c < FP / 16; ++c) dw0[a][c] = f32x4{0, 0, 0, 0};
-  }
+    for (int c = 0; c < NCW; ++c) dw0[a][c] = f32x4{0, 0, 0, 0};
 #pragma unroll
   for (int c = 0; c < NTO; ++c) dwo[c] = f32x4{0, 0, 0, 0};
   double sse = 0.0;
@@ -126,34 +271,13 @@ __device__ __forceinline__ void mlp_body(const SplitParams& P, const int vblk, c
   // loaded into registers while the previous pass computes its decoder
   // gradients and written into the X rows after them.
   constexpr int kXL = (kPass * 6 + kThreads - 1) / kThreads;
-  uint4 xr[kXL];
-  auto x_issue = [&](int64_t pass) {
-    const uint4* src = reinterpret_cast<const uint4*>(P.xg + pass * kPass * FXS);
-#pragma unroll
-    for (int k = 0; k < kXL; ++k)
-    {
-      const int c = tid + k * kThreads;
-      xr[k] = src[c < kPass * xchunks ? c : 0];
-    }
-  };
-  auto x_commit = [&]() {
-#pragma unroll
-    for (int k = 0; k < kXL; ++k)
-    {
-      const int c = tid + k * kThreads;
-      if (c < kPass * xchunks)
-      {
-        const int t = c / xchunks, q = c - t * xchunks;
-        *reinterpret_cast<uint4*>(&xnf[t * CF::XS + 8 * q]) = xr[k];
-      }
-    }
-  };
+  const int xn = kPass * xchunks;
   {
     uint32_t tg[kNT][4];
     const PassMap pm = pass_map((int64_t)vblk, T, kPass);
-    x_issue(vblk);
+    const XRows xr = x_issue<kXL, kThreads>(P.xg, vblk, kPass, FXS, xn, tid);
     targets_issue<CF, kNT>(tg, pm, (int64_t)vblk, P, orig_s, wave, n16, g);
-    x_commit();
+    x_commit<kXL, kThreads>(xr, xnf, CF::XS, xchunks, xn, tid);
     targets_commit(tg, tgt_s, wave, n16, g);
   }
 
@@ -195,17 +319,7 @@ __device__ __forceinline__ void mlp_body(const SplitParams& P, const int vblk, c
 #pragma unroll
     for (int j = 0; j < KH; ++j)
 #pragma unroll
-      for (int nt = 0; nt < kNT; ++nt)
-      {
-        float v[8];
-#pragma unroll
-        for (int i = 0; i < 4; ++i)
-        {
-          v[i] = act_fwd<float>(ACT, z0[2 * j][nt][i]);
-          v[4 + i] = act_fwd<float>(ACT, z0[2 * j + 1][nt][i]);
-        }
-        a0f[j][nt] = frag_pack(v);
-      }
+      for (int nt = 0; nt < kNT; ++nt) a0f[j][nt] = act_frag<ACT>(z0[2 * j][nt], z0[2 * j + 1][nt]);
     // layer 1
     f32x4 z1[MT][kNT];
     lds_ring<MT * KH, kRing>(
@@ -225,17 +339,7 @@ __device__ __forceinline__ void mlp_body(const SplitParams& P, const int vblk, c
 #pragma unroll
     for (int j = 0; j < KH; ++j)
 #pragma unroll
-      for (int nt = 0; nt < kNT; ++nt)
-      {
-        float v[8];
-#pragma unroll
-        for (int i = 0; i < 4; ++i)
-        {
-          v[i] = act_fwd<float>(ACT, z1[2 * j][nt][i]);
-          v[4 + i] = act_fwd<float>(ACT, z1[2 * j + 1][nt][i]);
-        }
-        a1f[j][nt] = frag_pack(v);
-      }
+      for (int nt = 0; nt < kNT; ++nt) a1f[j][nt] = act_frag<ACT>(z1[2 * j][nt], z1[2 * j + 1][nt]);
     // output layer + loss
     float dy[kNT][4];
     bf16x8 dyf[kNT];
@@ -259,7 +363,7 @@ __device__ __forceinline__ void mlp_body(const SplitParams& P, const int vblk, c
           float d = 0.0f;
           if (tvalid[nt] && o < P.O)
           {
-            const float yt = (float)((tgt_s[tloc[nt] * 4 + g] >> (8 * i)) & 0xFFu) / 255.0f;
+            const float yt = byte_over_255((tgt_s[tloc[nt] * 4 + g] >> (8 * i)) & 0xFFu);
             const float r = yt - yh[nt][i];
             sse += (double)r * (double)r;
             d = P.scale * r;
@@ -282,28 +386,14 @@ __device__ __forceinline__ void mlp_body(const SplitParams& P, const int vblk, c
           },
           [&](int r, const bf16x8 a) {
 #pragma unroll
-            for (int nt = 0; nt < kNT; ++nt)
-            {
-              const f32x4 da = mfma(a, dyf[nt], f32x4{0, 0, 0, 0});
-#pragma unroll
-              for (int i = 0; i < 4; ++i)
-                dz1[r][nt][i] = act_bwd<float>(ACT, pre_act<ACT>(z1, a1f, r, nt, i), da[i]);
-            }
+            for (int nt = 0; nt < kNT; ++nt) dz1[r][nt] = mfma(a, dyf[nt], f32x4{0, 0, 0, 0});
           });
 #pragma unroll
       for (int j = 0; j < KH; ++j)
 #pragma unroll
         for (int nt = 0; nt < kNT; ++nt)
-        {
-          float v[8];
-#pragma unroll
-          for (int i = 0; i < 4; ++i)
-          {
-            v[i] = dz1[2 * j][nt][i];
-            v[4 + i] = dz1[2 * j + 1][nt][i];
-          }
-          dz1f[j][nt] = frag_pack(v);
-        }
+          dz1f[j][nt] = act_bwd_frag<ACT>(dz1[2 * j][nt], dz1[2 * j + 1][nt], a1f[j][nt], z1[2 * j][nt],
+                                          z1[2 * j + 1][nt]);
     }
     // backward through layer 1 and the activation of layer 0
     bf16x8 dz0f[KH][kNT];
@@ -322,27 +412,13 @@ __device__ __forceinline__ void mlp_body(const SplitParams& P, const int vblk, c
               for (int nt = 0; nt < kNT; ++nt) dz0[r][nt] = f32x4{0, 0, 0, 0};
 #pragma unroll
             for (int nt = 0; nt < kNT; ++nt) dz0[r][nt] = mfma(a, dz1f[j][nt], dz0[r][nt]);
-            if (j == KH - 1)
-#pragma unroll
-              for (int nt = 0; nt < kNT; ++nt)
-#pragma unroll
-                for (int e = 0; e < 4; ++e)
-                  dz0[r][nt][e] = act_bwd<float>(ACT, pre_act<ACT>(z0, a0f, r, nt, e), dz0[r][nt][e]);
           });
 #pragma unroll
       for (int j = 0; j < KH; ++j)
 #pragma unroll
         for (int nt = 0; nt < kNT; ++nt)
-        {
-          float v[8];
-#pragma unroll
-          for (int i = 0; i < 4; ++i)
-          {
-            v[i] = dz0[2 * j][nt][i];
-            v[4 + i] = dz0[2 * j + 1][nt][i];
-          }
-          dz0f[j][nt] = frag_pack(v);
-        }
+          dz0f[j][nt] = act_bwd_frag<ACT>(dz0[2 * j][nt], dz0[2 * j + 1][nt], a0f[j][nt], z0[2 * j][nt],
+                                          z0[2 * j + 1][nt]);
     }
     // dX of the grid slots (FP32) into the pass's dX rows: lane l holds
     // slots 16 r + 4 g .. + 3 of texel l % 16, one 16-byte store
@@ -372,7 +448,7 @@ __device__ __forceinline__ void mlp_body(const SplitParams& P, const int vblk, c
     const int64_t next = pass + vgrid;
     const int64_t pn = next < npass ? next : pass;
     uint32_t tg[kNT][4];
-    x_issue(pn);
+    const XRows xr = x_issue<kXL, kThreads>(P.xg, pn, kPass, FXS, xn, tid);
     targets_issue<CF, kNT>(tg, pass_map(pn, T, kPass), pn, P, orig_s, wave, n16, g);
 #pragma unroll
     for (int rnd = 0; rnd < CF::ROUNDS; ++rnd)
@@ -401,26 +477,25 @@ __device__ __forceinline__ void mlp_body(const SplitParams& P, const int vblk, c
         const uint16_t* p = st + (32 * kc + trr0) * stride + c0 + tq;
         return frag_tr(p, p + 4 * stride);
       };
-      // dW1[p_out][p_in] += dZ1 A0^T; each A0^T fragment read once and used
-      // for all of this wave's row tiles
+      // dW1[p_out][p_in] += dZ1 A0^T over this wave's block of tiles; each
+      // A0^T fragment read once and used for the block's row tiles
+#pragma unroll
+      for (int kc = 0; kc < 2; ++kc)
       {
-        bf16x8 a[MTW][2];
+        bf16x8 a[BM1];
 #pragma unroll
-        for (int mt = 0; mt < MTW; ++mt)
+        for (int mt = 0; mt < BM1; ++mt) a[mt] = frag_st(stB, CF::SH, kc, 16 * (BM1 * mb1 + mt));
+        lds_ring_ld<BN1, kRing>([&](int c) { return frag_st(stA, CF::SH, kc, 16 * (BN1 * nb1 + c)); },
+                                [&](int c, const bf16x8 b) {
 #pragma unroll
-          for (int kc = 0; kc < 2; ++kc) a[mt][kc] = frag_st(stB, CF::SH, kc, CF::RW * wave + 16 * mt);
-        lds_ring_ld<2 * MT, kRing>([&](int i) { return frag_st(stA, CF::SH, i / MT, 16 * (i % MT)); },
-                                   [&](int i, const bf16x8 b) {
-                                     const int kc = i / MT, c = i % MT;
-#pragma unroll
-                                     for (int mt = 0; mt < MTW; ++mt) dw1[mt][c] = mfma(a[mt][kc], b, dw1[mt][c]);
-                                   });
+                                  for (int mt = 0; mt < BM1; ++mt) dw1[mt][c] = mfma(a[mt], b, dw1[mt][c]);
+                                });
       }
       // dW0[p][f] += dZ0 X^T, X^T by transposing reads of the X rows (K
       // position p of the round at pass texel (p / 16) 16 NT + 16 rnd + p % 16
       // for W = 4, 64 rnd + p for NT = 1)
       {
-        constexpr int NC = FP / 16;
+        constexpr int NC = NCW;
         auto xrow = [&](int kc, int h) {
           const int p = 32 * kc + 8 * g + 4 * h + (n16 >> 2);
           const int t = kNT == 1 ? 64 * rnd + p : (p >> 4) * 16 * kNT + 16 * rnd + (p & 15);
@@ -453,7 +528,7 @@ __device__ __forceinline__ void mlp_body(const SplitParams& P, const int vblk, c
       __syncthreads();
     }
     // the rounds ended with a barrier: X rows and target words are free
-    x_commit();
+    x_commit<kXL, kThreads>(xr, xnf, CF::XS, xchunks, xn, tid);
     targets_commit(tg, tgt_s, wave, n16, g);
   }
 
@@ -464,22 +539,27 @@ __device__ __forceinline__ void mlp_body(const SplitParams& P, const int vblk, c
   float* pw1 = part + H * F;
   float* pwo = pw1 + H * H;
 #pragma unroll
+  for (int mt = 0; mt < BM1; ++mt)
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+    {
+      const int row = kperm_pos(16 * (BM1 * mb1 + mt) + 4 * g + i);
+#pragma unroll
+      for (int c = 0; c < BN1; ++c) pw1[row * H + kperm_pos(16 * (BN1 * nb1 + c) + n16)] = dw1[mt][c][i];
+    }
+#pragma unroll
   for (int mt = 0; mt < MTW; ++mt)
-  {
 #pragma unroll
     for (int i = 0; i < 4; ++i)
     {
       const int row = kperm_pos(CF::RW * wave + 16 * mt + 4 * g + i);
 #pragma unroll
-      for (int c = 0; c < MT; ++c) pw1[row * H + kperm_pos(16 * c + n16)] = dw1[mt][c][i];
-#pragma unroll
-      for (int c = 0; c < FP / 16; ++c)
+      for (int c = 0; c < NCW; ++c)
       {
         const int col = 16 * c + n16;
         if (col < F) pw0[row * F + col] = dw0[mt][c][i];
       }
     }
-  }
 #pragma unroll
   for (int c = 0; c < NTO; ++c)
 #pragma unroll
