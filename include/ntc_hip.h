@@ -700,8 +700,9 @@ void* ntc_ctx_stream(ntc_context* ctx);
  * recorded. */
 int ntc_ctx_kernel_times(ntc_context* ctx, float* out, int32_t n);
 /* enabled: 0 off; 1 events around every kernel of each fused step; 2 around
- * the fused kernel only (fewer events between the step's kernels; the other
- * figures of ntc_ctx_kernel_times are then -1). Other values: InvalidState. */
+ * the fused kernel only, or only around the MLP launch of the three-launch
+ * step (fewer events between the step's kernels; the other figures of
+ * ntc_ctx_kernel_times are then -1). Other values: InvalidState. */
 int ntc_ctx_set_timing(ntc_context* ctx, int32_t enabled);
 /* Form of the fused path's step, taken at the next BeginTraining: 0 (the
  * default) the three launches of ntc_split_train_step where the layout has

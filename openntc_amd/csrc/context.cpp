@@ -858,7 +858,10 @@ Result Context::Impl::RunFusedStep(const StepPlan& st, int32_t step_index, hipEv
   caught_step = -1;
   const int nx = step_index + 1;
   const bool ahead = deferred && nx < train_end && plan[nx].grid_update && !plan[nx].lock;
-  if (ev) CTX_HIP(hipEventRecord(ev[0], stream), Result::CudaFailure);
+  // with the three launches, timing level 2 records only the MLP launch's
+  // pair: every event between two of the step's kernels delays the second
+  const bool ev_core = ev && (!split || timing == 1);
+  if (ev_core) CTX_HIP(hipEventRecord(ev[0], stream), Result::CudaFailure);
   if (split)
   {
     // the next step's catch-up starts once the MLP launch is done: beside the
@@ -877,7 +880,7 @@ Result Context::Impl::RunFusedStep(const StepPlan& st, int32_t step_index, hipEv
     if (ahead) CTX_HIP(hipEventRecord(ready_ev, stream), Result::CudaFailure);  // after the last step's launches
     CTX_NTC(ntc_fused_train_step(&a, stream));
   }
-  if (ev) CTX_HIP(hipEventRecord(ev[1], stream), Result::CudaFailure);
+  if (ev_core) CTX_HIP(hipEventRecord(ev[1], stream), Result::CudaFailure);
   // the next step's catch-up, on the second stream beside this whole step
   // (its waves fill the issue slots the fused launch leaves idle): it waits
   // for the last step's gradient step, leaves out this step's footprint cells
@@ -1586,7 +1589,7 @@ TrainProgress Context::Train(int32_t num_batches)
     {
       hipEvent_t* ev = &I.events[(size_t)s * Impl::kStepEvents];
       float ms = 0, ms2 = 0;
-      if (hipEventElapsedTime(&ms, ev[0], ev[1]) == hipSuccess) fused += ms;
+      if ((!I.split || I.timing == 1) && hipEventElapsedTime(&ms, ev[0], ev[1]) == hipSuccess) fused += ms;
       if (I.split && hipEventElapsedTime(&ms, ev[6], ev[7]) == hipSuccess) mlp += ms;
       if (I.timing != 1) continue;
       if (hipEventElapsedTime(&ms, ev[1], ev[2]) == hipSuccess) decoder += ms;
@@ -1610,7 +1613,7 @@ TrainProgress Context::Train(int32_t num_batches)
         ++grids;
       }
     }
-    I.kernel_ms = {(float)(fused / n), I.timing == 1 ? (float)(decoder / n) : -1.0f,
+    I.kernel_ms = {(!I.split || I.timing == 1) ? (float)(fused / n) : -1.0f, I.timing == 1 ? (float)(decoder / n) : -1.0f,
                    grids ? (float)(grid / grids) : -1.0f, (float)n,
                    (float)grids, (float)I.tiles_update_steps, deferred ? (float)(catchup / deferred) : -1.0f,
                    deferred ? (float)(hot / deferred) : -1.0f, (float)deferred,

@@ -159,8 +159,8 @@ struct Context::Impl
   // update the grid figure is catch-up + hot), then the MLP launch of the
   // three-launch step (-1 for the one-launch kernel). Level 2 records only the
   // fused kernel's pair (each event recorded between the step's kernels delays
-  // the next one by a few microseconds) and the MLP launch's; the other
-  // figures are then -1.
+  // the next one by a few microseconds), or with the three launches only the
+  // MLP launch's; the other figures are then -1.
   int32_t timing = 0;
   int32_t tiles_update_steps = 0;
   std::vector<float> kernel_ms;

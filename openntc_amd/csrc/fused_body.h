@@ -146,19 +146,101 @@ __device__ __forceinline__ uint16_t frag_elem(const bf16x8& f, int e)
   return (uint16_t)((e & 1) ? (w >> 16) : (w & 0xFFFFu));
 }
 
-// What the activation backward of hidden unit 16 r + 4 (l/16) + i (accumulator
-// layout) reads. HardGELU needs the FP32 pre-activation itself; ReLU only its
-// sign, and the BF16 activation of the same unit (fragment r/2, element
-// 4 (r%2) + i) is positive exactly when the pre-activation is (bar positive
-// values below 2^-133, which round to zero), so for ReLU the FP32
-// pre-activations need not stay live through the backward.
-template <int ACT, int MT, int KH, int NT>
-__device__ __forceinline__ float pre_act(const f32x4 (&z)[MT][NT], const bf16x8 (&af)[KH][NT], int r, int nt, int i)
+typedef short v2i16 __attribute__((ext_vector_type(2)));
+typedef unsigned short v2u16 __attribute__((ext_vector_type(2)));
+
+// ReLU of two packed BF16 values (one v_pk_max_i16): read as signed 16-bit
+// integers the negative values (sign bit set, -0 included) lie below zero,
+// the others keep their order, so max(v, 0) gives the BF16 ReLU. Rounding
+// to BF16 and the ReLU commute, so relu_pk(pk_bf16(a, b)) has the bits of
+// the ReLU applied in FP32 and then rounded.
+__device__ __forceinline__ uint32_t relu_pk(uint32_t w)
+{
+  return __builtin_bit_cast(uint32_t, __builtin_elementwise_max(__builtin_bit_cast(v2i16, w), v2i16{0, 0}));
+}
+
+// ReLU backward of two packed BF16 values: d where the paired activation a
+// (a ReLU output, so +0 or positive) is nonzero, else +0. The BF16
+// activation is nonzero exactly when the pre-activation is positive, bar
+// positive values below 2^-134 that round to zero, so the FP32
+// pre-activations need not stay live through the backward. Masking after
+// the rounding gives the bits of masking in FP32 first.
+__device__ __forceinline__ uint32_t relu_bwd_pk(uint32_t d, uint32_t a)
+{
+  // per half: 0 - a has its sign bit set exactly when 0 < a <= 0x7FFF; the
+  // arithmetic shift spreads it into an all-ones or all-zeros mask
+  const v2i16 m = (v2i16{0, 0} - __builtin_bit_cast(v2i16, a)) >> 15;
+  return d & __builtin_bit_cast(uint32_t, m);
+}
+
+// b / 255.0f for a byte b, correctly rounded, without a division: q = b r
+// with r = fl(1/255), then one FMA correction; equal to the IEEE quotient for
+// every b in 0..255 (checked exhaustively in exact arithmetic).
+__device__ __forceinline__ float byte_over_255(uint32_t b)
+{
+  constexpr float r = 1.0f / 255.0f;
+  const float x = (float)b;
+  const float q = x * r;
+  const float e = fmaf(-q, 255.0f, x);
+  return fmaf(e, r, q);
+}
+
+// Two accumulator tiles (hidden units 4 (l/16) + i of tiles 2 j and 2 j + 1)
+// as one B fragment of the next product (kperm order), after the activation.
+template <int ACT>
+__device__ __forceinline__ bf16x8 act_frag(const f32x4& lo, const f32x4& hi)
 {
   if constexpr (ACT == NTC_ACT_RELU)
-    return __uint_as_float((uint32_t)frag_elem(af[r >> 1][nt], 4 * (r & 1) + i) << 16);
+  {
+    uint4 u;
+    u.x = relu_pk(pk_bf16(lo[0], lo[1]));
+    u.y = relu_pk(pk_bf16(lo[2], lo[3]));
+    u.z = relu_pk(pk_bf16(hi[0], hi[1]));
+    u.w = relu_pk(pk_bf16(hi[2], hi[3]));
+    return __builtin_bit_cast(bf16x8, u);
+  }
   else
-    return z[r][nt][i];
+  {
+    float v[8];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+    {
+      v[i] = act_fwd<float>(ACT, lo[i]);
+      v[4 + i] = act_fwd<float>(ACT, hi[i]);
+    }
+    return frag_pack(v);
+  }
+}
+
+// The backward of the activation for two accumulator tiles of upstream
+// gradients (dA) as one B fragment: ReLU masks the packed BF16 gradient with
+// the activation fragment af of the same positions; other activations use
+// the FP32 pre-activations zlo / zhi.
+template <int ACT>
+__device__ __forceinline__ bf16x8 act_bwd_frag(const f32x4& dlo, const f32x4& dhi, const bf16x8& af,
+                                               const f32x4& zlo, const f32x4& zhi)
+{
+  if constexpr (ACT == NTC_ACT_RELU)
+  {
+    const uint4 a = __builtin_bit_cast(uint4, af);
+    uint4 u;
+    u.x = relu_bwd_pk(pk_bf16(dlo[0], dlo[1]), a.x);
+    u.y = relu_bwd_pk(pk_bf16(dlo[2], dlo[3]), a.y);
+    u.z = relu_bwd_pk(pk_bf16(dhi[0], dhi[1]), a.z);
+    u.w = relu_bwd_pk(pk_bf16(dhi[2], dhi[3]), a.w);
+    return __builtin_bit_cast(bf16x8, u);
+  }
+  else
+  {
+    float v[8];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+    {
+      v[i] = act_bwd<float>(ACT, zlo[i], dlo[i]);
+      v[4 + i] = act_bwd<float>(ACT, zhi[i], dhi[i]);
+    }
+    return frag_pack(v);
+  }
 }
 
 __device__ __forceinline__ f32x4 mfma(const bf16x8& a, const bf16x8& b, const f32x4& c)
@@ -1061,17 +1143,7 @@ __device__ __forceinline__ void fused_body(const FusedParams& P, const int vblk,
 #pragma unroll
     for (int j = 0; j < KH; ++j)
 #pragma unroll
-      for (int nt = 0; nt < kNT; ++nt)
-      {
-        float v[8];
-#pragma unroll
-        for (int i = 0; i < 4; ++i)
-        {
-          v[i] = act_fwd<float>(ACT, z0[2 * j][nt][i]);
-          v[4 + i] = act_fwd<float>(ACT, z0[2 * j + 1][nt][i]);
-        }
-        a0f[j][nt] = frag_pack(v);
-      }
+      for (int nt = 0; nt < kNT; ++nt) a0f[j][nt] = act_frag<ACT>(z0[2 * j][nt], z0[2 * j + 1][nt]);
     // layer 1
     f32x4 z1[MT][kNT];
     lds_ring<MT * KH, kRing>(
@@ -1091,17 +1163,7 @@ __device__ __forceinline__ void fused_body(const FusedParams& P, const int vblk,
 #pragma unroll
     for (int j = 0; j < KH; ++j)
 #pragma unroll
-      for (int nt = 0; nt < kNT; ++nt)
-      {
-        float v[8];
-#pragma unroll
-        for (int i = 0; i < 4; ++i)
-        {
-          v[i] = act_fwd<float>(ACT, z1[2 * j][nt][i]);
-          v[4 + i] = act_fwd<float>(ACT, z1[2 * j + 1][nt][i]);
-        }
-        a1f[j][nt] = frag_pack(v);
-      }
+      for (int nt = 0; nt < kNT; ++nt) a1f[j][nt] = act_frag<ACT>(z1[2 * j][nt], z1[2 * j + 1][nt]);
     // output layer + loss
     float dy[kNT][4];
     bf16x8 dyf[kNT];
@@ -1126,7 +1188,7 @@ __device__ __forceinline__ void fused_body(const FusedParams& P, const int vblk,
             if constexpr (MODE == kForward) P.yhat[(int64_t)o * P.B + pass_texel_index(pm, pass, tloc[nt], kPass, T)] = yh[i];
             if (use_targets)
             {
-              const float yt = (float)((tgt_s[tloc[nt] * 4 + g] >> (8 * i)) & 0xFFu) / 255.0f;
+              const float yt = byte_over_255((tgt_s[tloc[nt] * 4 + g] >> (8 * i)) & 0xFFu);
               const float r = yt - yh[i];
               sse += (double)r * (double)r;
               d = P.scale * r;
@@ -1151,28 +1213,14 @@ __device__ __forceinline__ void fused_body(const FusedParams& P, const int vblk,
           },
           [&](int r, const bf16x8 a) {
 #pragma unroll
-                            for (int nt = 0; nt < kNT; ++nt)
-                            {
-                              const f32x4 da = mfma(a, dyf[nt], f32x4{0, 0, 0, 0});
-#pragma unroll
-                              for (int i = 0; i < 4; ++i)
-                                dz1[r][nt][i] = act_bwd<float>(ACT, pre_act<ACT>(z1, a1f, r, nt, i), da[i]);
-                            }
-                          });
+            for (int nt = 0; nt < kNT; ++nt) dz1[r][nt] = mfma(a, dyf[nt], f32x4{0, 0, 0, 0});
+          });
 #pragma unroll
       for (int j = 0; j < KH; ++j)
 #pragma unroll
         for (int nt = 0; nt < kNT; ++nt)
-        {
-          float v[8];
-#pragma unroll
-          for (int i = 0; i < 4; ++i)
-          {
-            v[i] = dz1[2 * j][nt][i];
-            v[4 + i] = dz1[2 * j + 1][nt][i];
-          }
-          dz1f[j][nt] = frag_pack(v);
-        }
+          dz1f[j][nt] = act_bwd_frag<ACT>(dz1[2 * j][nt], dz1[2 * j + 1][nt], a1f[j][nt], z1[2 * j][nt],
+                                          z1[2 * j + 1][nt]);
     }
     // backward through layer 1 and the activation of layer 0
     bf16x8 dz0f[KH][kNT];
@@ -1192,27 +1240,13 @@ __device__ __forceinline__ void fused_body(const FusedParams& P, const int vblk,
               for (int nt = 0; nt < kNT; ++nt) dz0[r][nt] = f32x4{0, 0, 0, 0};
 #pragma unroll
             for (int nt = 0; nt < kNT; ++nt) dz0[r][nt] = mfma(a, dz1f[j][nt], dz0[r][nt]);
-            if (j == KH - 1)
-#pragma unroll
-              for (int nt = 0; nt < kNT; ++nt)
-#pragma unroll
-                for (int e = 0; e < 4; ++e)
-                  dz0[r][nt][e] = act_bwd<float>(ACT, pre_act<ACT>(z0, a0f, r, nt, e), dz0[r][nt][e]);
           });
 #pragma unroll
       for (int j = 0; j < KH; ++j)
 #pragma unroll
         for (int nt = 0; nt < kNT; ++nt)
-        {
-          float v[8];
-#pragma unroll
-          for (int i = 0; i < 4; ++i)
-          {
-            v[i] = dz0[2 * j][nt][i];
-            v[4 + i] = dz0[2 * j + 1][nt][i];
-          }
-          dz0f[j][nt] = frag_pack(v);
-        }
+          dz0f[j][nt] = act_bwd_frag<ACT>(dz0[2 * j][nt], dz0[2 * j + 1][nt], a0f[j][nt], z0[2 * j][nt],
+                                          z0[2 * j + 1][nt]);
     }
     // dX for the grid slots: kept per texel in LDS for the box gradients of
     // grids whose pass footprint fits a box, else scattered per corner with

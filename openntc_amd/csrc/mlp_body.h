@@ -80,101 +80,6 @@ struct MlpCfg
 // Feature row stride in HBM: F rounded up to 8 elements (16-byte chunks).
 __host__ __device__ constexpr int split_fxs(int F) { return (F + 7) & ~7; }
 
-typedef short v2i16 __attribute__((ext_vector_type(2)));
-typedef unsigned short v2u16 __attribute__((ext_vector_type(2)));
-
-// ReLU of two packed BF16 values (one v_pk_max_i16): read as signed 16-bit
-// integers the negative values (sign bit set, -0 included) lie below zero,
-// the others keep their order, so max(v, 0) gives the BF16 ReLU. Rounding
-// to BF16 and the ReLU commute, so relu_pk(pk_bf16(a, b)) has the bits of
-// the ReLU applied in FP32 and then rounded.
-__device__ __forceinline__ uint32_t relu_pk(uint32_t w)
-{
-  return __builtin_bit_cast(uint32_t, __builtin_elementwise_max(__builtin_bit_cast(v2i16, w), v2i16{0, 0}));
-}
-
-// ReLU backward of two packed BF16 values: d where the paired activation a
-// (a ReLU output, so +0 or positive: nonzero exactly when the pre-activation
-// was positive) is nonzero, else +0. Masking after the rounding gives the
-// bits of masking in FP32 first.
-__device__ __forceinline__ uint32_t relu_bwd_pk(uint32_t d, uint32_t a)
-{
-  // per half: 0 - a has its sign bit set exactly when 0 < a <= 0x7FFF; the
-  // arithmetic shift spreads it into an all-ones or all-zeros mask
-  const v2i16 m = (v2i16{0, 0} - __builtin_bit_cast(v2i16, a)) >> 15;
-  return d & __builtin_bit_cast(uint32_t, m);
-}
-
-// b / 255.0f for a byte b, correctly rounded, without a division: q = b r
-// with r = fl(1/255), then one FMA correction; equal to the IEEE quotient for
-// every b in 0..255 (checked exhaustively in exact arithmetic).
-__device__ __forceinline__ float byte_over_255(uint32_t b)
-{
-  constexpr float r = 1.0f / 255.0f;
-  const float x = (float)b;
-  const float q = x * r;
-  const float e = fmaf(-q, 255.0f, x);
-  return fmaf(e, r, q);
-}
-
-// Two accumulator tiles (hidden units 4 (l/16) + i of tiles 2 j and 2 j + 1)
-// as one B fragment of the next product (kperm order), after the activation.
-template <int ACT>
-__device__ __forceinline__ bf16x8 act_frag(const f32x4& lo, const f32x4& hi)
-{
-  if constexpr (ACT == NTC_ACT_RELU)
-  {
-    uint4 u;
-    u.x = relu_pk(pk_bf16(lo[0], lo[1]));
-    u.y = relu_pk(pk_bf16(lo[2], lo[3]));
-    u.z = relu_pk(pk_bf16(hi[0], hi[1]));
-    u.w = relu_pk(pk_bf16(hi[2], hi[3]));
-    return __builtin_bit_cast(bf16x8, u);
-  }
-  else
-  {
-    float v[8];
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
-    {
-      v[i] = act_fwd<float>(ACT, lo[i]);
-      v[4 + i] = act_fwd<float>(ACT, hi[i]);
-    }
-    return frag_pack(v);
-  }
-}
-
-// The backward of the activation for two accumulator tiles of upstream
-// gradients (dA) as one B fragment: ReLU masks the packed BF16 gradient with
-// the activation fragment af of the same positions; other activations use
-// the FP32 pre-activations zlo / zhi.
-template <int ACT>
-__device__ __forceinline__ bf16x8 act_bwd_frag(const f32x4& dlo, const f32x4& dhi, const bf16x8& af,
-                                               const f32x4& zlo, const f32x4& zhi)
-{
-  if constexpr (ACT == NTC_ACT_RELU)
-  {
-    const uint4 a = __builtin_bit_cast(uint4, af);
-    uint4 u;
-    u.x = relu_bwd_pk(pk_bf16(dlo[0], dlo[1]), a.x);
-    u.y = relu_bwd_pk(pk_bf16(dlo[2], dlo[3]), a.y);
-    u.z = relu_bwd_pk(pk_bf16(dhi[0], dhi[1]), a.z);
-    u.w = relu_bwd_pk(pk_bf16(dhi[2], dhi[3]), a.w);
-    return __builtin_bit_cast(bf16x8, u);
-  }
-  else
-  {
-    float v[8];
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
-    {
-      v[i] = act_bwd<float>(ACT, zlo[i], dlo[i]);
-      v[4 + i] = act_bwd<float>(ACT, zhi[i], dhi[i]);
-    }
-    return frag_pack(v);
-  }
-}
-
 // A pass's feature rows (n 16-byte chunks) held in registers between their
 // global loads and their LDS stores: up to three chunks per thread, as named
 // members (an array here was kept in scratch memory by the compiler, which
