@@ -307,7 +307,7 @@ def pmc_traffic(kernel):
     """HBM bytes per launch of `kernel` as measured with rocprofv3 FETCH_SIZE /
     WRITE_SIZE (copy-calibrated) and recorded in the committed profile summary;
     None when the summary holds no row for it."""
-    path = os.path.join(ROOT, "profiles", "grid_update_and_fused_4096_c3_hbm_traffic.md")
+    path = os.path.join(ROOT, "profiles", "train_step_4096_c3_hbm_traffic.md")
     try:
         lines = open(path).read().splitlines()
     except OSError:
