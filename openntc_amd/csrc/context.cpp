@@ -845,8 +845,17 @@ Result Context::Impl::RunFusedStep(const StepPlan& st, int32_t step_index, hipEv
                               g1v[lvl].as<float>(), st.lr_grid, 0.9f, 0.999f, st.grid_bias1, st.grid_bias2,
                               -1.0f, 1.0f - g0_delta, -1.0f, 1.0f - g1_delta};
   if (tiles_update) ++tiles_update_steps;
+  if (side_pending)
+  {
+    // the last step's work on the second stream (its decoder update with the
+    // three launches, this step's catch-up)
+    CTX_HIP(hipStreamWaitEvent(stream, catch_ev, 0), Result::CudaFailure);
+    side_pending = false;
+  }
   if (deferred && caught_step == step_index)
-    CTX_HIP(hipStreamWaitEvent(stream, catch_ev, 0), Result::CudaFailure);  // issued during the last step
+  {
+    // caught up during the last step
+  }
   else if (deferred)
   {
     if (evd) CTX_HIP(hipEventRecord(evd[3], stream), Result::CudaFailure);
@@ -872,7 +881,7 @@ Result Context::Impl::RunFusedStep(const StepPlan& st, int32_t step_index, hipEv
     if (ev) CTX_HIP(hipEventRecord(ev[6], stream), Result::CudaFailure);
     CTX_NTC(ntc_split_mlp(&a, split_ws.ptr, split_ws.bytes, stream));
     if (ev) CTX_HIP(hipEventRecord(ev[7], stream), Result::CudaFailure);
-    if (ahead) CTX_HIP(hipEventRecord(ready_ev, stream), Result::CudaFailure);
+    CTX_HIP(hipEventRecord(ready_ev, stream), Result::CudaFailure);
     CTX_NTC(ntc_split_scatter(&a, split_ws.ptr, split_ws.bytes, stream));
   }
   else
@@ -881,13 +890,35 @@ Result Context::Impl::RunFusedStep(const StepPlan& st, int32_t step_index, hipEv
     CTX_NTC(ntc_fused_train_step(&a, stream));
   }
   if (ev_core) CTX_HIP(hipEventRecord(ev[1], stream), Result::CudaFailure);
+  // With the three launches the decoder update runs on the second stream
+  // once the MLP launch is done, beside the scatter and the hot part (only
+  // the next step's MLP launch reads what it writes: the next step waits for
+  // it before its gather).
+  const ntc_feature_desc fd0 = FeatureDesc(0);
+  auto decoder_update = [&](hipStream_t s) -> Result {
+    CTX_NTC(ntc_decoder_update_bf16(&fd0, H, O, a.blocks, dw_part.as<float>(), st.lr_decoder, 0.9f, 0.999f,
+                                    st.bias1, st.bias2, dec_grad.as<float>(), dec_m.as<float>(), dec_v.as<float>(),
+                                    dec.as<float>(), wtab.as<uint16_t>(), sse_part.as<double>(),
+                                    scalars.as<double>() + 16, s));
+    return Result::Success;
+  };
+  if (split)
+  {
+    CTX_HIP(hipStreamWaitEvent(side, ready_ev, 0), Result::CudaFailure);
+    if (evd) CTX_HIP(hipEventRecord(evd[8], side), Result::CudaFailure);
+    Result r = decoder_update(side);
+    if (r != Result::Success) return r;
+    if (evd) CTX_HIP(hipEventRecord(evd[2], side), Result::CudaFailure);
+    side_pending = true;
+  }
   // the next step's catch-up, on the second stream beside this whole step
   // (its waves fill the issue slots the fused launch leaves idle): it waits
   // for the last step's gradient step, leaves out this step's footprint cells
   // when it is on the same level (the fused launch reads them and the hot
   // part below brings them current) and touches nothing else this step reads
   // or writes. Issued after the fused launch, so the fused workgroups, which
-  // need most of a CU's LDS, are placed first.
+  // need most of a CU's LDS, are placed first. With the three launches it
+  // follows the decoder update on the second stream.
   if (ahead)
   {
     const StepPlan& sn = plan[nx];
@@ -900,24 +931,25 @@ Result Context::Impl::RunFusedStep(const StepPlan& st, int32_t step_index, hipEv
                                -1.0f, 1.0f - g0_delta, -1.0f, 1.0f - g1_delta};
     // its step count: this step's hot part counts one more on the same level
     const int32_t upto = lazy_done[ln] + (ln == lvl ? 1 : 0);
-    CTX_HIP(hipStreamWaitEvent(side, ready_ev, 0), Result::CudaFailure);
+    if (!split) CTX_HIP(hipStreamWaitEvent(side, ready_ev, 0), Result::CudaFailure);
     hipEvent_t* evn = evd ? evd + kStepEvents : nullptr;
     if (evn) CTX_HIP(hipEventRecord(evn[3], side), Result::CudaFailure);
     CTX_NTC(ntc_grid_lazy_catchup_f32(&fdn, &tn, ln == lvl ? &a.tiles : nullptr, g0[ln].as<float>(),
                                       g1[ln].as<float>(), &adn, lazy_steps[ln].as<float>(), upto,
                                       lazy_last[ln][0].as<int32_t>(), lazy_last[ln][1].as<int32_t>(), side));
     if (evn) CTX_HIP(hipEventRecord(evn[4], side), Result::CudaFailure);
-    CTX_HIP(hipEventRecord(catch_ev, side), Result::CudaFailure);
     caught_step = nx;
+    side_pending = true;
   }
+  if (side_pending) CTX_HIP(hipEventRecord(catch_ev, side), Result::CudaFailure);
   // the decoder update refreshes the BF16 tables it changes (packed whole at
   // the start of training)
-  const ntc_feature_desc fd0 = FeatureDesc(0);
-  CTX_NTC(ntc_decoder_update_bf16(&fd0, H, O, a.blocks, dw_part.as<float>(), st.lr_decoder, 0.9f, 0.999f, st.bias1,
-                                  st.bias2, dec_grad.as<float>(), dec_m.as<float>(), dec_v.as<float>(),
-                                  dec.as<float>(), wtab.as<uint16_t>(), sse_part.as<double>(),
-                                  scalars.as<double>() + 16, stream));
-  if (evd) CTX_HIP(hipEventRecord(evd[2], stream), Result::CudaFailure);
+  if (!split)
+  {
+    Result r = decoder_update(stream);
+    if (r != Result::Success) return r;
+    if (evd) CTX_HIP(hipEventRecord(evd[2], stream), Result::CudaFailure);
+  }
   if (deferred)
   {
     CTX_NTC(ntc_grid_lazy_hot_f32(&fd, &a.tiles, g0[lvl].as<float>(), dg0[lvl].as<float>(), g1[lvl].as<float>(),
@@ -1580,6 +1612,11 @@ TrainProgress Context::Train(int32_t num_batches)
                    : I.RunPlainStep(st, I.batch_i);
     if (r != Result::Success) return error(r);
   }
+  if (I.side_pending)
+  {
+    if (hipStreamWaitEvent(I.stream, I.catch_ev, 0) != hipSuccess) return error(Result::CudaFailure);
+    I.side_pending = false;
+  }
   if (hipStreamSynchronize(I.stream) != hipSuccess) return error(Result::CudaFailure);
   if (timed)
   {
@@ -1592,13 +1629,13 @@ TrainProgress Context::Train(int32_t num_batches)
       if ((!I.split || I.timing == 1) && hipEventElapsedTime(&ms, ev[0], ev[1]) == hipSuccess) fused += ms;
       if (I.split && hipEventElapsedTime(&ms, ev[6], ev[7]) == hipSuccess) mlp += ms;
       if (I.timing != 1) continue;
-      if (hipEventElapsedTime(&ms, ev[1], ev[2]) == hipSuccess) decoder += ms;
+      if (hipEventElapsedTime(&ms, ev[I.split ? 8 : 1], ev[2]) == hipSuccess) decoder += ms;
       if (!I.plan[first + s].grid_update) continue;
       if (I.ev_lazy[(size_t)s])
       {
         // catch-up (before the fused kernel) + hot part
         if (hipEventElapsedTime(&ms, ev[3], ev[4]) == hipSuccess &&
-            hipEventElapsedTime(&ms2, ev[2], ev[5]) == hipSuccess)
+            hipEventElapsedTime(&ms2, ev[I.split ? 1 : 2], ev[5]) == hipSuccess)
         {
           catchup += ms;
           hot += ms2;

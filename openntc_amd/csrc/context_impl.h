@@ -75,8 +75,9 @@ struct Context::Impl
   static constexpr int32_t kOutPadded = 16;
   // timing events per fused step: fused kernel start/end, after the decoder
   // update + table pack, grid catch-up start/end, end of the grid update,
-  // MLP launch start/end of the three-launch step
-  static constexpr int32_t kStepEvents = 8;
+  // MLP launch start/end of the three-launch step, decoder update start (on
+  // the second stream with the three launches)
+  static constexpr int32_t kStepEvents = 9;
   int32_t min_dimension = Context::kMinDimension;
   int32_t mip_min_dim = 4;
   uint32_t seed = 123;
@@ -148,6 +149,10 @@ struct Context::Impl
   hipStream_t side = nullptr;
   hipEvent_t ready_ev = nullptr, catch_ev = nullptr;
   int32_t caught_step = -1, train_end = 0;
+  // work issued on side (the decoder update with the three launches, the
+  // next step's catch-up) that the context stream has not waited for yet:
+  // catch_ev is recorded after it
+  bool side_pending = false;
 
   // ---- timing (optional) ----
   // When enabled, Train records events around the fused step kernel, the
