@@ -375,6 +375,8 @@ Result Context::Impl::InitDevice()
           Result::CudaFailure);
   CTX_HIP(hipEventCreateWithFlags(&catch_ev, hipEventDisableTiming | hipEventDisableSystemFence),
           Result::CudaFailure);
+  CTX_HIP(hipEventCreateWithFlags(&dec_ev, hipEventDisableTiming | hipEventDisableSystemFence),
+          Result::CudaFailure);
   const size_t dbytes = sizeof(float) * (size_t)DecoderElems();
   CTX_HIP(dec.Alloc(dbytes), Result::AllocationFailure);
   CTX_HIP(dec_m.Alloc(dbytes), Result::AllocationFailure);
@@ -428,8 +430,10 @@ void Context::Impl::FreeAll()
   if (side) (void)hipStreamDestroy(side);
   if (ready_ev) (void)hipEventDestroy(ready_ev);
   if (catch_ev) (void)hipEventDestroy(catch_ev);
+  if (dec_ev) (void)hipEventDestroy(dec_ev);
   stream = side = nullptr;
-  ready_ev = catch_ev = nullptr;
+  ready_ev = catch_ev = dec_ev = nullptr;
+  side_pending = dec_pending = false;
 }
 
 Result Context::Impl::EnsurePlainBuffers(int64_t batch)
@@ -847,8 +851,9 @@ Result Context::Impl::RunFusedStep(const StepPlan& st, int32_t step_index, hipEv
   if (tiles_update) ++tiles_update_steps;
   if (side_pending)
   {
-    // the last step's work on the second stream (its decoder update with the
-    // three launches, this step's catch-up)
+    // the last step's work on the second stream: this step's catch-up (and
+    // with the three launches the last step's decoder update, which only the
+    // MLP launch below waits for)
     CTX_HIP(hipStreamWaitEvent(stream, catch_ev, 0), Result::CudaFailure);
     side_pending = false;
   }
@@ -878,6 +883,11 @@ Result Context::Impl::RunFusedStep(const StepPlan& st, int32_t step_index, hipEv
     // registers), not beside the MLP launch, whose waves fill every SIMD's
     // registers
     CTX_NTC(ntc_split_gather(&a, split_ws.ptr, split_ws.bytes, stream));
+    if (dec_pending)
+    {
+      CTX_HIP(hipStreamWaitEvent(stream, dec_ev, 0), Result::CudaFailure);
+      dec_pending = false;
+    }
     if (ev) CTX_HIP(hipEventRecord(ev[6], stream), Result::CudaFailure);
     CTX_NTC(ntc_split_mlp(&a, split_ws.ptr, split_ws.bytes, stream));
     if (ev) CTX_HIP(hipEventRecord(ev[7], stream), Result::CudaFailure);
@@ -902,15 +912,7 @@ Result Context::Impl::RunFusedStep(const StepPlan& st, int32_t step_index, hipEv
                                     scalars.as<double>() + 16, s));
     return Result::Success;
   };
-  if (split)
-  {
-    CTX_HIP(hipStreamWaitEvent(side, ready_ev, 0), Result::CudaFailure);
-    if (evd) CTX_HIP(hipEventRecord(evd[8], side), Result::CudaFailure);
-    Result r = decoder_update(side);
-    if (r != Result::Success) return r;
-    if (evd) CTX_HIP(hipEventRecord(evd[2], side), Result::CudaFailure);
-    side_pending = true;
-  }
+  if (split) CTX_HIP(hipStreamWaitEvent(side, ready_ev, 0), Result::CudaFailure);
   // the next step's catch-up, on the second stream beside this whole step
   // (its waves fill the issue slots the fused launch leaves idle): it waits
   // for the last step's gradient step, leaves out this step's footprint cells
@@ -938,10 +940,21 @@ Result Context::Impl::RunFusedStep(const StepPlan& st, int32_t step_index, hipEv
                                       g1[ln].as<float>(), &adn, lazy_steps[ln].as<float>(), upto,
                                       lazy_last[ln][0].as<int32_t>(), lazy_last[ln][1].as<int32_t>(), side));
     if (evn) CTX_HIP(hipEventRecord(evn[4], side), Result::CudaFailure);
+    CTX_HIP(hipEventRecord(catch_ev, side), Result::CudaFailure);
     caught_step = nx;
     side_pending = true;
   }
-  if (side_pending) CTX_HIP(hipEventRecord(catch_ev, side), Result::CudaFailure);
+  if (split)
+  {
+    // after the catch-up on the second stream: beside the hot part and the
+    // next step's gather
+    if (evd) CTX_HIP(hipEventRecord(evd[8], side), Result::CudaFailure);
+    Result r = decoder_update(side);
+    if (r != Result::Success) return r;
+    if (evd) CTX_HIP(hipEventRecord(evd[2], side), Result::CudaFailure);
+    CTX_HIP(hipEventRecord(dec_ev, side), Result::CudaFailure);
+    dec_pending = true;
+  }
   // the decoder update refreshes the BF16 tables it changes (packed whole at
   // the start of training)
   if (!split)
@@ -1616,6 +1629,11 @@ TrainProgress Context::Train(int32_t num_batches)
   {
     if (hipStreamWaitEvent(I.stream, I.catch_ev, 0) != hipSuccess) return error(Result::CudaFailure);
     I.side_pending = false;
+  }
+  if (I.dec_pending)
+  {
+    if (hipStreamWaitEvent(I.stream, I.dec_ev, 0) != hipSuccess) return error(Result::CudaFailure);
+    I.dec_pending = false;
   }
   if (hipStreamSynchronize(I.stream) != hipSuccess) return error(Result::CudaFailure);
   if (timed)

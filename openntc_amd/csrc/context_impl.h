@@ -149,10 +149,12 @@ struct Context::Impl
   hipStream_t side = nullptr;
   hipEvent_t ready_ev = nullptr, catch_ev = nullptr;
   int32_t caught_step = -1, train_end = 0;
-  // work issued on side (the decoder update with the three launches, the
-  // next step's catch-up) that the context stream has not waited for yet:
-  // catch_ev is recorded after it
-  bool side_pending = false;
+  // work issued on side that the context stream has not waited for yet: the
+  // next step's catch-up (catch_ev is recorded after it) and, with the three
+  // launches, the step's decoder update (dec_ev; waited for before the next
+  // MLP launch)
+  bool side_pending = false, dec_pending = false;
+  hipEvent_t dec_ev = nullptr;
 
   // ---- timing (optional) ----
   // When enabled, Train records events around the fused step kernel, the
