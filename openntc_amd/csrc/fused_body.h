@@ -268,22 +268,37 @@ __host__ __device__ __forceinline__ int kperm_col(int k)
 }
 
 // Column of element (row, col) of a table row of wd BF16 elements: the
-// 16-byte chunks of a row are XOR-permuted by row so that the sixteen rows one
-// MFMA operand read touches (same logical chunk) fall on different LDS banks
-// (64 banks of 4 bytes: without it rows of 256 bytes all start on bank 0).
-// An involution within the row; rows whose chunk count is not a power of two
-// are left as they are. For rows of 256 bytes or more the XOR key is row mod
-// 16 with its two 2-bit halves swapped: the transposing reads of a W^T
-// fragment take rows 4 g + k (k = 0..3) at chunks 4 a + q (q = 0..3) in one
-// 16-lane group, and the key 4 k + g puts those sixteen (k, q) on sixteen
-// different chunks (the plain key 4 g + k left four lanes on each).
+// 16-byte chunks of a row are XOR-permuted by a key of the row so that the
+// LDS reads of the MFMA operands spread over the 64 banks (4 bytes each).
+// gfx950 serves a 16-byte read in four cycles of 16 lanes, lanes {0-3, 8-11,
+// 20-23, 28-31} and {4-7, 12-15, 16-19, 24-27} of each half wave, and an
+// 8-byte (transposing) read in two cycles of 32 lanes (bank-conflict
+// counters of single-pattern kernels). An involution within the row; rows
+// whose chunk count is not a power of two are left as they are.
+//   * Rows of 256 bytes or more: key = row mod 16 with its bits 0..3 moved
+//     to key bits 0, 2, 3, 1. A forward fragment read (rows 16 r + i, chunk
+//     4 j + g) then puts each 16-lane cycle on 16 different chunks, and the
+//     transposing reads of a W^T fragment (rows 4 g + k, one 8-byte half of
+//     chunks 4 a + q) load every chunk of a 32-lane cycle exactly twice, the
+//     least those reads allow (they all take the same half of their chunk).
+//   * Rows of 128 bytes (two per 256 bytes of banks): key bit 1 = row bit 1,
+//     key bit 2 = row bit 2 ^ row bit 3. Forward fragment reads put each
+//     16-lane cycle on 16 different chunks of the two rows' 256 bytes, and
+//     the transposing reads of W^T (rows 4 g + k, both halves of chunk pairs
+//     2 t, 2 t + 1) put each 32-lane cycle on 64 different banks.
 __host__ __device__ __forceinline__ int swz(int row, int col, int wd)
 {
   const int nc = wd >> 3;
   if (nc & (nc - 1)) return col;
+  const int r = row & 15;
   if (nc >= 16)
   {
-    const int r = row & 15, key = ((r & 3) << 2) | (r >> 2);
+    const int key = (r & 1) | ((r >> 2) & 2) | ((r << 1) & 4) | ((r << 1) & 8);
+    return (((col >> 3) ^ key) << 3) | (col & 7);
+  }
+  if (nc == 8)
+  {
+    const int key = (r & 2) | ((((r >> 2) ^ (r >> 3)) & 1) << 2);
     return (((col >> 3) ^ key) << 3) | (col & 7);
   }
   const int per = 16 / nc;  // rows per 256 bytes

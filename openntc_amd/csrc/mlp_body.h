@@ -68,14 +68,53 @@ struct MlpCfg
   static_assert(MT % BM1 == 0 && MT % BN1 == 0 && (MT / BM1) * (MT / BN1) == W, "dW1 blocks");
   static_assert(FEAT <= FP && NCW <= FP / 16, "feature columns");
   static constexpr int WT = H * FP + H * H + 16 * H;
-  static constexpr int XS = FP + 8;      // LDS X row stride (elements)
+  // X rows of 64 slots and staging rows of 128 units and more are unpadded,
+  // their 16-byte chunks XOR-permuted by row (x_col, st_col); other widths
+  // keep a padded stride
+  static constexpr bool XSWZ = FP == 64;
+  static constexpr bool SSWZ = H >= 128 && (H & (H - 1)) == 0;
+  static constexpr int XS = XSWZ ? FP : FP + 8;  // LDS X row stride (elements)
   static constexpr int XR = PASS * XS;
-  static constexpr int SH = H + 16;      // staging row stride, as in the one-launch body
+  static constexpr int SH = SSWZ ? H : H + 16;   // staging row stride
   static constexpr int STAGE = 4 * 64 * SH + 64 * 16;
   static constexpr int LDS = WT + XR + STAGE;  // BF16 elements
   static_assert(2 * LDS + 16 * PASS <= 160 * 1024, "LDS budget");
   static_assert(STAGE * 2 >= 8 * 256, "final reduction reuses the staging region");
 };
+
+// Position in its row of column col of X row t (rows of 128 bytes): the
+// chunk XOR key has bit 1 = t bit 1, bit 2 = t bit 2 ^ t bit 3, as for the
+// 128-byte weight rows. The row reads of the layer-0 B operand (rows
+// 16 w + i, chunk 4 j + g) put each 16-lane cycle of a 16-byte read on 16
+// different chunks, and the transposing X^T reads of the decoder-gradient
+// rounds (rows 8 g' + 4 h + k, both halves of chunk pairs) put each 32-lane
+// cycle of an 8-byte read on 64 different banks.
+template <bool kSwz>
+__host__ __device__ __forceinline__ int x_col(int t, int col)
+{
+  if (!kSwz) return col;
+  const int key = (t & 2) | ((((t >> 2) ^ (t >> 3)) & 1) << 2);
+  return col ^ (key << 3);
+}
+
+// Position in its row of column col of staging row r (rows of 256 bytes or
+// more): chunk XOR key 2 (r mod 4) + (r & 8). The wave writes of a round
+// (rows 16 w + i, chunk 4 j + g; 16-byte writes go in eight cycles of lane
+// quads {0, 2}, {1, 3}, {4, 6}, {5, 7} of each half wave) and the
+// transposing reads of the operands (rows 8 g + k and 8 g + 4 + k, both
+// halves of a chunk pair) are free of bank conflicts. Rows r and r + 4 share
+// their key.
+template <bool kSwz>
+__host__ __device__ __forceinline__ int st_col(int r, int col)
+{
+  if (!kSwz) return col;
+  return col ^ ((((r & 3) << 1) | (r & 8)) << 3);
+}
+
+// dY staging rows (16 BF16, 32 bytes): row r sits in slot r ^ 4 (r bit 3),
+// so that the eight rows of a 32-lane transposing read (8 g + k) fill 256
+// different bytes.
+__host__ __device__ __forceinline__ int y_slot(int r) { return r ^ ((r >> 1) & 4); }
 
 // Feature row stride in HBM: F rounded up to 8 elements (16-byte chunks).
 __host__ __device__ constexpr int split_fxs(int F) { return (F + 7) & ~7; }
@@ -103,14 +142,14 @@ __device__ __forceinline__ XRows x_issue(const uint16_t* xg, int64_t pass, int p
   return x;
 }
 
-template <int kXL, int kThreads>
+template <int kXL, int kThreads, bool kSwz = false>
 __device__ __forceinline__ void x_commit(const XRows& x, uint16_t* xnf, int xs, int xchunks, int n, int tid)
 {
   auto put = [&](int c, const uint4& v) {
     if (c < n)
     {
       const int t = c / xchunks, q = c - t * xchunks;
-      *reinterpret_cast<uint4*>(&xnf[t * xs + 8 * q]) = v;
+      *reinterpret_cast<uint4*>(&xnf[t * xs + x_col<kSwz>(t, 8 * q)]) = v;
     }
   };
   put(tid, x.r0);
@@ -150,7 +189,10 @@ __device__ __forceinline__ void mlp_body(const SplitParams& P, const int vblk, c
     for (int i = tid; i < CF::WT / 8; i += kThreads) dst[i] = src[i];
     // X row columns past the stored features stay zero for the whole launch
     for (int i = tid; i < kPass * CF::XS; i += kThreads)
-      if (i % CF::XS >= FXS) xnf[i] = 0;
+    {
+      const int t = i / CF::XS;
+      if (x_col<CF::XSWZ>(t, i - t * CF::XS) >= FXS) xnf[i] = 0;
+    }
   }
 
   f32x4 dw1[BM1][BN1], dw0[MTW][NCW], dwo[NTO];
@@ -182,7 +224,7 @@ __device__ __forceinline__ void mlp_body(const SplitParams& P, const int vblk, c
     const PassMap pm = pass_map((int64_t)vblk, T, kPass);
     const XRows xr = x_issue<kXL, kThreads>(P.xg, vblk, kPass, FXS, xn, tid);
     targets_issue<CF, kNT>(tg, pm, (int64_t)vblk, P, orig_s, wave, n16, g);
-    x_commit<kXL, kThreads>(xr, xnf, CF::XS, xchunks, xn, tid);
+    x_commit<kXL, kThreads, CF::XSWZ>(xr, xnf, CF::XS, xchunks, xn, tid);
     targets_commit(tg, tgt_s, wave, n16, g);
   }
 
@@ -205,7 +247,8 @@ __device__ __forceinline__ void mlp_body(const SplitParams& P, const int vblk, c
 #pragma unroll
       for (int j = 0; j < KF; ++j)
 #pragma unroll
-        for (int nt = 0; nt < kNT; ++nt) xb[j][nt] = frag_lds(&xnf[tloc[nt] * CF::XS + 32 * j + 8 * g]);
+        for (int nt = 0; nt < kNT; ++nt)
+          xb[j][nt] = frag_lds(&xnf[tloc[nt] * CF::XS + x_col<CF::XSWZ>(tloc[nt], 32 * j + 8 * g)]);
       lds_ring<MT * KF, kRing>(
           [&](int i) {
             const int row = 16 * (i % MT) + n16;
@@ -367,20 +410,25 @@ __device__ __forceinline__ void mlp_body(const SplitParams& P, const int vblk, c
 #pragma unroll
         for (int j = 0; j < KH; ++j)
         {
-          const int at = row * CF::SH + 32 * j + 8 * g;
+          const int at = row * CF::SH + st_col<CF::SSWZ>(row, 32 * j + 8 * g);
           *reinterpret_cast<bf16x8*>(&stA[at]) = a0f[j][st];
           *reinterpret_cast<bf16x8*>(&stB[at]) = dz1f[j][st];
           *reinterpret_cast<bf16x8*>(&stC[at]) = dz0f[j][st];
           *reinterpret_cast<bf16x8*>(&stD[at]) = a1f[j][st];
         }
-        *reinterpret_cast<uint2*>(&stY[row * 16 + 4 * g]) =
+        *reinterpret_cast<uint2*>(&stY[y_slot(row) * 16 + 4 * g]) =
             make_uint2(pk_bf16(dy[st][0], dy[st][1]), pk_bf16(dy[st][2], dy[st][3]));
       }
       __syncthreads();
       const int trr0 = 8 * g + (n16 >> 2), tq = 4 * (n16 & 3);
-      auto frag_st = [&](const uint16_t* st, int stride, int kc, int c0) {
-        const uint16_t* p = st + (32 * kc + trr0) * stride + c0 + tq;
-        return frag_tr(p, p + 4 * stride);
+      auto frag_st = [&](const uint16_t* st, int kc, int c0) {
+        const int r = 32 * kc + trr0;
+        const uint16_t* p = st + r * CF::SH + st_col<CF::SSWZ>(r, c0 + tq);
+        return frag_tr(p, p + 4 * CF::SH);
+      };
+      auto frag_y = [&](int kc) {
+        const int r = 32 * kc + trr0;
+        return frag_tr(stY + y_slot(r) * 16 + tq, stY + y_slot(r + 4) * 16 + tq);
       };
       // dW1[p_out][p_in] += dZ1 A0^T over this wave's block of tiles; each
       // A0^T fragment read once and used for the block's row tiles
@@ -389,8 +437,8 @@ __device__ __forceinline__ void mlp_body(const SplitParams& P, const int vblk, c
       {
         bf16x8 a[BM1];
 #pragma unroll
-        for (int mt = 0; mt < BM1; ++mt) a[mt] = frag_st(stB, CF::SH, kc, 16 * (BM1 * mb1 + mt));
-        lds_ring_ld<BN1, kRing>([&](int c) { return frag_st(stA, CF::SH, kc, 16 * (BN1 * nb1 + c)); },
+        for (int mt = 0; mt < BM1; ++mt) a[mt] = frag_st(stB, kc, 16 * (BM1 * mb1 + mt));
+        lds_ring_ld<BN1, kRing>([&](int c) { return frag_st(stA, kc, 16 * (BN1 * nb1 + c)); },
                                 [&](int c, const bf16x8 b) {
 #pragma unroll
                                   for (int mt = 0; mt < BM1; ++mt) dw1[mt][c] = mfma(a[mt], b, dw1[mt][c]);
@@ -401,20 +449,20 @@ __device__ __forceinline__ void mlp_body(const SplitParams& P, const int vblk, c
       // for W = 4, 64 rnd + p for NT = 1)
       {
         constexpr int NC = NCW;
-        auto xrow = [&](int kc, int h) {
+        auto xt = [&](int kc, int h, int c) {
           const int p = 32 * kc + 8 * g + 4 * h + (n16 >> 2);
           const int t = kNT == 1 ? 64 * rnd + p : (p >> 4) * 16 * kNT + 16 * rnd + (p & 15);
-          return t * CF::XS;
+          return &xnf[t * CF::XS + x_col<CF::XSWZ>(t, c)];
         };
         bf16x8 a[MTW][2];
 #pragma unroll
         for (int mt = 0; mt < MTW; ++mt)
 #pragma unroll
-          for (int kc = 0; kc < 2; ++kc) a[mt][kc] = frag_st(stC, CF::SH, kc, CF::RW * wave + 16 * mt);
+          for (int kc = 0; kc < 2; ++kc) a[mt][kc] = frag_st(stC, kc, CF::RW * wave + 16 * mt);
         lds_ring_ld<2 * NC, kRing>(
             [&](int i) {
               const int c = 16 * (i % NC) + tq, kc = i / NC;
-              return frag_tr(&xnf[xrow(kc, 0) + c], &xnf[xrow(kc, 1) + c]);
+              return frag_tr(xt(kc, 0, c), xt(kc, 1, c));
             },
             [&](int i, const bf16x8 b) {
               const int kc = i / NC, c = i % NC;
@@ -426,14 +474,14 @@ __device__ __forceinline__ void mlp_body(const SplitParams& P, const int vblk, c
 #pragma unroll
       for (int kc = 0; kc < 2; ++kc)
       {
-        const bf16x8 a = frag_st(stY, 16, kc, 0);
+        const bf16x8 a = frag_y(kc);
 #pragma unroll
-        for (int c = 0; c < NTO; ++c) dwo[c] = mfma(a, frag_st(stD, CF::SH, kc, (wave * NTO + c) * 16), dwo[c]);
+        for (int c = 0; c < NTO; ++c) dwo[c] = mfma(a, frag_st(stD, kc, (wave * NTO + c) * 16), dwo[c]);
       }
       __syncthreads();
     }
     // the rounds ended with a barrier: X rows and target words are free
-    x_commit<kXL, kThreads>(xr, xnf, CF::XS, xchunks, xn, tid);
+    x_commit<kXL, kThreads, CF::XSWZ>(xr, xnf, CF::XS, xchunks, xn, tid);
     targets_commit(tg, tgt_s, wave, n16, g);
   }
 

@@ -29,6 +29,8 @@ import tempfile
 import numpy as np
 import pytest
 
+from tests.lds_banks import MIN_CYCLES, cycles
+
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 BODY = os.path.join(ROOT, "openntc_amd", "csrc", "fused_body.h")
 OBJ = os.path.join(ROOT, "openntc_amd", "_build", "kernels_fused.hip.o")
@@ -92,9 +94,13 @@ def swz(row, col, wd):
     nc = wd >> 3
     if nc & (nc - 1):
         return col
+    r = row & 15
     if nc >= 16:
-        r = row & 15
-        return (((col >> 3) ^ (((r & 3) << 2) | (r >> 2))) << 3) | (col & 7)
+        key = (r & 1) | ((r >> 2) & 2) | ((r << 1) & 4) | ((r << 1) & 8)
+        return (((col >> 3) ^ key) << 3) | (col & 7)
+    if nc == 8:
+        key = (r & 2) | ((((r >> 2) ^ (r >> 3)) & 1) << 2)
+        return (((col >> 3) ^ key) << 3) | (col & 7)
     per = 16 // nc
     return (((col >> 3) ^ ((row // per) & (nc - 1))) << 3) | (col & 7)
 
@@ -393,21 +399,38 @@ def _banks(byte_addrs, width):
 
 
 @pytest.mark.parametrize("j,r", [(j, r) for j in range(4) for r in range(8)])
-def test_transposing_reads_of_128_wide_tables_are_conflict_free(j, r):
-    """Within each 16-lane group, the 16 transposing reads (8 bytes each) of
-    a W1^T fragment at H = 128 cover 32 distinct banks, and the 16 plain
-    16-byte fragment reads of the forward W1 cover all 64."""
-    H = 128
-    for g in range(4):
-        for h in range(2):
-            addrs = []
-            for i in range(16):
-                row = tr_row(j, h, g, i)
-                col = kperm_col(16 * r + 4 * (i & 3))
-                addrs.append(2 * (row * H + swz(row, col, H)))
-            assert len(set(_banks(addrs, 8))) == 32, (g, h)
-        fwd = [2 * ((16 * r + i) * H + swz(16 * r + i, 32 * j + 8 * g, H)) for i in range(16)]
-        assert len(set(_banks(fwd, 16))) == 64
+def test_weight_table_reads_take_fewest_lds_cycles(j, r):
+    """gfx950 LDS cycles (tests/lds_banks.py) of the weight-table reads of one
+    fragment: the forward 16-byte reads of W1 (H = 128) and W0 (FP = 64)
+    take 4 cycles, free of bank conflicts; the transposing reads of W0^T
+    take 2; those of W1^T and Wout^T take 4, the least for reads whose 32
+    lanes all take the same 8-byte half of their 16-byte chunks (the kperm
+    column order), i.e. each bank holds two dwords and no more."""
+    H, FP = 128, 64
+    lanes = np.arange(64)
+    n16, g = lanes & 15, lanes >> 4
+    fwd1 = [2 * ((16 * r + i) * H + swz(16 * r + i, 32 * j + 8 * gg, H)) for i, gg in zip(n16, g)]
+    assert cycles("read16", fwd1) == MIN_CYCLES["read16"]
+    if j < FP // 32:
+        fwd0 = [2 * ((16 * r + i) * FP + swz(16 * r + i, 32 * j + 8 * gg, FP)) for i, gg in zip(n16, g)]
+        assert cycles("read16", fwd0) == MIN_CYCLES["read16"]
+    for h in range(2):
+        t1 = []
+        for i, gg in zip(n16, g):
+            row, col = tr_row(j, h, gg, i), kperm_col(16 * r + 4 * (i & 3))
+            t1.append(2 * (row * H + swz(row, col, H)))
+        assert cycles("read8", t1) == 2 * MIN_CYCLES["read8"]
+        if r < 2:
+            t0 = []
+            for i, gg in zip(n16, g):
+                row, col = tr_row(j, h, gg, i), 16 * r + 4 * (i & 3)
+                t0.append(2 * (row * FP + swz(row, col, FP)))
+            assert cycles("read8", t0) == MIN_CYCLES["read8"]
+    to = []
+    for i, gg in zip(n16, g):
+        o, col = 4 * gg + (i >> 2), kperm_col(16 * r + 4 * (i & 3))
+        to.append(2 * (16 * 0 + o * H + swz(o, col, H)))
+    assert cycles("read8", to) == 2 * MIN_CYCLES["read8"]
 
 
 @pytest.mark.parametrize("FP,NT", [(64, 1), (32, 2), (64, 2), (96, 2)])

@@ -29,7 +29,8 @@ import tempfile
 import numpy as np
 import pytest
 
-from tests.test_fused_lds_plan import LLVM, Region, kperm_col, kperm_pos, swz, tr_row
+from tests.lds_banks import MIN_CYCLES, cycles
+from tests.test_fused_lds_plan import LLVM, Region, kperm_col, kperm_pos, swz, tr_read, tr_row
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 OBJ = os.path.join(ROOT, "openntc_amd", "_build", "kernels_split.hip.o")
@@ -48,13 +49,28 @@ def _mlp_plan():
     BN1 = 4 if T1 >= 16 else (2 if T1 >= 4 else 1)
     BM1 = T1 // BN1
     WT = H * FP + H * H + 16 * H
-    XS = FP + 8
-    SH = H + 16
+    XS = FP if FP == 64 else FP + 8
+    SH = H if H >= 128 and H & (H - 1) == 0 else H + 16
     STAGE = 4 * 64 * SH + 64 * 16
     smem = WT + PASS * XS + STAGE
     total = 2 * smem + 4 * 128 + 4 * PASS * 4
     return dict(MT=MT, KF=KF, KH=KH, GT=GT, RW=RW, MTW=RW // 16, NTO=H // (16 * W), BM1=BM1, BN1=BN1,
                 NCW=(F + 15) // 16, WT=WT, XS=XS, SH=SH, STAGE=STAGE, total=total)
+
+
+def x_col(t, col):
+    """Column position in X row t (128-byte rows, chunk XOR key)."""
+    key = (t & 2) | ((((t >> 2) ^ (t >> 3)) & 1) << 2)
+    return col ^ (key << 3)
+
+
+def st_col(r, col):
+    """Column position in staging row r (256-byte rows, chunk XOR key)."""
+    return col ^ ((((r & 3) << 1) | (r & 8)) << 3)
+
+
+def y_slot(r):
+    return r ^ ((r >> 1) & 4)
 
 
 def _geo():
@@ -96,7 +112,7 @@ def test_mlp_lds_indices_in_bounds():
     for wave in range(W):
         tloc = wave * 16 + n16
         for j in range(P["KF"]):
-            xnf.touch(tloc * XS + 32 * j + 8 * g, 8)
+            xnf.touch(tloc * XS + x_col(tloc, 32 * j + 8 * g), 8)
             for r in range(P["MT"]):
                 row = 16 * r + n16
                 wt.touch(oW0 + row * FP + vswz(row, 32 * j + 8 * g, FP), 8)
@@ -120,41 +136,163 @@ def test_mlp_lds_indices_in_bounds():
         tgt.touch(tloc * 4 + g)
         trr0, tq = 8 * g + (n16 >> 2), 4 * (n16 & 3)
 
-        def frag_st(reg, stride, kc, c0):
+        def frag_st(reg, kc, c0):
             for h in range(2):
-                reg.touch((32 * kc + trr0 + 4 * h) * stride + c0 + tq, 4)
+                r = 32 * kc + trr0 + 4 * h
+                reg.touch(r * SH + st_col(r, c0 + tq), 4)
         mb1, nb1 = wave // (P["MT"] // P["BN1"]), wave % (P["MT"] // P["BN1"])
         for rnd in range(PASS // 64):
             if wave >> 2 == rnd:  # this wave's tile staged in round rnd
                 row = (wave & 3) * 16 + n16
                 for j in range(P["KH"]):
                     for st in (stA, stB, stC, stD):
-                        st.touch(row * SH + 32 * j + 8 * g, 8)
-                stY.touch(row * 16 + 4 * g, 4)
+                        st.touch(row * SH + st_col(row, 32 * j + 8 * g), 8)
+                stY.touch(y_slot(row) * 16 + 4 * g, 4)
             for kc in range(2):
                 for mt in range(P["BM1"]):
-                    frag_st(stB, SH, kc, 16 * (P["BM1"] * mb1 + mt))
+                    frag_st(stB, kc, 16 * (P["BM1"] * mb1 + mt))
                 for c in range(P["BN1"]):
-                    frag_st(stA, SH, kc, 16 * (P["BN1"] * nb1 + c))
+                    frag_st(stA, kc, 16 * (P["BN1"] * nb1 + c))
                 for mt in range(P["MTW"]):
-                    frag_st(stC, SH, kc, P["RW"] * wave + 16 * mt)
+                    frag_st(stC, kc, P["RW"] * wave + 16 * mt)
                 for h in range(2):
                     p = 32 * kc + 8 * g + 4 * h + (n16 >> 2)
                     t = 64 * rnd + p
                     for c in range(P["NCW"]):
-                        xnf.touch(t * XS + 16 * c + 4 * (n16 & 3), 4)
-                frag_st(stY, 16, kc, 0)
+                        xnf.touch(t * XS + x_col(t, 16 * c + 4 * (n16 & 3)), 4)
+                for h in range(2):
+                    stY.touch(y_slot(32 * kc + trr0 + 4 * h) * 16 + tq, 4)
                 for c in range(P["NTO"]):
-                    frag_st(stD, SH, kc, (wave * P["NTO"] + c) * 16)
+                    frag_st(stD, kc, (wave * P["NTO"] + c) * 16)
     # feature-row commit: chunk c of PASS * FXS / 8 -> row c / xchunks, 8 q
     xch = FXS // 8
     c = np.arange(PASS * xch)
-    xnf.touch((c // xch) * XS + 8 * (c % xch), 8)
+    xnf.touch((c // xch) * XS + x_col(c // xch, 8 * (c % xch)), 8)
     for r in (wt, xnf, stA, stB, stC, stD, stY, tgt):
         r.check()
     assert P["total"] <= LDS_LIMIT
     # the SSE reduction reuses the staging rows as 8 * 64 doubles
     assert 8 * 64 * W <= 2 * P["STAGE"]
+
+
+def _x_rows():
+    """X rows as the MLP launch lays them out: logical X[t][f] = 1000 t + f + 1
+    for the FXS stored features, zero padding up to FP, at x_col positions."""
+    XS = _mlp_plan()["XS"]
+    phys = np.full(PASS * XS, -1, np.int64)
+    xch = FXS // 8
+    for c in range(PASS * xch):
+        t, q = divmod(c, xch)
+        for e in range(8):
+            phys[t * XS + x_col(t, 8 * q) + e] = 1000 * t + 8 * q + e + 1
+    for i in range(PASS * XS):
+        t = i // XS
+        if x_col(t, i - t * XS) >= FXS:
+            phys[i] = 0
+    return phys
+
+
+def _want_x(t, f):
+    return 1000 * t + f + 1 if f < FXS else 0
+
+
+def test_mlp_swizzled_rows_form_the_operand_fragments():
+    """With the chunk-permuted X rows, staging rows and dY slots, every
+    operand fragment the MLP launch reads equals the logical one: layer-0 B
+    (X rows), the dW0 X^T fragments (transposing reads), the dW1 / dW0 /
+    dWout operands from the staging rows and the dY rows."""
+    P = _mlp_plan()
+    XS, SH = P["XS"], P["SH"]
+    assert XS == 64 and SH == H
+    phys = _x_rows()
+    assert (phys >= 0).all()  # every element written or zeroed
+    lanes = np.arange(64)
+    n16, g = lanes & 15, lanes >> 4
+    for wave in range(W):
+        tloc = wave * 16 + n16
+        for j in range(P["KF"]):
+            for lane in range(64):
+                t, c0 = tloc[lane], 32 * j + 8 * g[lane]
+                got = phys[t * XS + x_col(t, c0): t * XS + x_col(t, c0) + 8]
+                assert list(got) == [_want_x(t, c0 + e) for e in range(8)]
+    for rnd in range(PASS // 64):
+        for kc in range(2):
+            for c in range(P["NCW"]):
+                parts = []
+                for h in range(2):
+                    pp = 32 * kc + 8 * g + 4 * h + (n16 >> 2)
+                    t = 64 * rnd + pp
+                    parts.append(tr_read(phys, t * XS + np.array([x_col(a, 16 * c + 4 * (b & 3))
+                                                                  for a, b in zip(t, n16)])))
+                frag = np.concatenate(parts, axis=1)
+                for lane in range(64):
+                    want = [_want_x(64 * rnd + 32 * kc + 8 * g[lane] + e, 16 * c + n16[lane]) for e in range(8)]
+                    assert list(frag[lane]) == want, (rnd, kc, c, lane)
+    # staging: logical S[row][unit] = 1000 row + unit + 1 written by the waves
+    st = np.full(64 * SH, -1, np.int64)
+    for row in range(64):
+        for j in range(P["KH"]):
+            for gg in range(4):
+                for e in range(8):
+                    st[row * SH + st_col(row, 32 * j + 8 * gg) + e] = 1000 * row + 32 * j + 8 * gg + e + 1
+    assert (st >= 0).all()
+    trr0, tq = 8 * g + (n16 >> 2), 4 * (n16 & 3)
+    for kc in range(2):
+        for c0 in range(0, H, 16):
+            r0 = 32 * kc + trr0
+            a0 = r0 * SH + np.array([st_col(r, c0 + q) for r, q in zip(r0, tq)])
+            a1 = (r0 + 4) * SH + np.array([st_col(r + 4, c0 + q) for r, q in zip(r0, tq)])
+            frag = np.concatenate([tr_read(st, a0), tr_read(st, a1)], axis=1)
+            for lane in range(64):
+                want = [1000 * (32 * kc + 8 * g[lane] + e) + c0 + n16[lane] + 1 for e in range(8)]
+                assert list(frag[lane]) == want
+    y = np.full(64 * 16, -1, np.int64)
+    for row in range(64):
+        for gg in range(4):
+            for e in range(4):
+                y[y_slot(row) * 16 + 4 * gg + e] = 100 * row + 4 * gg + e + 1
+    assert (y >= 0).all()
+    for kc in range(2):
+        r0 = 32 * kc + trr0
+        frag = np.concatenate([tr_read(y, np.array([y_slot(r) for r in r0]) * 16 + tq),
+                               tr_read(y, np.array([y_slot(r + 4) for r in r0]) * 16 + tq)], axis=1)
+        for lane in range(64):
+            assert list(frag[lane]) == [100 * (32 * kc + 8 * g[lane] + e) + n16[lane] + 1 for e in range(8)]
+
+
+def test_mlp_lds_accesses_take_fewest_cycles():
+    """gfx950 LDS cycles (tests/lds_banks.py) of every per-wave access of the
+    MLP launch: X-row reads (layer-0 B) 4, X^T transposing reads 2, staging
+    writes 8, staging and dY transposing reads 2 — no bank conflicts. (The
+    weight-table reads are checked in test_fused_lds_plan.py.)"""
+    P = _mlp_plan()
+    XS, SH = P["XS"], P["SH"]
+    lanes = np.arange(64)
+    n16, g = lanes & 15, lanes >> 4
+    trr0, tq = 8 * g + (n16 >> 2), 4 * (n16 & 3)
+    for wave in range(W):
+        tloc = wave * 16 + n16
+        for j in range(P["KF"]):
+            a = [2 * (t * XS + x_col(t, 32 * j + 8 * gg)) for t, gg in zip(tloc, g)]
+            assert cycles("read16", a) == MIN_CYCLES["read16"]
+        row = (wave & 3) * 16 + n16
+        for j in range(P["KH"]):
+            a = [2 * (r * SH + st_col(r, 32 * j + 8 * gg)) for r, gg in zip(row, g)]
+            assert cycles("write16", a) == MIN_CYCLES["write16"]
+    for rnd in range(PASS // 64):
+        for kc in range(2):
+            for h in range(2):
+                pp = 32 * kc + 8 * g + 4 * h + (n16 >> 2)
+                t = 64 * rnd + pp
+                for c in range(P["NCW"]):
+                    a = [2 * (tt * XS + x_col(tt, 16 * c + 4 * (i & 3))) for tt, i in zip(t, n16)]
+                    assert cycles("read8", a) == MIN_CYCLES["read8"], (rnd, kc, h, c)
+                r = 32 * kc + trr0 + 4 * h
+                for c0 in range(0, H, 16):
+                    a = [2 * (rr * SH + st_col(rr, c0 + q)) for rr, q in zip(r, tq)]
+                    assert cycles("read8", a) == MIN_CYCLES["read8"], (kc, h, c0)
+                a = [2 * (y_slot(rr) * 16 + q) for rr, q in zip(r, tq)]
+                assert cycles("read8", a) == MIN_CYCLES["read8"]
 
 
 def test_mlp_global_rows_in_bounds():
