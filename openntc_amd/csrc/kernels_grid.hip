@@ -278,10 +278,51 @@ __device__ __forceinline__ float sqrt_rn_wide(float y)
   return r;
 }
 
+// Smallest / largest v over the wave (every lane active), wave-uniform.
+__device__ __forceinline__ float wave_minf(float v)
+{
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = fminf(v, __shfl_xor(v, off));
+  return __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, v)));
+}
+
+__device__ __forceinline__ float wave_maxf(float v)
+{
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off));
+  return __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, v)));
+}
+
+// a / b as the compiler's IEEE division computes it (reciprocal estimate,
+// one Newton step, quotient and two residual corrections: Markstein's
+// method) without its operand scaling (v_div_scale) and special-value
+// fix-up (v_div_fixup): equal to a / b bit for bit where both are the
+// identity, i.e. for a == 0 or 2^-90 <= |a|, b, |a / b| <= 2^60 (the scaling
+// only acts near the ends of the exponent range, the fix-up only on zeros,
+// infinities, NaNs and denormal or overflowing quotients). The residuals are
+// formed negated (-fma(b, q, -a) = fma(-b, q, a) except for an exact zero,
+// where it gives -0), so that a zero numerator keeps its sign as a / b does.
+__device__ __forceinline__ float div_rn_mid(float a, float b)
+{
+  const float r = __builtin_amdgcn_rcpf(b);
+  const float e = fmaf(-b, r, 1.0f);
+  const float r2 = fmaf(e, r, r);
+  const float q = a * r2;
+  const float rem = -fmaf(b, q, -a);
+  const float q2 = fmaf(rem, r2, q);
+  const float rem2 = -fmaf(b, q2, -a);
+  return fmaf(rem2, r2, q2);
+}
+
 // Zero-gradient steps lo .. upto - 1 of the lane's item (lo < upto, the
 // wave's smallest counter), applied from the lane's own counter on where run.
-__device__ __forceinline__ void lazy_steps(const LazyArgs& L, const GridSeg& s, int from, bool run, int lo, f4& mi,
-                                           f4& vi, f4& pi)
+// kCheckSqrt: some lane's y = v bias2 may fall below 2^-96 on the way (the
+// exact sqrtf's scaled range); fast_lo, fast_hi: the step's quotients take
+// div_rn_mid when lr is 0 or its bits lie in [fast_lo, fast_hi] (bounds over
+// every running lane of the wave, from lazy_steps).
+template <bool kCheckSqrt>
+__device__ __forceinline__ void lazy_step_loop(const LazyArgs& L, const GridSeg& s, int from, bool run, int lo,
+                                               uint32_t fast_lo, uint32_t fast_hi, f4& mi, f4& vi, f4& pi)
 {
   AdamConst c;
   c.b1 = L.b1;
@@ -300,6 +341,10 @@ __device__ __forceinline__ void lazy_steps(const LazyArgs& L, const GridSeg& s, 
     c.lr = lr;
     c.bias1 = bias1;
     c.bias2 = bias2;
+    // wave-uniform (scalar) choice of the division for this step: the bit
+    // patterns of non-negative floats order as the values do
+    const uint32_t lrb = (uint32_t)__builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, lr));
+    const bool fast_div = lrb == 0u || (lrb >= fast_lo && lrb <= fast_hi);
     if (run && o >= from)
     {
       // adam_one with g = 0, its gradient terms (1-b1) 0 and ((1-b2) 0) 0
@@ -313,12 +358,12 @@ __device__ __forceinline__ void lazy_steps(const LazyArgs& L, const GridSeg& s, 
         me[e] = fmaf(c.b1, mi[e], zm);
         vi[e] = fmaf(c.b2, vi[e], zv);
         y[e] = vi[e] * c.bias2;
-        low |= y[e] > 0.0f && y[e] < 0x1p-96f;
+        if (kCheckSqrt) low |= y[e] > 0.0f && y[e] < 0x1p-96f;
       }
       // the exact sqrtf where some lane of the wave has a tiny positive y,
       // else the same bits without its scaling and special-value steps
       float r[4];
-      if (__builtin_amdgcn_ballot_w64(low) != 0)
+      if (kCheckSqrt && __builtin_amdgcn_ballot_w64(low) != 0)
       {
 #pragma unroll
         for (int e = 0; e < 4; ++e) r[e] = __builtin_sqrtf(y[e]);
@@ -328,10 +373,21 @@ __device__ __forceinline__ void lazy_steps(const LazyArgs& L, const GridSeg& s, 
 #pragma unroll
         for (int e = 0; e < 4; ++e) r[e] = sqrt_rn_wide(y[e]);
       }
+      float q[4];
+      if (fast_div)
+      {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) q[e] = div_rn_mid(c.lr * (me[e] * c.bias1), r[e] + 1e-8f);
+      }
+      else
+      {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) q[e] = (c.lr * (me[e] * c.bias1)) / (r[e] + 1e-8f);
+      }
 #pragma unroll
       for (int e = 0; e < 4; ++e)
       {
-        const float pe = pi[e] - (c.lr * (me[e] * c.bias1)) / (r[e] + 1e-8f);
+        const float pe = pi[e] - q[e];
         mi[e] = me[e];
         // fmaxf(lo, fminf(hi, pe)) for every non-NaN pe (lo <= hi)
         pi[e] = __builtin_amdgcn_fmed3f(pe, s.lo, s.hi);
@@ -341,6 +397,69 @@ __device__ __forceinline__ void lazy_steps(const LazyArgs& L, const GridSeg& s, 
     bias1 = nb1;
     bias2 = nb2;
   }
+}
+
+// The step loop with the range checks it can leave out, from bounds of each
+// running lane over its k = upto - from steps (factors of 2 of margin
+// throughout; b1, b2 < 1 and every step's bias1 <= 1 / (1 - b1), bias2 <=
+// 1 / (1 - b2), as for t >= 1):
+//   * each step maps v to fl(b2 v), at least v b2' with b2' = b2 (1 - 2^-24),
+//     and y = fl(v bias2) >= v (1 - 2^-24), so y >= 2^-96 on every step when
+//     v >= 2^-95 b2'^-k (or v == 0: y stays 0, which the short sqrt gives);
+//   * |m| falls by at most b1' = b1 (1 - 2^-24) a step and never grows, so
+//     t = fl(m bias1) lies in [|m|min b1'^k / 2, 2 |m|max / (1 - b1)] and
+//     b = sqrt(y) + 1e-8 in [1e-8, 2 sqrt(v_max / (1 - b2)) + 1e-7];
+//     a = fl(lr t) and a / b stay in [2^-90, 2^60] for every lr in
+//     [2^-90 max(1, b_max) / t_min, 2^60 / (1e8 t_max)] (a == 0 where m or lr
+//     is zero).
+__device__ __forceinline__ void lazy_steps(const LazyArgs& L, const GridSeg& s, int from, bool run, int lo, f4& mi,
+                                           f4& vi, f4& pi)
+{
+  float tmin = __builtin_inff(), tmax = 0.0f, bmax = 0.0f;
+  bool sqrt_ok = true;
+  if (run)
+  {
+    const float k = (float)(L.upto - from);
+    // per-step decay in binades, rounded up (v_log_f32 is within 1 ulp)
+    const float d1 = -__builtin_amdgcn_logf(L.b1) * 1.001f + 1e-6f;
+    const float d2 = -__builtin_amdgcn_logf(L.b2) * 1.001f + 1e-6f;
+    float mmin = __builtin_inff(), mmax = 0.0f, vmin = __builtin_inff(), vmax = 0.0f;
+#pragma unroll
+    for (int e = 0; e < 4; ++e)
+    {
+      const float am = fabsf(mi[e]);
+      if (am != 0.0f) mmin = fminf(mmin, am);
+      mmax = fmaxf(mmax, am);
+      if (vi[e] != 0.0f) vmin = fminf(vmin, vi[e]);
+      vmax = fmaxf(vmax, vi[e]);
+    }
+    // log2 of the smallest v after k steps, one binade of margin
+    sqrt_ok = vmin == __builtin_inff() || __builtin_amdgcn_logf(vmin) - k * d2 - 1.0f >= -95.0f;
+    if (mmin != __builtin_inff()) tmin = __builtin_amdgcn_exp2f(__builtin_amdgcn_logf(mmin) - k * d1 - 2.0f);
+    tmax = 2.0f * mmax / (1.0f - L.b1);
+    bmax = 2.0f * __builtin_sqrtf(vmax / (1.0f - L.b2)) + 1e-7f;
+  }
+  const bool sqrt_ok_w = __builtin_amdgcn_ballot_w64(!sqrt_ok) == 0;
+  tmin = wave_minf(tmin);
+  tmax = wave_maxf(tmax);
+  bmax = wave_maxf(bmax);
+  // the lr range of the short division (empty: hi < lo); NaN bounds fail
+  // every comparison below and leave it empty
+  uint32_t fast_lo = 0xFFFFFFFFu, fast_hi = 0u;
+  if (bmax <= 0x1p60f && tmin > 0.0f)
+  {
+    const float lr_lo = 0x1p-90f * fmaxf(1.0f, bmax) / tmin;  // 0 when every m is 0
+    const float lr_hi = tmax > 0.0f ? 0x1p60f / (1e8f * tmax) : 0x1p60f;
+    if (lr_lo <= lr_hi)
+    {
+      fast_lo = __builtin_bit_cast(uint32_t, lr_lo);
+      fast_hi = __builtin_bit_cast(uint32_t, lr_hi);
+    }
+  }
+  if (sqrt_ok_w)
+    lazy_step_loop<false>(L, s, from, run, lo, fast_lo, fast_hi, mi, vi, pi);
+  else
+    lazy_step_loop<true>(L, s, from, run, lo, fast_lo, fast_hi, mi, vi, pi);
 }
 
 // Catch-up of one float4 item per lane to step L.upto (from: the item's

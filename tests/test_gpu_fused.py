@@ -564,6 +564,80 @@ def test_grid_lazy_matches_eager(case, pipelined):
     compare(K)
 
 
+@pytest.mark.parametrize("seed", [0, 1])
+def test_grid_lazy_flush_matches_eager_at_extreme_magnitudes(seed):
+    """The catch-up's short forms (the square root without its scaled range,
+    the division without operand scaling and fix-up, each taken only where
+    bounds over the wave's steps allow) give the bits of the eager update:
+    40 zero-gradient steps of every cell (the one-pass update with a zero
+    gradient) against one flush from step 0, for Adam states spread over
+    the whole FP32 range (m from denormal to 1e8 and zero of either sign,
+    v from denormal to 1e36, where v bias2 overflows) and learning rates
+    from 0 and 1e-38 to 1e10, with runs of ordinary values in between so
+    that whole waves take the short forms."""
+    D0, D1, C0, C1, M, T, org = _PART_CASES["coarse_q3_q2"]
+    n_tiles, K = len(org) // 2, 40
+    lib = G.lib()
+    rng = np.random.default_rng(100 + seed)
+    fdc = L.FeatureDesc(D0, D1, C0, C1, 1, 4, 2, 0, 128)
+    shapes = [(D0, D0, C0), (D1, D1, C1)]
+
+    def spread(shp, lo, hi, signed):
+        x = (10.0 ** rng.uniform(lo, hi, shp)).astype(np.float32)
+        if signed:
+            x *= rng.choice(np.array([-1.0, 1.0], np.float32), shp)
+        return x
+
+    init = []
+    for shp in shapes:
+        p0 = rng.uniform(-1, 1, shp).astype(np.float32)
+        m0 = spread(shp, -44, 8, True)
+        v0 = spread(shp, -45, 36, False)
+        D = shp[0]
+        # ordinary rows (every wave of them takes the short forms), zero rows
+        m0[: D // 2] = (rng.standard_normal((D // 2,) + shp[1:]) * 1e-3).astype(np.float32)
+        v0[: D // 2] = rng.uniform(1e-9, 1e-6, (D // 2,) + shp[1:]).astype(np.float32)
+        m0[D // 2: D // 2 + 4] = -0.0
+        v0[D // 2 + 4: D // 2 + 8] = 0.0
+        init.append([p0, m0, v0, np.zeros(shp, np.float32)])
+    lrs = rng.uniform(0.002, 0.01, K)
+    lrs[[3, 17]] = 0.0
+    lrs[[5, 29]] = 1e-38
+    lrs[[8, 33]] = 1e10
+    lrs[21] = 3e-20
+    consts = np.stack([lrs, 1.0 / (1 - 0.9 ** np.arange(1, K + 1)),
+                       1.0 / (1 - 0.999 ** np.arange(1, K + 1))], 1).astype(np.float32)
+    bounds = (-1.0, 1.0 - 2.0 / 16, -1.0, 1.0 - 2.0 / 16)
+    eager = [[G.dev(x) for x in grid] for grid in init]
+    lazy = [[G.dev(x) for x in grid] for grid in init]
+    last = [G.dev(np.zeros(D0 * D0, np.int32)), G.dev(np.zeros(D1 * D1, np.int32))]
+    tab = G.dev(consts.ravel())
+    og = G.dev(np.array(org, np.int32))
+    tl = G.tiles(og, n_tiles, T, M, 0.0)
+
+    def adam(st, k):
+        lr, b1c, b2c = (float(x) for x in consts[k])
+        return L.GridAdam(G.p(st[0][1]), G.p(st[0][2]), G.p(st[1][1]), G.p(st[1][2]), lr, 0.9, 0.999, b1c, b2c,
+                          *bounds)
+
+    for k in range(K):
+        ad = adam(eager, k)
+        G.ok(lib.ntc_grid_update_tiles_f32(G.byref(fdc), G.byref(tl), G.p(eager[0][0]), G.p(eager[0][3]),
+                                           G.p(eager[1][0]), G.p(eager[1][3]), G.byref(ad), None), "eager")
+    ad = adam(lazy, 0)
+    G.ok(lib.ntc_grid_lazy_catchup_f32(G.byref(fdc), None, None, G.p(lazy[0][0]), G.p(lazy[1][0]), G.byref(ad),
+                                       G.p(tab), K, G.p(last[0]), G.p(last[1]), None), "flush")
+    for g in range(2):
+        for j, name in enumerate("pmv"):
+            got, want = G.host(lazy[g][j]), G.host(eager[g][j])
+            # NaN-free states: compare the bits
+            assert not np.isnan(want).any()
+            assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), (
+                seed, g, name, int(np.count_nonzero(got.view(np.uint32) != want.view(np.uint32))))
+        # the extreme rows moved p too (the steps were not all no-ops)
+        assert not np.array_equal(G.host(lazy[g][0]), init[g][0])
+
+
 @pytest.mark.parametrize("kind,lod,n_tiles", [("c1_bilinear_relu", 0, 8), ("c1_bilinear_relu", 1, 4),
                                               ("c2_bilinear", 0, 8), ("c1_bilinear_relu", 2, 8)])
 def test_fused_sets_partials_repeat_bit_for_bit(oracle, kind, lod, n_tiles):
