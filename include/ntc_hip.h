@@ -394,6 +394,42 @@ int ntc_split_mlp(const ntc_fused_args* args, void* workspace,
                   size_t workspace_bytes, hipStream_t stream);
 int ntc_split_scatter(const ntc_fused_args* args, void* workspace,
                       size_t workspace_bytes, hipStream_t stream);
+/* What follows the MLP launch of one training step, in one launch on one
+ * stream: ntc_split_scatter's latent gradients (args->scatter = 1), the
+ * catch-up of ntc_grid_lazy_catchup_f32 for the next step's tiles
+ * (catch_tiles non-null and catch_upto > 0; catch_skip as its skip, which
+ * must hold this step's tiles when the next step is on the same level) and
+ * ntc_decoder_update_bf16 of this step's partials (dec_p non-null; blocks,
+ * dw_partials and sse_partials from args, the table layout from dec_feat).
+ * The same arithmetic as those calls (the scatter's FP32 atomics in any
+ * order, as there); the three touch disjoint memory, so their workgroups run
+ * beside each other with no event or second stream between them. Replaces
+ * the reference's accumulate_grid_gradients, zero-gradient update_adam /
+ * clamp_grid of the level and the decoder's update_adam
+ * (libopenntc.cpp:697-778, ntc_kernel.cu:261-410, 468-482). */
+typedef struct ntc_split_tail_args
+{
+  const ntc_feature_desc* catch_feat;
+  const ntc_tiles* catch_tiles;
+  const ntc_tiles* catch_skip;
+  float* catch_g0;
+  float* catch_g1;
+  const ntc_grid_adam* catch_adam;
+  const float* catch_steps;
+  int32_t catch_upto;
+  int32_t* catch_last0;
+  int32_t* catch_last1;
+  const ntc_feature_desc* dec_feat;
+  float dec_lr, dec_beta1, dec_beta2, dec_bias1, dec_bias2;
+  float* dec_grad;
+  float* dec_m;
+  float* dec_v;
+  float* dec_p;
+  uint16_t* dec_table;
+  double* sse_out;
+} ntc_split_tail_args;
+int ntc_split_tail(const ntc_fused_args* args, void* workspace, size_t workspace_bytes,
+                   const ntc_split_tail_args* tail, hipStream_t stream);
 /* Sums the workgroup partials in order into grad, then Adam on W0 | W1 |
  * Wout rows < O (update_adam x3, libopenntc.cpp:722-754); sse_out (may be
  * NULL) receives the ordered sum of sse_partials. */

@@ -55,6 +55,18 @@ class FusedArgs(C.Structure):
                 ("sse_partials", C.c_void_p), ("blocks", C.c_int32)]
 
 
+class SplitTailArgs(C.Structure):
+    """ntc_split_tail_args: the next step's catch-up (catch_tiles non-null)
+    and the decoder update (dec_p non-null) of ntc_split_tail."""
+    _fields_ = [("catch_feat", C.c_void_p), ("catch_tiles", C.c_void_p), ("catch_skip", C.c_void_p),
+                ("catch_g0", C.c_void_p), ("catch_g1", C.c_void_p), ("catch_adam", C.c_void_p),
+                ("catch_steps", C.c_void_p), ("catch_upto", C.c_int32), ("catch_last0", C.c_void_p),
+                ("catch_last1", C.c_void_p), ("dec_feat", C.c_void_p), ("dec_lr", C.c_float),
+                ("dec_beta1", C.c_float), ("dec_beta2", C.c_float), ("dec_bias1", C.c_float),
+                ("dec_bias2", C.c_float), ("dec_grad", C.c_void_p), ("dec_m", C.c_void_p), ("dec_v", C.c_void_p),
+                ("dec_p", C.c_void_p), ("dec_table", C.c_void_p), ("sse_out", C.c_void_p)]
+
+
 class LayoutInfo(C.Structure):
     _fields_ = [("g0_bits", C.c_int32), ("g1_bits", C.c_int32), ("g0_channels", C.c_int32),
                 ("g1_channels", C.c_int32), ("g0_scale", C.c_int32), ("hidden_dim", C.c_int32),
@@ -218,6 +230,7 @@ SIGNATURES = {
     "ntc_split_gather": (_I, [_P, _P, _SZ, _P]),
     "ntc_split_mlp": (_I, [_P, _P, _SZ, _P]),
     "ntc_split_scatter": (_I, [_P, _P, _SZ, _P]),
+    "ntc_split_tail": (_I, [_P, _P, _SZ, _P, _P]),
     "ntc_grid_update_tiles_f32": (_I, [_P, _P, _P, _P, _P, _P, _P, _P]),
     "ntc_grid_update_tiles_part_f32": (_I, [_P, _P, _P, _P, _P, _P, _P, _I, _I, _P]),
     "ntc_grid_lazy_catchup_f32": (_I, [_P, _P, _P, _P, _P, _P, _P, _I, _P, _P, _P]),

@@ -876,12 +876,42 @@ Result Context::Impl::RunFusedStep(const StepPlan& st, int32_t step_index, hipEv
   // pair: every event between two of the step's kernels delays the second
   const bool ev_core = ev && (!split || timing == 1);
   if (ev_core) CTX_HIP(hipEventRecord(ev[0], stream), Result::CudaFailure);
+  // The next step's catch-up: the cells of its footprints, leaving out this
+  // step's footprint cells when it is on the same level (the hot part below
+  // brings them current), to its step count (this step's hot part counts one
+  // more on the same level).
+  const StepPlan* sn = ahead ? &plan[nx] : nullptr;
+  const int ln = sn ? sn->level : 0;
+  const ntc_feature_desc fdn = FeatureDesc(ln);
+  const ntc_tiles tn = {origins_table.as<int32_t>() + (int64_t)nx * 16, sn ? sn->n_tiles : 0, sn ? sn->tile_dim : 1,
+                        sn ? mip_dim[sn->lod] : 1, sn ? (float)sn->lod / (float)(mip_count - 1) : 0.0f};
+  const ntc_grid_adam adn = {g0m[ln].as<float>(), g0v[ln].as<float>(), g1m[ln].as<float>(), g1v[ln].as<float>(),
+                             sn ? sn->lr_grid : 0.0f, 0.9f, 0.999f, sn ? sn->grid_bias1 : 0.0f,
+                             sn ? sn->grid_bias2 : 0.0f, -1.0f, 1.0f - g0_delta, -1.0f, 1.0f - g1_delta};
+  const int32_t upto_n = sn ? lazy_done[ln] + (ln == lvl ? 1 : 0) : 0;
+  const ntc_feature_desc fd0 = FeatureDesc(0);
+  // With the three launches and no per-kernel timing, everything after the
+  // MLP launch but the hot part is one launch on this stream (scatter, the
+  // next step's catch-up, the decoder update: disjoint memory, so their
+  // workgroups overlap as two streams would, with no events between streams)
+  // unless the catch-up is large: at more than 4 float4 items per texel of
+  // this step (the next step on a coarser mip, whose tiles cover more cells)
+  // it runs faster as a kernel of its own (the tail's registers allow fewer
+  // waves), on the second stream beside the scatter and the hot part.
+  int64_t catch_items = 0;
+  if (sn)
+  {
+    const int Mn = mip_dim[sn->lod], Tn = sn->tile_dim;
+    const int Dn[2] = {fdn.g0_dim, fdn.g1_dim}, Qn[2] = {fdn.g0_channels / 4, fdn.g1_channels / 4};
+    for (int g = 0; g < 2; ++g)
+    {
+      const int64_t w = std::min<int64_t>(((int64_t)(Tn - 1) * Dn[g] + Mn - 1) / Mn + 3, Dn[g]);
+      catch_items += w * w * Qn[g] * sn->n_tiles;
+    }
+  }
+  const bool tail = split && timing != 1 && catch_items <= 4 * B;
   if (split)
   {
-    // the next step's catch-up starts once the MLP launch is done: beside the
-    // scatter, the decoder update and the hot part (memory-bound, few
-    // registers), not beside the MLP launch, whose waves fill every SIMD's
-    // registers
     CTX_NTC(ntc_split_gather(&a, split_ws.ptr, split_ws.bytes, stream));
     if (dec_pending)
     {
@@ -891,8 +921,46 @@ Result Context::Impl::RunFusedStep(const StepPlan& st, int32_t step_index, hipEv
     if (ev) CTX_HIP(hipEventRecord(ev[6], stream), Result::CudaFailure);
     CTX_NTC(ntc_split_mlp(&a, split_ws.ptr, split_ws.bytes, stream));
     if (ev) CTX_HIP(hipEventRecord(ev[7], stream), Result::CudaFailure);
-    CTX_HIP(hipEventRecord(ready_ev, stream), Result::CudaFailure);
-    CTX_NTC(ntc_split_scatter(&a, split_ws.ptr, split_ws.bytes, stream));
+    if (tail)
+    {
+      ntc_split_tail_args t = {};
+      if (ahead)
+      {
+        t.catch_feat = &fdn;
+        t.catch_tiles = &tn;
+        t.catch_skip = ln == lvl ? &a.tiles : nullptr;
+        t.catch_g0 = g0[ln].as<float>();
+        t.catch_g1 = g1[ln].as<float>();
+        t.catch_adam = &adn;
+        t.catch_steps = lazy_steps[ln].as<float>();
+        t.catch_upto = upto_n;
+        t.catch_last0 = lazy_last[ln][0].as<int32_t>();
+        t.catch_last1 = lazy_last[ln][1].as<int32_t>();
+        caught_step = nx;
+      }
+      t.dec_feat = &fd0;
+      t.dec_lr = st.lr_decoder;
+      t.dec_beta1 = 0.9f;
+      t.dec_beta2 = 0.999f;
+      t.dec_bias1 = st.bias1;
+      t.dec_bias2 = st.bias2;
+      t.dec_grad = dec_grad.as<float>();
+      t.dec_m = dec_m.as<float>();
+      t.dec_v = dec_v.as<float>();
+      t.dec_p = dec.as<float>();
+      t.dec_table = wtab.as<uint16_t>();
+      t.sse_out = scalars.as<double>() + 16;
+      CTX_NTC(ntc_split_tail(&a, split_ws.ptr, split_ws.bytes, &t, stream));
+    }
+    else
+    {
+      // the next step's catch-up starts once the MLP launch is done: beside
+      // the scatter, the decoder update and the hot part (memory-bound, few
+      // registers), not beside the MLP launch, whose waves fill every SIMD's
+      // registers
+      CTX_HIP(hipEventRecord(ready_ev, stream), Result::CudaFailure);
+      CTX_NTC(ntc_split_scatter(&a, split_ws.ptr, split_ws.bytes, stream));
+    }
   }
   else
   {
@@ -904,7 +972,6 @@ Result Context::Impl::RunFusedStep(const StepPlan& st, int32_t step_index, hipEv
   // once the MLP launch is done, beside the scatter and the hot part (only
   // the next step's MLP launch reads what it writes: the next step waits for
   // it before its gather).
-  const ntc_feature_desc fd0 = FeatureDesc(0);
   auto decoder_update = [&](hipStream_t s) -> Result {
     CTX_NTC(ntc_decoder_update_bf16(&fd0, H, O, a.blocks, dw_part.as<float>(), st.lr_decoder, 0.9f, 0.999f,
                                     st.bias1, st.bias2, dec_grad.as<float>(), dec_m.as<float>(), dec_v.as<float>(),
@@ -912,7 +979,7 @@ Result Context::Impl::RunFusedStep(const StepPlan& st, int32_t step_index, hipEv
                                     scalars.as<double>() + 16, s));
     return Result::Success;
   };
-  if (split) CTX_HIP(hipStreamWaitEvent(side, ready_ev, 0), Result::CudaFailure);
+  if (split && !tail) CTX_HIP(hipStreamWaitEvent(side, ready_ev, 0), Result::CudaFailure);
   // the next step's catch-up, on the second stream beside this whole step
   // (its waves fill the issue slots the fused launch leaves idle): it waits
   // for the last step's gradient step, leaves out this step's footprint cells
@@ -921,30 +988,20 @@ Result Context::Impl::RunFusedStep(const StepPlan& st, int32_t step_index, hipEv
   // or writes. Issued after the fused launch, so the fused workgroups, which
   // need most of a CU's LDS, are placed first. With the three launches it
   // follows the decoder update on the second stream.
-  if (ahead)
+  if (ahead && !tail)
   {
-    const StepPlan& sn = plan[nx];
-    const int ln = sn.level;
-    const ntc_feature_desc fdn = FeatureDesc(ln);
-    const ntc_tiles tn = {origins_table.as<int32_t>() + (int64_t)nx * 16, sn.n_tiles, sn.tile_dim,
-                          mip_dim[sn.lod], (float)sn.lod / (float)(mip_count - 1)};
-    const ntc_grid_adam adn = {g0m[ln].as<float>(), g0v[ln].as<float>(), g1m[ln].as<float>(), g1v[ln].as<float>(),
-                               sn.lr_grid, 0.9f, 0.999f, sn.grid_bias1, sn.grid_bias2,
-                               -1.0f, 1.0f - g0_delta, -1.0f, 1.0f - g1_delta};
-    // its step count: this step's hot part counts one more on the same level
-    const int32_t upto = lazy_done[ln] + (ln == lvl ? 1 : 0);
     if (!split) CTX_HIP(hipStreamWaitEvent(side, ready_ev, 0), Result::CudaFailure);
     hipEvent_t* evn = evd ? evd + kStepEvents : nullptr;
     if (evn) CTX_HIP(hipEventRecord(evn[3], side), Result::CudaFailure);
     CTX_NTC(ntc_grid_lazy_catchup_f32(&fdn, &tn, ln == lvl ? &a.tiles : nullptr, g0[ln].as<float>(),
-                                      g1[ln].as<float>(), &adn, lazy_steps[ln].as<float>(), upto,
+                                      g1[ln].as<float>(), &adn, lazy_steps[ln].as<float>(), upto_n,
                                       lazy_last[ln][0].as<int32_t>(), lazy_last[ln][1].as<int32_t>(), side));
     if (evn) CTX_HIP(hipEventRecord(evn[4], side), Result::CudaFailure);
     CTX_HIP(hipEventRecord(catch_ev, side), Result::CudaFailure);
     caught_step = nx;
     side_pending = true;
   }
-  if (split)
+  if (split && !tail)
   {
     // after the catch-up on the second stream: beside the hot part and the
     // next step's gather

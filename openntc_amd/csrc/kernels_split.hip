@@ -2,6 +2,8 @@
 // H = 128 decoder shapes, and its C ABI. The MLP body is in mlp_body.h; the
 // gather and the box gradients are the one-launch body's (fused_body.h), run
 // here one 128-texel pass per workgroup with no MLP beside them.
+#include "decoder_update.h"
+#include "grid_lazy.h"
 #include "mlp_body.h"
 
 namespace
@@ -254,7 +256,8 @@ __global__ __launch_bounds__(kGatherThreads) void k_split_gather(FusedParams P, 
 // per (cell, channel)) for grids whose pass footprint fits a box, else one
 // atomic per texel, slot and corner, with the gather's corner weights.
 template <int FP, int G, int C0, int C1, int G0MODE, int PEC>
-__global__ __launch_bounds__(kThreads) void k_split_scatter(FusedParams P, const float* __restrict__ dxg)
+__device__ __forceinline__ void split_scatter_pass(const FusedParams& P, const float* __restrict__ dxg,
+                                                   const int64_t pass)
 {
   using GC = SplitGeo<FP, G, C0, C1, G0MODE, PEC>;
   __shared__ __attribute__((aligned(16))) float dxs[GC::PASS * GC::GS];
@@ -262,7 +265,6 @@ __global__ __launch_bounds__(kThreads) void k_split_scatter(FusedParams P, const
   __shared__ uint32_t orig_s[128];
   const int tid = threadIdx.x;
   const int wave = tid >> 6, lane = tid & 63;
-  const int64_t pass = blockIdx.x;
   const int T = P.tl.tile_dim, M = P.tl.mip_dim;
   const int D0 = P.fd.g0_dim, D1 = P.fd.g1_dim;
   if (tid < 2 * P.tl.n_tiles) orig_s[tid] = (uint32_t)P.tl.origins[tid] % (uint32_t)M;
@@ -327,6 +329,66 @@ __global__ __launch_bounds__(kThreads) void k_split_scatter(FusedParams P, const
   }
 }
 
+template <int FP, int G, int C0, int C1, int G0MODE, int PEC>
+__global__ __launch_bounds__(kThreads) void k_split_scatter(FusedParams P, const float* __restrict__ dxg)
+{
+  split_scatter_pass<FP, G, C0, C1, G0MODE, PEC>(P, dxg, blockIdx.x);
+}
+
+// What follows the MLP launch in one launch: the latent scatter, the next
+// step's catch-up of its footprint cells and the decoder update. They touch
+// disjoint memory (the scatter: dX rows and latent gradients; the catch-up:
+// p, m, v and step counters of cells outside this step's footprints; the
+// decoder update: the decoder partials, parameters, moments and BF16 tables),
+// so their workgroups may run in any order beside each other, as the same
+// kernels on two streams would, without the events between the streams
+// (each costs the command processor several microseconds between kernels).
+struct TailArgs
+{
+  int64_t n_scatter;  // scatter workgroups (one pass each)
+  int64_t n_catch;    // catch-up workgroups: catch_rb per rectangle slot
+  int32_t catch_rb;
+  int32_t n_dec_wg;   // decoder-update workgroups (one parameter per thread)
+  LazyArgs L;
+  int32_t n_dec, n_adam_w01, H, O, blocks, F, gslots;
+  const float* part;
+  DecAdam c;
+  float* grad;
+  float* m;
+  float* v;
+  float* p;
+  const double* sse_part;
+  double* sse_out;
+  TableLayout tly;
+  uint16_t* tab;
+};
+
+// Workgroups [0, n_dec_wg) update the decoder; the others are dealt to
+// scatter passes and catch-up blocks in proportion (workgroup r of them is a
+// scatter pass when floor((r + 1) n_scatter / n) steps past floor(r n_scatter
+// / n), n = n_scatter + n_catch: every pass once, and the catch-up index is
+// r minus the passes before it), so both kinds are in flight throughout.
+template <int FP, int G, int C0, int C1, int G0MODE, int PEC>
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(5))) void k_split_tail(FusedParams P, const float* __restrict__ dxg, TailArgs T)
+{
+  const int64_t b = blockIdx.x;
+  if (b < T.n_dec_wg)
+  {
+    decoder_update_elem((int)b * kThreads + (int)threadIdx.x, T.n_dec, T.n_adam_w01, T.H, T.O, T.blocks, T.part,
+                        T.c, T.grad, T.m, T.v, T.p, T.sse_part, T.sse_out, T.tly, T.F, T.gslots, T.tab);
+    return;
+  }
+  const int64_t r = b - T.n_dec_wg, n = T.n_scatter + T.n_catch;
+  const int64_t s0 = r * T.n_scatter / n, s1 = (r + 1) * T.n_scatter / n;
+  if (s1 > s0)
+    split_scatter_pass<FP, G, C0, C1, G0MODE, PEC>(P, dxg, s0);
+  else
+  {
+    const int64_t c = r - s0;
+    lazy_catchup_rects_body(T.L, (int)(c % T.catch_rb), T.catch_rb, (int)(c / T.catch_rb));
+  }
+}
+
 // MLP launch: 8 waves (two per SIMD, at most 256 registers each), one
 // 16-texel tile per wave, fragments read two ahead of their MFMA (the other
 // wave of the SIMD hides the rest of the latency).
@@ -340,6 +402,7 @@ __global__ __launch_bounds__(64 * kMlpWaves, 1) void k_mlp_train(SplitParams P)
 
 typedef void (*GatherKernel)(FusedParams, uint16_t*, int);
 typedef void (*ScatterKernel)(FusedParams, const float*);
+typedef void (*TailKernel)(FusedParams, const float*, TailArgs);
 typedef void (*MlpKernel)(SplitParams);
 
 struct SplitEntry
@@ -348,12 +411,13 @@ struct SplitEntry
   GatherKernel gather;
   MlpKernel mlp;
   ScatterKernel scatter;
+  TailKernel tail;
 };
 
 #define NTC_SPLIT(H, FP, G, C0, C1, MODE, PEC, LODS, ACT)                                                  \
   SplitEntry{shape_v<H, FP, G, C0, C1, MODE, PEC, LODS, ACT>(), k_split_gather<FP, G, C0, C1, MODE, PEC, LODS>, \
              k_mlp_train<H, FP, G, ACT, (MODE == NTC_G0_CORNERS ? 4 * C0 : C0) + C1 + 4 * PEC + LODS>,     \
-             k_split_scatter<FP, G, C0, C1, MODE, PEC>},
+             k_split_scatter<FP, G, C0, C1, MODE, PEC>, k_split_tail<FP, G, C0, C1, MODE, PEC>},
 
 // Layouts with the three-launch form: the H = 128 decoders with ReLU.
 const SplitEntry kSplitEntries[] = {NTC_SPLIT(128, 64, 32, 16, 16, NTC_G0_BILINEAR, 2, 0, NTC_ACT_RELU)};
@@ -488,6 +552,69 @@ int ntc_split_scatter(const ntc_fused_args* a, void* workspace, size_t workspace
   const int r = split_prepare(a, workspace, workspace_bytes, e, P, S, npass);
   if (r != NTC_OK) return r;
   if (a->scatter) hipLaunchKernelGGL(e->scatter, dim3((unsigned)npass), dim3(kThreads), 0, stream, P, S.dxg);
+  return NTC_LAUNCH_STATUS();
+}
+
+int ntc_split_tail(const ntc_fused_args* a, void* workspace, size_t workspace_bytes, const ntc_split_tail_args* t,
+                   hipStream_t stream)
+{
+  const SplitEntry* e;
+  FusedParams P;
+  SplitParams S;
+  int64_t npass;
+  int r = split_prepare(a, workspace, workspace_bytes, e, P, S, npass);
+  if (r != NTC_OK) return r;
+  NTC_RETURN_IF(!t, NTC_E_INVALID);
+  TailArgs T = {};
+  T.n_scatter = a->scatter ? npass : 0;
+  if (t->catch_tiles && t->catch_upto > 0)
+  {
+    // the catch-up touches no gradient: any non-null pointer passes the checks
+    r = lazy_args_of(t->catch_feat, t->catch_tiles, t->catch_g0, t->catch_g0, t->catch_g1, t->catch_g1,
+                     t->catch_adam, t->catch_steps, t->catch_upto, t->catch_last0, t->catch_last1, T.L);
+    if (r != NTC_OK) return r;
+    if (t->catch_skip)
+    {
+      NTC_RETURN_IF(!valid_tiles(t->catch_skip) || t->catch_skip->n_tiles > 16, NTC_E_INVALID);
+      T.L.skip = *t->catch_skip;
+      T.L.n_skip = t->catch_skip->n_tiles;
+    }
+    T.catch_rb = rect_blocks(t->catch_tiles, T.L.A, 256);
+    T.n_catch = (int64_t)T.catch_rb * 2 * t->catch_tiles->n_tiles;
+  }
+  if (t->dec_p)
+  {
+    const ntc_feature_desc* fd = t->dec_feat;
+    const int H = a->hidden_dim, O = a->out_dim;
+    NTC_RETURN_IF(!fd || !valid_feature_desc(fd) || fd->feature_dim != a->feat.feature_dim, NTC_E_INVALID);
+    NTC_RETURN_IF(!t->dec_grad || !t->dec_m || !t->dec_v || !t->dec_table, NTC_E_INVALID);
+    Shape s;
+    shape_of(*fd, H, 0, s);
+    const int F = fd->feature_dim;
+    T.n_dec = H * F + H * H + 16 * H;
+    T.n_adam_w01 = H * F + H * H;
+    T.H = H;
+    T.O = O;
+    T.blocks = a->blocks;
+    T.F = F;
+    T.gslots = g1_base(*fd) + fd->g1_channels;
+    T.part = a->dw_partials;
+    T.c = DecAdam{t->dec_lr, t->dec_beta1, t->dec_beta2, 1.0f - t->dec_beta1, 1.0f - t->dec_beta2, t->dec_bias1,
+                  t->dec_bias2};
+    T.grad = t->dec_grad;
+    T.m = t->dec_m;
+    T.v = t->dec_v;
+    T.p = t->dec_p;
+    T.sse_part = a->sse_partials;
+    T.sse_out = t->sse_out;
+    T.tly = TableLayout{s.H, s.FP, s.G};
+    T.tab = t->dec_table;
+    T.n_dec_wg = (T.n_dec + kThreads - 1) / kThreads;
+  }
+  const int64_t grid = T.n_dec_wg + T.n_scatter + T.n_catch;
+  NTC_RETURN_IF(grid > 0x7FFFFFFF, NTC_E_INVALID);
+  if (grid == 0) return NTC_OK;
+  hipLaunchKernelGGL(e->tail, dim3((unsigned)grid), dim3(kThreads), 0, stream, P, S.dxg, T);
   return NTC_LAUNCH_STATUS();
 }
 

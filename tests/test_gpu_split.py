@@ -10,6 +10,7 @@ import ctypes as C
 import numpy as np
 import pytest
 
+from openntc_amd import _lib as L
 from tests import gpu_util as G
 from tests.test_gpu_fused import REL_TOL_BF16, _check_step, _outputs, _step_case, bf16, rel
 
@@ -233,3 +234,110 @@ def test_context_uses_split_form_and_matches_one_launch():
         print(f"form {form}: {psnr[form]:.4f} dB", flush=True)
     assert psnr[0] > 20.0
     assert abs(psnr[0] - psnr[1]) < 0.1, psnr
+
+
+@pytest.mark.parametrize("dim,lod,n_tiles,n_next", [(1024, 0, 1, 3), (1024, 1, 2, 2), (2048, 1, 1, 4)])
+def test_split_tail_matches_separate_launches(oracle, dim, lod, n_tiles, n_next):
+    """ntc_split_tail (the scatter, the next step's catch-up and the decoder
+    update in one launch) against the three as separate launches
+    (ntc_split_scatter, ntc_grid_lazy_catchup_f32 with this step's tiles as
+    its skip, ntc_decoder_update_bf16) after the same gather and MLP launch:
+    the catch-up's p, m, v and step counters and the decoder's gradient,
+    moments, parameters, BF16 table and SSE bit for bit; the latent gradients
+    to FP32 atomic order. Mips larger than a tile, so that the next step's
+    rectangles reach cells outside this step's."""
+    lib = G.lib()
+    cs = _step_case(oracle, "c3_bilinear_h128", lod, n_tiles, dim)
+    assert cs["M"] > cs["T"]
+    fdc, H, act, B, Od, a = cs["fdc"], cs["H"], cs["act"], cs["B"], cs["Od"], cs["a"]
+    blocks = lib.ntc_split_blocks_for(G.byref(fdc), H, act, B, _max_blocks(lib))
+    out = _outputs(cs, blocks)
+    nbytes = lib.ntc_split_workspace_bytes(G.byref(fdc), H, act, B)
+    ws = G.empty((nbytes,), np.uint8)
+    a.dw_partials, a.sse_partials, a.blocks = G.p(out["part"]), G.p(out["ssep"]), blocks
+    a.dg0, a.dg1 = G.p(out["dg0"]), G.p(out["dg1"])  # set per run below; the gather checks them
+    G.ok(lib.ntc_split_gather(G.byref(a), G.p(ws), nbytes, None), "gather")
+    G.ok(lib.ntc_split_mlp(G.byref(a), G.p(ws), nbytes, None), "mlp")
+
+    rng = np.random.default_rng(100 + lod)
+    shapes = [cs["g0"].shape, cs["g1"].shape]
+    grids = []
+    for k, shp in enumerate(shapes):
+        p0 = rng.uniform(-1, 1, shp).astype(np.float32)
+        m0 = (rng.standard_normal(shp) * 1e-3).astype(np.float32)
+        v0 = rng.uniform(0, 1e-6, shp).astype(np.float32)
+        D = shp[0]
+        m0[: D // 4] = 0.0  # cells no gradient has reached yet
+        v0[: D // 4] = 0.0
+        grids.append((p0, m0, v0))
+    K, upto = 20, 13
+    consts = np.stack([rng.uniform(0.002, 0.01, K), 1.0 / (1 - 0.9 ** np.arange(1, K + 1)),
+                       1.0 / (1 - 0.999 ** np.arange(1, K + 1))], 1).astype(np.float32)
+    last = [rng.integers(0, upto + 1, shp[0] * shp[1]).astype(np.int32) for shp in shapes]
+    n_dec = cs["dec"].size
+    dec_m = (rng.standard_normal(n_dec) * 1e-3).astype(np.float32)
+    dec_v = rng.uniform(0, 1e-6, n_dec).astype(np.float32)
+    table0 = G.host(_table_copy(lib, cs))
+    M, T = cs["M"], cs["T"]
+    nxt = G.dev(rng.integers(0, M, 2 * n_next).astype(np.int32))
+    tn = G.tiles(nxt, n_next, T, M, 0.0)
+    bounds = (-1.0, 1.0 - 2.0 / 16, -1.0, 1.0 - 2.0 / 16)
+    tab = G.dev(consts.ravel())
+    lr, b1c, b2c = 0.004, 1.0 / (1 - 0.9 ** 6), 1.0 / (1 - 0.999 ** 6)
+
+    def state():
+        st = dict(g=[[G.dev(x) for x in g] for g in grids], last=[G.dev(x) for x in last],
+                  dg=[G.dev(np.zeros(shp, np.float32)) for shp in shapes], grad=G.dev(np.zeros(n_dec, np.float32)),
+                  m=G.dev(dec_m), v=G.dev(dec_v), p=G.dev(cs["dec"]), table=G.dev(table0),
+                  sse=G.dev(np.zeros(1, np.float64)))
+        st["adam"] = L.GridAdam(G.p(st["g"][0][1]), G.p(st["g"][0][2]), G.p(st["g"][1][1]), G.p(st["g"][1][2]),
+                                0.0, 0.9, 0.999, 0.0, 0.0, *bounds)
+        return st
+
+    def result(st):
+        r = {f"{n}{k}": G.host(st["g"][k][j]) for k in range(2) for j, n in enumerate("pmv")}
+        r.update({f"last{k}": G.host(st["last"][k]) for k in range(2)})
+        r.update({n: G.host(st[n]) for n in ("grad", "m", "v", "p", "table", "sse")})
+        return r, [G.host(d) for d in st["dg"]]
+
+    sep, tail = state(), state()
+    # separate launches
+    a.dg0, a.dg1 = G.p(sep["dg"][0]), G.p(sep["dg"][1])
+    G.ok(lib.ntc_split_scatter(G.byref(a), G.p(ws), nbytes, None), "scatter")
+    G.ok(lib.ntc_grid_lazy_catchup_f32(G.byref(fdc), G.byref(tn), G.byref(a.tiles), G.p(sep["g"][0][0]),
+                                       G.p(sep["g"][1][0]), G.byref(sep["adam"]), G.p(tab), upto,
+                                       G.p(sep["last"][0]), G.p(sep["last"][1]), None), "catch-up")
+    G.ok(lib.ntc_decoder_update_bf16(G.byref(fdc), H, Od, blocks, G.p(out["part"]), lr, 0.9, 0.999, b1c, b2c,
+                                     G.p(sep["grad"]), G.p(sep["m"]), G.p(sep["v"]), G.p(sep["p"]),
+                                     G.p(sep["table"]), G.p(out["ssep"]), G.p(sep["sse"]), None), "decoder update")
+    # one launch
+    a.dg0, a.dg1 = G.p(tail["dg"][0]), G.p(tail["dg"][1])
+    t = L.SplitTailArgs()
+    t.catch_feat, t.catch_tiles, t.catch_skip = C.addressof(fdc), C.addressof(tn), C.addressof(a.tiles)
+    t.catch_g0, t.catch_g1 = G.p(tail["g"][0][0]), G.p(tail["g"][1][0])
+    t.catch_adam, t.catch_steps, t.catch_upto = C.addressof(tail["adam"]), G.p(tab), upto
+    t.catch_last0, t.catch_last1 = G.p(tail["last"][0]), G.p(tail["last"][1])
+    t.dec_feat = C.addressof(fdc)
+    t.dec_lr, t.dec_beta1, t.dec_beta2, t.dec_bias1, t.dec_bias2 = lr, 0.9, 0.999, b1c, b2c
+    t.dec_grad, t.dec_m, t.dec_v, t.dec_p = G.p(tail["grad"]), G.p(tail["m"]), G.p(tail["v"]), G.p(tail["p"])
+    t.dec_table, t.sse_out = G.p(tail["table"]), G.p(tail["sse"])
+    G.ok(lib.ntc_split_tail(G.byref(a), G.p(ws), nbytes, G.byref(t), None), "ntc_split_tail")
+    rs, dgs = result(sep)
+    rt, dgt = result(tail)
+    for name in rs:
+        assert np.array_equal(rs[name], rt[name]), (name, int(np.count_nonzero(rs[name] != rt[name])))
+    # the catch-up did work: counters moved to upto inside the next rectangles
+    assert any((rt[f"last{k}"] == upto).sum() > (last[k] == upto).sum() for k in range(2))
+    for k in range(2):
+        scale = float(np.abs(dgs[k]).max())
+        assert scale > 0
+        np.testing.assert_allclose(dgt[k], dgs[k], rtol=0, atol=1e-5 * scale)
+
+
+def _table_copy(lib, cs):
+    """A device copy of the case's packed BF16 decoder table."""
+    n = lib.ntc_fused_table_elems(G.byref(cs["fdc"]), cs["H"])
+    d_dec = G.dev(cs["dec"])
+    tab = G.empty((n,), np.uint16)
+    G.ok(lib.ntc_pack_decoder_bf16(G.byref(cs["fdc"]), cs["H"], cs["Od"], G.p(d_dec), G.p(tab), None))
+    return tab
