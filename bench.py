@@ -38,6 +38,7 @@ LAYOUT = dict(g0_bits=4, g1_bits=4, g0_channels=16, g1_channels=16, g0_scale=2, 
 DIM = 4096
 TOTAL_STEPS = 8000
 TILES = 8
+TIMING_STRIDE = 8  # timed region: events around the MLP launch of every 8th step
 SEED_SET = 20262
 
 
@@ -584,10 +585,12 @@ def main():
     if prog.result != ntc.Result.Success:
         raise SystemExit(f"warmup failed: {prog}")
     log(f"warmup {a.warmup} steps done")
-    # events around the fused kernel only inside the timed region: an event
-    # between two of the step's kernels delays the second by a few
-    # microseconds; the other kernels are timed over a sample after it
+    # events around the fused kernel (the MLP launch of the three-launch
+    # step) only, and only in every TIMING_STRIDE-th step inside the timed
+    # region: each event between two of the step's kernels leaves the GPU idle
+    # for a few microseconds; the other kernels are timed over a sample after it
     lib.ntc_ctx_set_timing(ctx._h, 2)
+    lib.ntc_ctx_set_timing_stride(ctx._h, TIMING_STRIDE)
     lib.ntc_dev_synchronize()
     t0 = time.perf_counter()
     prog = ctx.Train(a.steps)
@@ -687,20 +690,33 @@ def main():
         # the dX rows and adds into the footprint cells' gradients (8 B/param)
         fxs = (F + 7) // 8 * 8
         gsl = -(-(lay.g0_channels + lay.g1_channels) // 16) * 16
-        flops, fbytes, gbytes, cbytes, hbytes, blks, dense, mbytes = [], [], [], [], [], [], [], []
-        for si in infos:
-            B = si.n_tiles * si.tile_dim ** 2
-            pl, pf = cells(si)
-            blocks = min(256, (B + (127 if split else 63)) // (128 if split else 64))
-            flops.append(6.0 * (F * H + H * H + H * O) * B)
-            blks.append(blocks)
-            if split:
-                mbytes.append(2.0 * fxs * B + O * B + 4.0 * gsl * B + 4.0 * blocks * n_dec)
-                fbytes.append(4.0 * pf + 2.0 * fxs * B + mbytes[-1] + 4.0 * gsl * B + 8.0 * pf)
-            else:
-                fbytes.append(4.0 * pf + O * B + 4.0 * blocks * n_dec)
-            if si.grid_update:
-                dense.append(grid_bytes(si, pl, pf)[1])
+        def launch_work(steps):
+            """Per step: flops, bytes of the fused launch (the three launches),
+            bytes of the MLP launch, its workgroups, dense-equivalent grid bytes."""
+            flops, fbytes, mbytes, blks, dense = [], [], [], [], []
+            for si in steps:
+                B = si.n_tiles * si.tile_dim ** 2
+                pl, pf = cells(si)
+                blocks = min(256, (B + (127 if split else 63)) // (128 if split else 64))
+                flops.append(6.0 * (F * H + H * H + H * O) * B)
+                blks.append(blocks)
+                if split:
+                    mbytes.append(2.0 * fxs * B + O * B + 4.0 * gsl * B + 4.0 * blocks * n_dec)
+                    fbytes.append(4.0 * pf + 2.0 * fxs * B + mbytes[-1] + 4.0 * gsl * B + 8.0 * pf)
+                else:
+                    fbytes.append(4.0 * pf + O * B + 4.0 * blocks * n_dec)
+                if si.grid_update:
+                    dense.append(grid_bytes(si, pl, pf)[1])
+            return flops, fbytes, mbytes, blks, dense
+
+        # the events of the timed region: every TIMING_STRIDE-th timed step
+        flops, fbytes, mbytes, blks, _ = launch_work(infos[::TIMING_STRIDE])
+        if split:
+            # the three launches together were timed over the sample after it
+            flops, fbytes, _, blks, _ = launch_work(dinfos)
+            mflops, _, mbytes, mblks, _ = launch_work(infos[::TIMING_STRIDE])
+        dense = launch_work(infos)[4]
+        gbytes, cbytes, hbytes = [], [], []
         for si in dinfos:
             if not si.grid_update:
                 continue
@@ -719,11 +735,11 @@ def main():
                                      "bytes_per_launch": float(np.mean(fbytes)),
                                      "workgroups": float(np.mean(blks))},
                 "decoder_update": {"ms": dec_ms},
-                "sample": (f"MLP launch: the {a.steps} timed steps; the three launches together, decoder update "
-                           f"and grid kernels: the {a.detail_steps} steps after them, an event around each"
-                           if split else
-                           f"fused kernel: the {a.steps} timed steps; decoder update and grid kernels: the "
-                           f"{a.detail_steps} steps after them, an event around each kernel")}
+                "sample": (f"MLP launch: every {TIMING_STRIDE}th of the {a.steps} timed steps; the three launches "
+                           f"together, decoder update and grid kernels: the {a.detail_steps} steps after them, an "
+                           f"event around each" if split else
+                           f"fused kernel: every {TIMING_STRIDE}th of the {a.steps} timed steps; decoder update and "
+                           f"grid kernels: the {a.detail_steps} steps after them, an event around each kernel")}
         if gbytes and lazy:
             c_gbs = float(np.mean(cbytes)) / (catch_ms * 1e-3) / 1e9
             hot_gbs = float(np.mean(hbytes)) / (hot_ms * 1e-3) / 1e9
@@ -744,18 +760,19 @@ def main():
         # level plus the fused launch would move, over the measured step time
         # (with deferred steps the grid part is not moved every step: this is
         # the equivalent rate, not measured traffic)
-        step_bytes = float(np.mean(fbytes)) + (float(np.sum(dense)) / len(infos) if dense else 0.0)
+        step_bytes = float(np.mean(launch_work(infos)[1])) + (float(np.sum(dense)) / len(infos) if dense else 0.0)
         kern["step"] = {"ms": dt * 1e3 / a.steps, "dense_equivalent_bytes": step_bytes,
                         "achieved": step_bytes / (dt / a.steps) / 1e9, "unit": "GB/s",
                         "frac": step_bytes / (dt / a.steps) / 1e9 / HBM_PEAK_GBS}
         if split:
-            mlp_tflops = np.mean(flops) / (mlp_ms * 1e-3) / 1e12
+            mlp_tflops = np.mean(mflops) / (mlp_ms * 1e-3) / 1e12
             mlp_gbs = np.mean(mbytes) / (mlp_ms * 1e-3) / 1e9
             kern["mlp_train"] = {"kernel": "k_mlp_train", "ms": mlp_ms, "tflops": mlp_tflops,
                                  "mfma_frac": mlp_tflops / BF16_PEAK_TFLOPS, "GBps": mlp_gbs,
-                                 "hbm_frac": mlp_gbs / HBM_PEAK_GBS, "flop_per_launch": float(np.mean(flops)),
-                                 "bytes_per_launch": float(np.mean(mbytes)), "workgroups": float(np.mean(blks)),
-                                 "sample": f"the {a.steps} timed steps, events right around the launch"}
+                                 "hbm_frac": mlp_gbs / HBM_PEAK_GBS, "flop_per_launch": float(np.mean(mflops)),
+                                 "bytes_per_launch": float(np.mean(mbytes)), "workgroups": float(np.mean(mblks)),
+                                 "sample": f"every {TIMING_STRIDE}th of the {a.steps} timed steps "
+                                           f"({len(infos[::TIMING_STRIDE])} launches), events right around the launch"}
         out["kernels"] = kern
         # the roofline names the longest-running kernel of the step
         if split:

@@ -740,6 +740,11 @@ int ntc_ctx_kernel_times(ntc_context* ctx, float* out, int32_t n);
  * step (fewer events between the step's kernels; the other figures of
  * ntc_ctx_kernel_times are then -1). Other values: InvalidState. */
 int ntc_ctx_set_timing(ntc_context* ctx, int32_t enabled);
+/* With timing 2: events only in the first of every `stride` steps of a Train
+ * call (stride >= 1; 1, the default: every step), each event pair costing
+ * the stream a few microseconds between two kernels; the averages of
+ * ntc_ctx_kernel_times and its count of steps timed are over those steps. */
+int ntc_ctx_set_timing_stride(ntc_context* ctx, int32_t stride);
 /* Form of the fused path's step, taken at the next BeginTraining: 0 (the
  * default) the three launches of ntc_split_train_step where the layout has
  * them, else the one-launch ntc_fused_train_step; 1 always the one-launch

@@ -205,6 +205,7 @@ SIGNATURES = {
     "ntc_ctx_kernel_times": (_I, [_P, _P, _I]),
     "ntc_ctx_flush_grids": (_I, [_P]),
     "ntc_ctx_set_timing": (_I, [_P, _I]),
+    "ntc_ctx_set_timing_stride": (_I, [_P, _I]),
     "ntc_ctx_set_fused_form": (_I, [_P, _I]),
     "ntc_ctx_fused_form": (_I, [_P]),
     "ntc_debug_inject_fault": (_I, [_I, _I]),

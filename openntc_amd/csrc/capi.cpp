@@ -420,6 +420,14 @@ int ntc_ctx_set_timing(ntc_context* c, int32_t enabled)
   return 0;
 }
 
+int ntc_ctx_set_timing_stride(ntc_context* c, int32_t stride)
+{
+  if (!c) return (int)Result::InvalidState;
+  if (stride < 1) return (int)Result::InvalidState;
+  c->ctx.GetImpl()->timing_stride = stride;
+  return 0;
+}
+
 int ntc_ctx_set_fused_form(ntc_context* c, int32_t form)
 {
   if (!c) return (int)Result::InvalidState;

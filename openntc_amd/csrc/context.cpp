@@ -1677,8 +1677,11 @@ TrainProgress Context::Train(int32_t num_batches)
   for (; I.batch_i < target; ++I.batch_i)
   {
     const StepPlan& st = I.plan[I.batch_i];
+    // timing 2: events only in the sampled steps
+    const bool sampled = timed && (I.timing != 2 || (I.batch_i - first) % I.timing_stride == 0);
     Result r = I.path == TrainPath::Fused
-                   ? I.RunFusedStep(st, I.batch_i, timed ? &I.events[(size_t)(I.batch_i - first) * Impl::kStepEvents] : nullptr)
+                   ? I.RunFusedStep(st, I.batch_i,
+                                    sampled ? &I.events[(size_t)(I.batch_i - first) * Impl::kStepEvents] : nullptr)
                    : I.RunPlainStep(st, I.batch_i);
     if (r != Result::Success) return error(r);
   }
@@ -1696,9 +1699,11 @@ TrainProgress Context::Train(int32_t num_batches)
   if (timed)
   {
     double fused = 0, decoder = 0, grid = 0, catchup = 0, hot = 0, mlp = 0;
-    int grids = 0, deferred = 0;
+    int grids = 0, deferred = 0, nsamp = 0;
     for (int s = 0; s < n; ++s)
     {
+      if (I.timing == 2 && s % I.timing_stride) continue;
+      ++nsamp;
       hipEvent_t* ev = &I.events[(size_t)s * Impl::kStepEvents];
       float ms = 0, ms2 = 0;
       if ((!I.split || I.timing == 1) && hipEventElapsedTime(&ms, ev[0], ev[1]) == hipSuccess) fused += ms;
@@ -1725,11 +1730,11 @@ TrainProgress Context::Train(int32_t num_batches)
         ++grids;
       }
     }
-    I.kernel_ms = {(!I.split || I.timing == 1) ? (float)(fused / n) : -1.0f, I.timing == 1 ? (float)(decoder / n) : -1.0f,
-                   grids ? (float)(grid / grids) : -1.0f, (float)n,
+    I.kernel_ms = {(!I.split || I.timing == 1) ? (float)(fused / nsamp) : -1.0f,
+                   I.timing == 1 ? (float)(decoder / nsamp) : -1.0f, grids ? (float)(grid / grids) : -1.0f, (float)nsamp,
                    (float)grids, (float)I.tiles_update_steps, deferred ? (float)(catchup / deferred) : -1.0f,
                    deferred ? (float)(hot / deferred) : -1.0f, (float)deferred,
-                   I.split ? (float)(mlp / n) : -1.0f};
+                   I.split ? (float)(mlp / nsamp) : -1.0f};
   }
   tp.phase_ = TrainPhase::TrainInProgress;
   tp.result_ = Result::Success;

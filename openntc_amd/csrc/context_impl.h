@@ -169,6 +169,7 @@ struct Context::Impl
   // the next one by a few microseconds), or with the three launches only the
   // MLP launch's; the other figures are then -1.
   int32_t timing = 0;
+  int32_t timing_stride = 1;  // timing 2: events in the first of every timing_stride steps
   int32_t tiles_update_steps = 0;
   std::vector<float> kernel_ms;
   std::vector<hipEvent_t> events;
